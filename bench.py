@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Time-to-1e-10 of Jacobi-PCG on one MI355X: FP64 vs mixed-precision refined.
+
+Workload: 3D Laplace stiffness of Q2 hexahedra on a 64^3 unit-cube mesh,
+Dirichlet rows eliminated (2,048,383 DOFs, 127,263,527 nonzeros, int32
+columns), b ~ U(-1,1) from SplitMix64 seed 1, x0 = 0, stop at relative
+residual 1e-10. Inputs are generated on the host and resident in HBM before
+timing starts.
+
+One step = one complete solve to 1e-10 with the mixed-precision refined PCG
+(the headline value, seconds per solve). The FP64 fused PCG is timed the same
+way and reported beside it, with the standalone SpMV bandwidths, the roofline
+of the dominant kernel (the FP32 SpMV q = A p inside the mixed solve, timed
+with HIP events on the solver stream over the timed steps) and the threaded
+CPU restatement of the reference CG on a bounded sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def spmv_bytes(rows, nnz, value_bytes):
+    """Algorithmic bytes of one SpMV y = A x: entries + columns, row extents, x read once, y written once."""
+    return (value_bytes + 4) * nnz + 4 * (rows + 1) + 2 * value_bytes * rows
+
+
+def time_spmv(torch, linalg, matrix, precision, repetitions=50):
+    import ctypes
+
+    lib = linalg.library()
+    dtype = torch.float64 if precision == 64 else torch.float32
+    x = torch.ones(matrix.columns, dtype=dtype, device="cuda:0")
+    y = torch.zeros(matrix.rows, dtype=dtype, device="cuda:0")
+    fn = lib.cie_hip_csr_product_f64 if precision == 64 else lib.cie_hip_csr_product_f32
+    stream = ctypes.c_void_p()
+    lib.cie_hip_context_stream(matrix.context.handle, ctypes.byref(stream))
+    torch.cuda.synchronize()
+    ext_stream = torch.cuda.ExternalStream(stream.value)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        fn(matrix.context.handle, matrix.handle, 0.0, ctypes.c_void_p(x.data_ptr()), matrix.columns, 1.0,
+           ctypes.c_void_p(y.data_ptr()), matrix.rows)
+    start.record(ext_stream)
+    for _ in range(repetitions):
+        fn(matrix.context.handle, matrix.handle, 0.0, ctypes.c_void_p(x.data_ptr()), matrix.columns, 1.0,
+           ctypes.c_void_p(y.data_ptr()), matrix.rows)
+    stop.record(ext_stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) * 1e-3 / repetitions
+
+
+def cpu_baseline(workload, iterations_to_tolerance, sample_iterations, threads):
+    import oracle
+
+    _, stats, _, seconds = oracle.pcg(workload.matrix, workload.rhs, max_iterations=sample_iterations,
+                                      relative_tolerance=1e-30, threads=threads)
+    per_iteration = seconds / max(1, stats.iterationCount)
+    return {
+        "value": per_iteration * (iterations_to_tolerance + 1),
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle threaded FP64 Jacobi-PCG (restated ConjugateGradients_impl.hpp:49-201) on the same "
+                   f"system, first {stats.iterationCount} iterations in {seconds:.2f} s on {threads} threads, "
+                   f"extrapolated to the {iterations_to_tolerance + 1} loop trips FP64 PCG needs for 1e-10"),
+    }
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=5)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--elements", type=int, default=64, help="hexahedra per dimension (64 = the benchmark)")
+    parser.add_argument("--tolerance", type=float, default=1e-10)
+    parser.add_argument("--cpu-iterations", type=int, default=20, help="CPU baseline sample (0 disables)")
+    parser.add_argument("--cpu-threads", type=int, default=0)
+    parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
+    args = parser.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(the solve path is single-device; see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cie_amd import linalg, systems
+    from cie_amd._lib import Statistics
+
+    t0 = time.perf_counter()
+    workload = systems.q2_laplace_3d(args.elements, tolerance=args.tolerance)
+    generate_seconds = time.perf_counter() - t0
+    csr = workload.matrix
+    rows, nnz = csr.rows, csr.nnz
+    print(f"[bench] generated {workload.name}: {rows} rows, {nnz} nonzeros in {generate_seconds:.1f}s",
+          file=sys.stderr, flush=True)
+
+    context = linalg.Context.default()
+    t0 = time.perf_counter()
+    matrix = linalg.CSRMatrix(csr, context)
+    upload_seconds = time.perf_counter() - t0
+    tune64 = matrix.autotune(64, 10)
+    tune32 = matrix.autotune(32, 10)
+    b = torch.from_numpy(workload.rhs).to("cuda:0")
+    x = torch.zeros_like(b)
+    settings = Statistics(100000, 0.0, args.tolerance)
+    fp64 = linalg.FusedPCG(matrix, settings)
+    mixed = linalg.MixedPCG(matrix, settings)
+    print(f"[bench] uploaded in {upload_seconds:.1f}s; launch fp64 {matrix.launch(64)} ({tune64 * 1e3:.3f} ms), "
+          f"fp32 {matrix.launch(32)} ({tune32 * 1e3:.3f} ms)", file=sys.stderr, flush=True)
+
+    def solve(solver):
+        x.zero_()
+        solver.product(0, b, 1, x)
+        return solver.info
+
+    for _ in range(args.warmup):
+        solve(mixed)
+        solve(fp64)
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    # Timed region: K mixed solves (the value), in-solve SpMV events on.
+    if not args.no_profile:
+        linalg.library().cie_hip_context_set_profiling(context.handle, 1)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    mixed_infos = [solve(mixed) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    mixed_wall = (time.perf_counter() - start) / args.steps
+    linalg.library().cie_hip_context_set_profiling(context.handle, 0)
+    print(f"[bench] mixed: {mixed_wall:.4f} s/solve", file=sys.stderr, flush=True)
+    mixed_x = x.cpu().numpy()
+
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    fp64_infos = [solve(fp64) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    fp64_wall = (time.perf_counter() - start) / args.steps
+    fp64_x = x.cpu().numpy()
+    print(f"[bench] fp64: {fp64_wall:.4f} s/solve", file=sys.stderr, flush=True)
+
+    # Tight FP64 reference solution for the error of both results.
+    tight = linalg.FusedPCG(matrix, Statistics(100000, 0.0, 1e-14))
+    solve(tight)
+    reference_x = x.cpu().numpy()
+    import numpy as np
+
+    def error(v):
+        return float(np.linalg.norm(v - reference_x) / np.linalg.norm(reference_x))
+
+    spmv64 = time_spmv(torch, linalg, matrix, 64)
+    spmv32 = time_spmv(torch, linalg, matrix, 32)
+    bytes64 = spmv_bytes(rows, nnz, 8)
+    bytes32 = spmv_bytes(rows, nnz, 4)
+
+    m = mixed_infos[-1]
+    f = fp64_infos[-1]
+    product_seconds = statistics.mean(i.productSeconds for i in mixed_infos if i.productSamples) \
+        if any(i.productSamples for i in mixed_infos) else spmv32
+    achieved = bytes32 / product_seconds / 1e9
+
+    result = {
+        "metric": "time-to-1e-10 residual (s) + SpMV HBM GB/s, FP64 vs mixed-refined PCG, 1 GPU",
+        "value": mixed_wall,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed_wall * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement",
+        "data": "synthetic (structured Q2 stiffness, SplitMix64 rhs)",
+        "config": {
+            "workload": f"3D Q2 Laplace, {args.elements}^3 hexahedra, Dirichlet eliminated, Jacobi-PCG to "
+                        f"relative residual {args.tolerance:g}",
+            "rows": rows, "nnz": nnz, "rhs_seed": 1, "x0": "zero",
+            "parallelism": "single GPU",
+        },
+        "mixed": {
+            "seconds": mixed_wall, "device_seconds": m.seconds, "iterations": int(m.stats.iterationCount),
+            "refinements": int(m.refinements), "relative_residual_fp64": m.stats.relativeResidual,
+            "true_relative_residual": m.trueRelativeResidual, "error_vs_tight_fp64": error(mixed_x),
+        },
+        "fp64": {
+            "seconds": fp64_wall, "device_seconds": f.seconds, "iterations": int(f.stats.iterationCount),
+            "relative_residual": f.stats.relativeResidual, "true_relative_residual": f.trueRelativeResidual,
+            "error_vs_tight_fp64": error(fp64_x),
+        },
+        "speedup_mixed_vs_fp64": fp64_wall / mixed_wall,
+        "spmv": {
+            "fp64_seconds": spmv64, "fp64_GBps": bytes64 / spmv64 / 1e9, "fp64_bytes": bytes64,
+            "fp32_seconds": spmv32, "fp32_GBps": bytes32 / spmv32 / 1e9, "fp32_bytes": bytes32,
+            "launch_fp64": list(matrix.launch(64)), "launch_fp32": list(matrix.launch(32)),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "kernel": "productKernel<float> (q = A p, FP32 plane, fused p.q) inside the mixed solve",
+            "bytes_per_launch": bytes32, "mean_launch_seconds": product_seconds,
+        },
+    }
+    if args.cpu_iterations > 0:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        result["cpu_baseline"] = cpu_baseline(workload, int(f.stats.iterationCount), args.cpu_iterations, threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

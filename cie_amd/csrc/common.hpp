@@ -1,0 +1,115 @@
+// Internal definitions shared by the HIP translation units of libcie_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/cie_hip.h"
+
+namespace ciehip {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kReduceBlock = 256;  // threads per block of the vector/reduction kernels
+constexpr int kMaxPartials = 4096; // block partials one grid reduction may produce
+constexpr int kXcds = 8;           // MI355X accelerator complex dies (blocks are dealt round-robin)
+
+// Error state: the C ABI never throws; each failure sets a thread-local message.
+int setError(int status, const std::string& message);
+
+#define CIE_HIP_CHECK(call)                                                               \
+    do {                                                                                  \
+        hipError_t cieHipError_ = (call);                                                 \
+        if (cieHipError_ != hipSuccess)                                                   \
+            return ::ciehip::setError(cieHipError_ == hipErrorOutOfMemory                 \
+                                          ? CIE_HIP_ERROR_OUT_OF_MEMORY                   \
+                                          : CIE_HIP_ERROR_HIP,                            \
+                                      std::string(#call) + ": " + hipGetErrorString(cieHipError_)); \
+    } while (0)
+
+// Device-side scalar state of one solve. Lives in device memory; kernels read
+// and update it so that a whole PCG iteration needs no host round trip.
+struct SolverState {
+    double rho;            // r^T z of the current iterate
+    double alpha;          // search scale p^T r / p^T A p
+    double beta;           // direction update scale
+    double rr;             // r^T r
+    double initialNorm;    // ||r_0||
+    double absolute;       // ||r||
+    double relative;       // ||r|| / ||r_0||
+    double absoluteTolerance;
+    double relativeTolerance;
+    double updateNorm;     // mixed: r^T r at the last FP64 refinement step
+    double updateThreshold;// mixed: refine when r^T r <= threshold * updateNorm
+    uint64_t iteration;    // Statistics::iterationCount
+    uint64_t maxIterations;
+    uint64_t refinements;  // mixed: FP64 refinement steps taken
+    int32_t done;          // 0 running, else a SolveEnd value
+    int32_t refine;        // mixed: 1 if the next step recomputes the true residual
+    int32_t restart;       // mixed: 1 resets the search direction at each refinement
+    int32_t pad;
+};
+
+enum SolveEnd : int32_t {
+    kRunning = 0,
+    kConverged = 1,
+    kMaxIterations = 2,
+    kNotSpd = 3,      // p^T A p <= 0
+    kBreakdown = 4,   // r^T z <= 0 before the loop
+};
+
+// Per-context scratch for grid reductions and scalar results.
+struct Scratch {
+    double* partials = nullptr;   // kMaxPartials * 4 doubles
+    unsigned* counters = nullptr; // 64 counters for last-block-done reductions
+    double* scalars = nullptr;    // 64 doubles of device scalar results
+    double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
+    SolverState* state = nullptr; // one solver state
+    SolverState* pinnedState = nullptr; // two pinned host slots (double-buffered polling)
+};
+
+}  // namespace ciehip
+
+struct cie_hip_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownsStream = false;
+    int computeUnits = 0;
+    ciehip::Scratch scratch;
+    bool profiling = false;
+    static constexpr int kProfileEvents = 64;
+    hipEvent_t profileEvents[2 * kProfileEvents] = {};
+};
+
+// Launch configuration of a CSR SpMV for one value precision.
+struct CsrLaunch {
+    int lanes = 0;    // lanes per row (a power of two up to 64)
+    int unroll = 0;   // entries per lane loaded before they are used
+    int block = 0;    // threads per block
+    int grid = 0;     // blocks of the row-chunked kernels (multiple of 8)
+};
+
+struct cie_hip_csr_s {
+    cie_hip_context context = nullptr;
+    int64_t rows = 0, columns = 0, nnz = 0;
+    int32_t* rowExtents = nullptr;
+    int32_t* columnIndices = nullptr;
+    double* entries64 = nullptr;
+    float* entries32 = nullptr;
+    bool ownsIndices = false, ownsEntries64 = false;
+    double* inverseDiagonal64 = nullptr;  // built lazily for the fused solvers
+    float* inverseDiagonal32 = nullptr;
+    double meanRowLength = 0.0;
+    int maxRowLength = 0;
+    CsrLaunch launch64, launch32;
+    void* workspace = nullptr;      // solver vectors, grown on demand, freed with the matrix
+    size_t workspaceBytes = 0;
+};
+
+namespace ciehip {
+// Returns workspace of at least `bytes` bytes owned by the matrix handle.
+int matrixWorkspace(cie_hip_csr matrix, size_t bytes, void** pointer);
+// Builds the FP64/FP32 inverse diagonal of the matrix if it is missing.
+int ensureInverseDiagonal(cie_hip_csr matrix, int precision);
+}  // namespace ciehip

@@ -1,0 +1,392 @@
+// Device CSR matrices, SpMV (LinearOperator::product), the Jacobi inverse
+// diagonal and the SpMV launch autotuner.
+#include "dispatch.hpp"
+#include "kernels.hpp"
+
+#include <limits>
+#include <vector>
+
+using namespace ciehip;
+
+namespace {
+
+template <class T>
+int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
+                  const T* entries) {
+    const int perChunk = rowsPerChunk(m->rows, launch);
+    const bool ok = withShape(launch.lanes, launch.unroll, [&](auto lanes, auto unroll) {
+        constexpr int L = decltype(lanes)::value;
+        constexpr int U = decltype(unroll)::value;
+        hipLaunchKernelGGL((csrProductKernel<T, T, L, U, kSpmvBlock>), dim3(launch.grid), dim3(kSpmvBlock), 0,
+                           m->context->stream, int(m->rows), perChunk, m->rowExtents, m->columnIndices, entries,
+                           in, out, inScale, outScale);
+    });
+    if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    CIE_HIP_CHECK(hipGetLastError());
+    return CIE_HIP_OK;
+}
+
+CsrLaunch& launchOf(cie_hip_csr m, int precision) { return precision == 32 ? m->launch32 : m->launch64; }
+
+void releaseMatrix(cie_hip_csr m) {
+    if (!m) return;
+    if (m->context) {
+        hipSetDevice(m->context->device);
+        hipStreamSynchronize(m->context->stream);
+    }
+    if (m->ownsIndices) {
+        hipFree(m->rowExtents);
+        hipFree(m->columnIndices);
+    }
+    if (m->ownsEntries64) hipFree(m->entries64);
+    if (m->entries32) hipFree(m->entries32);
+    if (m->inverseDiagonal64) hipFree(m->inverseDiagonal64);
+    if (m->inverseDiagonal32) hipFree(m->inverseDiagonal32);
+    if (m->workspace) hipFree(m->workspace);
+    delete m;
+}
+
+int rowStatistics(cie_hip_csr m, const int32_t* hostExtents) {
+    int maxLength = 0;
+    for (int64_t row = 0; row < m->rows; ++row)
+        maxLength = std::max(maxLength, hostExtents[row + 1] - hostExtents[row]);
+    m->maxRowLength = maxLength;
+    m->meanRowLength = m->rows ? double(m->nnz) / double(m->rows) : 0.0;
+    return CIE_HIP_OK;
+}
+
+int buildPlane32(cie_hip_csr m) {
+    if (m->entries32) return CIE_HIP_OK;
+    CIE_HIP_CHECK(hipMalloc(&m->entries32, sizeof(float) * std::max<int64_t>(1, m->nnz)));
+    if (m->nnz) {
+        hipLaunchKernelGGL((convertKernel<double, float>), dim3(vectorGrid(m->nnz)), dim3(kReduceBlock), 0,
+                           m->context->stream, m->nnz, m->entries64, m->entries32);
+        CIE_HIP_CHECK(hipGetLastError());
+    }
+    return CIE_HIP_OK;
+}
+
+template <class T>
+int inverseDiagonal(cie_hip_csr m, T* inverse, int64_t* failedRow) {
+    cie_hip_context ctx = m->context;
+    unsigned long long* failure = reinterpret_cast<unsigned long long*>(ctx->scratch.scalars);
+    unsigned long long* pinned = reinterpret_cast<unsigned long long*>(ctx->scratch.pinned);
+    pinned[0] = pinned[1] = std::numeric_limits<unsigned long long>::max();
+    CIE_HIP_CHECK(hipMemcpyAsync(failure, pinned, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
+    if (m->rows) {
+        hipLaunchKernelGGL((inverseDiagonalKernel<T>), dim3(vectorGrid(m->rows)), dim3(kReduceBlock), 0, ctx->stream,
+                           int(m->rows), m->rowExtents, m->columnIndices, m->entries64, inverse, failure);
+        CIE_HIP_CHECK(hipGetLastError());
+    }
+    CIE_HIP_CHECK(hipMemcpyAsync(pinned, failure, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const unsigned long long zero = pinned[0], missing = pinned[1];
+    if (zero == std::numeric_limits<unsigned long long>::max() && missing == zero) return CIE_HIP_OK;
+    // Report the first failing row, as the CPU loop does (DiagonalOperator.cpp:64-83).
+    const bool missingFirst = missing < zero;
+    const unsigned long long row = missingFirst ? missing : zero;
+    if (failedRow) *failedRow = int64_t(row);
+    return missingFirst
+        ? setError(CIE_HIP_ERROR_MISSING_DIAGONAL, "row " + std::to_string(row) + " has no diagonal entry")
+        : setError(CIE_HIP_ERROR_ZERO_DIAGONAL, "row " + std::to_string(row) + " has an explicit zero on its diagonal");
+}
+
+}  // namespace
+
+namespace ciehip {
+
+int matrixWorkspace(cie_hip_csr m, size_t bytes, void** pointer) {
+    if (m->workspaceBytes < bytes) {
+        if (m->workspace) {
+            CIE_HIP_CHECK(hipStreamSynchronize(m->context->stream));
+            CIE_HIP_CHECK(hipFree(m->workspace));
+            m->workspace = nullptr;
+            m->workspaceBytes = 0;
+        }
+        CIE_HIP_CHECK(hipMalloc(&m->workspace, bytes));
+        m->workspaceBytes = bytes;
+    }
+    *pointer = m->workspace;
+    return CIE_HIP_OK;
+}
+
+int ensureInverseDiagonal(cie_hip_csr m, int precision) {
+    if (m->rows != m->columns)
+        return setError(CIE_HIP_ERROR_SHAPE, "the Jacobi preconditioner needs a square matrix");
+    if (precision == 32) {
+        if (m->inverseDiagonal32) return CIE_HIP_OK;
+        float* inverse = nullptr;
+        CIE_HIP_CHECK(hipMalloc(&inverse, sizeof(float) * std::max<int64_t>(1, m->rows)));
+        const int status = inverseDiagonal<float>(m, inverse, nullptr);
+        if (status != CIE_HIP_OK) {
+            hipFree(inverse);
+            return status;
+        }
+        m->inverseDiagonal32 = inverse;
+        return CIE_HIP_OK;
+    }
+    if (m->inverseDiagonal64) return CIE_HIP_OK;
+    double* inverse = nullptr;
+    CIE_HIP_CHECK(hipMalloc(&inverse, sizeof(double) * std::max<int64_t>(1, m->rows)));
+    const int status = inverseDiagonal<double>(m, inverse, nullptr);
+    if (status != CIE_HIP_OK) {
+        hipFree(inverse);
+        return status;
+    }
+    m->inverseDiagonal64 = inverse;
+    return CIE_HIP_OK;
+}
+
+}  // namespace ciehip
+
+extern "C" {
+
+int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, int64_t nnz,
+                       const int32_t* rowExtents, const int32_t* columnIndices, const double* entries,
+                       int location, int planes, cie_hip_csr* matrix) {
+    if (!matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_create: null output");
+    *matrix = nullptr;
+    if (!context || rows < 0 || columns < 0 || nnz < 0 || !rowExtents || (nnz && (!columnIndices || !entries)))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_create: invalid arguments");
+    if (rows > INT32_MAX - 1 || columns > INT32_MAX || nnz > INT32_MAX)
+        return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_create: sizes exceed int32 indexing");
+    if (location != CIE_HIP_HOST && location != CIE_HIP_DEVICE)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_create: location must be host or device");
+    if (!(planes & (CIE_HIP_PLANE_F64 | CIE_HIP_PLANE_F32)))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_create: no value plane requested");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+
+    std::vector<int32_t> hostExtents;
+    const int32_t* extents = rowExtents;
+    if (location == CIE_HIP_HOST) {
+        // CSROperator.cpp:21-29 checks; here also monotone extents and column range.
+        if (rowExtents[0] != 0 || rowExtents[rows] != nnz)
+            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_create: row extents must start at 0 and end at nnz");
+        for (int64_t row = 0; row < rows; ++row)
+            if (rowExtents[row + 1] < rowExtents[row])
+                return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_create: row extents decrease at row " + std::to_string(row));
+        for (int64_t k = 0; k < nnz; ++k)
+            if (columnIndices[k] < 0 || columnIndices[k] >= columns)
+                return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_create: column index out of range at entry " + std::to_string(k));
+    } else {
+        hostExtents.resize(size_t(rows) + 1);
+        CIE_HIP_CHECK(hipMemcpy(hostExtents.data(), rowExtents, sizeof(int32_t) * (rows + 1), hipMemcpyDeviceToHost));
+        if (hostExtents[0] != 0 || hostExtents[rows] != nnz)
+            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_create: row extents must start at 0 and end at nnz");
+        extents = hostExtents.data();
+    }
+
+    auto* m = new cie_hip_csr_s;
+    m->context = context;
+    m->rows = rows;
+    m->columns = columns;
+    m->nnz = nnz;
+    auto bail = [&](int status, const std::string& message) {
+        releaseMatrix(m);
+        return setError(status, message);
+    };
+    const size_t extentBytes = sizeof(int32_t) * size_t(rows + 1);
+    const size_t indexBytes = sizeof(int32_t) * size_t(std::max<int64_t>(1, nnz));
+    const size_t entryBytes = sizeof(double) * size_t(std::max<int64_t>(1, nnz));
+    if (location == CIE_HIP_HOST) {
+        m->ownsIndices = m->ownsEntries64 = true;
+        if (hipMalloc(&m->rowExtents, extentBytes) != hipSuccess || hipMalloc(&m->columnIndices, indexBytes) != hipSuccess
+            || hipMalloc(&m->entries64, entryBytes) != hipSuccess)
+            return bail(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_csr_create: device allocation failed");
+        if (hipMemcpyAsync(m->rowExtents, rowExtents, extentBytes, hipMemcpyHostToDevice, context->stream) != hipSuccess
+            || (nnz && hipMemcpyAsync(m->columnIndices, columnIndices, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, context->stream) != hipSuccess)
+            || (nnz && hipMemcpyAsync(m->entries64, entries, sizeof(double) * nnz, hipMemcpyHostToDevice, context->stream) != hipSuccess))
+            return bail(CIE_HIP_ERROR_HIP, "cie_hip_csr_create: upload failed");
+    } else {
+        m->rowExtents = const_cast<int32_t*>(rowExtents);
+        m->columnIndices = const_cast<int32_t*>(columnIndices);
+        m->entries64 = const_cast<double*>(entries);
+    }
+    rowStatistics(m, extents);
+    if (planes & CIE_HIP_PLANE_F32) {
+        const int status = buildPlane32(m);
+        if (status != CIE_HIP_OK) return bail(status, cie_hip_last_error());
+    }
+    if (hipStreamSynchronize(context->stream) != hipSuccess)
+        return bail(CIE_HIP_ERROR_HIP, "cie_hip_csr_create: stream synchronisation failed");
+    for (CsrLaunch* launch : {&m->launch64, &m->launch32}) {
+        *launch = defaultLaunch(m->meanRowLength, context->computeUnits, rows);
+        launch->grid = spmvGrid(rows, *launch, context->computeUnits);
+    }
+    *matrix = m;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_destroy(cie_hip_csr matrix) {
+    releaseMatrix(matrix);
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64_t* nnz) {
+    if (!matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_shape: null matrix");
+    if (rows) *rows = matrix->rows;
+    if (columns) *columns = matrix->columns;
+    if (nnz) *nnz = matrix->nnz;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries) {
+    if (!matrix || (!entries && matrix->nnz)) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_update_entries: null argument");
+    cie_hip_context ctx = matrix->context;
+    CIE_HIP_CHECK(hipSetDevice(ctx->device));
+    if (!matrix->nnz) return CIE_HIP_OK;
+    if (matrix->ownsEntries64) {
+        if (entries != matrix->entries64)
+            CIE_HIP_CHECK(hipMemcpyAsync(matrix->entries64, entries, sizeof(double) * matrix->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        matrix->entries64 = const_cast<double*>(entries);
+    }
+    if (matrix->entries32) {
+        hipLaunchKernelGGL((convertKernel<double, float>), dim3(vectorGrid(matrix->nnz)), dim3(kReduceBlock), 0,
+                           ctx->stream, matrix->nnz, matrix->entries64, matrix->entries32);
+        CIE_HIP_CHECK(hipGetLastError());
+    }
+    CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (matrix->inverseDiagonal64) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal64)); matrix->inverseDiagonal64 = nullptr; }
+    if (matrix->inverseDiagonal32) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal32)); matrix->inverseDiagonal32 = nullptr; }
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int lanes, int unroll, int block) {
+    if (!matrix || (precision != 32 && precision != 64))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: invalid matrix or precision");
+    if (!validShape(lanes, unroll))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: lanes must be 1..64 (power of two), unroll 1, 2, 4 or 8");
+    if (block != kSpmvBlock)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: supported block size is 256");
+    CsrLaunch& launch = launchOf(matrix, precision);
+    launch.lanes = lanes;
+    launch.unroll = unroll;
+    launch.block = block;
+    launch.grid = spmvGrid(matrix->rows, launch, matrix->context->computeUnits);
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* lanes, int* unroll, int* block) {
+    if (!matrix || (precision != 32 && precision != 64))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_get_launch: invalid matrix or precision");
+    const CsrLaunch& launch = launchOf(matrix, precision);
+    if (lanes) *lanes = launch.lanes;
+    if (unroll) *unroll = launch.unroll;
+    if (block) *block = launch.block;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, double* bestSeconds) {
+    if (!matrix || (precision != 32 && precision != 64))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: invalid matrix or precision");
+    if (precision == 32 && !matrix->entries32)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
+    cie_hip_context ctx = matrix->context;
+    CIE_HIP_CHECK(hipSetDevice(ctx->device));
+    repetitions = std::max(1, repetitions);
+    const size_t valueSize = precision == 32 ? sizeof(float) : sizeof(double);
+    void* buffer = nullptr;
+    const size_t inBytes = valueSize * size_t(std::max<int64_t>(1, matrix->columns));
+    const size_t outBytes = valueSize * size_t(std::max<int64_t>(1, matrix->rows));
+    CIE_HIP_CHECK(hipMalloc(&buffer, inBytes + outBytes));
+    CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, inBytes + outBytes, ctx->stream));
+    hipEvent_t start, stop;
+    CIE_HIP_CHECK(hipEventCreate(&start));
+    CIE_HIP_CHECK(hipEventCreate(&stop));
+
+    // Candidates: lanes within a factor 4 of the starting shape, unroll 1..8.
+    const CsrLaunch seed = defaultLaunch(matrix->meanRowLength, ctx->computeUnits, matrix->rows);
+    std::vector<CsrLaunch> candidates;
+    for (int lanes = std::max(1, seed.lanes / 4); lanes <= std::min(64, seed.lanes * 4); lanes *= 2)
+        for (int unroll : {1, 2, 4, 8}) {
+            CsrLaunch c;
+            c.lanes = lanes;
+            c.unroll = unroll;
+            c.block = kSpmvBlock;
+            c.grid = spmvGrid(matrix->rows, c, ctx->computeUnits);
+            candidates.push_back(c);
+        }
+    double best = std::numeric_limits<double>::infinity();
+    CsrLaunch bestLaunch = seed;
+    bestLaunch.grid = spmvGrid(matrix->rows, seed, ctx->computeUnits);
+    int status = CIE_HIP_OK;
+    for (const CsrLaunch& c : candidates) {
+        for (int pass = 0; pass < 2 && status == CIE_HIP_OK; ++pass) {  // pass 0 warms up
+            if (hipEventRecord(start, ctx->stream) != hipSuccess) { status = CIE_HIP_ERROR_HIP; break; }
+            for (int r = 0; r < (pass ? repetitions : 1) && status == CIE_HIP_OK; ++r) {
+                if (precision == 32) {
+                    float* in = static_cast<float*>(buffer);
+                    status = launchProduct<float>(matrix, c, 0.0f, in, 1.0f, in + std::max<int64_t>(1, matrix->columns), matrix->entries32);
+                } else {
+                    double* in = static_cast<double*>(buffer);
+                    status = launchProduct<double>(matrix, c, 0.0, in, 1.0, in + std::max<int64_t>(1, matrix->columns), matrix->entries64);
+                }
+            }
+            if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
+                status = CIE_HIP_ERROR_HIP;
+                break;
+            }
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, start, stop);
+            if (pass && ms * 1e-3 / repetitions < best) {
+                best = ms * 1e-3 / repetitions;
+                bestLaunch = c;
+            }
+        }
+        if (status != CIE_HIP_OK) break;
+    }
+    hipEventDestroy(start);
+    hipEventDestroy(stop);
+    hipFree(buffer);
+    if (status != CIE_HIP_OK) return setError(status, std::string("cie_hip_csr_autotune: ") + cie_hip_last_error());
+    launchOf(matrix, precision) = bestLaunch;
+    if (bestSeconds) *bestSeconds = best;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_product_f64(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* in,
+                            int64_t inSize, double outScale, double* out, int64_t outSize) {
+    if (!context || !matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f64: null handle");
+    if (matrix->rows != outSize || matrix->columns != inSize)
+        return setError(CIE_HIP_ERROR_SHAPE, "Incompatible matrix-vector product: [" + std::to_string(matrix->rows) + "x"
+                                                 + std::to_string(matrix->columns) + "] * [" + std::to_string(inSize)
+                                                 + "] = [" + std::to_string(outSize) + "]");
+    if (!matrix->rows) return CIE_HIP_OK;
+    if (!in || !out) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f64: null vector");
+    if (!matrix->entries64) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f64: matrix has no FP64 plane");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    return launchProduct<double>(matrix, matrix->launch64, inScale, in, outScale, out, matrix->entries64);
+}
+
+int cie_hip_csr_product_f32(cie_hip_context context, cie_hip_csr matrix, float inScale, const float* in,
+                            int64_t inSize, float outScale, float* out, int64_t outSize) {
+    if (!context || !matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f32: null handle");
+    if (matrix->rows != outSize || matrix->columns != inSize)
+        return setError(CIE_HIP_ERROR_SHAPE, "Incompatible matrix-vector product: [" + std::to_string(matrix->rows) + "x"
+                                                 + std::to_string(matrix->columns) + "] * [" + std::to_string(inSize)
+                                                 + "] = [" + std::to_string(outSize) + "]");
+    if (!matrix->rows) return CIE_HIP_OK;
+    if (!in || !out) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f32: null vector");
+    if (!matrix->entries32) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f32: matrix has no FP32 plane");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    return launchProduct<float>(matrix, matrix->launch32, inScale, in, outScale, out, matrix->entries32);
+}
+
+int cie_hip_inverse_diagonal_f64(cie_hip_context context, cie_hip_csr matrix, double* inverse, int64_t* failedRow) {
+    if (!context || !matrix || (!inverse && matrix->rows))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_inverse_diagonal_f64: null argument");
+    if (matrix->rows != matrix->columns) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_inverse_diagonal_f64: matrix is not square");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    return inverseDiagonal<double>(matrix, inverse, failedRow);
+}
+
+int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, float* inverse, int64_t* failedRow) {
+    if (!context || !matrix || (!inverse && matrix->rows))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_inverse_diagonal_f32: null argument");
+    if (matrix->rows != matrix->columns) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_inverse_diagonal_f32: matrix is not square");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    return inverseDiagonal<float>(matrix, inverse, failedRow);
+}
+
+}  // extern "C"

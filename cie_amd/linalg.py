@@ -1,0 +1,418 @@
+"""Python mirror of the reference's solver interface, backed by the HIP C ABI.
+
+Class and method names follow linalg/packages/solvers (paths relative to the
+reference repository):
+
+    HIPSpace              LinalgSpaceLike (inc/LinalgSpace.hpp:10-35), like
+                          DefaultSpace / SYCLSpace (src/SYCLSpace.cpp:114-343)
+    LinearOperator        inc/LinearOperator.hpp:10-34
+    HIPCSROperator        CSROperator / SYCLCSROperator (src/SYCLCSROperator.cpp:13-89)
+    DiagonalOperator,
+    makeDiagonalOperator  src/DiagonalOperator.cpp:17-162
+    ConjugateGradients    impl/ConjugateGradients_impl.hpp:22-201 (generic, over
+                          any space; every step is one space/operator call)
+    FusedPCG, MixedPCG    the fused device solvers (cie_hip_pcg_f64 / _mixed)
+
+Vectors are torch tensors on a HIP device; torch only provides the memory.
+Every compute call goes through libcie_hip.so.
+"""
+import ctypes
+import math
+import sys
+from ctypes import byref, c_double, c_float, c_int, c_int64, c_void_p
+
+import numpy as np
+
+from . import _lib
+from ._lib import MixedSettings, SolveInfo, Statistics, check, library
+
+_DBL_MIN = sys.float_info.min
+_DBL_EPS = sys.float_info.epsilon
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class Context:
+    """One device + one HIP stream (cie_hip_context)."""
+
+    _defaults = {}
+
+    def __init__(self, device=0, stream=None):
+        self.device = device
+        self.handle = c_void_p()
+        check(library().cie_hip_context_create(device, stream, byref(self.handle)), "cie_hip_context_create")
+
+    @classmethod
+    def default(cls, device=0):
+        if device not in cls._defaults:
+            cls._defaults[device] = cls(device)
+        return cls._defaults[device]
+
+    def synchronize(self):
+        check(library().cie_hip_context_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_context_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _sync_torch():
+    torch = _torch()
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+
+def _dptr(tensor):
+    return c_void_p(tensor.data_ptr())
+
+
+class CSRMatrix:
+    """Device-resident CSR (int32 extents/columns, FP64 and optional FP32 planes).
+
+    Built from host arrays (uploaded) -- the device counterpart of a
+    CSRView<const double, const int> (utilities/inc/CSRView.hpp:11-86).
+    """
+
+    def __init__(self, csr, context=None, planes=_lib.PLANE_F64 | _lib.PLANE_F32):
+        self.context = context or Context.default()
+        self.rows, self.columns, self.nnz = int(csr.rows), int(csr.columns), int(csr.row_extents[-1])
+        extents = np.ascontiguousarray(csr.row_extents, dtype=np.int32)
+        cols = np.ascontiguousarray(csr.column_indices, dtype=np.int32)
+        vals = np.ascontiguousarray(csr.entries, dtype=np.float64)
+        self.handle = c_void_p()
+        check(library().cie_hip_csr_create(self.context.handle, self.rows, self.columns, self.nnz,
+                                           extents.ctypes.data_as(c_void_p), cols.ctypes.data_as(c_void_p),
+                                           vals.ctypes.data_as(c_void_p), _lib.HOST, planes, byref(self.handle)),
+              "cie_hip_csr_create")
+
+    def rowCount(self):
+        return self.rows
+
+    def columnCount(self):
+        return self.columns
+
+    def launch(self, precision=64):
+        lanes, unroll, block = c_int(), c_int(), c_int()
+        check(library().cie_hip_csr_get_launch(self.handle, precision, byref(lanes), byref(unroll), byref(block)))
+        return lanes.value, unroll.value, block.value
+
+    def set_launch(self, precision, lanes, unroll, block=256):
+        check(library().cie_hip_csr_set_launch(self.handle, precision, lanes, unroll, block), "cie_hip_csr_set_launch")
+
+    def autotune(self, precision=64, repetitions=10):
+        best = c_double()
+        check(library().cie_hip_csr_autotune(self.handle, precision, repetitions, byref(best)), "cie_hip_csr_autotune")
+        return best.value
+
+    def update_entries(self, entries):
+        """Replace the FP64 entries (device tensor, same pattern); refreshes the FP32 plane."""
+        _sync_torch()
+        check(library().cie_hip_csr_update_entries(self.handle, _dptr(entries)), "cie_hip_csr_update_entries")
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_csr_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HIPSpace:
+    """LinalgSpaceLike over device vectors of one value type (float64 or float32)."""
+
+    def __init__(self, dtype=None, context=None):
+        torch = _torch()
+        self.dtype = dtype or torch.float64
+        if self.dtype not in (torch.float64, torch.float32):
+            raise TypeError("HIPSpace supports float64 and float32")
+        self.context = context or Context.default()
+        self.device = torch.device("cuda", self.context.device)
+        self._suffix = "f64" if self.dtype == torch.float64 else "f32"
+        self._scalar = c_double if self.dtype == torch.float64 else c_float
+
+    # --- LinalgSpace.hpp:17-22 ---
+    @staticmethod
+    def view(vector):
+        return vector
+
+    @staticmethod
+    def size(view):
+        return int(view.numel())
+
+    def makeVector(self, size):
+        return _torch().empty(int(size), dtype=self.dtype, device=self.device)
+
+    def _call(self, name, *args):
+        _sync_torch()
+        return check(getattr(library(), f"cie_hip_{name}_{self._suffix}")(self.context.handle, *args), name)
+
+    def _check(self, *vectors):
+        for v in vectors:
+            if v.dtype != self.dtype or v.device != self.device or not v.is_contiguous():
+                raise TypeError(f"expected a contiguous {self.dtype} vector on {self.device}")
+
+    # --- LinalgSpace.hpp:24-29 ---
+    def innerProduct(self, left, right):
+        self._check(left, right)
+        if left.numel() != right.numel():
+            raise _lib.ShapeError(_lib.ERROR_SHAPE, "innerProduct: sizes differ")
+        result = self._scalar()
+        self._call("inner_product", _dptr(left), _dptr(right), left.numel(), byref(result))
+        return result.value
+
+    def scale(self, target, source_or_value, value=None):
+        """scale(t, s, a): t = a * t .* s;  scale(v, a): v *= a (DefaultSpace.cpp:80-122)."""
+        if value is None:
+            self._check(target)
+            self._call("scale_uniform", _dptr(target), target.numel(), self._scalar(source_or_value))
+        else:
+            self._check(target, source_or_value)
+            self._call("scale", _dptr(target), _dptr(source_or_value), target.numel(), self._scalar(value))
+
+    def add(self, target, source, value):
+        self._check(target, source)
+        self._call("add", _dptr(target), _dptr(source), target.numel(), self._scalar(value))
+
+    def assign(self, target, source):
+        """Device<-device copy, or host<->device when one side is a numpy array (SYCLSpace.hpp:116-122)."""
+        if isinstance(source, np.ndarray):
+            self._check(target)
+            target.copy_(_torch().from_numpy(np.ascontiguousarray(source, dtype=np.float64 if self._suffix == "f64"
+                                                                  else np.float32)))
+            _sync_torch()
+        elif isinstance(target, np.ndarray):
+            self._check(source)
+            _sync_torch()
+            target[...] = source.cpu().numpy()
+        else:
+            self._check(target, source)
+            self._call("assign", _dptr(target), _dptr(source), target.numel())
+
+    def fill(self, target, value):
+        self._check(target)
+        self._call("fill", _dptr(target), target.numel(), self._scalar(value))
+
+    def getContext(self):
+        return self.context
+
+
+class LinearOperator:
+    """out = inScale * out + outScale * A * in (LinearOperator.hpp:15-33)."""
+
+    def product(self, inScale, in_, outScale, out):
+        raise NotImplementedError
+
+
+class HIPCSROperator(LinearOperator):
+    def __init__(self, matrix, space):
+        self.matrix = matrix
+        self.space = space
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        suffix = self.space._suffix
+        _sync_torch()
+        fn = getattr(library(), f"cie_hip_csr_product_{suffix}")
+        check(fn(self.space.context.handle, self.matrix.handle, inScale, _dptr(in_), in_.numel(), outScale,
+                 _dptr(out), out.numel()), "product")
+
+
+class DiagonalOperator(LinearOperator):
+    """Jacobi preconditioner holding the inverse diagonal (DiagonalOperator.cpp:37-53)."""
+
+    def __init__(self, inverseDiagonal, space):
+        self.inverseDiagonal = inverseDiagonal
+        self.space = space
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        _sync_torch()
+        fn = getattr(library(), f"cie_hip_diagonal_product_{self.space._suffix}")
+        check(fn(self.space.context.handle, _dptr(self.inverseDiagonal), inScale, _dptr(in_), outScale, _dptr(out),
+                 out.numel()), "DiagonalOperator.product")
+
+
+def makeDiagonalOperator(matrix, space):
+    """inverse_i = 1 / a_ii; raises ZeroDiagonal / MissingDiagonal like DiagonalOperator.cpp:76-82."""
+    inverse = space.makeVector(matrix.rows)
+    failed = c_int64(-1)
+    _sync_torch()
+    fn = getattr(library(), f"cie_hip_inverse_diagonal_{space._suffix}")
+    check(fn(space.context.handle, matrix.handle, _dptr(inverse), byref(failed)), "makeDiagonalOperator")
+    return DiagonalOperator(inverse, space)
+
+
+def _float_equal(left, right):
+    """cieutils/packages/maths/impl/Comparison_impl.hpp:49-56 (default tolerances)."""
+    norm = min(abs(left) + abs(right), sys.float_info.max)
+    return abs(left - right) < max(_DBL_MIN, _DBL_EPS * norm)
+
+
+def _float_less(left, right):
+    return left < right and not _float_equal(left, right)
+
+
+class IterativeSolver(LinearOperator):
+    def __init__(self, settings=None):
+        self._settings = settings or Statistics()
+        self._results = None
+
+    def getConfiguration(self):
+        return self._settings
+
+    def configure(self, settings):
+        self._settings = settings
+
+    def getStats(self):
+        return self._results
+
+    def report(self, stats):
+        self._results = stats
+
+
+class ConjugateGradients(IterativeSolver):
+    """Generic (P)CG over any LinalgSpace; mirrors ConjugateGradients_impl.hpp:49-201 step by step.
+
+    Every operation is one HIPSpace / operator call, so this path exercises the
+    backend exactly as the reference's solver template would.
+    """
+
+    def __init__(self, lhs, space, preconditioner=None, settings=None, verbosity=0):
+        super().__init__(settings)
+        self._lhs, self._space, self._preconditioner, self._verbosity = lhs, space, preconditioner, verbosity
+        self.history = []
+
+    def product(self, inScale, in_, outScale, out):
+        space = self._space
+        size = space.size(out)
+        if space.size(in_) != size:
+            raise _lib.ShapeError(_lib.ERROR_SHAPE, f"incompatible vector sizes {space.size(in_)} != {size}")
+        settings = self.getConfiguration()
+        stats = Statistics(0, sys.float_info.max, sys.float_info.max)
+        self.history = []
+        solution = space.makeVector(size)
+        space.assign(solution, out)
+        if settings.iterationCount:
+            search = space.makeVector(size)
+            searchProduct = space.makeVector(size)
+            space.fill(searchProduct, 0)
+            preconditioned = None
+            residual = space.makeVector(size)
+            space.assign(residual, in_)
+            self._lhs.product(1, solution, -1, residual)
+            residualNorm = space.innerProduct(residual, residual)
+            initialResidualNorm = math.sqrt(residualNorm)
+            stats.absoluteResidual = initialResidualNorm
+            stats.relativeResidual = 1.0
+            if _float_less(stats.absoluteResidual, settings.absoluteResidual) or _float_less(
+                    stats.relativeResidual, settings.relativeResidual):
+                self.report(stats)
+                return self._finish(inScale, outScale, out, solution)
+            if self._preconditioner is not None:
+                preconditioned = space.makeVector(size)
+                self._preconditioner.product(0, residual, 1, preconditioned)
+                preconditionedNorm = space.innerProduct(residual, preconditioned)
+                space.assign(search, preconditioned)
+                if not 0 < preconditionedNorm:
+                    raise _lib.Breakdown(_lib.ERROR_BREAKDOWN, "preconditioned norm vanished in iteration 0 (r^T @ z = 0)")
+            else:
+                preconditionedNorm = residualNorm
+                space.assign(search, residual)
+                if not 0 < preconditionedNorm:
+                    raise _lib.Breakdown(_lib.ERROR_BREAKDOWN, "residual norm vanished in iteration 0 (r^T @ r = 0)")
+            while stats.iterationCount < settings.iterationCount:
+                self._lhs.product(0, search, 1, searchProduct)
+                denominator = space.innerProduct(search, searchProduct)
+                if not 0 < denominator:
+                    raise _lib.NotPositiveDefinite(_lib.ERROR_NOT_SPD,
+                                                   f"p^T @ A @ p vanished in iteration {stats.iterationCount}")
+                searchScale = preconditionedNorm / denominator
+                space.add(solution, search, searchScale)
+                space.add(residual, searchProduct, -searchScale)
+                residualNorm = space.innerProduct(residual, residual)
+                previous = preconditionedNorm
+                if preconditioned is not None:
+                    self._preconditioner.product(0, residual, 1, preconditioned)
+                    preconditionedNorm = space.innerProduct(residual, preconditioned)
+                else:
+                    preconditionedNorm = residualNorm
+                searchScale = preconditionedNorm / previous
+                space.scale(search, searchScale)
+                space.add(search, preconditioned if preconditioned is not None else residual, 1)
+                stats.absoluteResidual = math.sqrt(residualNorm)
+                stats.relativeResidual = stats.absoluteResidual / initialResidualNorm
+                self.history.append(stats.absoluteResidual)
+                if _float_less(stats.absoluteResidual, settings.absoluteResidual) or _float_less(
+                        stats.relativeResidual, settings.relativeResidual):
+                    break
+                stats.iterationCount += 1
+        self.report(stats)
+        return self._finish(inScale, outScale, out, solution)
+
+    def _finish(self, inScale, outScale, out, solution):
+        space = self._space
+        if inScale == 1:
+            space.add(out, solution, outScale)
+        elif inScale == 0:
+            space.assign(out, solution)
+            space.scale(out, outScale)
+        else:
+            space.scale(out, inScale)
+            space.add(out, solution, outScale)
+
+
+class FusedPCG(IterativeSolver):
+    """Jacobi-PCG in three fused kernels per iteration (cie_hip_pcg_f64)."""
+
+    def __init__(self, matrix, settings=None, true_residual=True):
+        super().__init__(settings)
+        self.matrix = matrix
+        self.true_residual = true_residual
+        self.info = None
+
+    def product(self, inScale, in_, outScale, out):
+        settings = self.getConfiguration()
+        info = SolveInfo()
+        _sync_torch()
+        status = library().cie_hip_pcg_f64(self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_),
+                                           outScale, _dptr(out), out.numel(), byref(settings),
+                                           1 if self.true_residual else 0, byref(info))
+        self.info = info
+        check(status, "cie_hip_pcg_f64")
+        self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+
+
+class MixedPCG(IterativeSolver):
+    """FP32 inner PCG with FP64 solution accumulation and FP64 residual refinement (cie_hip_pcg_mixed)."""
+
+    def __init__(self, matrix, settings=None, refinement_factor=0.1, restart=False):
+        super().__init__(settings)
+        self.matrix = matrix
+        self.refinement_factor = refinement_factor
+        self.restart = restart
+        self.info = None
+
+    def product(self, inScale, in_, outScale, out):
+        mixed = MixedSettings(self.getConfiguration(), self.refinement_factor, 1 if self.restart else 0, 0)
+        info = SolveInfo()
+        _sync_torch()
+        status = library().cie_hip_pcg_mixed(self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_),
+                                             outScale, _dptr(out), out.numel(), byref(mixed), byref(info))
+        self.info = info
+        check(status, "cie_hip_pcg_mixed")
+        self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))

@@ -1,0 +1,177 @@
+"""Structured CSR systems and seeded inputs of the solver workloads (host side).
+
+Thin numpy wrappers over include/cie_hip_systems.h. Node numbering is
+lexicographic with x fastest and Dirichlet boundary rows/columns eliminated, so
+each row's column indices are sorted (as the reference's assembler produces,
+fem/packages/graph/src/Assembler.cpp:131-142).
+"""
+from ctypes import byref, c_int32, c_int64
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from ._lib import LagrangeSpec, check_sys, library
+
+
+@dataclass
+class CSR:
+    """Host CSR arrays in the layout of CSRView (linalg/packages/utilities/inc/CSRView.hpp:11-86)."""
+
+    rows: int
+    columns: int
+    row_extents: np.ndarray
+    column_indices: np.ndarray
+    entries: np.ndarray
+
+    @property
+    def nnz(self):
+        return int(self.row_extents[-1])
+
+    def to_scipy(self):
+        import scipy.sparse
+
+        return scipy.sparse.csr_matrix((self.entries, self.column_indices, self.row_extents),
+                                       shape=(self.rows, self.columns))
+
+
+def _ptr(array):
+    return array.ctypes.data_as(__import__("ctypes").c_void_p)
+
+
+def splitmix64(seed, count):
+    out = np.empty(count, dtype=np.uint64)
+    check_sys(library().cie_sys_splitmix64(seed, count, _ptr(out)), "splitmix64")
+    return out
+
+
+def uniform(seed, count):
+    """U(-1, 1): 2 * (x >> 11) * 2^-53 - 1 over the SplitMix64 stream of `seed`."""
+    out = np.empty(count, dtype=np.float64)
+    check_sys(library().cie_sys_uniform(seed, count, _ptr(out)), "uniform")
+    return out
+
+
+def normal(seed, sigma, count):
+    """N(0, sigma^2) by Box-Muller, two SplitMix64 draws per value."""
+    out = np.empty(count, dtype=np.float64)
+    check_sys(library().cie_sys_normal(seed, float(sigma), count, _ptr(out)), "normal")
+    return out
+
+
+def poisson5(m):
+    """2D 5-point Laplacian (4 on the diagonal, -1 off it) on an m x m interior grid."""
+    rows, nnz = c_int64(), c_int64()
+    lib = library()
+    check_sys(lib.cie_sys_poisson5_size(m, byref(rows), byref(nnz)), "poisson5_size")
+    extents = np.empty(rows.value + 1, dtype=np.int32)
+    cols = np.empty(nnz.value, dtype=np.int32)
+    vals = np.empty(nnz.value, dtype=np.float64)
+    check_sys(lib.cie_sys_poisson5_fill(m, _ptr(extents), _ptr(cols), _ptr(vals)), "poisson5_fill")
+    return CSR(rows.value, rows.value, extents, cols, vals)
+
+
+def _spec(dim, order, elements, eliminate=True):
+    return LagrangeSpec(dim, order, elements, 1 if eliminate else 0)
+
+
+def lagrange_size(dim, order, elements, eliminate=True):
+    spec = _spec(dim, order, elements, eliminate)
+    rows, nnz, count, npe = c_int64(), c_int64(), c_int64(), c_int32()
+    check_sys(library().cie_sys_lagrange_size(byref(spec), byref(rows), byref(nnz), byref(count), byref(npe)),
+              "lagrange_size")
+    return rows.value, nnz.value, count.value, npe.value
+
+
+def lagrange(dim, order, elements, conductivity=None, eliminate=True, threads=0):
+    """Laplace stiffness of order-`order` Lagrange elements on an elements^dim unit mesh."""
+    spec = _spec(dim, order, elements, eliminate)
+    rows, nnz, count, _ = lagrange_size(dim, order, elements, eliminate)
+    if conductivity is not None:
+        conductivity = np.ascontiguousarray(conductivity, dtype=np.float64)
+        if conductivity.shape != (count,):
+            raise ValueError(f"expected {count} element conductivities, got {conductivity.shape}")
+    extents = np.empty(rows + 1, dtype=np.int32)
+    cols = np.empty(nnz, dtype=np.int32)
+    vals = np.empty(nnz, dtype=np.float64)
+    check_sys(library().cie_sys_lagrange_fill(byref(spec), None if conductivity is None else _ptr(conductivity),
+                                              _ptr(extents), _ptr(cols), _ptr(vals), threads), "lagrange_fill")
+    return CSR(rows, rows, extents, cols, vals)
+
+
+def element_matrix(dim, order, elements):
+    """Element stiffness for k_e = 1 (nodes per element squared, row-major)."""
+    _, _, _, npe = lagrange_size(dim, order, elements)
+    out = np.empty((npe, npe), dtype=np.float64)
+    check_sys(library().cie_sys_lagrange_element_matrix(byref(_spec(dim, order, elements)), _ptr(out)),
+              "element_matrix")
+    return out
+
+
+def element_dofs(dim, order, elements, eliminate=True):
+    """Row index of each element node, -1 for eliminated (Dirichlet) nodes."""
+    _, _, count, npe = lagrange_size(dim, order, elements, eliminate)
+    out = np.empty((count, npe), dtype=np.int32)
+    check_sys(library().cie_sys_lagrange_element_dofs(byref(_spec(dim, order, elements, eliminate)), _ptr(out)),
+              "element_dofs")
+    return out
+
+
+def csr_slots(csr, rows, cols):
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    out = np.empty(rows.shape, dtype=np.int64)
+    check_sys(library().cie_sys_csr_slots(csr.rows, _ptr(csr.row_extents), _ptr(csr.column_indices), rows.size,
+                                          _ptr(rows), _ptr(cols), _ptr(out)), "csr_slots")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Workload:
+    name: str
+    matrix: CSR
+    rhs: np.ndarray
+    relative_tolerance: float
+    description: str
+    extra: dict = field(default_factory=dict)
+
+
+def five_point_2d(m=256, seed=0, tolerance=1e-8):
+    """2D 5-point Laplacian on an m x m interior grid, b ~ U(-1,1) from `seed`."""
+    matrix = poisson5(m)
+    return Workload(f"five_point_2d_{m}", matrix, uniform(seed, matrix.rows), tolerance,
+                    f"2D 5-point Laplacian, {m}x{m} interior grid, b~U(-1,1) seed {seed}")
+
+
+def q2_laplace_3d(elements=64, seed=1, tolerance=1e-10, threads=0):
+    """Q2 hexahedra on an elements^3 unit-cube mesh, k_e = 1, b ~ U(-1,1) from `seed`."""
+    matrix = lagrange(3, 2, elements, threads=threads)
+    return Workload(f"q2_laplace_3d_{elements}", matrix, uniform(seed, matrix.rows), tolerance,
+                    f"3D Q2 Laplace stiffness, {elements}^3 hexahedra, Dirichlet eliminated, b~U(-1,1) seed {seed}")
+
+
+def q4_lognormal_3d(elements=32, seed=2, rhs_seed=1, tolerance=1e-10, threads=0):
+    """Q4 hexahedra with k_e = exp(g_e), g_e ~ N(0,1) from `seed`; b ~ U(-1,1) from `rhs_seed`."""
+    _, _, count, _ = lagrange_size(3, 4, elements)
+    g = normal(seed, 1.0, count)
+    matrix = lagrange(3, 4, elements, conductivity=np.exp(g), threads=threads)
+    return Workload(f"q4_lognormal_3d_{elements}", matrix, uniform(rhs_seed, matrix.rows), tolerance,
+                    f"3D Q4 Laplace, {elements}^3 hexahedra, k_e=exp(N(0,1)) seed {seed}, b~U(-1,1) seed {rhs_seed}",
+                    {"log_conductivity": g})
+
+
+def q2_adjoint_3d(elements=48, seed=4, target_seed=5, rhs_seed=1, variance=0.5, tolerance=1e-11, threads=0):
+    """Q2 hexahedra with k_e = exp(g_e), g_e ~ N(0, variance) from `seed`; the target
+    field uses `target_seed`; b ~ U(-1,1) from `rhs_seed`."""
+    _, _, count, _ = lagrange_size(3, 2, elements)
+    sigma = math.sqrt(variance)
+    g = normal(seed, sigma, count)
+    g_target = normal(target_seed, sigma, count)
+    matrix = lagrange(3, 2, elements, conductivity=np.exp(g), threads=threads)
+    return Workload(f"q2_adjoint_3d_{elements}", matrix, uniform(rhs_seed, matrix.rows), tolerance,
+                    f"3D Q2 Laplace, {elements}^3 hexahedra, k_e=exp(N(0,{variance})) seed {seed}",
+                    {"log_conductivity": g, "target_log_conductivity": g_target})

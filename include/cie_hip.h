@@ -1,0 +1,195 @@
+/* C ABI of the MI355X (gfx950) sparse linear-solve backend.
+ *
+ * This is the boundary the reference's solver layer would bind for a HIP
+ * backend beside DefaultSpace (threaded) and SYCLSpace (SYCL). Paths below are
+ * relative to the reference repository; each entry point names the interface
+ * it stands in for.
+ *
+ * Conventions
+ *   - Every function returns a status (CIE_HIP_OK = 0) and never throws; the
+ *     message of the last failure on the calling thread is cie_hip_last_error().
+ *     The reference reports the same conditions as cie::Exception via CIE_CHECK
+ *     (cieutils/packages/macros/inc/checks.hpp:17-23).
+ *   - Vector arguments are DEVICE pointers (hipMalloc'd, or e.g. torch tensors)
+ *     unless a parameter says otherwise. Host arrays are borrowed for the call.
+ *   - All work of a context is ordered on its one HIP stream. Functions that
+ *     return a host value (inner products, solver statistics) synchronise it.
+ *   - One context is not thread-safe; separate contexts are independent.
+ */
+#ifndef CIE_HIP_H
+#define CIE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CIE_HIP_OK 0
+#define CIE_HIP_ERROR_INVALID_ARGUMENT 1
+#define CIE_HIP_ERROR_SHAPE 2            /* ConjugateGradients_impl.hpp:57-61, CSROperator.cpp:40-44 */
+#define CIE_HIP_ERROR_NOT_SPD 3          /* p^T A p <= 0: ConjugateGradients_impl.hpp:145-149 */
+#define CIE_HIP_ERROR_BREAKDOWN 4        /* r^T z <= 0 before the loop: ConjugateGradients_impl.hpp:119-131 */
+#define CIE_HIP_ERROR_ZERO_DIAGONAL 5    /* DiagonalOperator.cpp:76-78 */
+#define CIE_HIP_ERROR_MISSING_DIAGONAL 6 /* DiagonalOperator.cpp:79-82 */
+#define CIE_HIP_ERROR_HIP 7
+#define CIE_HIP_ERROR_OUT_OF_MEMORY 8
+#define CIE_HIP_ERROR_NO_DEVICE 9
+
+/* memory locations for cie_hip_csr_create / cie_hip_memcpy */
+#define CIE_HIP_HOST 0
+#define CIE_HIP_DEVICE 1
+
+/* value planes of a device CSR matrix */
+#define CIE_HIP_PLANE_F64 1
+#define CIE_HIP_PLANE_F32 2
+
+typedef struct cie_hip_context_s* cie_hip_context;
+typedef struct cie_hip_csr_s* cie_hip_csr;
+
+/* IterativeSolver<TSpace>::Statistics (linalg/packages/solvers/inc/IterativeSolver.hpp:21-25):
+ * used both as settings (caps and tolerances) and as results. iterationCount
+ * follows ConjugateGradients_impl.hpp:134,184-185: on convergence it is NOT
+ * incremented for the final loop trip. */
+typedef struct cie_hip_statistics {
+    uint64_t iterationCount;
+    double absoluteResidual;
+    double relativeResidual;
+} cie_hip_statistics;
+
+/* Results of the fused solvers beyond Statistics. */
+typedef struct cie_hip_solve_info {
+    cie_hip_statistics stats;      /* as ConjugateGradients::getStats() */
+    int32_t converged;             /* 1 if the tolerance test passed */
+    int32_t status;                /* same value the call returns */
+    double trueRelativeResidual;   /* ||b - A x|| / ||b - A x0|| in FP64 (if requested) */
+    double seconds;                /* device time of the solve (HIP events) */
+    uint64_t refinements;          /* mixed: FP64 true-residual refinement steps */
+    uint64_t spmvCount;            /* SpMV launches that did work */
+    double productSeconds;         /* profiling on: mean duration of the loop's SpMV (q = A p) kernel */
+    uint64_t productSamples;       /* profiling on: launches that mean is taken over */
+} cie_hip_solve_info;
+
+/* Settings of the mixed-precision refined PCG. */
+typedef struct cie_hip_mixed_settings {
+    cie_hip_statistics settings;   /* caps and FP64 tolerances, as for cie_hip_pcg_f64 */
+    double refinementFactor;       /* refine when ||r_32|| <= factor * ||r|| at the last refinement (default 0.1) */
+    int32_t restart;               /* 1: reset the search direction at each refinement (classic restarted IR) */
+    int32_t pad;
+} cie_hip_mixed_settings;
+
+/* ---- errors, devices, contexts ------------------------------------------ */
+const char* cie_hip_last_error(void);
+const char* cie_hip_status_string(int status);
+int cie_hip_device_count(int* count);
+
+/* A context = one device + one stream (+ reduction scratch). stream may be an
+ * existing hipStream_t (borrowed, e.g. torch's current stream) or NULL to
+ * create one. Replaces the sycl::queue held by SYCLSpace (SYCLSpace.hpp:128). */
+int cie_hip_context_create(int device, void* stream, cie_hip_context* context);
+int cie_hip_context_destroy(cie_hip_context context);
+int cie_hip_context_synchronize(cie_hip_context context);
+int cie_hip_context_stream(cie_hip_context context, void** stream);
+/* enable != 0: the fused solvers bracket the first 64 loop SpMV kernels of
+ * each solve with HIP events and report their mean duration in
+ * cie_hip_solve_info.productSeconds. */
+int cie_hip_context_set_profiling(cie_hip_context context, int enable);
+
+/* Device memory for callers without an allocator of their own
+ * (cieutils/packages/concurrency/inc/sycl.hpp:35-70 DeviceMemory). */
+int cie_hip_malloc(cie_hip_context context, size_t bytes, void** pointer);
+int cie_hip_free(cie_hip_context context, void* pointer);
+/* kind: 0 host->device, 1 device->host, 2 device->device (SYCLSpace.cpp:316-343). */
+int cie_hip_memcpy(cie_hip_context context, void* destination, const void* source, size_t bytes, int kind);
+
+/* ---- CSR matrix (linalg/packages/utilities/inc/CSRView.hpp:11-86) -------- */
+/* rowExtents[rows+1], columnIndices[nnz] (sorted per row), entries[nnz].
+ * location CIE_HIP_HOST: arrays are uploaded to library-owned device memory.
+ * location CIE_HIP_DEVICE: index arrays and FP64 entries are borrowed.
+ * planes: CIE_HIP_PLANE_F64 and/or CIE_HIP_PLANE_F32 (FP32 copy of entries). */
+int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, int64_t nnz,
+                       const int32_t* rowExtents, const int32_t* columnIndices, const double* entries,
+                       int location, int planes, cie_hip_csr* matrix);
+int cie_hip_csr_destroy(cie_hip_csr matrix);
+int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64_t* nnz);
+/* Rewrite the FP64 entries (and the FP32 plane if present) from device memory;
+ * the sparsity pattern is unchanged. */
+int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
+
+/* SpMV launch shape. precision: 64 or 32. lanes: lanes per row (1..64, power
+ * of two), unroll: entries per lane in flight (1, 2, 4, 8), block: threads per
+ * block (64..1024). cie_hip_csr_autotune times candidates on the device and
+ * keeps the fastest (SYCLCSROperator.cpp:26-27 fixes lanes = sub-group size). */
+int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int lanes, int unroll, int block);
+int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* lanes, int* unroll, int* block);
+int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, double* bestSeconds);
+
+/* LinearOperator<TSpace>::product (linalg/packages/solvers/inc/LinearOperator.hpp:15-33),
+ * CSROperator.cpp:33-111 / SYCLCSROperator.cpp:32-89:
+ *     out = inScale * out + outScale * A * in
+ * inScale == 0 does not read out. Row sums use a fixed order: bitwise
+ * reproducible run to run. */
+int cie_hip_csr_product_f64(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* in,
+                            int64_t inSize, double outScale, double* out, int64_t outSize);
+int cie_hip_csr_product_f32(cie_hip_context context, cie_hip_csr matrix, float inScale, const float* in,
+                            int64_t inSize, float outScale, float* out, int64_t outSize);
+
+/* ---- LinalgSpaceLike operations (solvers/inc/LinalgSpace.hpp:10-35) ------ */
+/* DefaultSpace.cpp:50-187, SYCLSpace.cpp:114-313. innerProduct returns a host
+ * value (deterministic two-stage reduction). */
+int cie_hip_inner_product_f64(cie_hip_context context, const double* left, const double* right, int64_t size,
+                              double* result);
+int cie_hip_scale_f64(cie_hip_context context, double* target, const double* source, int64_t size, double scale);
+int cie_hip_scale_uniform_f64(cie_hip_context context, double* target, int64_t size, double scale);
+int cie_hip_add_f64(cie_hip_context context, double* target, const double* source, int64_t size, double scale);
+int cie_hip_assign_f64(cie_hip_context context, double* target, const double* source, int64_t size);
+int cie_hip_fill_f64(cie_hip_context context, double* target, int64_t size, double value);
+int cie_hip_inner_product_f32(cie_hip_context context, const float* left, const float* right, int64_t size,
+                              float* result);
+int cie_hip_scale_f32(cie_hip_context context, float* target, const float* source, int64_t size, float scale);
+int cie_hip_scale_uniform_f32(cie_hip_context context, float* target, int64_t size, float scale);
+int cie_hip_add_f32(cie_hip_context context, float* target, const float* source, int64_t size, float scale);
+int cie_hip_assign_f32(cie_hip_context context, float* target, const float* source, int64_t size);
+int cie_hip_fill_f32(cie_hip_context context, float* target, int64_t size, float value);
+
+/* ---- Jacobi preconditioner (solvers/src/DiagonalOperator.cpp) ------------ */
+/* makeDiagonalOperator: inverse[i] = 1 / a_ii. Fails like the CPU version
+ * (DiagonalOperator.cpp:76-82) on the first row with a zero or missing
+ * diagonal; *failedRow (optional, host) receives that row. */
+int cie_hip_inverse_diagonal_f64(cie_hip_context context, cie_hip_csr matrix, double* inverse, int64_t* failedRow);
+int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, float* inverse, int64_t* failedRow);
+/* DiagonalOperator::product (DiagonalOperator.cpp:37-53): out = inScale*out + outScale*D^-1*in. */
+int cie_hip_diagonal_product_f64(cie_hip_context context, const double* inverse, double inScale, const double* in,
+                                 double outScale, double* out, int64_t size);
+int cie_hip_diagonal_product_f32(cie_hip_context context, const float* inverse, float inScale, const float* in,
+                                 float outScale, float* out, int64_t size);
+
+/* ---- Solvers -------------------------------------------------------------- */
+/* ConjugateGradients<TSpace>::product(inScale, b, outScale, x) with the Jacobi
+ * preconditioner (solvers/impl/ConjugateGradients_impl.hpp:49-201): x is the
+ * initial guess; on return x = inScale * x + outScale * solution. Same step
+ * order, breakdown checks and convergence test (Comparison_impl.hpp:32-64) as
+ * the reference; three fused kernels per iteration, no host round trip inside
+ * the loop. trueResidual != 0 also computes info->trueRelativeResidual. */
+int cie_hip_pcg_f64(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* b,
+                    double outScale, double* x, int64_t size, const cie_hip_statistics* settings,
+                    int trueResidual, cie_hip_solve_info* info);
+
+/* Mixed-precision refined PCG (no reference counterpart; the reference only
+ * instantiates all-FP32 CG, ConjugateGradients.cpp:10). The inner PCG keeps
+ * matrix values, Krylov vectors and dot products in FP32; the solution is
+ * accumulated in FP64, and the true residual b - A x (FP64 matrix plane) and
+ * the convergence test are recomputed in FP64 on the device whenever the FP32
+ * residual has fallen by refinementFactor, or looks converged. The matrix
+ * needs both planes. Statistics report the FP64 true residual. */
+int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* b,
+                      double outScale, double* x, int64_t size, const cie_hip_mixed_settings* settings,
+                      cie_hip_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIE_HIP_H */

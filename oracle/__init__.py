@@ -1,0 +1,130 @@
+"""CPU oracle of the sparse PCG path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers over oracle/liboracle.so (a plain-C restatement of the
+reference's threaded CG path; see oracle.c for the file:line each function
+follows). Only the tests, the smoke check and the benchmark's CPU-baseline leg
+use this package, and only as the checker / baseline -- never as the product.
+"""
+import ctypes
+from ctypes import POINTER, byref, c_double, c_int, c_int32, c_int64, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY = Path(__file__).resolve().parent / "liboracle.so"
+
+OP_SCALE, OP_SCALE_UNIFORM, OP_ADD, OP_ASSIGN, OP_FILL = range(5)
+STATUS = {0: "ok", 2: "shape", 3: "not spd", 4: "breakdown", 5: "zero diagonal", 6: "missing diagonal"}
+
+
+class Statistics(ctypes.Structure):
+    _fields_ = [("iterationCount", c_uint64), ("absoluteResidual", c_double), ("relativeResidual", c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            from .build import build
+
+            build()
+        l = ctypes.CDLL(str(LIBRARY))
+        P = c_void_p
+        l.oracle_csr_product.argtypes = [c_int64, P, P, P, c_double, P, c_double, P, c_int]
+        l.oracle_inner_product.argtypes = [c_int64, P, P, c_int]
+        l.oracle_inner_product.restype = c_double
+        l.oracle_vector_op.argtypes = [c_int, P, P, c_double, c_int64, c_int]
+        l.oracle_inverse_diagonal.argtypes = [c_int64, P, P, P, P, POINTER(c_int64)]
+        l.oracle_diagonal_product.argtypes = [P, c_double, P, c_double, P, c_int64]
+        l.oracle_pcg.argtypes = [c_int64, P, P, P, c_double, P, c_double, P, c_uint64, c_double, c_double, c_int,
+                                 c_int, POINTER(Statistics), P, c_int64, POINTER(c_double)]
+        l.oracle_element_gradient.argtypes = [c_int64, c_int32, P, P, P, P, P]
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, message=""):
+        super().__init__(f"[{status}] {STATUS.get(status, 'error')} {message}".strip())
+        self.status = status
+
+
+def csr_product(csr, in_scale, x, out_scale, y, threads=1):
+    """y = in_scale * y + out_scale * A x (in place on y, float64)."""
+    ext, cols, vals = _i32(csr.row_extents), _i32(csr.column_indices), _f64(csr.entries)
+    assert y.dtype == np.float64 and y.flags.c_contiguous
+    x = _f64(x)
+    lib().oracle_csr_product(csr.rows, _p(ext), _p(cols), _p(vals), in_scale, _p(x), out_scale, _p(y), threads)
+    return y
+
+
+def inner_product(left, right, threads=1):
+    left, right = _f64(left), _f64(right)
+    return lib().oracle_inner_product(left.size, _p(left), _p(right), threads)
+
+
+def vector_op(op, target, source=None, value=0.0, threads=1):
+    assert target.dtype == np.float64 and target.flags.c_contiguous
+    src = None if source is None else _f64(source)
+    lib().oracle_vector_op(op, _p(target), None if src is None else _p(src), value, target.size, threads)
+    return target
+
+
+def inverse_diagonal(csr):
+    out = np.empty(csr.rows, dtype=np.float64)
+    failed = c_int64(-1)
+    ext, cols, vals = _i32(csr.row_extents), _i32(csr.column_indices), _f64(csr.entries)
+    status = lib().oracle_inverse_diagonal(csr.rows, _p(ext), _p(cols), _p(vals), _p(out), byref(failed))
+    if status:
+        raise OracleError(status, f"row {failed.value}")
+    return out
+
+
+def diagonal_product(inverse, in_scale, x, out_scale, y):
+    inverse, x = _f64(inverse), _f64(x)
+    lib().oracle_diagonal_product(_p(inverse), in_scale, _p(x), out_scale, _p(y), y.size)
+    return y
+
+
+def pcg(csr, b, x0=None, max_iterations=5000, absolute_tolerance=0.0, relative_tolerance=1e-10, precondition=True,
+        threads=1, in_scale=0.0, out_scale=1.0, history=False):
+    """ConjugateGradients::product(in_scale, b, out_scale, x) with Jacobi; returns (x, stats, history, seconds)."""
+    ext, cols, vals = _i32(csr.row_extents), _i32(csr.column_indices), _f64(csr.entries)
+    b = _f64(b)
+    x = np.zeros(csr.rows) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    stats = Statistics()
+    capacity = max_iterations if history else 0
+    hist = np.zeros(max(1, capacity))
+    seconds = c_double()
+    status = lib().oracle_pcg(csr.rows, _p(ext), _p(cols), _p(vals), in_scale, _p(b), out_scale, _p(x),
+                              max_iterations, absolute_tolerance, relative_tolerance, 1 if precondition else 0,
+                              threads, byref(stats), _p(hist), capacity, byref(seconds))
+    if status:
+        raise OracleError(status)
+    count = min(capacity, stats.iterationCount + 1)
+    return x, stats, hist[:count] if history else None, seconds.value
+
+
+def element_gradient(dofs, element_matrix, lam, u):
+    """dJ/dk_e = -lambda_e^T K_e0 u_e for every element (dofs: E x npe, -1 = eliminated)."""
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    ke = _f64(element_matrix)
+    lam, u = _f64(lam), _f64(u)
+    out = np.empty(dofs.shape[0])
+    lib().oracle_element_gradient(dofs.shape[0], dofs.shape[1], _p(dofs), _p(ke), _p(lam), _p(u), _p(out))
+    return out

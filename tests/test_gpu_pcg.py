@@ -1,0 +1,178 @@
+"""HIP Jacobi-PCG (generic over HIPSpace, fused FP64, mixed refined) against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from cie_amd import systems
+from cie_amd._lib import Statistics
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda:0")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def true_relative_residual(csr, x, b):
+    r = oracle.csr_product(csr, 1.0, x, -1.0, np.array(b, dtype=np.float64))  # r = b - A x (in place on b copy)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def five_point(hip):
+    w = systems.five_point_2d(256)
+    x, stats, hist, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=w.relative_tolerance, history=True)
+    return w, hip.CSRMatrix(w.matrix), x, stats, hist
+
+
+def test_generic_cg_over_hipspace_matches_oracle(hip, torch_gpu, five_point):
+    """ConjugateGradients<HIPSpace<double>> with makeDiagonalOperator, as poisson2D/solver.cpp:82-108 builds it."""
+    w, matrix, x_ref, s_ref, h_ref = five_point
+    space = hip.HIPSpace()
+    solver = hip.ConjugateGradients(hip.HIPCSROperator(matrix, space), space, hip.makeDiagonalOperator(matrix, space),
+                                    Statistics(5000, 0.0, w.relative_tolerance))
+    x = space.makeVector(w.matrix.rows)
+    space.fill(x, 0)
+    solver.product(0, to_dev(torch_gpu, w.rhs), 1, x)
+    stats = solver.getStats()
+    assert abs(int(stats.iterationCount) - int(s_ref.iterationCount)) <= 1
+    assert rel(x.cpu().numpy(), x_ref) < 1e-10
+    n = min(len(h_ref), len(solver.history))
+    assert np.allclose(solver.history[:n], h_ref[:n], rtol=1e-6)
+
+
+def test_fused_pcg_matches_oracle(hip, torch_gpu, five_point):
+    w, matrix, x_ref, s_ref, _ = five_point
+    solver = hip.FusedPCG(matrix, Statistics(5000, 0.0, w.relative_tolerance))
+    x = torch_gpu.zeros(w.matrix.rows, dtype=torch_gpu.float64, device="cuda:0")
+    solver.product(0, to_dev(torch_gpu, w.rhs), 1, x)
+    stats = solver.getStats()
+    assert solver.info.converged == 1
+    assert abs(int(stats.iterationCount) - int(s_ref.iterationCount)) <= 1
+    assert stats.relativeResidual < w.relative_tolerance
+    assert rel(x.cpu().numpy(), x_ref) < 1e-10
+    assert abs(solver.info.trueRelativeResidual - true_relative_residual(w.matrix, x.cpu().numpy(), w.rhs)) < 1e-12
+
+
+def test_fused_pcg_bitwise_reproducible(hip, torch_gpu, five_point):
+    w, matrix, _, _, _ = five_point
+    outs = []
+    for _ in range(2):
+        solver = hip.FusedPCG(matrix, Statistics(5000, 0.0, w.relative_tolerance))
+        x = torch_gpu.zeros(w.matrix.rows, dtype=torch_gpu.float64, device="cuda:0")
+        solver.product(0, to_dev(torch_gpu, w.rhs), 1, x)
+        outs.append((x.cpu().numpy(), solver.getStats().iterationCount, solver.getStats().absoluteResidual))
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1:] == outs[1][1:]
+
+
+@pytest.mark.parametrize("in_scale,out_scale", [(1.0, 1.0), (0.5, -2.0), (1.0, -1.0)])
+def test_fused_pcg_output_scaling_and_warm_start(hip, torch_gpu, in_scale, out_scale):
+    w = systems.five_point_2d(40)
+    matrix = hip.CSRMatrix(w.matrix)
+    x0 = np.random.default_rng(2).standard_normal(w.matrix.rows)
+    x_ref, s_ref, _, _ = oracle.pcg(w.matrix, w.rhs, x0=x0, relative_tolerance=1e-9, in_scale=in_scale,
+                                    out_scale=out_scale)
+    solver = hip.FusedPCG(matrix, Statistics(5000, 0.0, 1e-9))
+    x = to_dev(torch_gpu, x0)
+    solver.product(in_scale, to_dev(torch_gpu, w.rhs), out_scale, x)
+    assert abs(int(solver.getStats().iterationCount) - int(s_ref.iterationCount)) <= 1
+    assert rel(x.cpu().numpy(), x_ref) < 1e-9
+
+
+@pytest.mark.parametrize("dim,order,n", [(3, 2, 10), (2, 4, 12), (3, 4, 4)])
+def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n):
+    count = n ** dim
+    k = np.exp(systems.normal(2, 1.0, count))
+    csr = systems.lagrange(dim, order, n, conductivity=k)
+    b = systems.uniform(1, csr.rows)
+    x_ref, s_ref, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-10)
+    matrix = hip.CSRMatrix(csr)
+    matrix.autotune(64, 3)
+    solver = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
+    x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
+    solver.product(0, to_dev(torch_gpu, b), 1, x)
+    assert abs(int(solver.getStats().iterationCount) - int(s_ref.iterationCount)) <= 2
+    assert rel(x.cpu().numpy(), x_ref) < 1e-8
+    assert solver.info.trueRelativeResidual < 2e-10
+
+
+@pytest.mark.parametrize("restart", [False, True])
+def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart):
+    count = 8 ** 3
+    k = np.exp(systems.normal(2, 1.0, count))
+    csr = systems.lagrange(3, 4, 8, conductivity=k)
+    b = systems.uniform(1, csr.rows)
+    x_ref, _, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-14, max_iterations=20000)
+    matrix = hip.CSRMatrix(csr)
+    solver = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), restart=restart)
+    x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
+    solver.product(0, to_dev(torch_gpu, b), 1, x)
+    info = solver.info
+    assert info.converged == 1 and info.refinements >= 1
+    got = x.cpu().numpy()
+    truth = true_relative_residual(csr, got, b)
+    assert truth < 1e-10
+    assert abs(info.trueRelativeResidual - truth) < 1e-12
+    assert solver.getStats().relativeResidual < 1e-10
+    assert rel(got, x_ref) < 1e-6
+
+
+def test_fused_pcg_errors(hip, torch_gpu):
+    # r^T z <= 0 before the loop (ConjugateGradients_impl.hpp:119-123)
+    neg = systems.CSR(4, 4, np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32), -np.ones(4))
+    solver = hip.FusedPCG(hip.CSRMatrix(neg), Statistics(100, 0.0, 1e-10))
+    x = torch_gpu.zeros(4, dtype=torch_gpu.float64, device="cuda:0")
+    with pytest.raises(hip._lib.Breakdown):
+        solver.product(0, to_dev(torch_gpu, np.ones(4)), 1, x)
+    # p^T A p <= 0 (:145-149): symmetric indefinite with a positive diagonal
+    a = np.array([[1.0, 2.0], [2.0, 1.0]])
+    indefinite = systems.CSR(2, 2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), a.ravel())
+    solver = hip.FusedPCG(hip.CSRMatrix(indefinite), Statistics(100, 0.0, 1e-10))
+    with pytest.raises(hip._lib.NotPositiveDefinite):
+        solver.product(0, to_dev(torch_gpu, np.array([1.0, -1.0])), 1, torch_gpu.zeros(2, dtype=torch_gpu.float64,
+                                                                                        device="cuda:0"))
+    # iteration cap: stats, no exception (:134, :200)
+    w = systems.five_point_2d(32)
+    solver = hip.FusedPCG(hip.CSRMatrix(w.matrix), Statistics(7, 0.0, 1e-30))
+    x = torch_gpu.zeros(w.matrix.rows, dtype=torch_gpu.float64, device="cuda:0")
+    solver.product(0, to_dev(torch_gpu, w.rhs), 1, x)
+    assert solver.getStats().iterationCount == 7 and solver.info.converged == 0
+    _, capped, _, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-30, max_iterations=7)
+    assert solver.getStats().absoluteResidual == pytest.approx(capped.absoluteResidual, rel=1e-10)
+    # zero iterations: x is only rescaled
+    solver = hip.FusedPCG(hip.CSRMatrix(w.matrix), Statistics(0, 0.0, 1e-8))
+    solver.product(0, to_dev(torch_gpu, w.rhs), 1, x)
+    assert solver.getStats().iterationCount == 0
+    # exact initial guess: early exit before the loop
+    solver = hip.FusedPCG(hip.CSRMatrix(w.matrix), Statistics(100, 1e-12, 1e-8))
+    solver.product(0, torch_gpu.zeros_like(x), 1, torch_gpu.zeros_like(x))
+    assert solver.getStats().iterationCount == 0 and solver.info.converged == 1
+    with pytest.raises(hip._lib.ShapeError):
+        hip.FusedPCG(hip.CSRMatrix(w.matrix), Statistics(10, 0.0, 1e-8)).product(0, x[:-1], 1, x[:-1])
+
+
+def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu):
+    """3D Q2 64^3 (2,048,383 DOFs): both solvers reach FP64 true relative residual 1e-10; the mixed
+    solution agrees with the FP64 one to the conditioning-limited level."""
+    w = systems.q2_laplace_3d(64)
+    matrix = hip.CSRMatrix(w.matrix)
+    matrix.autotune(64, 3)
+    matrix.autotune(32, 3)
+    b = to_dev(torch_gpu, w.rhs)
+    fp64 = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
+    x64 = torch_gpu.zeros_like(b)
+    fp64.product(0, b, 1, x64)
+    assert fp64.info.converged == 1 and fp64.info.trueRelativeResidual < 2e-10
+    mixed = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10))
+    xm = torch_gpu.zeros_like(b)
+    mixed.product(0, b, 1, xm)
+    assert mixed.info.converged == 1 and mixed.info.trueRelativeResidual < 1e-10
+    host = xm.cpu().numpy()
+    assert true_relative_residual(w.matrix, host, w.rhs) < 1e-10
+    assert rel(host, x64.cpu().numpy()) < 1e-6
+    print(f"fp64 iterations {fp64.getStats().iterationCount} {fp64.info.seconds:.4f}s; mixed iterations "
+          f"{mixed.getStats().iterationCount} refinements {mixed.info.refinements} {mixed.info.seconds:.4f}s")

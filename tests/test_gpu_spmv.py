@@ -1,0 +1,208 @@
+"""HIP SpMV and space operations against the oracle and the reference's KATs."""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from cie_amd import systems
+
+pytestmark = pytest.mark.gpu
+KAT = json.loads((Path(__file__).parent / "golden" / "kat.json").read_text())
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+
+
+def to_dev(torch, a, dtype=None):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0", dtype=dtype or torch.float64)
+
+
+def random_csr(rng, rows, cols, density, empty_rows=()):
+    a = rng.standard_normal((rows, cols)) * (rng.random((rows, cols)) < density)
+    for r in empty_rows:
+        a[r, :] = 0.0
+    rr, cc = np.nonzero(a)
+    ext = np.zeros(rows + 1, dtype=np.int64)
+    np.add.at(ext, rr + 1, 1)
+    return systems.CSR(rows, cols, np.cumsum(ext).astype(np.int32), cc.astype(np.int32), a[rr, cc])
+
+
+def spmv_bound(csr, x, y0, in_scale, out_scale, eps):
+    absolute = np.abs(csr.entries) * np.abs(x[csr.column_indices])
+    lengths = np.diff(csr.row_extents).astype(np.int64)
+    sums = np.bincount(np.repeat(np.arange(csr.rows), lengths), weights=absolute, minlength=csr.rows)
+    return 2 * (lengths + 3) * eps * (abs(out_scale) * sums + abs(in_scale) * np.abs(y0)) + 1e-300
+
+
+def test_spmv_kat(hip, torch_gpu):
+    k = KAT["spmv"]                        # SYCLSpace_test.cpp:140-186 -> [233, 462]
+    csr = systems.CSR(2, k["columns"], np.array(k["row_extents"], np.int32), np.array(k["column_indices"], np.int32),
+                      np.array(k["entries"]))
+    matrix = hip.CSRMatrix(csr)
+    for dtype in (torch_gpu.float64, torch_gpu.float32):
+        space = hip.HIPSpace(dtype)
+        op = hip.HIPCSROperator(matrix, space)
+        x = to_dev(torch_gpu, k["in"], dtype)
+        y = to_dev(torch_gpu, k["out"], dtype)
+        op.product(k["in_scale"], x, k["out_scale"], y)
+        assert y.cpu().tolist() == k["expected"]
+
+
+SCALES = [(0, 1), (0, -1), (0, 2.5), (1, 1), (1, -1), (1, 0.5), (-1, 1), (-1, 3.0), (0.25, 1), (0.25, -2.0)]
+
+
+@pytest.mark.parametrize("lanes,unroll", [(1, 1), (2, 4), (4, 2), (8, 8), (16, 4), (32, 1), (64, 2)])
+def test_spmv_random_against_oracle(hip, torch_gpu, lanes, unroll):
+    rng = np.random.default_rng(lanes * 10 + unroll)
+    csr = random_csr(rng, 3001, 2500, 0.02, empty_rows=(0, 17, 3000))
+    matrix = hip.CSRMatrix(csr)
+    matrix.set_launch(64, lanes, unroll)
+    matrix.set_launch(32, lanes, unroll)
+    x = rng.standard_normal(2500)
+    y0 = rng.standard_normal(3001)
+    for in_scale, out_scale in SCALES:
+        expected = oracle.csr_product(csr, in_scale, x, out_scale, y0.copy())
+        space = hip.HIPSpace(torch_gpu.float64)
+        y = to_dev(torch_gpu, y0)
+        hip.HIPCSROperator(matrix, space).product(in_scale, to_dev(torch_gpu, x), out_scale, y)
+        got = y.cpu().numpy()
+        assert np.all(np.abs(got - expected) <= spmv_bound(csr, x, y0, in_scale, out_scale, EPS64))
+        space32 = hip.HIPSpace(torch_gpu.float32)
+        y32 = to_dev(torch_gpu, y0, torch_gpu.float32)
+        hip.HIPCSROperator(matrix, space32).product(in_scale, to_dev(torch_gpu, x, torch_gpu.float32), out_scale, y32)
+        bound32 = spmv_bound(csr, x, y0, in_scale, out_scale, EPS32) * 4
+        assert np.all(np.abs(y32.cpu().numpy() - expected) <= bound32)
+
+
+def test_spmv_deterministic_and_autotuned(hip, torch_gpu):
+    w = systems.q2_laplace_3d(16)
+    matrix = hip.CSRMatrix(w.matrix)
+    best = matrix.autotune(64, repetitions=5)
+    assert best > 0
+    space = hip.HIPSpace()
+    op = hip.HIPCSROperator(matrix, space)
+    x = to_dev(torch_gpu, w.rhs)
+    outs = []
+    for _ in range(3):
+        y = space.makeVector(w.matrix.rows)
+        op.product(0, x, 1, y)
+        outs.append(y.cpu().numpy())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])  # bitwise run to run
+    expected = oracle.csr_product(w.matrix, 0.0, w.rhs, 1.0, np.zeros(w.matrix.rows))
+    assert np.all(np.abs(outs[0] - expected) <= spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64))
+
+
+def test_spmv_full_size_q2_64(hip, torch_gpu):
+    """3D Q2, 64^3 hexahedra (2,048,383 rows, 127,263,527 nonzeros): FP64 and FP32 planes vs the oracle."""
+    w = systems.q2_laplace_3d(64)
+    assert w.matrix.nnz == 127263527
+    matrix = hip.CSRMatrix(w.matrix)
+    expected = oracle.csr_product(w.matrix, 0.0, w.rhs, 1.0, np.zeros(w.matrix.rows), threads=8)
+    space = hip.HIPSpace()
+    y = space.makeVector(w.matrix.rows)
+    hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
+    got = y.cpu().numpy()
+    assert np.all(np.abs(got - expected) <= spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64))
+    space32 = hip.HIPSpace(torch_gpu.float32)
+    y32 = space32.makeVector(w.matrix.rows)
+    hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
+    rel = np.linalg.norm(y32.cpu().numpy() - expected) / np.linalg.norm(expected)
+    assert rel < 1e-5
+
+
+def test_spmv_shape_error(hip, torch_gpu):
+    matrix = hip.CSRMatrix(systems.poisson5(4))
+    space = hip.HIPSpace()
+    with pytest.raises(hip._lib.ShapeError):
+        hip.HIPCSROperator(matrix, space).product(0, space.makeVector(15), 1, space.makeVector(16))
+
+
+def test_space_kats(hip, torch_gpu):
+    for dtype in (torch_gpu.float64, torch_gpu.float32):
+        space = hip.HIPSpace(dtype)
+        for n in KAT["inner_product_sizes"]:     # SYCLSpace_test.cpp:27-47
+            left = torch_gpu.arange(n, dtype=torch_gpu.float64, device="cuda:0").to(dtype)
+            right = torch_gpu.full((n,), -1.0, dtype=dtype, device="cuda:0")
+            if dtype == torch_gpu.float64:
+                assert space.innerProduct(left, right) == -n * (n - 1) / 2
+            else:
+                assert space.innerProduct(left, right) == pytest.approx(-n * (n - 1) / 2, rel=1e-5)
+        for n in KAT["assign_add_scale_sizes"]:
+            source_host = np.arange(n, dtype=np.float64)
+            source = space.makeVector(n)
+            space.assign(source, source_host)            # :49-68
+            copy = space.makeVector(n)
+            space.assign(copy, source)
+            back = np.empty(n)
+            space.assign(back, copy)
+            assert np.array_equal(back, source_host)
+            target = space.makeVector(n)
+            space.assign(target, 10.0 * source_host)
+            space.add(target, source, 100.0)             # :70-98
+            assert np.allclose(target.cpu().numpy(), 110.0 * source_host, rtol=1e-6, atol=0)
+            space.add(target, source, 1.0)
+            assert np.allclose(target.cpu().numpy(), 111.0 * source_host, rtol=1e-6, atol=0)
+            v = space.makeVector(n)
+            space.assign(v, source)
+            space.scale(v, 2.0)                          # :100-115
+            assert np.array_equal(v.cpu().numpy(), 2.0 * source_host)
+            t = space.makeVector(n)
+            space.assign(t, 10.0 * source_host)
+            space.scale(t, source, 100.0)                # :117-138
+            assert np.allclose(t.cpu().numpy(), 1e3 * source_host ** 2, rtol=1e-6, atol=0)
+            space.fill(t, -2.5)
+            assert np.all(t.cpu().numpy() == -2.5)
+
+
+def test_space_ops_match_oracle_bitwise(hip, torch_gpu):
+    rng = np.random.default_rng(11)
+    n = 100003
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    space = hip.HIPSpace()
+    for op, scale in [(oracle.OP_SCALE, 1.0), (oracle.OP_SCALE, -1.0), (oracle.OP_SCALE, 0.3), (oracle.OP_ADD, 1.0),
+                      (oracle.OP_ADD, -1.0), (oracle.OP_ADD, 0.7), (oracle.OP_SCALE_UNIFORM, 1.7)]:
+        expected = oracle.vector_op(op, a.copy(), b, scale)
+        t = to_dev(torch_gpu, a)
+        if op == oracle.OP_SCALE:
+            space.scale(t, to_dev(torch_gpu, b), scale)
+        elif op == oracle.OP_ADD:
+            space.add(t, to_dev(torch_gpu, b), scale)
+        else:
+            space.scale(t, scale)
+        got = t.cpu().numpy()
+        # elementwise ops: equal up to one fused multiply-add rounding
+        assert np.all(np.abs(got - expected) <= 2 * EPS64 * (np.abs(a) + np.abs(scale * a * b) + np.abs(scale * b)))
+    dot = space.innerProduct(to_dev(torch_gpu, a), to_dev(torch_gpu, b))
+    ref = oracle.inner_product(a, b)
+    assert abs(dot - ref) <= 2 * n * EPS64 * np.sum(np.abs(a * b))
+    again = space.innerProduct(to_dev(torch_gpu, a), to_dev(torch_gpu, b))
+    assert dot == again
+
+
+def test_diagonal_operator(hip, torch_gpu):
+    csr = systems.poisson5(32)
+    matrix = hip.CSRMatrix(csr)
+    space = hip.HIPSpace()
+    diag = hip.makeDiagonalOperator(matrix, space)
+    assert np.all(diag.inverseDiagonal.cpu().numpy() == 0.25)
+    rng = np.random.default_rng(5)
+    x, y0 = rng.standard_normal(csr.rows), rng.standard_normal(csr.rows)
+    for in_scale, out_scale in [(0, 1), (0, 2.0), (1, 1), (0.5, -3.0)]:
+        expected = oracle.diagonal_product(np.full(csr.rows, 0.25), in_scale, x, out_scale, y0.copy())
+        y = to_dev(torch_gpu, y0)
+        diag.product(in_scale, to_dev(torch_gpu, x), out_scale, y)
+        assert np.allclose(y.cpu().numpy(), expected, rtol=4 * EPS64, atol=1e-300)
+
+
+def test_diagonal_errors(hip, torch_gpu):
+    space = hip.HIPSpace()
+    missing = systems.CSR(3, 3, np.array([0, 2, 3, 5], np.int32), np.array([0, 1, 0, 0, 2], np.int32),
+                          np.array([2.0, 1.0, 1.0, 1.0, 3.0]))
+    with pytest.raises(hip._lib.MissingDiagonal, match="row 1"):
+        hip.makeDiagonalOperator(hip.CSRMatrix(missing), space)
+    zero = systems.CSR(3, 3, np.array([0, 1, 2, 3], np.int32), np.array([0, 1, 2], np.int32),
+                       np.array([1.0, 2.0, 0.0]))
+    with pytest.raises(hip._lib.ZeroDiagonal, match="row 2"):
+        hip.makeDiagonalOperator(hip.CSRMatrix(zero), space)

@@ -1,0 +1,215 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Checks: the known answers of the reference's SYCL space test
+(linalg/packages/solvers/test/SYCLSpace_test.cpp:27-186), closed forms
+(5-point eigenpairs, dense solves, finite termination), invariants (scale
+invariance, symmetric products) and the reference's 1D Dirichlet problem with
+exact solution u = 1 - x (fem/test/test_1d.cpp:291-313).
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from cie_amd import systems
+
+KAT = json.loads((Path(__file__).parent / "golden" / "kat.json").read_text())
+
+
+def csr_from_dense(a):
+    rows, cols = np.nonzero(a)
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    extents = np.zeros(a.shape[0] + 1, dtype=np.int32)
+    np.add.at(extents, rows + 1, 1)
+    return systems.CSR(a.shape[0], a.shape[1], np.cumsum(extents).astype(np.int32), cols.astype(np.int32),
+                       a[rows, cols].astype(np.float64))
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_inner_product_kat(threads):
+    # SYCLSpace_test.cpp:27-47: iota(0..n-1) . (-1) == -n(n-1)/2
+    for n in KAT["inner_product_sizes"]:
+        left = np.arange(n, dtype=np.float64)
+        right = -np.ones(n)
+        assert oracle.inner_product(left, right, threads) == -n * (n - 1) / 2
+
+
+def test_vector_ops_kat():
+    for n in KAT["assign_add_scale_sizes"]:
+        source = np.arange(n, dtype=np.float64)
+        target = 10.0 * source
+        oracle.vector_op(oracle.OP_ADD, target, source, 100.0)                  # :70-98
+        assert np.allclose(target, 110.0 * source, rtol=1e-15, atol=0)
+        oracle.vector_op(oracle.OP_ADD, target, source, 1.0)
+        assert np.allclose(target, 111.0 * source, rtol=1e-15, atol=0)
+        v = source.copy()
+        oracle.vector_op(oracle.OP_SCALE_UNIFORM, v, None, 2.0)                  # :100-115
+        assert np.array_equal(v, 2.0 * source)
+        t = 10.0 * source
+        oracle.vector_op(oracle.OP_SCALE, t, source, 100.0)                      # :117-138
+        assert np.allclose(t, 1e3 * source * source, rtol=1e-15, atol=0)
+        copy = np.empty(n)
+        oracle.vector_op(oracle.OP_ASSIGN, copy, source)                          # :49-68
+        assert np.array_equal(copy, source)
+        oracle.vector_op(oracle.OP_FILL, copy, None, -3.5)
+        assert np.all(copy == -3.5)
+
+
+def test_spmv_kat():
+    k = KAT["spmv"]                                                               # :140-186
+    csr = systems.CSR(2, k["columns"], np.array(k["row_extents"], np.int32), np.array(k["column_indices"], np.int32),
+                      np.array(k["entries"]))
+    out = np.array(k["out"])
+    oracle.csr_product(csr, k["in_scale"], np.array(k["in"]), k["out_scale"], out)
+    assert out.tolist() == k["expected"]
+
+
+@pytest.mark.parametrize("in_scale,out_scale", [(0, 1), (0, -1), (0, 2.5), (1, 1), (1, -1), (1, 0.5), (-1, 1),
+                                                (-1, -1), (-1, 3.0), (0.25, 1), (0.25, -1), (0.25, -2.0)])
+def test_spmv_scale_classes_against_dense(in_scale, out_scale):
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((40, 30)) * (rng.random((40, 30)) < 0.2)
+    a[5, :] = 0.0  # an empty row
+    csr = csr_from_dense(a)
+    x = rng.standard_normal(30)
+    y0 = rng.standard_normal(40)
+    y = y0.copy()
+    oracle.csr_product(csr, in_scale, x, out_scale, y)
+    # LinearOperator.hpp:15-33 contract (the restatement takes the contract for inScale=outScale=-1).
+    assert np.allclose(y, in_scale * y0 + out_scale * (a @ x), rtol=1e-13, atol=1e-13)
+
+
+def test_spmv_five_point_eigenvectors():
+    m = 32
+    csr = systems.poisson5(m)
+    i = np.arange(1, m + 1)
+    for j, k in [(1, 1), (3, 7), (m, 2)]:
+        v = np.outer(np.sin(k * np.pi * i / (m + 1)), np.sin(j * np.pi * i / (m + 1))).ravel()
+        lam = 4 - 2 * math.cos(j * math.pi / (m + 1)) - 2 * math.cos(k * math.pi / (m + 1))
+        out = np.zeros(m * m)
+        oracle.csr_product(csr, 0.0, v, 1.0, out)
+        assert np.allclose(out, lam * v, atol=1e-12)
+
+
+def test_inverse_diagonal_and_errors():
+    csr = systems.poisson5(8)
+    assert np.all(oracle.inverse_diagonal(csr) == 0.25)
+    a = np.diag([2.0, 0.0, 3.0]) + np.diag([1.0, 1.0], 1)
+    a[1, 1] = 0.0
+    bad = csr_from_dense(a)
+    with pytest.raises(oracle.OracleError) as missing:
+        oracle.inverse_diagonal(bad)   # row 1 has no stored diagonal
+    assert missing.value.status == 6
+    zero = systems.CSR(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([1.0, 0.0]))
+    with pytest.raises(oracle.OracleError) as explicit:
+        oracle.inverse_diagonal(zero)
+    assert explicit.value.status == 5
+
+
+def test_pcg_dense_spd():
+    rng = np.random.default_rng(3)
+    for n in (5, 17, 40):
+        m = rng.standard_normal((n, n))
+        a = m @ m.T + n * np.eye(n)
+        a[np.abs(a) < 0.5] = 0.0
+        a = 0.5 * (a + a.T)
+        a += n * np.eye(n)
+        b = rng.standard_normal(n)
+        x, stats, _, _ = oracle.pcg(csr_from_dense(a), b, relative_tolerance=1e-14, max_iterations=500)
+        exact = np.linalg.solve(a, b)
+        assert np.linalg.norm(x - exact) / np.linalg.norm(exact) < 1e-12
+        assert stats.relativeResidual < 1e-14
+
+
+def test_pcg_finite_termination_diagonal():
+    # A diagonal matrix with k distinct eigenvalues: plain CG ends in at most k steps.
+    values = np.repeat([1.0, 4.0, 9.0, 25.0], 10)
+    csr = csr_from_dense(np.diag(values))
+    b = np.random.default_rng(1).standard_normal(values.size)
+    _, stats, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-12, precondition=False)
+    assert stats.iterationCount + 1 <= 4
+    # Jacobi-preconditioned: D^-1 A = I, one step.
+    _, stats, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-12, precondition=True)
+    assert stats.iterationCount == 0
+
+
+def test_pcg_scale_invariance_and_jacobi_equivalence():
+    w = systems.five_point_2d(48)
+    _, s1, h1, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-8, history=True)
+    scaled = systems.CSR(w.matrix.rows, w.matrix.columns, w.matrix.row_extents, w.matrix.column_indices,
+                         w.matrix.entries * 8.0)
+    _, s2, h2, _ = oracle.pcg(scaled, w.rhs, relative_tolerance=1e-8, history=True)
+    assert s1.iterationCount == s2.iterationCount
+    assert np.allclose(h1, h2, rtol=1e-9)
+    # constant diagonal: Jacobi-PCG == CG up to the scale of z
+    _, s3, h3, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-8, precondition=False, history=True)
+    assert s3.iterationCount == s1.iterationCount
+    assert np.allclose(h1, h3, rtol=1e-9)
+
+
+def test_pcg_one_dimensional_dirichlet_exact_solution():
+    # fem/test/test_1d.cpp: 10 nodes, u(0) = 1, u(1) = 0 -> u = 1 - x at the nodes.
+    nodes = 10
+    h = 1.0 / (nodes - 1)
+    inner = nodes - 2
+    a = (np.diag(np.full(inner, 2.0)) - np.diag(np.ones(inner - 1), 1) - np.diag(np.ones(inner - 1), -1)) / h
+    b = np.zeros(inner)
+    b[0] = 1.0 / h  # lifted Dirichlet value u(0) = 1
+    x, stats, _, _ = oracle.pcg(csr_from_dense(a), b, relative_tolerance=1e-14)
+    xs = np.linspace(0, 1, nodes)[1:-1]
+    assert np.allclose(x, 1 - xs, atol=1e-13)
+
+
+def test_pcg_iteration_count_rules():
+    w = systems.five_point_2d(16)
+    _, stats, hist, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-8, history=True)
+    # converged: the final trip is not counted (ConjugateGradients_impl.hpp:184-185)
+    assert len(hist) == stats.iterationCount + 1
+    assert hist[-1] / np.linalg.norm(w.rhs) < 1e-8 <= hist[-2] / np.linalg.norm(w.rhs)
+    _, capped, _, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-30, max_iterations=7)
+    assert capped.iterationCount == 7
+    _, none, _, _ = oracle.pcg(w.matrix, w.rhs, max_iterations=0)
+    assert none.iterationCount == 0 and none.absoluteResidual == np.finfo(float).max
+
+
+def test_pcg_errors():
+    neg = csr_from_dense(-np.eye(4))
+    with pytest.raises(oracle.OracleError) as breakdown:
+        oracle.pcg(neg, np.ones(4))
+    assert breakdown.value.status == 4
+    indefinite = csr_from_dense(np.diag([1.0, 1.0, 1.0, -1.0]) + 0.0)
+    with pytest.raises(oracle.OracleError) as notspd:
+        oracle.pcg(indefinite, np.array([0.1, 0.1, 0.1, 1.0]), precondition=False)
+    assert notspd.value.status == 3
+
+
+def test_threaded_pcg_matches_sequential():
+    w = systems.five_point_2d(64)
+    x1, s1, _, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-8, threads=1)
+    x4, s4, _, _ = oracle.pcg(w.matrix, w.rhs, relative_tolerance=1e-8, threads=4)
+    assert abs(int(s1.iterationCount) - int(s4.iterationCount)) <= 1
+    assert np.linalg.norm(x1 - x4) / np.linalg.norm(x1) < 1e-10
+
+
+def test_element_gradient_against_dense_assembly():
+    # dJ/dk_e for J = lambda^T K(k) u is lambda_e^T K_e0 u_e; the oracle returns its negative.
+    elements = 3
+    ke = systems.element_matrix(3, 2, elements)
+    dofs = systems.element_dofs(3, 2, elements)
+    count = dofs.shape[0]
+    rng = np.random.default_rng(0)
+    k = np.exp(rng.standard_normal(count) * 0.3)
+    csr = systems.lagrange(3, 2, elements, conductivity=k)
+    lam = rng.standard_normal(csr.rows)
+    u = rng.standard_normal(csr.rows)
+    grad = oracle.element_gradient(dofs, ke, lam, u)
+    a = csr.to_scipy()
+    for e in (0, 5, count - 1):
+        bumped = k.copy()
+        bumped[e] += 1.0
+        delta = systems.lagrange(3, 2, elements, conductivity=bumped).to_scipy() - a
+        assert np.isclose(-grad[e], lam @ (delta @ u), rtol=1e-11, atol=1e-12)

@@ -1,0 +1,127 @@
+"""The structured system generator: closed-form sizes, element matrices and
+assembly invariants (symmetry, zero row sums before elimination, linearity in
+the element conductivities), and the seeded RNG against its fixture."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cie_amd import systems
+
+RNG = json.loads((Path(__file__).parent / "golden" / "rng.json").read_text())
+
+
+def test_rng_fixture():
+    for seed, data in RNG.items():
+        raw = systems.splitmix64(int(seed), 16)
+        assert [str(v) for v in raw] == data["raw"]
+        assert systems.uniform(int(seed), 16).tolist() == data["uniform"]
+
+
+def test_normal_moments():
+    g = systems.normal(2, 1.0, 200000)
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
+    g = systems.normal(4, 0.5, 200000)
+    assert abs(g.std() - 0.5) < 0.005
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_closed_form_sizes(n, order):
+    # per-dimension couplings: full 2*order*n + 1 - ... ; eliminated Q2: 8n-9, Q4: 24n-17 (3D cube root)
+    rows, nnz, count, npe = systems.lagrange_size(3, order, n, eliminate=False)
+    assert rows == (order * n + 1) ** 3 and count == n ** 3 and npe == (order + 1) ** 3
+    csr = systems.lagrange(2, order, n, eliminate=False)
+    per_dim = np.diff(csr.row_extents).reshape(order * n + 1, order * n + 1)
+    assert csr.rows == (order * n + 1) ** 2
+    if order == 2:
+        assert systems.lagrange_size(3, 2, n, eliminate=False)[1] == (8 * n + 1) ** 3
+        if n > 1:
+            assert systems.lagrange_size(3, 2, n)[1] == (8 * n - 9) ** 3
+    if order == 4:
+        assert systems.lagrange_size(3, 4, n, eliminate=False)[1] == (24 * n + 1) ** 3
+        if n > 1:
+            assert systems.lagrange_size(3, 4, n)[1] == (24 * n - 17) ** 3
+    assert per_dim.sum() == csr.nnz
+
+
+def test_benchmark_sizes():
+    assert systems.lagrange_size(3, 2, 64) == (2048383, 127263527, 262144, 27)
+    assert systems.lagrange_size(3, 4, 32)[:2] == (2048383, 423564751)
+    assert systems.lagrange_size(3, 2, 48)[:2] == (857375, 52734375)
+    assert systems.poisson5(256).nnz == 326656
+
+
+def test_q2_element_matrix_closed_form():
+    h = 1.0 / 3
+    k1 = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / (3 * h)
+    m1 = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) * h / 30
+    expected = np.kron(np.kron(m1, m1), k1) + np.kron(np.kron(m1, k1), m1) + np.kron(np.kron(k1, m1), m1)
+    assert np.allclose(systems.element_matrix(3, 2, 3), expected, rtol=0, atol=1e-14)
+    expected2 = np.kron(m1 / h, k1 * h) + np.kron(k1 * h, m1 / h)
+    assert np.allclose(systems.element_matrix(2, 2, 3), expected2, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (2, 4), (3, 2), (3, 4)])
+def test_element_matrix_properties(dim, order):
+    ke = systems.element_matrix(dim, order, 2)
+    assert np.allclose(ke, ke.T, atol=1e-14)
+    assert np.allclose(ke.sum(axis=1), 0.0, atol=1e-12)
+    eig = np.linalg.eigvalsh(ke)
+    assert eig[0] > -1e-12 and eig[1] > 1e-8  # positive semidefinite, rank npe - 1
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 2, 4), (2, 4, 3), (3, 2, 3), (3, 4, 2)])
+def test_assembly_symmetry_row_sums_and_dense(dim, order, n):
+    rng = np.random.default_rng(1)
+    count = n ** dim
+    k = np.exp(rng.standard_normal(count))
+    full = systems.lagrange(dim, order, n, conductivity=k, eliminate=False).to_scipy().toarray()
+    assert np.allclose(full, full.T, atol=1e-12)
+    assert np.allclose(full.sum(axis=1), 0.0, atol=1e-10)
+    # dense element-by-element assembly from the element matrix and dof map
+    ke = systems.element_matrix(dim, order, n)
+    dofs = systems.element_dofs(dim, order, n, eliminate=False)
+    dense = np.zeros_like(full)
+    for e in range(count):
+        d = dofs[e]
+        dense[np.ix_(d, d)] += k[e] * ke
+    assert np.allclose(full, dense, atol=1e-12)
+    # elimination = deleting boundary rows and columns
+    elim = systems.lagrange(dim, order, n, conductivity=k).to_scipy().toarray()
+    m = order * n + 1
+    grid = np.indices((m,) * dim).reshape(dim, -1)
+    interior = np.all((grid > 0) & (grid < m - 1), axis=0)
+    assert np.allclose(elim, full[np.ix_(interior, interior)], atol=1e-12)
+    assert np.all(np.linalg.eigvalsh(elim) > 0)
+
+
+def test_rows_sorted_and_unique():
+    csr = systems.lagrange(3, 2, 5)
+    for row in range(csr.rows):
+        c = csr.column_indices[csr.row_extents[row]:csr.row_extents[row + 1]]
+        assert np.all(np.diff(c) > 0)
+        assert row in c
+
+
+def test_thread_count_independence():
+    k = np.exp(systems.normal(4, 0.7, 4 ** 3))
+    a = systems.lagrange(3, 4, 4, conductivity=k, threads=1)
+    b = systems.lagrange(3, 4, 4, conductivity=k, threads=7)
+    assert np.array_equal(a.entries, b.entries) and np.array_equal(a.column_indices, b.column_indices)
+
+
+def test_five_point_structure():
+    csr = systems.poisson5(5)
+    a = csr.to_scipy().toarray()
+    t = 2 * np.eye(5) - np.eye(5, k=1) - np.eye(5, k=-1)
+    assert np.array_equal(a, np.kron(np.eye(5), t) + np.kron(t, np.eye(5)))
+
+
+def test_csr_slots():
+    csr = systems.poisson5(4)
+    slots = systems.csr_slots(csr, [0, 5, 5, 15], [0, 4, 7, 14])
+    assert csr.entries[slots[0]] == 4.0 and csr.entries[slots[1]] == -1.0
+    assert slots[2] == -1
+    assert csr.column_indices[slots[3]] == 14
