@@ -111,8 +111,8 @@ def main():
     t0 = time.perf_counter()
     matrix = linalg.CSRMatrix(csr, context)
     upload_seconds = time.perf_counter() - t0
-    tune64 = matrix.autotune(64, 10)
-    tune32 = matrix.autotune(32, 10)
+    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector"))
+    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector"))
     b = torch.from_numpy(workload.rhs).to("cuda:0")
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, args.tolerance)

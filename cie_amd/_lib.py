@@ -8,7 +8,11 @@ import ctypes
 from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
 from pathlib import Path
 
-LIBRARY_PATH = Path(__file__).resolve().parent / "libcie_hip.so"
+import os
+
+# CIE_HIP_LIBRARY selects another build of the same C ABI (e.g. one compiled
+# with different load or launch options); by default the in-tree build.
+LIBRARY_PATH = Path(os.environ.get("CIE_HIP_LIBRARY", Path(__file__).resolve().parent / "libcie_hip.so"))
 
 OK = 0
 ERROR_INVALID_ARGUMENT = 1
@@ -23,6 +27,8 @@ ERROR_NO_DEVICE = 9
 
 HOST = 0
 DEVICE = 1
+SPMV_VECTOR = 0
+SPMV_STAGED = 1
 PLANE_F64 = 1
 PLANE_F32 = 2
 
@@ -116,7 +122,7 @@ _SIGNATURES = {
     "cie_hip_csr_update_entries": (c_int, [_P, _P]),
     "cie_hip_csr_set_launch": (c_int, [_P, c_int, c_int, c_int, c_int]),
     "cie_hip_csr_get_launch": (c_int, [_P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
-    "cie_hip_csr_autotune": (c_int, [_P, c_int, c_int, POINTER(c_double)]),
+    "cie_hip_csr_autotune": (c_int, [_P, c_int, c_int, c_int, POINTER(c_double)]),
     "cie_hip_csr_product_f64": (c_int, [_P, _P, c_double, _P, c_int64, c_double, _P, c_int64]),
     "cie_hip_csr_product_f32": (c_int, [_P, _P, c_float, _P, c_int64, c_float, _P, c_int64]),
     # spaces
@@ -142,6 +148,17 @@ _SIGNATURES = {
                                 POINTER(SolveInfo)]),
     "cie_hip_pcg_mixed": (c_int, [_P, _P, c_double, _P, c_double, _P, c_int64, POINTER(MixedSettings),
                                   POINTER(SolveInfo)]),
+    # batched
+    "cie_hip_batch_create": (c_int, [_P, _P, c_int32, _P, POINTER(_P)]),
+    "cie_hip_batch_destroy": (c_int, [_P]),
+    "cie_hip_batch_set_tile_rows": (c_int, [_P, _P]),
+    "cie_hip_batch_get_tile_rows": (c_int, [_P, _P]),
+    "cie_hip_batch_autotune": (c_int, [_P, c_int, c_int]),
+    "cie_hip_pcg_batched": (c_int, [_P, _P, _P, _P, POINTER(MixedSettings), c_int, c_int, _P, _P, _P,
+                                    POINTER(SolveInfo)]),
+    # adjoint
+    "cie_hip_element_gradient": (c_int, [_P, c_int64, c_int32, _P, _P, _P, _P, _P, _P]),
+    "cie_hip_csr_add_element_matrices": (c_int, [_P, c_int64, c_int32, _P, _P, _P]),
     # structured systems (include/cie_hip_systems.h)
     "cie_sys_last_error": (c_char_p, []),
     "cie_sys_splitmix64": (c_int, [c_uint64, c_int64, _P]),

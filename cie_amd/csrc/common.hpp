@@ -84,10 +84,12 @@ struct cie_hip_context_s {
 
 // Launch configuration of a CSR SpMV for one value precision.
 struct CsrLaunch {
-    int lanes = 0;    // lanes per row (a power of two up to 64)
+    int kernel = 0;   // CIE_HIP_SPMV_STAGED or CIE_HIP_SPMV_VECTOR
+    int width = 0;    // staged: LDS entries per wavefront segment; vector: lanes per row
     int unroll = 0;   // entries per lane loaded before they are used
     int block = 0;    // threads per block
-    int grid = 0;     // blocks of the row-chunked kernels (multiple of 8)
+    int grid = 0;     // blocks (multiple of 8, one contiguous run per XCD)
+    int param = 0;    // staged: tiles per block; vector: rows per block
 };
 
 struct cie_hip_csr_s {

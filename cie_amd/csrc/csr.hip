@@ -4,6 +4,8 @@
 #include "kernels.hpp"
 
 #include <limits>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 using namespace ciehip;
@@ -13,13 +15,12 @@ namespace {
 template <class T>
 int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
                   const T* entries) {
-    const int perChunk = rowsPerChunk(m->rows, launch);
-    const bool ok = withShape(launch.lanes, launch.unroll, [&](auto lanes, auto unroll) {
-        constexpr int L = decltype(lanes)::value;
-        constexpr int U = decltype(unroll)::value;
-        hipLaunchKernelGGL((csrProductKernel<T, T, L, U, kSpmvBlock>), dim3(launch.grid), dim3(kSpmvBlock), 0,
-                           m->context->stream, int(m->rows), perChunk, m->rowExtents, m->columnIndices, entries,
-                           in, out, inScale, outScale);
+    const bool ok = withWalker<T, T>(launch, [&](auto walker) {
+        using Walker = decltype(walker);
+        const GridShape shape = gridFor(csrProductKernel<Walker>, launch, m->rows, m->context->computeUnits);
+        hipLaunchKernelGGL((csrProductKernel<Walker>), dim3(shape.grid), dim3(Walker::kBlock), 0, m->context->stream,
+                           int(m->rows), shape.param, m->rowExtents, m->columnIndices, entries, in, out, inScale,
+                           outScale);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     CIE_HIP_CHECK(hipGetLastError());
@@ -94,6 +95,18 @@ int inverseDiagonal(cie_hip_csr m, T* inverse, int64_t* failedRow) {
 }  // namespace
 
 namespace ciehip {
+
+int residentBlocks(const void* kernel, int block) {
+    static std::mutex mutex;
+    static std::unordered_map<const void*, int> cache;
+    std::lock_guard<std::mutex> lock(mutex);
+    const auto found = cache.find(kernel);
+    if (found != cache.end()) return found->second;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, block, 0) != hipSuccess || blocks < 1) blocks = 1;
+    cache.emplace(kernel, blocks);
+    return blocks;
+}
 
 int matrixWorkspace(cie_hip_csr m, size_t bytes, void** pointer) {
     if (m->workspaceBytes < bytes) {
@@ -209,10 +222,7 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
     }
     if (hipStreamSynchronize(context->stream) != hipSuccess)
         return bail(CIE_HIP_ERROR_HIP, "cie_hip_csr_create: stream synchronisation failed");
-    for (CsrLaunch* launch : {&m->launch64, &m->launch32}) {
-        *launch = defaultLaunch(m->meanRowLength, context->computeUnits, rows);
-        launch->grid = spmvGrid(rows, *launch, context->computeUnits);
-    }
+    m->launch64 = m->launch32 = defaultLaunch(m->meanRowLength, m->maxRowLength, context->computeUnits, rows);
     *matrix = m;
     return CIE_HIP_OK;
 }
@@ -252,64 +262,91 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries) {
     return CIE_HIP_OK;
 }
 
-int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int lanes, int unroll, int block) {
+int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll) {
     if (!matrix || (precision != 32 && precision != 64))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: invalid matrix or precision");
-    if (!validShape(lanes, unroll))
-        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: lanes must be 1..64 (power of two), unroll 1, 2, 4 or 8");
-    if (block != kSpmvBlock)
-        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: supported block size is 256");
+    if (kernel == CIE_HIP_SPMV_STAGED) {
+        if (!validStage(width, unroll))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: staged width must be 512 or 1024, unroll 4 or 8");
+    } else if (kernel == CIE_HIP_SPMV_VECTOR) {
+        if (!validShape(width, unroll))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: vector width must be 1..64 (power of two), unroll 1, 2, 4 or 8");
+    } else {
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: unknown kernel");
+    }
     CsrLaunch& launch = launchOf(matrix, precision);
-    launch.lanes = lanes;
+    launch.kernel = kernel;
+    launch.width = width;
     launch.unroll = unroll;
-    launch.block = block;
-    launch.grid = spmvGrid(matrix->rows, launch, matrix->context->computeUnits);
+    launch.block = kSpmvBlock;
+    sizeLaunch(launch, matrix->rows, matrix->context->computeUnits);
     return CIE_HIP_OK;
 }
 
-int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* lanes, int* unroll, int* block) {
+int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll) {
     if (!matrix || (precision != 32 && precision != 64))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_get_launch: invalid matrix or precision");
     const CsrLaunch& launch = launchOf(matrix, precision);
-    if (lanes) *lanes = launch.lanes;
+    if (kernel) *kernel = launch.kernel;
+    if (width) *width = launch.width;
     if (unroll) *unroll = launch.unroll;
-    if (block) *block = launch.block;
     return CIE_HIP_OK;
 }
 
-int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, double* bestSeconds) {
+int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds) {
     if (!matrix || (precision != 32 && precision != 64))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: invalid matrix or precision");
     if (precision == 32 && !matrix->entries32)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
+    if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED))))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: empty kernel mask");
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
     repetitions = std::max(1, repetitions);
+    const int computeUnits = ctx->computeUnits;
+
+    std::vector<CsrLaunch> candidates;
+    if ((kernels & (1 << CIE_HIP_SPMV_STAGED)) && matrix->maxRowLength <= kStagedMaxRow) {
+        for (int width : {512, 1024})
+            for (int unroll : {4, 8}) {
+                CsrLaunch c;
+                c.kernel = CIE_HIP_SPMV_STAGED;
+                c.width = width;
+                c.unroll = unroll;
+                c.block = kSpmvBlock;
+                sizeLaunch(c, matrix->rows, computeUnits);
+                candidates.push_back(c);
+            }
+    }
+    if (kernels & (1 << CIE_HIP_SPMV_VECTOR)) {
+        // Lanes within a factor 4 of about one pass of 4 entries per lane per row.
+        const int seed = std::min(64, nextPow2(int(std::ceil(std::max(1.0, matrix->meanRowLength) / 4))));
+        for (int lanes = std::max(1, seed / 4); lanes <= std::min(64, seed * 4); lanes *= 2)
+            for (int unroll : {1, 2, 4, 8}) {
+                CsrLaunch c;
+                c.kernel = CIE_HIP_SPMV_VECTOR;
+                c.width = lanes;
+                c.unroll = unroll;
+                c.block = kSpmvBlock;
+                sizeLaunch(c, matrix->rows, computeUnits);
+                candidates.push_back(c);
+            }
+    }
+    if (candidates.empty())
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: no candidate for this matrix");
+
     const size_t valueSize = precision == 32 ? sizeof(float) : sizeof(double);
     void* buffer = nullptr;
-    const size_t inBytes = valueSize * size_t(std::max<int64_t>(1, matrix->columns));
-    const size_t outBytes = valueSize * size_t(std::max<int64_t>(1, matrix->rows));
-    CIE_HIP_CHECK(hipMalloc(&buffer, inBytes + outBytes));
-    CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, inBytes + outBytes, ctx->stream));
+    const int64_t inCount = std::max<int64_t>(1, matrix->columns);
+    const size_t bytes = valueSize * size_t(inCount + std::max<int64_t>(1, matrix->rows));
+    CIE_HIP_CHECK(hipMalloc(&buffer, bytes));
+    CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, bytes, ctx->stream));
     hipEvent_t start, stop;
     CIE_HIP_CHECK(hipEventCreate(&start));
     CIE_HIP_CHECK(hipEventCreate(&stop));
 
-    // Candidates: lanes within a factor 4 of the starting shape, unroll 1..8.
-    const CsrLaunch seed = defaultLaunch(matrix->meanRowLength, ctx->computeUnits, matrix->rows);
-    std::vector<CsrLaunch> candidates;
-    for (int lanes = std::max(1, seed.lanes / 4); lanes <= std::min(64, seed.lanes * 4); lanes *= 2)
-        for (int unroll : {1, 2, 4, 8}) {
-            CsrLaunch c;
-            c.lanes = lanes;
-            c.unroll = unroll;
-            c.block = kSpmvBlock;
-            c.grid = spmvGrid(matrix->rows, c, ctx->computeUnits);
-            candidates.push_back(c);
-        }
     double best = std::numeric_limits<double>::infinity();
-    CsrLaunch bestLaunch = seed;
-    bestLaunch.grid = spmvGrid(matrix->rows, seed, ctx->computeUnits);
+    CsrLaunch bestLaunch = candidates.front();
     int status = CIE_HIP_OK;
     for (const CsrLaunch& c : candidates) {
         for (int pass = 0; pass < 2 && status == CIE_HIP_OK; ++pass) {  // pass 0 warms up
@@ -317,10 +354,10 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, dou
             for (int r = 0; r < (pass ? repetitions : 1) && status == CIE_HIP_OK; ++r) {
                 if (precision == 32) {
                     float* in = static_cast<float*>(buffer);
-                    status = launchProduct<float>(matrix, c, 0.0f, in, 1.0f, in + std::max<int64_t>(1, matrix->columns), matrix->entries32);
+                    status = launchProduct<float>(matrix, c, 0.0f, in, 1.0f, in + inCount, matrix->entries32);
                 } else {
                     double* in = static_cast<double*>(buffer);
-                    status = launchProduct<double>(matrix, c, 0.0, in, 1.0, in + std::max<int64_t>(1, matrix->columns), matrix->entries64);
+                    status = launchProduct<double>(matrix, c, 0.0, in, 1.0, in + inCount, matrix->entries64);
                 }
             }
             if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
@@ -328,7 +365,7 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, dou
                 break;
             }
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, start, stop);
+            (void)hipEventElapsedTime(&ms, start, stop);
             if (pass && ms * 1e-3 / repetitions < best) {
                 best = ms * 1e-3 / repetitions;
                 bestLaunch = c;
@@ -336,9 +373,9 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, dou
         }
         if (status != CIE_HIP_OK) break;
     }
-    hipEventDestroy(start);
-    hipEventDestroy(stop);
-    hipFree(buffer);
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    (void)hipFree(buffer);
     if (status != CIE_HIP_OK) return setError(status, std::string("cie_hip_csr_autotune: ") + cie_hip_last_error());
     launchOf(matrix, precision) = bestLaunch;
     if (bestSeconds) *bestSeconds = best;

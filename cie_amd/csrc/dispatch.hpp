@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -10,6 +11,7 @@
 namespace ciehip {
 
 constexpr int kSpmvBlock = 256;
+constexpr int kStagedMaxRow = 8192;  // longest row the staged kernel is chosen for by default
 
 // Lanes per row x entries per lane in flight.
 #define CIEHIP_SHAPES(X)                                              \
@@ -39,6 +41,36 @@ inline bool validShape(int lanes, int unroll) {
     return withShape(lanes, unroll, [](auto, auto) {});
 }
 
+#define CIEHIP_STAGES(X) X(512, 4) X(512, 8) X(1024, 4) X(1024, 8)
+
+inline bool validStage(int capacity, int unroll) {
+    switch (capacity * 16 + unroll) {
+#define CIEHIP_STAGE_VALID(C, U) case C * 16 + U: return true;
+        CIEHIP_STAGES(CIEHIP_STAGE_VALID)
+#undef CIEHIP_STAGE_VALID
+    default: return false;
+    }
+}
+
+// Calls f(Walker{}) with the row walker the launch selects; false if unsupported.
+template <class TV, class TM, class F>
+bool withWalker(const CsrLaunch& launch, F&& f) {
+    if (launch.kernel == CIE_HIP_SPMV_STAGED) {
+        switch (launch.width * 16 + launch.unroll) {
+#define CIEHIP_STAGE_CASE(C, U)                                   \
+    case C * 16 + U:                                              \
+        f(StagedRows<TV, TM, C, U, kSpmvBlock>{});                \
+        return true;
+            CIEHIP_STAGES(CIEHIP_STAGE_CASE)
+#undef CIEHIP_STAGE_CASE
+        default: return false;
+        }
+    }
+    return withShape(launch.width, launch.unroll, [&](auto lanes, auto unroll) {
+        f(VectorRows<TV, TM, decltype(lanes)::value, decltype(unroll)::value, kSpmvBlock>{});
+    });
+}
+
 inline int nextPow2(int v) {
     int p = 1;
     while (p < v) p *= 2;
@@ -47,39 +79,83 @@ inline int nextPow2(int v) {
 
 inline int roundUp(int64_t value, int64_t multiple) { return int((value + multiple - 1) / multiple * multiple); }
 
-// Starting point before autotuning: about one pass of L*U entries per row.
-inline CsrLaunch defaultLaunch(double meanRowLength, int computeUnits, int64_t rows) {
+// Fills grid and param of a launch for a matrix with `rows` rows: a multiple
+// of 8 blocks (one contiguous run per XCD), at most 8 blocks per compute unit.
+inline void sizeLaunch(CsrLaunch& launch, int64_t rows, int computeUnits) {
+    const int64_t cap = std::max(8, computeUnits * 8);
+    if (launch.kernel == CIE_HIP_SPMV_STAGED) {
+        const int64_t tiles = std::max<int64_t>(1, (rows + kWave - 1) / kWave);
+        const int64_t wavesPerBlock = launch.block / kWave;
+        int grid = int(std::min<int64_t>((tiles + wavesPerBlock - 1) / wavesPerBlock, cap));
+        launch.grid = std::min(roundUp(grid, kXcds), kMaxPartials);
+        launch.param = int((tiles + launch.grid - 1) / launch.grid);
+    } else {
+        const int64_t rowsPerStep = launch.block / launch.width;
+        const int64_t steps = std::max<int64_t>(1, (rows + rowsPerStep - 1) / rowsPerStep);
+        int grid = int(std::min<int64_t>(steps, cap));
+        launch.grid = std::min(roundUp(grid, kXcds), kMaxPartials);
+        const int64_t perBlock = (rows + launch.grid - 1) / launch.grid;
+        launch.param = std::max<int>(int(rowsPerStep), roundUp(perBlock, rowsPerStep));
+    }
+}
+
+// Blocks of `kernel` that are resident on one compute unit at `block` threads
+// (HIP occupancy query, cached per kernel).
+int residentBlocks(const void* kernel, int block);
+
+struct GridShape {
+    int grid;   // blocks
+    int param;  // staged: tiles per block; vector: rows per block
+};
+
+// One resident round of blocks: compute units x blocks per unit (rounded down
+// to a multiple of 8 so each XCD gets a contiguous run), but no more blocks
+// than there is work for.
+template <class Kernel>
+GridShape gridFor(Kernel kernel, const CsrLaunch& launch, int64_t rows, int computeUnits) {
+    const int perUnit = residentBlocks(reinterpret_cast<const void*>(kernel), launch.block);
+    int cap = std::max(1, computeUnits) * perUnit;
+    if (cap >= kXcds) cap -= cap % kXcds;
+    cap = std::min(cap, kMaxPartials);
+    GridShape shape;
+    if (launch.kernel == CIE_HIP_SPMV_STAGED) {
+        const int64_t tiles = std::max<int64_t>(1, (rows + kWave - 1) / kWave);
+        const int64_t wavesPerBlock = launch.block / kWave;
+        shape.grid = int(std::min<int64_t>((tiles + wavesPerBlock - 1) / wavesPerBlock, cap));
+        shape.param = int((tiles + shape.grid - 1) / shape.grid);
+    } else {
+        const int64_t rowsPerStep = launch.block / launch.width;
+        const int64_t steps = std::max<int64_t>(1, (rows + rowsPerStep - 1) / rowsPerStep);
+        shape.grid = int(std::min<int64_t>(steps, cap));
+        const int64_t perBlock = (rows + shape.grid - 1) / shape.grid;
+        shape.param = std::max<int>(int(rowsPerStep), roundUp(perBlock, rowsPerStep));
+    }
+    return shape;
+}
+
+// Starting point before autotuning: the staged kernel (bit-identical rows),
+// unless some row is too long for one lane to sum it, then vector rows with
+// about one pass of L * U entries per row.
+inline CsrLaunch defaultLaunch(double meanRowLength, int maxRowLength, int computeUnits, int64_t rows) {
     CsrLaunch launch;
     launch.block = kSpmvBlock;
-    int unroll = 4;
-    int lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
-    if (lanes < 4) {
-        unroll = 2;
-        lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
+    if (maxRowLength <= kStagedMaxRow) {
+        launch.kernel = CIE_HIP_SPMV_STAGED;
+        launch.width = 1024;
+        launch.unroll = 8;
+    } else {
+        launch.kernel = CIE_HIP_SPMV_VECTOR;
+        int unroll = 4;
+        int lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
+        if (lanes < 4) {
+            unroll = 2;
+            lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
+        }
+        launch.width = std::min(64, std::max(1, lanes));
+        launch.unroll = unroll;
     }
-    launch.lanes = std::min(64, std::max(1, lanes));
-    launch.unroll = unroll;
-    launch.grid = 0;  // filled by spmvGrid
-    (void)computeUnits;
-    (void)rows;
+    sizeLaunch(launch, rows, computeUnits);
     return launch;
-}
-
-// Grid of the row-chunked SpMV kernels: a multiple of 8 (one run per XCD),
-// at most 8 blocks per compute unit, at least one row group per block.
-inline int spmvGrid(int64_t rows, const CsrLaunch& launch, int computeUnits) {
-    const int64_t rowsPerIteration = launch.block / launch.lanes;
-    const int64_t groups = std::max<int64_t>(1, (rows + rowsPerIteration - 1) / rowsPerIteration);
-    const int64_t cap = std::max(8, computeUnits * 8);
-    int grid = int(std::min<int64_t>(groups, cap));
-    grid = roundUp(grid, kXcds);
-    return std::min(grid, kMaxPartials);
-}
-
-inline int rowsPerChunk(int64_t rows, const CsrLaunch& launch) {
-    const int64_t rowsPerIteration = launch.block / launch.lanes;
-    const int64_t perBlock = (rows + launch.grid - 1) / launch.grid;
-    return std::max<int>(int(rowsPerIteration), roundUp(perBlock, rowsPerIteration));
 }
 
 // Grid of the elementwise/reduction kernels: depends only on n.

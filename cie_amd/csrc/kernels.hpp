@@ -6,8 +6,8 @@
 //     in increasing order, then an xor-butterfly combines the L lane sums
 //     (every lane of the group ends with the same bits, since a+b == b+a);
 //   * a grid reduction writes one partial per block (in-block order fixed),
-//     and the last block to arrive sums the partials in index order. Only an
-//     integer ticket counter is updated atomically, never a value.
+//     and a one-block kernel launched after it sums the partials in block
+//     order; no value is ever accumulated with atomics.
 #pragma once
 
 #include "common.hpp"
@@ -42,24 +42,6 @@ __device__ __forceinline__ T blockSum(T value, T* lds) {
     return total;
 }
 
-// After thread 0 has stored this block's partials: returns true (in every
-// thread) in the last block to arrive, which may then read all partials.
-__device__ __forceinline__ bool arriveLast(unsigned* counter) {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned ticket = atomicAdd(counter, 1u);
-        last = ticket == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-__device__ __forceinline__ void resetCounter(unsigned* counter) {
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-}
-
 // Sum partials[0..count) with stride `stride` in a fixed order; valid in thread 0.
 template <int BS, class T>
 __device__ __forceinline__ T sumPartials(const T* partials, int count, int stride, T* lds) {
@@ -78,10 +60,22 @@ __device__ __forceinline__ bool floatLess(double left, double right) {
     return left < right && !floatEqual(left, right);
 }
 
+// Matrix entries and column indices are read once per product. Plain loads
+// measured faster than non-temporal ones on gfx950 (the streaming hint is
+// kept as a build option).
+#ifdef CIEHIP_STREAMING_MATRIX_LOADS
+#define CIEHIP_STREAM_LOAD(pointer) __builtin_nontemporal_load(pointer)
+#else
+#define CIEHIP_STREAM_LOAD(pointer) (*(pointer))
+#endif
+
 // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run
 // of chunks so that its L2 sees a contiguous window of the gathered vector.
 __device__ __forceinline__ int xcdChunk() {
     const int grid = gridDim.x;
+#ifdef CIEHIP_NO_XCD_REMAP
+    return blockIdx.x;
+#endif
     if (grid % kXcds) return blockIdx.x;
     return (blockIdx.x % kXcds) * (grid / kXcds) + blockIdx.x / kXcds;
 }
@@ -104,8 +98,8 @@ __device__ __forceinline__ TV rowProduct(int row, int lane, const int32_t* __res
         for (int u = 0; u < U; ++u) {
             const int k = base + u * L + lane;
             if (k < end) {
-                column[u] = __builtin_nontemporal_load(columnIndices + k);
-                entry[u] = __builtin_nontemporal_load(entries + k);
+                column[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
+                entry[u] = CIEHIP_STREAM_LOAD(entries + k);
             } else {
                 // Inside the loop the row has at least one entry, so the
                 // matrix has a column 0 to gather harmlessly from.
@@ -123,31 +117,162 @@ __device__ __forceinline__ TV rowProduct(int row, int lane, const int32_t* __res
     return groupSum<L>(sum);
 }
 
-struct RowChunks {
-    int rowBegin, rowEnd;
-};
+// ---------------------------------------------------------------------------
+// Row walkers. run() visits every row of the block's share of the matrix and
+// calls epilogue(row, sum) once per row, from one lane, where sum = A[row,:] in.
+// Kernels keep per-thread partials (dot products) inside their epilogues.
+// ---------------------------------------------------------------------------
 
-__device__ __forceinline__ RowChunks rowChunk(int rows, int rowsPerChunk, int chunk) {
-    const int begin = min(rows, chunk * rowsPerChunk);
-    return {begin, min(rows, begin + rowsPerChunk)};
-}
-
-// out = inScale * out + outScale * A * in
+// L lanes per row, BS / L rows per block step; rows split into one contiguous
+// chunk per block. Sum order: per-lane strided FMAs, then an xor butterfly.
 template <class TV, class TM, int L, int U, int BS>
-__global__ __launch_bounds__(BS) void csrProductKernel(int rows, int rowsPerChunk, const int32_t* __restrict__ rowExtents,
-                                                       const int32_t* __restrict__ columnIndices,
-                                                       const TM* __restrict__ entries, const TV* __restrict__ in,
-                                                       TV* __restrict__ out, TV inScale, TV outScale) {
-    const RowChunks range = rowChunk(rows, rowsPerChunk, xcdChunk());
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
-    for (int row0 = range.rowBegin; row0 < range.rowEnd; row0 += BS / L) {
-        const int row = row0 + group;
-        if (row < range.rowEnd) {
-            const TV sum = rowProduct<TV, TM, L, U>(row, lane, rowExtents, columnIndices, entries, in);
-            if (lane == 0) out[row] = inScale == TV(0) ? outScale * sum : inScale * out[row] + outScale * sum;
+struct VectorRows {
+    using Value = TV;
+    using Matrix = TM;
+    static constexpr int kBlock = BS;
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void run(int rows, int rowsPerChunk, const int32_t* __restrict__ rowExtents,
+                                               const int32_t* __restrict__ columnIndices,
+                                               const TM* __restrict__ entries, const TV* __restrict__ in,
+                                               Epilogue&& epilogue) {
+        const int chunk = xcdChunk();
+        const int rowBegin = min(rows, chunk * rowsPerChunk);
+        const int rowEnd = min(rows, rowBegin + rowsPerChunk);
+        const int group = threadIdx.x / L;
+        const int lane = threadIdx.x % L;
+        for (int row0 = rowBegin; row0 < rowEnd; row0 += BS / L) {
+            const int row = row0 + group;
+            if (row < rowEnd) {
+                const TV sum = rowProduct<TV, TM, L, U>(row, lane, rowExtents, columnIndices, entries, in);
+                if (lane == 0) epilogue(row, sum);
+            }
         }
     }
+};
+
+// Orders a wavefront's LDS writes before its later LDS reads of other lanes' data.
+__device__ __forceinline__ void stageFence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One wavefront per tile of 64 consecutive rows. The tile's entries are
+// streamed in coalesced passes (lane l loads entry base + l, U loads in
+// flight per lane), each product a_ij * x_j is rounded and staged in LDS
+// (CAP entries per segment), and lane r then adds the products of row r in
+// CSR entry order. That is the summation of CSROperator.cpp:50-55
+// (contribution += entry * in[column], no fused multiply-add), so every row
+// comes out bit-identical to a sequential CPU loop over the same CSR.
+// Sum of row first + lane (lane < count <= 64) of one tile, by one wavefront;
+// `stage` holds CAP values private to the wavefront. Every lane must call it.
+// The tile's entries are consumed in passes of 64 * U (CAP is a multiple of a
+// pass, so passes never straddle a segment). Software pipelined: the column
+// and entry loads of pass i + 1 are issued before the gathers of pass i are
+// consumed and before the row sums of a finished segment run, so the matrix
+// stream stays in flight while the wavefront does its LDS work.
+template <class TV, class TM, int CAP, int U>
+__device__ __forceinline__ TV stagedTileSum(int first, int count, int lane, const int32_t* __restrict__ rowExtents,
+                                            const int32_t* __restrict__ columnIndices,
+                                            const TM* __restrict__ entries, const TV* __restrict__ in, TV* stage) {
+    constexpr int kPass = kWave * U;
+    static_assert(CAP % kPass == 0, "a segment holds whole passes");
+    const bool active = lane < count;
+    const int rowBegin = active ? rowExtents[first + lane] : 0;
+    const int rowEnd = active ? rowExtents[first + lane + 1] : 0;
+    const int begin = __shfl(rowBegin, 0);
+    const int end = __shfl(rowEnd, count - 1);
+    int column[U], nextColumn[U];
+    TM entry[U], nextEntry[U];
+    auto load = [&](int base, int* c, TM* e) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * kWave + lane;
+            if (k < end) {
+                c[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
+                e[u] = CIEHIP_STREAM_LOAD(entries + k);
+            } else {
+                c[u] = 0;  // the tile is non-empty, so column 0 exists
+                e[u] = TM(0);
+            }
+        }
+    };
+    TV sum = TV(0);
+    if (begin < end) load(begin, column, entry);
+    int base = begin;
+    for (int segment = begin; segment < end; segment += CAP) {
+        const int segmentEnd = min(segment + CAP, end);
+        for (; base < segmentEnd; base += kPass) {
+            TV gathered[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) gathered[u] = in[column[u]];
+            const bool more = base + kPass < end;
+            if (more) load(base + kPass, nextColumn, nextEntry);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = base + u * kWave + lane;
+                if (k < segmentEnd) stage[k - segment] = TV(entry[u]) * gathered[u];
+            }
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    column[u] = nextColumn[u];
+                    entry[u] = nextEntry[u];
+                }
+            }
+        }
+        stageFence();
+        const int lo = max(rowBegin, segment);
+        const int hi = min(rowEnd, segmentEnd);
+        for (int k = lo; k < hi; ++k) sum += stage[k - segment];
+        stageFence();
+    }
+    return sum;
+}
+
+template <class TV, class TM, int CAP, int U, int BS>
+struct StagedRows {
+    using Value = TV;
+    using Matrix = TM;
+    static constexpr int kBlock = BS;
+    static constexpr int kTile = kWave;
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void run(int rows, int tilesPerBlock, const int32_t* __restrict__ rowExtents,
+                                               const int32_t* __restrict__ columnIndices,
+                                               const TM* __restrict__ entries, const TV* __restrict__ in,
+                                               Epilogue&& epilogue) {
+        __shared__ TV stage[BS / kWave][CAP];
+        const int tiles = (rows + kTile - 1) / kTile;
+        const int chunk = xcdChunk();
+        const int tileBegin = min(tiles, chunk * tilesPerBlock);
+        const int tileEnd = min(tiles, tileBegin + tilesPerBlock);
+        const int wave = threadIdx.x / kWave;
+        const int lane = threadIdx.x % kWave;
+        for (int tile = tileBegin + wave; tile < tileEnd; tile += BS / kWave) {
+            const int first = tile * kTile;
+            const int count = min(kTile, rows - first);
+            const TV sum = stagedTileSum<TV, TM, CAP, U>(first, count, lane, rowExtents, columnIndices, entries, in,
+                                                          stage[wave]);
+            if (lane < count) epilogue(first + lane, sum);
+        }
+    }
+};
+
+// out = inScale * out + outScale * A * in. With contraction off this equals
+// each special case of CSROperator.cpp:78-110 bit for bit (1 * y, -1 * y and
+// 0 + x are exact), except inScale = outScale = -1, where the reference
+// computes A*in - out and this follows the documented contract, -A*in - out.
+template <class Walker>
+__global__ __launch_bounds__(Walker::kBlock) void csrProductKernel(
+    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
+    const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ in,
+    typename Walker::Value* __restrict__ out, typename Walker::Value inScale, typename Walker::Value outScale) {
+    using TV = typename Walker::Value;
+    Walker::run(rows, param, rowExtents, columnIndices, entries, in, [&](int row, TV sum) {
+        out[row] = inScale == TV(0) ? outScale * sum : inScale * out[row] + outScale * sum;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -191,23 +316,25 @@ __global__ __launch_bounds__(kReduceBlock) void diagonalProductKernel(int64_t n,
     }
 }
 
-// Two-stage deterministic inner product; the result lands in *result (device).
+// Two-stage deterministic inner product: per-block partials in a fixed
+// order, then innerProductFinishKernel (one block) sums them in block order.
 template <class T>
 __global__ __launch_bounds__(kReduceBlock) void innerProductKernel(int64_t n, const T* __restrict__ left,
-                                                                   const T* __restrict__ right, T* partials,
-                                                                   unsigned* counter, T* result) {
+                                                                   const T* __restrict__ right, T* partials) {
     __shared__ T lds[kReduceBlock / kWave];
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     T sum = T(0);
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) sum += left[i] * right[i];
     sum = blockSum<kReduceBlock>(sum, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = sum;
-    if (!arriveLast(counter)) return;
-    const T total = sumPartials<kReduceBlock>(partials, gridDim.x, 1, lds);
-    if (threadIdx.x == 0) {
-        *result = total;
-        resetCounter(counter);
-    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kReduceBlock) void innerProductFinishKernel(int count, const T* __restrict__ partials,
+                                                                         T* result) {
+    __shared__ T lds[kReduceBlock / kWave];
+    const T total = sumPartials<kReduceBlock>(partials, count, 1, lds);
+    if (threadIdx.x == 0) *result = total;
 }
 
 // makeDiagonalOperator: inverse = 1 / a_ii; the first failing row (zero or

@@ -7,9 +7,10 @@
 //   loop: q = A p; alpha = rho / p.q; x += alpha p; r -= alpha q; rr = r.r;
 //         z = D^-1 r; rho' = r.z; beta = rho'/rho; p = beta p + z;
 //         abs = sqrt(rr), rel = abs / ||r0||; stop on Comparison::less.
-// Each loop trip is three kernels that keep all scalars on the device:
-//   product:   q = A p and p.q (last block: alpha, p.q <= 0 check)
-//   update:    x, r updates, z, r.r and r.z (last block: beta, convergence)
+// Each loop trip is three streaming kernels, each followed (where it reduces)
+// by a one-block finish kernel that keeps the scalars on the device:
+//   product:   q = A p and p.q            -> finish: alpha, p.q <= 0 check
+//   update:    x, r updates, z, r.r, r.z  -> finish: beta, convergence test
 //   direction: p = beta p + D^-1 r
 // The host only polls the device state once per chunk of iterations.
 //
@@ -29,139 +30,70 @@ using namespace ciehip;
 
 namespace {
 
-enum Counter { kStartCounter = 8, kProductCounter = 9, kUpdateCounter = 10, kRefineCounter = 11 };
-
 // ---------------------------------------------------------------------------
-// Kernels
+// Kernels. The row-walking and elementwise kernels only write one partial
+// per block (two values: slots 2b and 2b + 1); a single-block finish kernel
+// then sums the partials in block order and updates the device state. Kernel
+// boundaries order the two, so no cross-block fence is needed.
 // ---------------------------------------------------------------------------
 
-// r = b - A x (FP64), z = D^-1 r, start: p = z. Partials: r.r (FP64), r.z (TR).
-template <class TR, int L, int U, int BS, bool kStart>
-__global__ __launch_bounds__(BS) void residualKernel(int rows, int rowsPerChunk, const int32_t* __restrict__ rowExtents,
-                                                     const int32_t* __restrict__ columnIndices,
-                                                     const double* __restrict__ entries, const double* __restrict__ x,
-                                                     const double* __restrict__ b, TR* __restrict__ r,
-                                                     const TR* __restrict__ inverse, TR* __restrict__ p,
-                                                     double* partials, unsigned* counter, SolverState* state) {
+// r = b - A x (FP64 matrix and x), z = D^-1 r, start: p = z.
+// Partials: r.r in FP64, r.z in the recurrence precision TR.
+template <class Walker, class TR, bool kStart>
+__global__ __launch_bounds__(Walker::kBlock) void residualKernel(
+    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
+    const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
+    TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p, double* __restrict__ partials,
+    const SolverState* __restrict__ state) {
+    constexpr int BS = Walker::kBlock;
     __shared__ double ldsRR[BS / kWave];
     __shared__ TR ldsRZ[BS / kWave];
     if (state->done || (!kStart && !state->refine)) return;
-    const int chunk = xcdChunk();
-    const RowChunks range = rowChunk(rows, rowsPerChunk, chunk);
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
     double rr = 0.0;
     TR rz = TR(0);
-    for (int row0 = range.rowBegin; row0 < range.rowEnd; row0 += BS / L) {
-        const int row = row0 + group;
-        if (row < range.rowEnd) {
-            const double sum = rowProduct<double, double, L, U>(row, lane, rowExtents, columnIndices, entries, x);
-            if (lane == 0) {
-                const double t = b[row] - sum;
-                const TR residual = TR(t);
-                const TR z = residual * inverse[row];
-                r[row] = residual;
-                if (kStart) p[row] = z;
-                rr += t * t;
-                rz += residual * z;
-            }
-        }
-    }
+    Walker::run(rows, param, rowExtents, columnIndices, entries, x, [&](int row, double sum) {
+        const double t = b[row] - sum;
+        const TR residual = TR(t);
+        const TR z = residual * inverse[row];
+        r[row] = residual;
+        if (kStart) p[row] = z;
+        rr += t * t;
+        rz += residual * z;
+    });
+    const int chunk = xcdChunk();
     rr = blockSum<BS>(rr, ldsRR);
     rz = blockSum<BS>(rz, ldsRZ);
     if (threadIdx.x == 0) {
         partials[2 * chunk] = rr;
         partials[2 * chunk + 1] = double(rz);
     }
-    if (!arriveLast(counter)) return;
-    const double RR = sumPartials<BS>(partials, gridDim.x, 2, ldsRR);
-    TR rzSum = TR(0);
-    for (int i = threadIdx.x; i < int(gridDim.x); i += BS) rzSum += TR(partials[2 * i + 1]);
-    const TR RZ = blockSum<BS>(rzSum, ldsRZ);
-    if (threadIdx.x != 0) return;
-    resetCounter(counter);
-    SolverState& s = *state;
-    s.rr = RR;
-    if (kStart) {
-        // ConjugateGradients_impl.hpp:93-132
-        const double initial = sqrt(RR);
-        s.initialNorm = initial;
-        s.absolute = initial;
-        s.relative = 1.0;
-        s.updateNorm = RR;
-        if (floatLess(initial, s.absoluteTolerance) || floatLess(1.0, s.relativeTolerance)) {
-            s.done = kConverged;
-        } else if (!(TR(0) < RZ)) {
-            s.done = kBreakdown;
-        } else {
-            s.rho = double(RZ);
-        }
-    } else {
-        // FP64 refinement: true residual, FP64 convergence test, restart the FP32 recurrence.
-        s.refine = 0;
-        s.refinements += 1;
-        s.absolute = sqrt(RR);
-        s.relative = s.absolute / s.initialNorm;
-        if (floatLess(s.absolute, s.absoluteTolerance) || floatLess(s.relative, s.relativeTolerance)) {
-            s.done = kConverged;
-        } else {
-            const TR previous = TR(s.rho);
-            s.rho = double(RZ);
-            s.beta = s.restart ? 0.0 : double(RZ / previous);
-            s.updateNorm = RR;
-            s.iteration += 1;
-            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
-        }
-    }
 }
 
-// q = A p, partial p.q; last block: alpha = rho / p.q (ConjugateGradients_impl.hpp:136-151).
-template <class T, int L, int U, int BS>
-__global__ __launch_bounds__(BS) void productKernel(int rows, int rowsPerChunk, const int32_t* __restrict__ rowExtents,
-                                                    const int32_t* __restrict__ columnIndices,
-                                                    const T* __restrict__ entries, const T* __restrict__ p,
-                                                    T* __restrict__ q, double* partials, unsigned* counter,
-                                                    SolverState* state) {
+// q = A p, partial p.q (ConjugateGradients_impl.hpp:136-143).
+template <class Walker>
+__global__ __launch_bounds__(Walker::kBlock) void productKernel(
+    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
+    const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ p,
+    typename Walker::Value* __restrict__ q, double* __restrict__ partials, const SolverState* __restrict__ state) {
+    using T = typename Walker::Value;
+    constexpr int BS = Walker::kBlock;
     __shared__ T lds[BS / kWave];
     if (state->done) return;
-    const int chunk = xcdChunk();
-    const RowChunks range = rowChunk(rows, rowsPerChunk, chunk);
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
     T pq = T(0);
-    for (int row0 = range.rowBegin; row0 < range.rowEnd; row0 += BS / L) {
-        const int row = row0 + group;
-        if (row < range.rowEnd) {
-            const T sum = rowProduct<T, T, L, U>(row, lane, rowExtents, columnIndices, entries, p);
-            if (lane == 0) {
-                q[row] = sum;
-                pq += p[row] * sum;
-            }
-        }
-    }
+    Walker::run(rows, param, rowExtents, columnIndices, entries, p, [&](int row, T sum) {
+        q[row] = sum;
+        pq += p[row] * sum;
+    });
     pq = blockSum<BS>(pq, lds);
-    if (threadIdx.x == 0) partials[chunk] = double(pq);
-    if (!arriveLast(counter)) return;
-    T total = T(0);
-    for (int i = threadIdx.x; i < int(gridDim.x); i += BS) total += T(partials[i]);
-    total = blockSum<BS>(total, lds);
-    if (threadIdx.x != 0) return;
-    resetCounter(counter);
-    if (!(T(0) < total)) {
-        state->done = kNotSpd;
-    } else {
-        state->alpha = double(T(state->rho) / total);
-    }
+    if (threadIdx.x == 0) partials[2 * xcdChunk()] = double(pq);
 }
 
-// x += alpha p; r -= alpha q; z = D^-1 r; partials r.r and r.z.
-// Plain: last block computes beta and the convergence test.
-// Mixed: last block either continues the FP32 recurrence or requests a refinement.
-template <class T, class TX, bool kMixed>
+// x += alpha p; r -= alpha q; z = D^-1 r; partials r.r and r.z (:154-163).
+template <class T, class TX>
 __global__ __launch_bounds__(kReduceBlock) void updateKernel(int64_t n, TX* __restrict__ x, T* __restrict__ r,
                                                              const T* __restrict__ p, const T* __restrict__ q,
-                                                             const T* __restrict__ inverse, double* partials,
-                                                             unsigned* counter, SolverState* state) {
+                                                             const T* __restrict__ inverse, double* __restrict__ partials,
+                                                             const SolverState* __restrict__ state) {
     __shared__ T lds[kReduceBlock / kWave];
     __shared__ T lds2[kReduceBlock / kWave];
     if (state->done) return;
@@ -183,51 +115,9 @@ __global__ __launch_bounds__(kReduceBlock) void updateKernel(int64_t n, TX* __re
         partials[2 * blockIdx.x] = double(rr);
         partials[2 * blockIdx.x + 1] = double(rz);
     }
-    if (!arriveLast(counter)) return;
-    T rrSum = T(0), rzSum = T(0);
-    for (int i = threadIdx.x; i < int(gridDim.x); i += kReduceBlock) {
-        rrSum += T(partials[2 * i]);
-        rzSum += T(partials[2 * i + 1]);
-    }
-    const T RR = blockSum<kReduceBlock>(rrSum, lds);
-    const T RZ = blockSum<kReduceBlock>(rzSum, lds2);
-    if (threadIdx.x != 0) return;
-    resetCounter(counter);
-    SolverState& s = *state;
-    const double absolute = sqrt(double(RR));
-    const double relative = absolute / s.initialNorm;
-    if (kMixed) {
-        const bool refine = double(RR) <= s.updateThreshold * s.updateNorm || floatLess(absolute, s.absoluteTolerance)
-                         || floatLess(relative, s.relativeTolerance) || s.iteration + 1 >= s.maxIterations;
-        if (refine) {
-            s.refine = 1;
-            return;
-        }
-        const T previous = T(s.rho);
-        s.rho = double(RZ);
-        s.beta = double(RZ / previous);
-        s.rr = double(RR);
-        s.absolute = absolute;
-        s.relative = relative;
-        s.iteration += 1;
-        return;
-    }
-    // ConjugateGradients_impl.hpp:158-185
-    const double previous = s.rho;
-    s.rho = double(RZ);
-    s.beta = double(RZ) / previous;
-    s.rr = double(RR);
-    s.absolute = absolute;
-    s.relative = relative;
-    if (floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance)) {
-        s.done = kConverged;
-    } else {
-        s.iteration += 1;
-        if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
-    }
 }
 
-// p = beta p + D^-1 r (ConjugateGradients_impl.hpp:168-170).
+// p = beta p + D^-1 r (ConjugateGradients_impl.hpp:168-170: scale, then add).
 template <class T>
 __global__ __launch_bounds__(kReduceBlock) void directionKernel(int64_t n, T* __restrict__ p, const T* __restrict__ r,
                                                                 const T* __restrict__ inverse,
@@ -239,47 +129,161 @@ __global__ __launch_bounds__(kReduceBlock) void directionKernel(int64_t n, T* __
         p[i] = p[i] * beta + r[i] * inverse[i];
 }
 
+enum Phase : int { kPhaseStart = 0, kPhaseAlpha = 1, kPhaseUpdate = 2, kPhaseUpdateMixed = 3, kPhaseRefine = 4 };
+
+// Single block: sums `count` block partials in block order and advances the
+// solver state. TR is the precision of the recurrence's dot products; the
+// residual phases carry r.r in FP64.
+template <class TR, int kPhase>
+__global__ __launch_bounds__(kReduceBlock) void finishKernel(int count, const double* __restrict__ partials,
+                                                             SolverState* __restrict__ state) {
+    __shared__ double lds64[kReduceBlock / kWave];
+    __shared__ TR ldsR[kReduceBlock / kWave];
+    SolverState& s = *state;
+    if (s.done) return;
+    if (kPhase == kPhaseRefine && !s.refine) return;
+    constexpr bool kFirst64 = kPhase == kPhaseStart || kPhase == kPhaseRefine;
+    double first64 = 0.0;
+    TR firstR = TR(0), second = TR(0);
+    for (int i = threadIdx.x; i < count; i += kReduceBlock) {
+        if (kFirst64) first64 += partials[2 * i];
+        else firstR += TR(partials[2 * i]);
+        if (kPhase != kPhaseAlpha) second += TR(partials[2 * i + 1]);
+    }
+    double A64 = 0.0;
+    TR AR = TR(0), B = TR(0);
+    if (kFirst64) A64 = blockSum<kReduceBlock>(first64, lds64);
+    else AR = blockSum<kReduceBlock>(firstR, ldsR);
+    if (kPhase != kPhaseAlpha) B = blockSum<kReduceBlock>(second, ldsR);
+    if (threadIdx.x != 0) return;
+    if (kPhase == kPhaseStart) {
+        // ConjugateGradients_impl.hpp:93-132
+        const double initial = sqrt(A64);
+        s.rr = A64;
+        s.initialNorm = initial;
+        s.absolute = initial;
+        s.relative = 1.0;
+        s.updateNorm = A64;
+        if (floatLess(initial, s.absoluteTolerance) || floatLess(1.0, s.relativeTolerance)) {
+            s.done = kConverged;
+        } else if (!(TR(0) < B)) {
+            s.done = kBreakdown;
+        } else {
+            s.rho = double(B);
+        }
+    } else if (kPhase == kPhaseAlpha) {
+        // :139-151
+        if (!(TR(0) < AR)) s.done = kNotSpd;
+        else s.alpha = double(TR(s.rho) / AR);
+    } else if (kPhase == kPhaseUpdate) {
+        // :158-185
+        const double absolute = sqrt(double(AR));
+        const double relative = absolute / s.initialNorm;
+        const double previous = s.rho;
+        s.rho = double(B);
+        s.beta = double(B) / previous;
+        s.rr = double(AR);
+        s.absolute = absolute;
+        s.relative = relative;
+        if (floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance)) {
+            s.done = kConverged;
+        } else {
+            s.iteration += 1;
+            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
+        }
+    } else if (kPhase == kPhaseUpdateMixed) {
+        // FP32 recurrence: continue, or ask for an FP64 refinement of the residual.
+        const double absolute = sqrt(double(AR));
+        const double relative = absolute / s.initialNorm;
+        const bool refine = double(AR) <= s.updateThreshold * s.updateNorm ||
+                            floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance) ||
+                            s.iteration + 1 >= s.maxIterations;
+        if (refine) {
+            s.refine = 1;
+        } else {
+            const TR previous = TR(s.rho);
+            s.rho = double(B);
+            s.beta = double(B / previous);
+            s.rr = double(AR);
+            s.absolute = absolute;
+            s.relative = relative;
+            s.iteration += 1;
+        }
+    } else {
+        // FP64 refinement: true residual, FP64 convergence test, restart the FP32 recurrence.
+        s.refine = 0;
+        s.refinements += 1;
+        s.rr = A64;
+        s.absolute = sqrt(A64);
+        s.relative = s.absolute / s.initialNorm;
+        if (floatLess(s.absolute, s.absoluteTolerance) || floatLess(s.relative, s.relativeTolerance)) {
+            s.done = kConverged;
+        } else {
+            const TR previous = TR(s.rho);
+            s.rho = double(B);
+            s.beta = s.restart ? 0.0 : double(B / previous);
+            s.updateNorm = A64;
+            s.iteration += 1;
+            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
 
 size_t alignUp(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
+template <class TR, int kPhase>
+void launchFinish(cie_hip_context ctx, int count) {
+    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kReduceBlock), 0, ctx->stream, count,
+                       ctx->scratch.partials, ctx->scratch.state);
+}
+
+// r = b - A x and its dot products, then the start or refinement finish.
 template <class TR>
 int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, TR* r, const TR* inverse, TR* p) {
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
-    const int perChunk = rowsPerChunk(m->rows, launch);
-    const bool ok = withShape(launch.lanes, launch.unroll, [&](auto lanes, auto unroll) {
-        constexpr int L = decltype(lanes)::value;
-        constexpr int U = decltype(unroll)::value;
-        if (start)
-            hipLaunchKernelGGL((residualKernel<TR, L, U, kSpmvBlock, true>), dim3(launch.grid), dim3(kSpmvBlock), 0,
-                               ctx->stream, int(m->rows), perChunk, m->rowExtents, m->columnIndices, m->entries64, x, b,
-                               r, inverse, p, ctx->scratch.partials, ctx->scratch.counters + kStartCounter,
-                               ctx->scratch.state);
-        else
-            hipLaunchKernelGGL((residualKernel<TR, L, U, kSpmvBlock, false>), dim3(launch.grid), dim3(kSpmvBlock), 0,
-                               ctx->stream, int(m->rows), perChunk, m->rowExtents, m->columnIndices, m->entries64, x, b,
-                               r, inverse, p, ctx->scratch.partials, ctx->scratch.counters + kRefineCounter,
-                               ctx->scratch.state);
+    int grid = 0;
+    const bool ok = withWalker<double, double>(launch, [&](auto walker) {
+        using Walker = decltype(walker);
+        if (start) {
+            const GridShape shape = gridFor(residualKernel<Walker, TR, true>, launch, m->rows, ctx->computeUnits);
+            grid = shape.grid;
+            hipLaunchKernelGGL((residualKernel<Walker, TR, true>), dim3(shape.grid), dim3(Walker::kBlock), 0,
+                               ctx->stream, int(m->rows), shape.param, m->rowExtents, m->columnIndices, m->entries64,
+                               x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
+        } else {
+            const GridShape shape = gridFor(residualKernel<Walker, TR, false>, launch, m->rows, ctx->computeUnits);
+            grid = shape.grid;
+            hipLaunchKernelGGL((residualKernel<Walker, TR, false>), dim3(shape.grid), dim3(Walker::kBlock), 0,
+                               ctx->stream, int(m->rows), shape.param, m->rowExtents, m->columnIndices, m->entries64,
+                               x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
+        }
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    if (start) launchFinish<TR, kPhaseStart>(ctx, grid);
+    else launchFinish<TR, kPhaseRefine>(ctx, grid);
     return CIE_HIP_OK;
 }
 
+// q = A p with its p.q, then the alpha finish.
 template <class T>
 int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q) {
     cie_hip_context ctx = m->context;
-    const int perChunk = rowsPerChunk(m->rows, launch);
-    const bool ok = withShape(launch.lanes, launch.unroll, [&](auto lanes, auto unroll) {
-        constexpr int L = decltype(lanes)::value;
-        constexpr int U = decltype(unroll)::value;
-        hipLaunchKernelGGL((productKernel<T, L, U, kSpmvBlock>), dim3(launch.grid), dim3(kSpmvBlock), 0, ctx->stream,
-                           int(m->rows), perChunk, m->rowExtents, m->columnIndices, entries, p, q,
-                           ctx->scratch.partials, ctx->scratch.counters + kProductCounter, ctx->scratch.state);
+    int grid = 0;
+    const bool ok = withWalker<T, T>(launch, [&](auto walker) {
+        using Walker = decltype(walker);
+        const GridShape shape = gridFor(productKernel<Walker>, launch, m->rows, ctx->computeUnits);
+        grid = shape.grid;
+        hipLaunchKernelGGL((productKernel<Walker>), dim3(shape.grid), dim3(Walker::kBlock), 0, ctx->stream,
+                           int(m->rows), shape.param, m->rowExtents, m->columnIndices, entries, p, q,
+                           ctx->scratch.partials, ctx->scratch.state);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    launchFinish<T, kPhaseAlpha>(ctx, grid);
     return CIE_HIP_OK;
 }
 
@@ -327,7 +331,8 @@ struct Timer {
     hipEvent_t start = nullptr, marks[2] = {nullptr, nullptr};
     ~Timer() {
         if (start) (void)hipEventDestroy(start);
-        for (auto& e : marks) if (e) (void)hipEventDestroy(e);
+        for (auto& e : marks)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -344,8 +349,7 @@ struct ProductProfile {
     void end(cie_hip_context ctx, int mark) {
         if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
     }
-    // Mean duration of the bracketed kernels whose state was still running is
-    // not separable from early exits; the caller passes how many did work.
+    // Mean over the first `samples` bracketed launches (those that did work).
     double mean(cie_hip_context ctx, int samples) const {
         double total = 0.0;
         int count = 0;
@@ -360,7 +364,9 @@ struct ProductProfile {
     }
 };
 
-// Runs the loop until the device state says done; returns the solve time.
+// Enqueues chunks of iterations until the device state says done. The state
+// of chunk k is read back while chunk k + 1 is already queued, so the device
+// never idles on a poll; kernels of a finished solve return immediately.
 template <class Enqueue>
 int runLoop(cie_hip_context ctx, Timer& timer, Enqueue&& enqueueIteration, double* seconds) {
     SolverState* pinned = ctx->scratch.pinnedState;
@@ -474,9 +480,9 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
             int s = launchProductStep<double>(m, m->entries64, m->launch64, p64, q64);
             if (s) return s;
             profile.end(ctx, mark);
-            hipLaunchKernelGGL((updateKernel<double, double, false>), vgrid, vblock, 0, ctx->stream, n, solution, r64,
-                               p64, q64, m->inverseDiagonal64, ctx->scratch.partials,
-                               ctx->scratch.counters + kUpdateCounter, ctx->scratch.state);
+            hipLaunchKernelGGL((updateKernel<double, double>), vgrid, vblock, 0, ctx->stream, n, solution, r64, p64,
+                               q64, m->inverseDiagonal64, ctx->scratch.partials, ctx->scratch.state);
+            launchFinish<double, kPhaseUpdate>(ctx, int(vgrid.x));
             hipLaunchKernelGGL((directionKernel<double>), vgrid, vblock, 0, ctx->stream, n, p64, r64,
                                m->inverseDiagonal64, ctx->scratch.state);
             return CIE_HIP_OK;
@@ -492,9 +498,9 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
             int s = launchProductStep<float>(m, m->entries32, m->launch32, p32, q32);
             if (s) return s;
             profile.end(ctx, mark);
-            hipLaunchKernelGGL((updateKernel<float, double, true>), vgrid, vblock, 0, ctx->stream, n, solution, r32,
-                               p32, q32, m->inverseDiagonal32, ctx->scratch.partials,
-                               ctx->scratch.counters + kUpdateCounter, ctx->scratch.state);
+            hipLaunchKernelGGL((updateKernel<float, double>), vgrid, vblock, 0, ctx->stream, n, solution, r32, p32,
+                               q32, m->inverseDiagonal32, ctx->scratch.partials, ctx->scratch.state);
+            launchFinish<float, kPhaseUpdateMixed>(ctx, int(vgrid.x));
             s = launchResidual<float>(m, false, solution, q.b, r32, m->inverseDiagonal32, p32);
             if (s) return s;
             hipLaunchKernelGGL((directionKernel<float>), vgrid, vblock, 0, ctx->stream, n, p32, r32,

@@ -31,8 +31,11 @@ int innerProduct(cie_hip_context ctx, const T* left, const T* right, int64_t siz
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
     T* partials = reinterpret_cast<T*>(ctx->scratch.partials);
     T* deviceResult = reinterpret_cast<T*>(ctx->scratch.scalars);
-    hipLaunchKernelGGL((innerProductKernel<T>), dim3(vectorGrid(size)), dim3(kReduceBlock), 0, ctx->stream, size, left,
-                       right, partials, ctx->scratch.counters, deviceResult);
+    const int grid = vectorGrid(size);
+    hipLaunchKernelGGL((innerProductKernel<T>), dim3(grid), dim3(kReduceBlock), 0, ctx->stream, size, left, right,
+                       partials);
+    hipLaunchKernelGGL((innerProductFinishKernel<T>), dim3(1), dim3(kReduceBlock), 0, ctx->stream, grid, partials,
+                       deviceResult);
     CIE_HIP_CHECK(hipGetLastError());
     T* pinned = reinterpret_cast<T*>(ctx->scratch.pinned);
     CIE_HIP_CHECK(hipMemcpyAsync(pinned, deviceResult, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));

@@ -103,16 +103,21 @@ class CSRMatrix:
         return self.columns
 
     def launch(self, precision=64):
-        lanes, unroll, block = c_int(), c_int(), c_int()
-        check(library().cie_hip_csr_get_launch(self.handle, precision, byref(lanes), byref(unroll), byref(block)))
-        return lanes.value, unroll.value, block.value
+        """(kernel, width, unroll) of the SpMV for `precision` (64 or 32)."""
+        kernel, width, unroll = c_int(), c_int(), c_int()
+        check(library().cie_hip_csr_get_launch(self.handle, precision, byref(kernel), byref(width), byref(unroll)))
+        return ("staged" if kernel.value == _lib.SPMV_STAGED else "vector"), width.value, unroll.value
 
-    def set_launch(self, precision, lanes, unroll, block=256):
-        check(library().cie_hip_csr_set_launch(self.handle, precision, lanes, unroll, block), "cie_hip_csr_set_launch")
+    def set_launch(self, precision, kernel, width, unroll):
+        code = _lib.SPMV_STAGED if kernel == "staged" else _lib.SPMV_VECTOR
+        check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
-    def autotune(self, precision=64, repetitions=10):
+    def autotune(self, precision=64, repetitions=10, kernels=("staged",)):
+        """Time the candidate launches of `kernels` ("staged", "vector") and keep the fastest."""
+        mask = sum(1 << (_lib.SPMV_STAGED if k == "staged" else _lib.SPMV_VECTOR) for k in kernels)
         best = c_double()
-        check(library().cie_hip_csr_autotune(self.handle, precision, repetitions, byref(best)), "cie_hip_csr_autotune")
+        check(library().cie_hip_csr_autotune(self.handle, precision, mask, repetitions, byref(best)),
+              "cie_hip_csr_autotune")
         return best.value
 
     def update_entries(self, entries):
@@ -158,8 +163,11 @@ class HIPSpace:
         return _torch().empty(int(size), dtype=self.dtype, device=self.device)
 
     def _call(self, name, *args):
+        # Synchronous like SYCLSpace (every op ends in wait_and_throw, SYCLSpace.cpp:166-311):
+        # torch's stream is drained before and the library stream after the call.
         _sync_torch()
-        return check(getattr(library(), f"cie_hip_{name}_{self._suffix}")(self.context.handle, *args), name)
+        check(getattr(library(), f"cie_hip_{name}_{self._suffix}")(self.context.handle, *args), name)
+        self.context.synchronize()
 
     def _check(self, *vectors):
         for v in vectors:
@@ -230,6 +238,7 @@ class HIPCSROperator(LinearOperator):
         fn = getattr(library(), f"cie_hip_csr_product_{suffix}")
         check(fn(self.space.context.handle, self.matrix.handle, inScale, _dptr(in_), in_.numel(), outScale,
                  _dptr(out), out.numel()), "product")
+        self.space.context.synchronize()
 
 
 class DiagonalOperator(LinearOperator):
@@ -245,6 +254,7 @@ class DiagonalOperator(LinearOperator):
         fn = getattr(library(), f"cie_hip_diagonal_product_{self.space._suffix}")
         check(fn(self.space.context.handle, _dptr(self.inverseDiagonal), inScale, _dptr(in_), outScale, _dptr(out),
                  out.numel()), "DiagonalOperator.product")
+        self.space.context.synchronize()
 
 
 def makeDiagonalOperator(matrix, space):
@@ -416,3 +426,54 @@ class MixedPCG(IterativeSolver):
         self.info = info
         check(status, "cie_hip_pcg_mixed")
         self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+
+
+class Batch:
+    """Many independent systems on the diagonal blocks of one CSRMatrix (cie_hip_batch)."""
+
+    def __init__(self, matrix, offsets):
+        self.matrix = matrix
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.systems = len(self.offsets) - 1
+        self.handle = c_void_p()
+        check(library().cie_hip_batch_create(matrix.context.handle, matrix.handle, self.systems,
+                                             self.offsets.ctypes.data_as(c_void_p), byref(self.handle)),
+              "cie_hip_batch_create")
+
+    def autotune(self, precision=32, repetitions=3):
+        check(library().cie_hip_batch_autotune(self.handle, precision, repetitions), "cie_hip_batch_autotune")
+
+    def tile_rows(self):
+        out = np.empty(self.systems, dtype=np.int32)
+        check(library().cie_hip_batch_get_tile_rows(self.handle, out.ctypes.data_as(c_void_p)))
+        return out
+
+    def set_tile_rows(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        check(library().cie_hip_batch_set_tile_rows(self.handle, rows.ctypes.data_as(c_void_p)))
+
+    def solve(self, b, x, settings, mixed=True, graph=True, refinement_factor=0.1, restart=False):
+        """Solve every system in place on x; returns (stats, status, refinements, summary)."""
+        stats = (Statistics * self.systems)()
+        status = np.zeros(self.systems, dtype=np.int32)
+        refinements = np.zeros(self.systems, dtype=np.uint64)
+        summary = SolveInfo()
+        mixed_settings = MixedSettings(settings, refinement_factor, 1 if restart else 0, 0)
+        _sync_torch()
+        code = library().cie_hip_pcg_batched(self.matrix.context.handle, self.handle, _dptr(b), _dptr(x),
+                                             byref(mixed_settings), 1 if mixed else 0, 1 if graph else 0, stats,
+                                             status.ctypes.data_as(c_void_p), refinements.ctypes.data_as(c_void_p),
+                                             byref(summary))
+        check(code, "cie_hip_pcg_batched")
+        return list(stats), status, refinements, summary
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_batch_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -175,3 +175,46 @@ def q2_adjoint_3d(elements=48, seed=4, target_seed=5, rhs_seed=1, variance=0.5, 
     return Workload(f"q2_adjoint_3d_{elements}", matrix, uniform(rhs_seed, matrix.rows), tolerance,
                     f"3D Q2 Laplace, {elements}^3 hexahedra, k_e=exp(N(0,{variance})) seed {seed}",
                     {"log_conductivity": g, "target_log_conductivity": g_target})
+
+
+def q2_quad_batch(count=256, seed=3, smallest=16, largest=256, sigma=0.5, tolerance=1e-10, threads=0):
+    """`count` independent 2D Q2 Laplace systems on n_i x n_i element grids.
+
+    n_i = smallest + floor(u_i (largest - smallest + 1)) for the first `count`
+    SplitMix64 uniforms u_i in [0, 1) of `seed`; system i has k_e = exp(g_e)
+    with g_e ~ N(0, sigma^2) from seed 1000 * seed + i, and b ~ U(-1, 1) from
+    seed 1000 * seed + 500 + i. Returned as one block-diagonal CSR (system i
+    owns rows offsets[i]..offsets[i+1]) with the concatenated right-hand side.
+    """
+    raw = splitmix64(seed, count)
+    unit = (raw >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    sizes = (smallest + np.floor(unit * (largest - smallest + 1))).astype(np.int64)
+    blocks, rhs = [], []
+    for i, n in enumerate(sizes):
+        _, _, elements, _ = lagrange_size(2, 2, int(n))
+        g = normal(1000 * seed + i, sigma, elements)
+        blocks.append(lagrange(2, 2, int(n), conductivity=np.exp(g), threads=threads))
+        rhs.append(uniform(1000 * seed + 500 + i, blocks[-1].rows))
+    matrix, offsets = block_diagonal(blocks)
+    return Workload(f"q2_quad_batch_{count}", matrix, np.concatenate(rhs), tolerance,
+                    f"{count} 2D Q2 Laplace systems, n_i~U{{{smallest}..{largest}}} seed {seed}, "
+                    f"k_e lognormal(0,{sigma})", {"offsets": offsets, "sizes": sizes})
+
+
+def block_diagonal(blocks):
+    """One CSR holding `blocks` on its diagonal; returns (csr, row offsets)."""
+    offsets = np.zeros(len(blocks) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([b.rows for b in blocks])
+    nnz = np.zeros(len(blocks) + 1, dtype=np.int64)
+    nnz[1:] = np.cumsum([b.nnz for b in blocks])
+    if nnz[-1] > np.iinfo(np.int32).max:
+        raise ValueError("the batch exceeds int32 nonzero indexing")
+    extents = np.empty(offsets[-1] + 1, dtype=np.int32)
+    cols = np.empty(nnz[-1], dtype=np.int32)
+    vals = np.empty(nnz[-1], dtype=np.float64)
+    for i, b in enumerate(blocks):
+        extents[offsets[i]:offsets[i + 1]] = b.row_extents[:-1] + nnz[i]
+        cols[nnz[i]:nnz[i + 1]] = b.column_indices + offsets[i]
+        vals[nnz[i]:nnz[i + 1]] = b.entries
+    extents[-1] = nnz[-1]
+    return CSR(int(offsets[-1]), int(offsets[-1]), extents, cols, vals), offsets

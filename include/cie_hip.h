@@ -118,13 +118,24 @@ int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64
  * the sparsity pattern is unchanged. */
 int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 
-/* SpMV launch shape. precision: 64 or 32. lanes: lanes per row (1..64, power
- * of two), unroll: entries per lane in flight (1, 2, 4, 8), block: threads per
- * block (64..1024). cie_hip_csr_autotune times candidates on the device and
- * keeps the fastest (SYCLCSROperator.cpp:26-27 fixes lanes = sub-group size). */
-int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int lanes, int unroll, int block);
-int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* lanes, int* unroll, int* block);
-int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int repetitions, double* bestSeconds);
+/* SpMV launch shape, per value precision (64 or 32). Two kernels:
+ *   CIE_HIP_SPMV_STAGED: one wavefront per 64-row tile; coalesced entry
+ *     streams, products staged in LDS, each row summed in CSR entry order by
+ *     one lane -- rows bit-identical to the sequential CPU loop of
+ *     CSROperator.cpp:50-55. width = LDS entries per segment (512, 1024),
+ *     unroll = entries per lane in flight (4, 8).
+ *   CIE_HIP_SPMV_VECTOR: `width` lanes per row (1..64, power of two) with
+ *     strided FMAs and a butterfly sum; unroll 1, 2, 4 or 8. The
+ *     SYCLCSROperator layout (SYCLCSROperator.cpp:26-27,71-80) is width = 64.
+ * Both are bitwise reproducible run to run for a fixed launch. The default is
+ * STAGED/1024/8 (VECTOR if a row exceeds 8192 entries). cie_hip_csr_autotune
+ * times the candidates of the kernels in `kernels` (a mask of 1 << kernel)
+ * on the device and keeps the fastest. */
+#define CIE_HIP_SPMV_VECTOR 0
+#define CIE_HIP_SPMV_STAGED 1
+int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll);
+int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll);
+int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds);
 
 /* LinearOperator<TSpace>::product (linalg/packages/solvers/inc/LinearOperator.hpp:15-33),
  * CSROperator.cpp:33-111 / SYCLCSROperator.cpp:32-89:
@@ -187,6 +198,57 @@ int cie_hip_pcg_f64(cie_hip_context context, cie_hip_csr matrix, double inScale,
 int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* b,
                       double outScale, double* x, int64_t size, const cie_hip_mixed_settings* settings,
                       cie_hip_solve_info* info);
+
+/* ---- Batched solves (no reference counterpart) ---------------------------- */
+/* Many independent SPD systems of uneven size, stored as the diagonal blocks
+ * of one CSR matrix (rows of system s: [offsets[s], offsets[s+1]), columns in
+ * the same range; checked at creation). Both value planes are required. Each
+ * system runs the reference's Jacobi-PCG (ConjugateGradients_impl.hpp:49-201),
+ * all systems advance together, one kernel launch per step for the whole
+ * batch, with per-system scalars and convergence masks on the device; one
+ * iteration's kernels are captured in a HIP graph (useGraph != 0) and
+ * replayed until every system is done. Rows are processed in tiles of 16, 32
+ * or 64 rows per wavefront (never straddling two systems), chosen per system
+ * by cie_hip_batch_autotune from timed runs (default 64). */
+typedef struct cie_hip_batch_s* cie_hip_batch;
+int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t systems, const int64_t* offsets,
+                         cie_hip_batch* batch);
+int cie_hip_batch_destroy(cie_hip_batch batch);
+int cie_hip_batch_set_tile_rows(cie_hip_batch batch, const int32_t* rowsPerTile);
+int cie_hip_batch_get_tile_rows(cie_hip_batch batch, int32_t* rowsPerTile);
+int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions);
+/* Solves every system in place: x (device, all rows) is the initial guess and
+ * the result. mixed != 0 runs the mixed-precision refined form (settings as
+ * for cie_hip_pcg_mixed), else FP64 (settings.settings only). Host outputs
+ * (each optional): stats[s] (Statistics as for the single solve; for mixed,
+ * the FP64 true residual), status[s] (1 converged, 2 iteration cap, 3 p^T A p
+ * <= 0, 4 r^T z <= 0), refinements[s]; summary holds the batch time and the
+ * largest iteration count. Returns the first breakdown status, if any. */
+int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const double* b, double* x,
+                        const cie_hip_mixed_settings* settings, int mixed, int useGraph, cie_hip_statistics* stats,
+                        int32_t* status, uint64_t* refinements, cie_hip_solve_info* summary);
+
+/* ---- Adjoint gradient (no reference counterpart) -------------------------- */
+/* For a stiffness assembled from per-element conductivities,
+ *     K(k) = sum_e k_e P_e^T K_e0 P_e
+ * (the element integrand is linear in the cell modulus,
+ * fem/packages/integrands/impl/LinearIsotropicStiffnessIntegrand_impl.hpp:74,
+ * scattered by fem/packages/graph/impl/Assembler_impl.hpp:156-194), and an
+ * objective J(u) with K u = f, the adjoint lambda solves K lambda = dJ/du
+ * (K is symmetric: cie_hip_pcg_f64 solves it) and
+ *     gradient[e] = scale[e] * (-lambda_e^T K_e0 u_e)
+ * with scale NULL meaning 1 (dJ/dk), or scale = k for dJ/dg with k = exp(g).
+ * dofs holds elements x nodesPerElement row indices, -1 for eliminated
+ * (Dirichlet) nodes; K_e0 is nodesPerElement^2, row-major. Device arrays. */
+int cie_hip_element_gradient(cie_hip_context context, int64_t elements, int32_t nodesPerElement, const int32_t* dofs,
+                             const double* elementMatrix, const double* scale, const double* lambda, const double* u,
+                             double* gradient);
+/* entries[slots[i][a][b]] += factors[i] * K_e0[a][b] for `count` elements, in
+ * list order (slot -1 skipped): the change of K when k_e changes by factors[i].
+ * Refreshes the FP32 plane and drops the cached Jacobi diagonal. The matrix
+ * must own its FP64 entries. Device arrays. */
+int cie_hip_csr_add_element_matrices(cie_hip_csr matrix, int64_t count, int32_t nodesPerElement, const int64_t* slots,
+                                     const double* elementMatrix, const double* factors);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,94 @@
+"""Batched PCG: many uneven systems in one launch per step, against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from cie_amd import systems
+from cie_amd._lib import Statistics
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(count=12, seed=7):
+    return systems.q2_quad_batch(count=count, seed=seed, smallest=3, largest=24)
+
+
+def per_system_residuals(w, x):
+    r = oracle.csr_product(w.matrix, 1.0, x, -1.0, w.rhs.copy(), threads=8)
+    offsets = w.extra["offsets"]
+    return np.array([np.linalg.norm(r[a:b]) / np.linalg.norm(w.rhs[a:b]) for a, b in zip(offsets[:-1], offsets[1:])])
+
+
+def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu):
+    w = small_batch()
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x = torch_gpu.zeros_like(b)
+    stats, status, _, summary = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=False)
+    assert np.all(status == 1) and summary.converged == 1
+    got = x.cpu().numpy()
+    offsets = w.extra["offsets"]
+    for i, (a, c) in enumerate(zip(offsets[:-1], offsets[1:])):
+        sub = systems.CSR(int(c - a), int(c - a), w.matrix.row_extents[a:c + 1] - w.matrix.row_extents[a],
+                          w.matrix.column_indices[w.matrix.row_extents[a]:w.matrix.row_extents[c]] - a,
+                          w.matrix.entries[w.matrix.row_extents[a]:w.matrix.row_extents[c]])
+        x_ref, s_ref, _, _ = oracle.pcg(sub, w.rhs[a:c], relative_tolerance=1e-10)
+        assert abs(int(stats[i].iterationCount) - int(s_ref.iterationCount)) <= 1
+        assert np.linalg.norm(got[a:c] - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+
+
+@pytest.mark.parametrize("restart", [False, True])
+def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart):
+    w = small_batch(seed=8)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x = torch_gpu.zeros_like(b)
+    stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, restart=restart)
+    assert np.all(status == 1) and np.all(refinements >= 1)
+    assert np.all(per_system_residuals(w, x.cpu().numpy()) < 1e-10)
+    assert all(s.relativeResidual < 1e-10 for s in stats)
+
+
+def test_batched_graph_replays_match_eager(hip, torch_gpu):
+    w = small_batch(seed=9)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.autotune(32, 2)
+    assert set(batch.tile_rows().tolist()) <= {16, 32, 64}
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    results = []
+    for graph in (False, True, True, True):   # the graph is captured once and replayed on later calls
+        x = torch_gpu.zeros_like(b)
+        stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, graph=graph)
+        torch_gpu.cuda.synchronize()
+        _ = (b * 2).sum().item()                # eager work between replays
+        results.append((x.cpu().numpy(), [s.iterationCount for s in stats]))
+    for x, iterations in results[1:]:
+        assert np.array_equal(x, results[0][0]) and iterations == results[0][1]
+
+
+def test_batched_rejects_coupled_systems(hip, torch_gpu):
+    csr = systems.poisson5(8)
+    matrix = hip.CSRMatrix(csr)
+    with pytest.raises(hip._lib.ShapeError):
+        hip.Batch(matrix, [0, 32, 64])   # the 5-point stencil couples rows 31 and 32
+
+
+def test_full_batch_256_uneven_systems(hip, torch_gpu):
+    """256 2D Q2 systems, n_i in [16, 256] (seed 3), lognormal(0, 0.5) conductivity, mixed refinement to 1e-10."""
+    w = systems.q2_quad_batch(256)
+    assert len(w.extra["sizes"]) == 256
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.autotune(32, 2)
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x = torch_gpu.zeros_like(b)
+    stats, status, refinements, summary = batch.solve(b, x, Statistics(100000, 0.0, 1e-10), mixed=True)
+    assert np.all(status == 1)
+    residuals = per_system_residuals(w, x.cpu().numpy())
+    assert np.all(residuals < 1e-10)
+    print(f"batch: {w.matrix.rows} rows, {w.matrix.nnz} nnz, {summary.seconds:.3f} s, max iterations "
+          f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, tile rows "
+          f"{np.bincount(batch.tile_rows(), minlength=65)[[16, 32, 64]].tolist()}")

@@ -53,13 +53,18 @@ def test_spmv_kat(hip, torch_gpu):
 SCALES = [(0, 1), (0, -1), (0, 2.5), (1, 1), (1, -1), (1, 0.5), (-1, 1), (-1, 3.0), (0.25, 1), (0.25, -2.0)]
 
 
-@pytest.mark.parametrize("lanes,unroll", [(1, 1), (2, 4), (4, 2), (8, 8), (16, 4), (32, 1), (64, 2)])
-def test_spmv_random_against_oracle(hip, torch_gpu, lanes, unroll):
-    rng = np.random.default_rng(lanes * 10 + unroll)
+LAUNCHES = [("staged", 512, 4), ("staged", 512, 8), ("staged", 1024, 4), ("staged", 1024, 8), ("vector", 1, 1),
+            ("vector", 2, 4), ("vector", 4, 2), ("vector", 8, 8), ("vector", 16, 4), ("vector", 32, 1),
+            ("vector", 64, 2)]
+
+
+@pytest.mark.parametrize("kernel,width,unroll", LAUNCHES)
+def test_spmv_random_against_oracle(hip, torch_gpu, kernel, width, unroll):
+    rng = np.random.default_rng(width * 10 + unroll)
     csr = random_csr(rng, 3001, 2500, 0.02, empty_rows=(0, 17, 3000))
     matrix = hip.CSRMatrix(csr)
-    matrix.set_launch(64, lanes, unroll)
-    matrix.set_launch(32, lanes, unroll)
+    matrix.set_launch(64, kernel, width, unroll)
+    matrix.set_launch(32, kernel, width, unroll)
     x = rng.standard_normal(2500)
     y0 = rng.standard_normal(3001)
     for in_scale, out_scale in SCALES:
@@ -68,7 +73,10 @@ def test_spmv_random_against_oracle(hip, torch_gpu, lanes, unroll):
         y = to_dev(torch_gpu, y0)
         hip.HIPCSROperator(matrix, space).product(in_scale, to_dev(torch_gpu, x), out_scale, y)
         got = y.cpu().numpy()
-        assert np.all(np.abs(got - expected) <= spmv_bound(csr, x, y0, in_scale, out_scale, EPS64))
+        if kernel == "staged":
+            assert np.array_equal(got, expected)  # same rounding sequence as the CPU row loop
+        else:
+            assert np.all(np.abs(got - expected) <= spmv_bound(csr, x, y0, in_scale, out_scale, EPS64))
         space32 = hip.HIPSpace(torch_gpu.float32)
         y32 = to_dev(torch_gpu, y0, torch_gpu.float32)
         hip.HIPCSROperator(matrix, space32).product(in_scale, to_dev(torch_gpu, x, torch_gpu.float32), out_scale, y32)
@@ -91,7 +99,17 @@ def test_spmv_deterministic_and_autotuned(hip, torch_gpu):
         outs.append(y.cpu().numpy())
     assert all(np.array_equal(outs[0], o) for o in outs[1:])  # bitwise run to run
     expected = oracle.csr_product(w.matrix, 0.0, w.rhs, 1.0, np.zeros(w.matrix.rows))
-    assert np.all(np.abs(outs[0] - expected) <= spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64))
+    assert np.array_equal(outs[0], expected)
+    # vector rows: deterministic too, within the rounding bound of the CPU order
+    matrix.autotune(64, repetitions=3, kernels=("vector",))
+    assert matrix.launch(64)[0] == "vector"
+    first = space.makeVector(w.matrix.rows)
+    op.product(0, x, 1, first)
+    second = space.makeVector(w.matrix.rows)
+    op.product(0, x, 1, second)
+    assert torch_gpu.equal(first, second)
+    bound = spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64)
+    assert np.all(np.abs(first.cpu().numpy() - expected) <= bound)
 
 
 def test_spmv_full_size_q2_64(hip, torch_gpu):
@@ -104,7 +122,8 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
     y = space.makeVector(w.matrix.rows)
     hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
     got = y.cpu().numpy()
-    assert np.all(np.abs(got - expected) <= spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64))
+    assert matrix.launch(64)[0] == "staged"
+    assert np.array_equal(got, expected)  # bit-identical to the sequential CPU rows
     space32 = hip.HIPSpace(torch_gpu.float32)
     y32 = space32.makeVector(w.matrix.rows)
     hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)

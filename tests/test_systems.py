@@ -125,3 +125,17 @@ def test_csr_slots():
     assert csr.entries[slots[0]] == 4.0 and csr.entries[slots[1]] == -1.0
     assert slots[2] == -1
     assert csr.column_indices[slots[3]] == 14
+
+
+def test_batch_workload_layout():
+    w = systems.q2_quad_batch(count=6, seed=3, smallest=2, largest=9)
+    sizes = w.extra["sizes"]
+    raw = systems.splitmix64(3, 6)
+    unit = (raw >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    assert sizes.tolist() == (2 + np.floor(unit * 8)).astype(int).tolist()
+    offsets = w.extra["offsets"]
+    assert np.array_equal(np.diff(offsets), (2 * sizes - 1) ** 2)
+    a = w.matrix.to_scipy()
+    for lo, hi in zip(offsets[:-1], offsets[1:]):  # block diagonal: nothing outside the diagonal blocks
+        assert a[lo:hi, :lo].nnz == 0 and a[lo:hi, hi:].nnz == 0
+    assert w.matrix.nnz == sum((8 * n - 9) ** 2 for n in sizes)
