@@ -12,7 +12,7 @@ import os
 
 # CIE_HIP_LIBRARY selects another build of the same C ABI (e.g. one compiled
 # with different load or launch options); by default the in-tree build.
-LIBRARY_PATH = Path(os.environ.get("CIE_HIP_LIBRARY", Path(__file__).resolve().parent / "libcie_hip.so"))
+LIBRARY_PATH = Path(os.environ.get("CIE_HIP_LIBRARY") or Path(__file__).resolve().parent / "libcie_hip.so")
 
 OK = 0
 ERROR_INVALID_ARGUMENT = 1
