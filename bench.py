@@ -87,6 +87,8 @@ def main():
     parser.add_argument("--cpu-iterations", type=int, default=20, help="CPU baseline sample (0 disables)")
     parser.add_argument("--cpu-threads", type=int, default=0)
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
+    parser.add_argument("--calibration-copy", action="store_true",
+                        help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -208,7 +210,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(matrix.launch(32)),
             "kernel": "productKernel<float> (q = A p, FP32 plane, fused p.q) inside the mixed solve",
             "bytes_per_launch": bytes32, "mean_launch_seconds": product_seconds,
         },
@@ -217,6 +219,23 @@ def main():
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         result["cpu_baseline"] = cpu_baseline(workload, int(f.stats.iterationCount), args.cpu_iterations, threads)
     print(json.dumps(result), flush=True)
+    if args.calibration_copy:
+        source = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        torch.empty_like(source).copy_(source)
+        torch.cuda.synchronize()
+
+
+def measured_traffic(launch):
+    """HBM bytes per launch of the in-solve FP32 product kernel for this launch shape, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy), else None."""
+    path = ROOT / "profiles" / "spmv_q2_64_traffic.json"
+    if not path.exists():
+        return None
+    kernel, width, unroll = launch
+    walker = "StagedRows" if kernel == "staged" else "VectorRows"
+    name = f"productKernel<{walker}<float, float, {width}, {unroll}, 256> >"
+    entry = json.loads(path.read_text()).get("kernels", {}).get(name)
+    return None if entry is None else entry["traffic_bytes"]
 
 
 if __name__ == "__main__":

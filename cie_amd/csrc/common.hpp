@@ -62,7 +62,6 @@ enum SolveEnd : int32_t {
 // Per-context scratch for grid reductions and scalar results.
 struct Scratch {
     double* partials = nullptr;   // kMaxPartials * 4 doubles
-    unsigned* counters = nullptr; // 64 counters for last-block-done reductions
     double* scalars = nullptr;    // 64 doubles of device scalar results
     double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
     SolverState* state = nullptr; // one solver state

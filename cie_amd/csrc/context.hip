@@ -74,13 +74,10 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     }
     auto& s = ctx->scratch;
     bool ok = hipMalloc(&s.partials, sizeof(double) * ciehip::kMaxPartials * 4) == hipSuccess
-           && hipMalloc(&s.counters, sizeof(unsigned) * 64) == hipSuccess
            && hipMalloc(&s.scalars, sizeof(double) * 64) == hipSuccess
            && hipMalloc(&s.state, sizeof(ciehip::SolverState)) == hipSuccess
            && hipHostMalloc(&s.pinned, sizeof(double) * 64) == hipSuccess
-           && hipHostMalloc(&s.pinnedState, 2 * sizeof(ciehip::SolverState)) == hipSuccess
-           && hipMemsetAsync(s.counters, 0, sizeof(unsigned) * 64, ctx->stream) == hipSuccess
-           && hipStreamSynchronize(ctx->stream) == hipSuccess;
+           && hipHostMalloc(&s.pinnedState, 2 * sizeof(ciehip::SolverState)) == hipSuccess;
     if (!ok) {
         cie_hip_context_destroy(ctx);
         return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_context_create: scratch allocation failed");
@@ -95,7 +92,6 @@ int cie_hip_context_destroy(cie_hip_context context) {
     if (context->stream) hipStreamSynchronize(context->stream);
     auto& s = context->scratch;
     if (s.partials) hipFree(s.partials);
-    if (s.counters) hipFree(s.counters);
     if (s.scalars) hipFree(s.scalars);
     if (s.state) hipFree(s.state);
     if (s.pinned) hipHostFree(s.pinned);

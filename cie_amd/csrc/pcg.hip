@@ -269,11 +269,40 @@ int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, 
     return CIE_HIP_OK;
 }
 
+// Brackets the first kProfileEvents loop SpMV kernels (not their finish
+// kernels) with events on the solver stream (profiling on).
+struct ProductProfile {
+    int used = 0;
+    int begin(cie_hip_context ctx) {
+        if (!ctx->profiling || used >= cie_hip_context_s::kProfileEvents) return -1;
+        (void)hipEventRecord(ctx->profileEvents[2 * used], ctx->stream);
+        return used++;
+    }
+    void end(cie_hip_context ctx, int mark) {
+        if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
+    }
+    // Mean over the first `samples` bracketed launches (those that did work).
+    double mean(cie_hip_context ctx, int samples) const {
+        double total = 0.0;
+        int count = 0;
+        for (int i = 0; i < std::min(used, samples); ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->profileEvents[2 * i], ctx->profileEvents[2 * i + 1]) == hipSuccess) {
+                total += ms * 1e-3;
+                ++count;
+            }
+        }
+        return count ? total / count : 0.0;
+    }
+};
+
 // q = A p with its p.q, then the alpha finish.
 template <class T>
-int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q) {
+int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q,
+                      ProductProfile& profile) {
     cie_hip_context ctx = m->context;
     int grid = 0;
+    const int mark = profile.begin(ctx);
     const bool ok = withWalker<T, T>(launch, [&](auto walker) {
         using Walker = decltype(walker);
         const GridShape shape = gridFor(productKernel<Walker>, launch, m->rows, ctx->computeUnits);
@@ -283,6 +312,7 @@ int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, 
                            ctx->scratch.partials, ctx->scratch.state);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    profile.end(ctx, mark);
     launchFinish<T, kPhaseAlpha>(ctx, grid);
     return CIE_HIP_OK;
 }
@@ -337,32 +367,6 @@ struct Timer {
 };
 
 constexpr int kChunk = 16;
-
-// Brackets the first kProfileEvents loop SpMV launches with events (profiling on).
-struct ProductProfile {
-    int used = 0;
-    int begin(cie_hip_context ctx) {
-        if (!ctx->profiling || used >= cie_hip_context_s::kProfileEvents) return -1;
-        (void)hipEventRecord(ctx->profileEvents[2 * used], ctx->stream);
-        return used++;
-    }
-    void end(cie_hip_context ctx, int mark) {
-        if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
-    }
-    // Mean over the first `samples` bracketed launches (those that did work).
-    double mean(cie_hip_context ctx, int samples) const {
-        double total = 0.0;
-        int count = 0;
-        for (int i = 0; i < std::min(used, samples); ++i) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ctx->profileEvents[2 * i], ctx->profileEvents[2 * i + 1]) == hipSuccess) {
-                total += ms * 1e-3;
-                ++count;
-            }
-        }
-        return count ? total / count : 0.0;
-    }
-};
 
 // Enqueues chunks of iterations until the device state says done. The state
 // of chunk k is read back while chunk k + 1 is already queued, so the device
@@ -476,10 +480,8 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         status = launchResidual<double>(m, true, solution, q.b, r64, m->inverseDiagonal64, p64);
         if (status) return status;
         status = runLoop(ctx, timer, [&]() -> int {
-            const int mark = profile.begin(ctx);
-            int s = launchProductStep<double>(m, m->entries64, m->launch64, p64, q64);
+            int s = launchProductStep<double>(m, m->entries64, m->launch64, p64, q64, profile);
             if (s) return s;
-            profile.end(ctx, mark);
             hipLaunchKernelGGL((updateKernel<double, double>), vgrid, vblock, 0, ctx->stream, n, solution, r64, p64,
                                q64, m->inverseDiagonal64, ctx->scratch.partials, ctx->scratch.state);
             launchFinish<double, kPhaseUpdate>(ctx, int(vgrid.x));
@@ -494,10 +496,8 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         status = launchResidual<float>(m, true, solution, q.b, r32, m->inverseDiagonal32, p32);
         if (status) return status;
         status = runLoop(ctx, timer, [&]() -> int {
-            const int mark = profile.begin(ctx);
-            int s = launchProductStep<float>(m, m->entries32, m->launch32, p32, q32);
+            int s = launchProductStep<float>(m, m->entries32, m->launch32, p32, q32, profile);
             if (s) return s;
-            profile.end(ctx, mark);
             hipLaunchKernelGGL((updateKernel<float, double>), vgrid, vblock, 0, ctx->stream, n, solution, r32, p32,
                                q32, m->inverseDiagonal32, ctx->scratch.partials, ctx->scratch.state);
             launchFinish<float, kPhaseUpdateMixed>(ctx, int(vgrid.x));

@@ -1,0 +1,107 @@
+"""Summaries of rocprofv3 output for the solver kernels.
+
+    python -m cie_amd.profiling stats <kernel_stats.csv> <out.md> "<command>"
+    python -m cie_amd.profiling traffic <fetch dir> <write dir> <out.json> <out.md> "<command>"
+
+``stats`` turns a --kernel-trace --stats CSV into a short table.
+``traffic`` reads two --pmc runs (FETCH_SIZE, then WRITE_SIZE; both in KiB per
+dispatch) of a program that ends with a device-to-device copy of a known
+size (bench.py --calibration-copy): the counters of that copy give the scale
+factor of each counter, and the scaled mean per dispatch of every kernel is
+its HBM traffic in bytes.
+"""
+import csv
+import glob
+import json
+import re
+import sys
+from collections import defaultdict
+from pathlib import Path
+
+CALIBRATION_BYTES = 1 << 30  # bench.py --calibration-copy moves 1 GiB device to device
+CALIBRATION_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _short(name):
+    name = re.sub(r"\(anonymous namespace\)::", "", name)
+    name = re.sub(r"\(int, int, .*", "", name)
+    name = re.sub(r"\(long, .*", "", name)
+    name = name.replace("ciehip::", "").replace("void ", "")
+    return name[:100]
+
+
+def kernel_stats(path, out, command):
+    rows = list(csv.DictReader(open(path)))
+    lines = [f"# Kernel time: `{command}`", "", "| kernel | calls | mean (us) | total share |", "|---|---:|---:|---:|"]
+    for r in rows[:20]:
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['Percentage']):.1f}% |")
+    Path(out).write_text("\n".join(lines) + "\n")
+
+
+def _counter_means(directory, counter):
+    """Mean counter value per kernel name, and the value of the last dispatch of the copy
+    kernel: the calibration copy ends the program, and the runtime launches other copies
+    under that name before it."""
+    files = glob.glob(str(Path(directory) / "*counter_collection.csv"))
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {directory}")
+    values = defaultdict(list)
+    calibration = None
+    for row in csv.DictReader(open(files[0])):
+        if row["Counter_Name"] != counter:
+            continue
+        if row["Kernel_Name"] == CALIBRATION_KERNEL:
+            dispatch = int(row["Dispatch_Id"])
+            if calibration is None or dispatch > calibration[0]:
+                calibration = (dispatch, float(row["Counter_Value"]))
+        else:
+            values[row["Kernel_Name"]].append(float(row["Counter_Value"]))
+    if calibration is None:
+        raise ValueError(f"{directory}: no {CALIBRATION_KERNEL} dispatch to calibrate {counter} with")
+    return {name: sum(v) / len(v) for name, v in values.items()}, calibration[1]
+
+
+def traffic(fetch_dir, write_dir, out_json, out_md, command):
+    fetch, copy_fetch = _counter_means(fetch_dir, "FETCH_SIZE")
+    write, copy_write = _counter_means(write_dir, "WRITE_SIZE")
+    copy_fetch *= 1024
+    copy_write *= 1024
+    fetch_scale = CALIBRATION_BYTES / copy_fetch
+    write_scale = CALIBRATION_BYTES / copy_write
+    kernels = {}
+    for name in fetch:
+        if name not in write:
+            continue
+        kernels[_short(name)] = {
+            "fetch_bytes": fetch[name] * 1024 * fetch_scale,
+            "write_bytes": write[name] * 1024 * write_scale,
+            "traffic_bytes": fetch[name] * 1024 * fetch_scale + write[name] * 1024 * write_scale,
+        }
+    result = {"command": command, "calibration": {"bytes": CALIBRATION_BYTES, "fetch_scale": fetch_scale,
+                                                  "write_scale": write_scale}, "kernels": kernels}
+    Path(out_json).write_text(json.dumps(result, indent=1) + "\n")
+    lines = [f"# HBM traffic per dispatch: `{command}`", "",
+             f"Calibration: a {CALIBRATION_BYTES} B device copy read FETCH_SIZE x{fetch_scale:.3f}, "
+             f"WRITE_SIZE x{write_scale:.3f} of its true bytes; the figures below are scaled by those factors.", "",
+             "| kernel | fetch (MB) | write (MB) | traffic (MB) |", "|---|---:|---:|---:|"]
+    # solver kernels first, then the standalone/autotune SpMV candidates
+    for name, k in sorted(kernels.items(), key=lambda kv: (kv[0].startswith("csrProductKernel"), kv[0])):
+        lines.append(f"| `{name}` | {k['fetch_bytes'] / 1e6:.1f} | {k['write_bytes'] / 1e6:.1f} | "
+                     f"{k['traffic_bytes'] / 1e6:.1f} |")
+    Path(out_md).write_text("\n".join(lines) + "\n")
+
+
+def main(argv):
+    if len(argv) >= 4 and argv[0] == "stats":
+        kernel_stats(argv[1], argv[2], argv[3])
+    elif len(argv) >= 6 and argv[0] == "traffic":
+        traffic(argv[1], argv[2], argv[3], argv[4], argv[5])
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
