@@ -113,8 +113,8 @@ def main():
     t0 = time.perf_counter()
     matrix = linalg.CSRMatrix(csr, context)
     upload_seconds = time.perf_counter() - t0
-    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector"))
-    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector"))
+    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled"))
+    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled"))
     b = torch.from_numpy(workload.rhs).to("cuda:0")
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, args.tolerance)
@@ -211,7 +211,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(matrix.launch(32)),
-            "kernel": "productKernel<float> (q = A p, FP32 plane, fused p.q) inside the mixed solve",
+            "kernel": f"productKernel {list(matrix.launch(32))} (q = A p, FP32 plane, fused p.q) inside the mixed solve",
             "bytes_per_launch": bytes32, "mean_launch_seconds": product_seconds,
         },
     }
@@ -232,8 +232,11 @@ def measured_traffic(launch):
     if not path.exists():
         return None
     kernel, width, unroll = launch
-    walker = "StagedRows" if kernel == "staged" else "VectorRows"
-    name = f"productKernel<{walker}<float, float, {width}, {unroll}, 256> >"
+    if kernel == "tiled":
+        name = f"productKernel<BlockTiles<float, float, {unroll}, {width}> >"
+    else:
+        walker = "StagedRows" if kernel == "staged" else "VectorRows"
+        name = f"productKernel<{walker}<float, float, {width}, {unroll}, 256> >"
     entry = json.loads(path.read_text()).get("kernels", {}).get(name)
     return None if entry is None else entry["traffic_bytes"]
 

@@ -29,6 +29,7 @@ HOST = 0
 DEVICE = 1
 SPMV_VECTOR = 0
 SPMV_STAGED = 1
+SPMV_TILED = 2
 PLANE_F64 = 1
 PLANE_F32 = 2
 

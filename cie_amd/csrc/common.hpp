@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/cie_hip.h"
 
@@ -61,7 +62,8 @@ enum SolveEnd : int32_t {
 
 // Per-context scratch for grid reductions and scalar results.
 struct Scratch {
-    double* partials = nullptr;   // kMaxPartials * 4 doubles
+    double* partials = nullptr;   // partialSlots * 4 doubles (grown on demand)
+    int partialSlots = 0;
     double* scalars = nullptr;    // 64 doubles of device scalar results
     double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
     SolverState* state = nullptr; // one solver state
@@ -81,14 +83,20 @@ struct cie_hip_context_s {
     hipEvent_t profileEvents[2 * kProfileEvents] = {};
 };
 
-// Launch configuration of a CSR SpMV for one value precision.
+// Launch configuration of a CSR SpMV for one value precision. The grid is
+// sized per launch from the kernel's occupancy (one resident round).
 struct CsrLaunch {
-    int kernel = 0;   // CIE_HIP_SPMV_STAGED or CIE_HIP_SPMV_VECTOR
-    int width = 0;    // staged: LDS entries per wavefront segment; vector: lanes per row
+    int kernel = 0;   // CIE_HIP_SPMV_VECTOR, CIE_HIP_SPMV_STAGED or CIE_HIP_SPMV_TILED
+    int width = 0;    // vector: lanes per row; staged: LDS entries per segment; tiled: threads per block
     int unroll = 0;   // entries per lane loaded before they are used
     int block = 0;    // threads per block
-    int grid = 0;     // blocks (multiple of 8, one contiguous run per XCD)
-    int param = 0;    // staged: tiles per block; vector: rows per block
+};
+
+// Block tiles of a matrix for one tile capacity (entries per tile).
+struct TileSet {
+    int capacity = 0;
+    int count = 0;          // tiles; the device array holds count + 1 entries
+    int2* tiles = nullptr;  // {first row, first entry}
 };
 
 struct cie_hip_csr_s {
@@ -106,6 +114,8 @@ struct cie_hip_csr_s {
     CsrLaunch launch64, launch32;
     void* workspace = nullptr;      // solver vectors, grown on demand, freed with the matrix
     size_t workspaceBytes = 0;
+    std::vector<int32_t> hostExtents;  // host copy of rowExtents (tile cuts are made on the host)
+    std::vector<TileSet> tileSets;
 };
 
 namespace ciehip {
@@ -113,4 +123,9 @@ namespace ciehip {
 int matrixWorkspace(cie_hip_csr matrix, size_t bytes, void** pointer);
 // Builds the FP64/FP32 inverse diagonal of the matrix if it is missing.
 int ensureInverseDiagonal(cie_hip_csr matrix, int precision);
+// The block tiles of `capacity` entries (built and uploaded on first use).
+int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
+// Grows the context's partials to at least `slots` slots of 4 doubles. A
+// growth waits for the stream first (queued kernels may use the old buffer).
+int ensurePartials(cie_hip_context context, int slots);
 }  // namespace ciehip

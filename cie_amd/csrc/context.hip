@@ -14,6 +14,19 @@ int setError(int status, const std::string& message) {
     return status;
 }
 
+int ensurePartials(cie_hip_context ctx, int slots) {
+    Scratch& s = ctx->scratch;
+    if (slots <= s.partialSlots) return CIE_HIP_OK;
+    CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double* grown = nullptr;
+    if (hipMalloc(&grown, sizeof(double) * size_t(slots) * 4) != hipSuccess)
+        return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "partials allocation failed");
+    CIE_HIP_CHECK(hipFree(s.partials));
+    s.partials = grown;
+    s.partialSlots = slots;
+    return CIE_HIP_OK;
+}
+
 }  // namespace ciehip
 
 using ciehip::setError;
@@ -73,6 +86,7 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
         ctx->ownsStream = true;
     }
     auto& s = ctx->scratch;
+    s.partialSlots = ciehip::kMaxPartials;
     bool ok = hipMalloc(&s.partials, sizeof(double) * ciehip::kMaxPartials * 4) == hipSuccess
            && hipMalloc(&s.scalars, sizeof(double) * 64) == hipSuccess
            && hipMalloc(&s.state, sizeof(ciehip::SolverState)) == hipSuccess

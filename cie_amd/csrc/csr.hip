@@ -15,14 +15,18 @@ namespace {
 template <class T>
 int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
                   const T* entries) {
+    int status = CIE_HIP_OK;
     const bool ok = withWalker<T, T>(launch, [&](auto walker) {
         using Walker = decltype(walker);
-        const GridShape shape = gridFor(csrProductKernel<Walker>, launch, m->rows, m->context->computeUnits);
-        hipLaunchKernelGGL((csrProductKernel<Walker>), dim3(shape.grid), dim3(Walker::kBlock), 0, m->context->stream,
-                           int(m->rows), shape.param, m->rowExtents, m->columnIndices, entries, in, out, inScale,
-                           outScale);
+        RowSource source;
+        int grid = 0;
+        status = prepareLaunch(m, launch, csrProductKernel<Walker>, source, grid);
+        if (status != CIE_HIP_OK) return;
+        hipLaunchKernelGGL((csrProductKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, m->context->stream,
+                           source, entries, in, out, inScale, outScale);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    if (status != CIE_HIP_OK) return status;
     CIE_HIP_CHECK(hipGetLastError());
     return CIE_HIP_OK;
 }
@@ -44,6 +48,8 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->inverseDiagonal64) hipFree(m->inverseDiagonal64);
     if (m->inverseDiagonal32) hipFree(m->inverseDiagonal32);
     if (m->workspace) hipFree(m->workspace);
+    for (TileSet& set : m->tileSets)
+        if (set.tiles) hipFree(set.tiles);
     delete m;
 }
 
@@ -120,6 +126,43 @@ int matrixWorkspace(cie_hip_csr m, size_t bytes, void** pointer) {
         m->workspaceBytes = bytes;
     }
     *pointer = m->workspace;
+    return CIE_HIP_OK;
+}
+
+// Cuts the rows into tiles of whole consecutive rows with at most `capacity`
+// entries and at most kSpmvBlock rows (a longer row is a tile of its own).
+int ensureTiles(cie_hip_csr m, int capacity, const TileSet** tiles) {
+    for (const TileSet& set : m->tileSets)
+        if (set.capacity == capacity) {
+            *tiles = &set;
+            return CIE_HIP_OK;
+        }
+    const int32_t* extents = m->hostExtents.data();
+    const int rows = int(m->rows);
+    std::vector<int2> cut;
+    cut.reserve(size_t(m->nnz / std::max(1, capacity / 2) + rows / kSpmvBlock + 2));
+    int row = 0;
+    while (row < rows) {
+        const int first = row;
+        if (extents[row + 1] - extents[row] > capacity) {
+            ++row;
+        } else {
+            while (row < rows && row - first < kSpmvBlock && extents[row + 1] - extents[first] <= capacity) ++row;
+        }
+        cut.push_back(make_int2(first, extents[first]));
+    }
+    cut.push_back(make_int2(rows, extents[rows]));
+    TileSet set;
+    set.capacity = capacity;
+    set.count = int(cut.size()) - 1;
+    CIE_HIP_CHECK(hipMalloc(&set.tiles, sizeof(int2) * cut.size()));
+    const hipError_t copied = hipMemcpy(set.tiles, cut.data(), sizeof(int2) * cut.size(), hipMemcpyHostToDevice);
+    if (copied != hipSuccess) {
+        (void)hipFree(set.tiles);
+        return setError(CIE_HIP_ERROR_HIP, std::string("tile upload failed: ") + hipGetErrorString(copied));
+    }
+    m->tileSets.push_back(set);
+    *tiles = &m->tileSets.back();
     return CIE_HIP_OK;
 }
 
@@ -216,6 +259,7 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
         m->entries64 = const_cast<double*>(entries);
     }
     rowStatistics(m, extents);
+    m->hostExtents.assign(extents, extents + rows + 1);
     if (planes & CIE_HIP_PLANE_F32) {
         const int status = buildPlane32(m);
         if (status != CIE_HIP_OK) return bail(status, cie_hip_last_error());
@@ -271,6 +315,9 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     } else if (kernel == CIE_HIP_SPMV_VECTOR) {
         if (!validShape(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: vector width must be 1..64 (power of two), unroll 1, 2, 4 or 8");
+    } else if (kernel == CIE_HIP_SPMV_TILED) {
+        if (!validTiles(width, unroll))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: tiled width must be 256, unroll 4, 8 or 16");
     } else {
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: unknown kernel");
     }
@@ -279,7 +326,6 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     launch.width = width;
     launch.unroll = unroll;
     launch.block = kSpmvBlock;
-    sizeLaunch(launch, matrix->rows, matrix->context->computeUnits);
     return CIE_HIP_OK;
 }
 
@@ -298,12 +344,11 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: invalid matrix or precision");
     if (precision == 32 && !matrix->entries32)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
-    if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED))))
+    if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED) | (1 << CIE_HIP_SPMV_TILED))))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: empty kernel mask");
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
     repetitions = std::max(1, repetitions);
-    const int computeUnits = ctx->computeUnits;
 
     std::vector<CsrLaunch> candidates;
     if ((kernels & (1 << CIE_HIP_SPMV_STAGED)) && matrix->maxRowLength <= kStagedMaxRow) {
@@ -314,9 +359,18 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
                 c.width = width;
                 c.unroll = unroll;
                 c.block = kSpmvBlock;
-                sizeLaunch(c, matrix->rows, computeUnits);
                 candidates.push_back(c);
             }
+    }
+    if (kernels & (1 << CIE_HIP_SPMV_TILED)) {
+        for (int unroll : {4, 8, 16}) {
+            CsrLaunch c;
+            c.kernel = CIE_HIP_SPMV_TILED;
+            c.width = kSpmvBlock;
+            c.unroll = unroll;
+            c.block = kSpmvBlock;
+            candidates.push_back(c);
+        }
     }
     if (kernels & (1 << CIE_HIP_SPMV_VECTOR)) {
         // Lanes within a factor 4 of about one pass of 4 entries per lane per row.
@@ -328,7 +382,6 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
                 c.width = lanes;
                 c.unroll = unroll;
                 c.block = kSpmvBlock;
-                sizeLaunch(c, matrix->rows, computeUnits);
                 candidates.push_back(c);
             }
     }

@@ -52,9 +52,34 @@ inline bool validStage(int capacity, int unroll) {
     }
 }
 
+// Entries per lane in flight of the block-tile kernel (tiles of 256 * U entries).
+#define CIEHIP_TILES(X) X(4) X(8) X(16)
+
+inline bool validTiles(int block, int unroll) {
+    if (block != kSpmvBlock) return false;
+    switch (unroll) {
+#define CIEHIP_TILES_VALID(U) case U: return true;
+        CIEHIP_TILES(CIEHIP_TILES_VALID)
+#undef CIEHIP_TILES_VALID
+    default: return false;
+    }
+}
+
 // Calls f(Walker{}) with the row walker the launch selects; false if unsupported.
 template <class TV, class TM, class F>
 bool withWalker(const CsrLaunch& launch, F&& f) {
+    if (launch.kernel == CIE_HIP_SPMV_TILED) {
+        if (launch.width != kSpmvBlock) return false;
+        switch (launch.unroll) {
+#define CIEHIP_TILES_CASE(U)                               \
+    case U:                                                \
+        f(BlockTiles<TV, TM, U, kSpmvBlock>{});            \
+        return true;
+            CIEHIP_TILES(CIEHIP_TILES_CASE)
+#undef CIEHIP_TILES_CASE
+        default: return false;
+        }
+    }
     if (launch.kernel == CIE_HIP_SPMV_STAGED) {
         switch (launch.width * 16 + launch.unroll) {
 #define CIEHIP_STAGE_CASE(C, U)                                   \
@@ -79,58 +104,50 @@ inline int nextPow2(int v) {
 
 inline int roundUp(int64_t value, int64_t multiple) { return int((value + multiple - 1) / multiple * multiple); }
 
-// Fills grid and param of a launch for a matrix with `rows` rows: a multiple
-// of 8 blocks (one contiguous run per XCD), at most 8 blocks per compute unit.
-inline void sizeLaunch(CsrLaunch& launch, int64_t rows, int computeUnits) {
-    const int64_t cap = std::max(8, computeUnits * 8);
-    if (launch.kernel == CIE_HIP_SPMV_STAGED) {
-        const int64_t tiles = std::max<int64_t>(1, (rows + kWave - 1) / kWave);
-        const int64_t wavesPerBlock = launch.block / kWave;
-        int grid = int(std::min<int64_t>((tiles + wavesPerBlock - 1) / wavesPerBlock, cap));
-        launch.grid = std::min(roundUp(grid, kXcds), kMaxPartials);
-        launch.param = int((tiles + launch.grid - 1) / launch.grid);
-    } else {
-        const int64_t rowsPerStep = launch.block / launch.width;
-        const int64_t steps = std::max<int64_t>(1, (rows + rowsPerStep - 1) / rowsPerStep);
-        int grid = int(std::min<int64_t>(steps, cap));
-        launch.grid = std::min(roundUp(grid, kXcds), kMaxPartials);
-        const int64_t perBlock = (rows + launch.grid - 1) / launch.grid;
-        launch.param = std::max<int>(int(rowsPerStep), roundUp(perBlock, rowsPerStep));
-    }
-}
-
 // Blocks of `kernel` that are resident on one compute unit at `block` threads
 // (HIP occupancy query, cached per kernel).
 int residentBlocks(const void* kernel, int block);
 
-struct GridShape {
-    int grid;   // blocks
-    int param;  // staged: tiles per block; vector: rows per block
-};
-
-// One resident round of blocks: compute units x blocks per unit (rounded down
+// Grid and row source of a launch of `kernel`. Vector and staged walkers get
+// one resident round of blocks: compute units x blocks per unit (rounded down
 // to a multiple of 8 so each XCD gets a contiguous run), but no more blocks
-// than there is work for.
+// than there is work for; param is each block's share. The tiled walker gets
+// one block per tile (the tiles are cut on first use). A grid reduction over
+// the launch writes one partial per block (Walker::slot()).
 template <class Kernel>
-GridShape gridFor(Kernel kernel, const CsrLaunch& launch, int64_t rows, int computeUnits) {
+int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSource& source, int& grid) {
     const int perUnit = residentBlocks(reinterpret_cast<const void*>(kernel), launch.block);
-    int cap = std::max(1, computeUnits) * perUnit;
+    int cap = std::max(1, m->context->computeUnits) * perUnit;
     if (cap >= kXcds) cap -= cap % kXcds;
     cap = std::min(cap, kMaxPartials);
-    GridShape shape;
-    if (launch.kernel == CIE_HIP_SPMV_STAGED) {
+    source.rows = int(m->rows);
+    source.rowExtents = m->rowExtents;
+    source.columnIndices = m->columnIndices;
+    source.tiles = nullptr;
+    source.tileCount = 0;
+    source.pad = 0;
+    const int64_t rows = m->rows;
+    if (launch.kernel == CIE_HIP_SPMV_TILED) {
+        const TileSet* set = nullptr;
+        const int status = ensureTiles(m, launch.block * launch.unroll, &set);
+        if (status != CIE_HIP_OK) return status;
+        source.tiles = set->tiles;
+        source.tileCount = set->count;
+        source.param = 1;
+        grid = std::max(1, set->count);  // one block per tile
+    } else if (launch.kernel == CIE_HIP_SPMV_STAGED) {
         const int64_t tiles = std::max<int64_t>(1, (rows + kWave - 1) / kWave);
         const int64_t wavesPerBlock = launch.block / kWave;
-        shape.grid = int(std::min<int64_t>((tiles + wavesPerBlock - 1) / wavesPerBlock, cap));
-        shape.param = int((tiles + shape.grid - 1) / shape.grid);
+        grid = int(std::min<int64_t>((tiles + wavesPerBlock - 1) / wavesPerBlock, cap));
+        source.param = int((tiles + grid - 1) / grid);
     } else {
         const int64_t rowsPerStep = launch.block / launch.width;
         const int64_t steps = std::max<int64_t>(1, (rows + rowsPerStep - 1) / rowsPerStep);
-        shape.grid = int(std::min<int64_t>(steps, cap));
-        const int64_t perBlock = (rows + shape.grid - 1) / shape.grid;
-        shape.param = std::max<int>(int(rowsPerStep), roundUp(perBlock, rowsPerStep));
+        grid = int(std::min<int64_t>(steps, cap));
+        const int64_t perBlock = (rows + grid - 1) / grid;
+        source.param = std::max<int>(int(rowsPerStep), roundUp(perBlock, rowsPerStep));
     }
-    return shape;
+    return CIE_HIP_OK;
 }
 
 // Starting point before autotuning: the staged kernel (bit-identical rows),
@@ -154,7 +171,6 @@ inline CsrLaunch defaultLaunch(double meanRowLength, int maxRowLength, int compu
         launch.width = std::min(64, std::max(1, lanes));
         launch.unroll = unroll;
     }
-    sizeLaunch(launch, rows, computeUnits);
     return launch;
 }
 

@@ -123,6 +123,20 @@ __device__ __forceinline__ TV rowProduct(int row, int lane, const int32_t* __res
 // Kernels keep per-thread partials (dot products) inside their epilogues.
 // ---------------------------------------------------------------------------
 
+// The sparsity pattern as the walkers read it. param is the block's share:
+// rows per block (VectorRows), 64-row tiles per block (StagedRows) or block
+// tiles per block (BlockTiles). tiles/tileCount describe the block tiles of
+// BlockTiles (tileCount + 1 entries {first row, first entry}); unused otherwise.
+struct RowSource {
+    int rows;
+    int param;
+    const int32_t* __restrict__ rowExtents;
+    const int32_t* __restrict__ columnIndices;
+    const int2* __restrict__ tiles;
+    int tileCount;
+    int pad;
+};
+
 // L lanes per row, BS / L rows per block step; rows split into one contiguous
 // chunk per block. Sum order: per-lane strided FMAs, then an xor butterfly.
 template <class TV, class TM, int L, int U, int BS>
@@ -131,14 +145,17 @@ struct VectorRows {
     using Matrix = TM;
     static constexpr int kBlock = BS;
 
+    // Index of this block's partial in a grid reduction.
+    __device__ __forceinline__ static int slot() { return xcdChunk(); }
+
     template <class Epilogue>
-    __device__ __forceinline__ static void run(int rows, int rowsPerChunk, const int32_t* __restrict__ rowExtents,
-                                               const int32_t* __restrict__ columnIndices,
-                                               const TM* __restrict__ entries, const TV* __restrict__ in,
-                                               Epilogue&& epilogue) {
+    __device__ __forceinline__ static void run(const RowSource& source, const TM* __restrict__ entries,
+                                               const TV* __restrict__ in, Epilogue&& epilogue) {
+        const int32_t* __restrict__ rowExtents = source.rowExtents;
+        const int32_t* __restrict__ columnIndices = source.columnIndices;
         const int chunk = xcdChunk();
-        const int rowBegin = min(rows, chunk * rowsPerChunk);
-        const int rowEnd = min(rows, rowBegin + rowsPerChunk);
+        const int rowBegin = min(source.rows, chunk * source.param);
+        const int rowEnd = min(source.rows, rowBegin + source.param);
         const int group = threadIdx.x / L;
         const int lane = threadIdx.x % L;
         for (int row0 = rowBegin; row0 < rowEnd; row0 += BS / L) {
@@ -238,16 +255,20 @@ struct StagedRows {
     static constexpr int kBlock = BS;
     static constexpr int kTile = kWave;
 
+    // Index of this block's partial in a grid reduction.
+    __device__ __forceinline__ static int slot() { return xcdChunk(); }
+
     template <class Epilogue>
-    __device__ __forceinline__ static void run(int rows, int tilesPerBlock, const int32_t* __restrict__ rowExtents,
-                                               const int32_t* __restrict__ columnIndices,
-                                               const TM* __restrict__ entries, const TV* __restrict__ in,
-                                               Epilogue&& epilogue) {
+    __device__ __forceinline__ static void run(const RowSource& source, const TM* __restrict__ entries,
+                                               const TV* __restrict__ in, Epilogue&& epilogue) {
         __shared__ TV stage[BS / kWave][CAP];
+        const int rows = source.rows;
+        const int32_t* __restrict__ rowExtents = source.rowExtents;
+        const int32_t* __restrict__ columnIndices = source.columnIndices;
         const int tiles = (rows + kTile - 1) / kTile;
         const int chunk = xcdChunk();
-        const int tileBegin = min(tiles, chunk * tilesPerBlock);
-        const int tileEnd = min(tiles, tileBegin + tilesPerBlock);
+        const int tileBegin = min(tiles, chunk * source.param);
+        const int tileEnd = min(tiles, tileBegin + source.param);
         const int wave = threadIdx.x / kWave;
         const int lane = threadIdx.x % kWave;
         for (int tile = tileBegin + wave; tile < tileEnd; tile += BS / kWave) {
@@ -260,17 +281,116 @@ struct StagedRows {
     }
 };
 
+// Block tiles (the CSR-stream scheme of Greathouse and Daga, SC14). The rows
+// are cut once, on the host, into tiles of whole consecutive rows with at most
+// BS * U entries and at most BS rows; tiles[t] = {first row, first entry}.
+// Block t handles tile t (one block per tile, so the hardware balances tiles
+// over the compute units as blocks retire). It streams the tile's columns and
+// entries with every lane (entry first + lane + BS * u, all U loads in flight
+// before any is used), gathers in[column], stages the rounded products
+// a_ij * x_j in LDS, and then sums each row with a group of G lanes, G the
+// largest power of two (<= 64) that gives every row of the tile its own group:
+// lane g of a group adds products g, g + G, ... in order, then an xor butterfly
+// combines the group. The order depends only on the tile cut, so results are
+// bitwise reproducible. A row longer than a tile is a tile of its own, summed
+// in passes of BS * U.
+template <class TV, class TM, int U, int BS>
+struct BlockTiles {
+    using Value = TV;
+    using Matrix = TM;
+    static constexpr int kBlock = BS;
+    static constexpr int kCapacity = BS * U;
+
+    // Index of this block's partial in a grid reduction.
+    __device__ __forceinline__ static int slot() { return blockIdx.x; }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void run(const RowSource& source, const TM* __restrict__ entries,
+                                               const TV* __restrict__ in, Epilogue&& epilogue) {
+        __shared__ TV stage[kCapacity];
+        __shared__ int32_t extents[BS + 1];
+        const int32_t* __restrict__ columnIndices = source.columnIndices;
+        const int t = blockIdx.x;
+        if (t >= source.tileCount) return;
+        const int tid = threadIdx.x;
+        const int2 head = source.tiles[t];
+        const int2 next = source.tiles[t + 1];
+        const int first = head.x, rows = next.x - head.x, begin = head.y, end = next.y;
+        if (end - begin > kCapacity) {
+            const TV sum = longRow(begin, end, columnIndices, entries, in, stage);
+            if (tid == 0) epilogue(first, sum);
+            return;
+        }
+        int column[U];
+        TM entry[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = begin + tid + BS * u;
+            column[u] = 0;  // a tile holds at least one row; gathering in[0] is harmless
+            entry[u] = TM(0);
+            if (k < end) {
+                column[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
+                entry[u] = CIEHIP_STREAM_LOAD(entries + k);
+            }
+        }
+        if (tid < rows) extents[tid] = source.rowExtents[first + tid] - begin;
+        if (tid == 0) extents[rows] = end - begin;
+        TV gathered[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) gathered[u] = in[column[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) stage[tid + BS * u] = TV(entry[u]) * gathered[u];
+        __syncthreads();
+        int group = kWave;
+        while (group > 1 && rows * group > BS) group >>= 1;
+        const int row = tid / group;
+        const int g = tid % group;
+        TV sum = TV(0);
+        if (row < rows) {
+            const int rowEnd = extents[row + 1];
+            for (int k = extents[row] + g; k < rowEnd; k += group) sum += stage[k];
+        }
+        for (int offset = group / 2; offset > 0; offset /= 2) sum += __shfl_xor(sum, offset, kWave);
+        if (row < rows && g == 0) epilogue(first + row, sum);
+    }
+
+    // Sum of one row of more than kCapacity entries, valid in thread 0: per
+    // pass, lane sums over u, wavefront butterflies, then waves in order.
+    __device__ __forceinline__ static TV longRow(int begin, int end, const int32_t* __restrict__ columnIndices,
+                                                 const TM* __restrict__ entries, const TV* __restrict__ in,
+                                                 TV* stage) {
+        const int tid = threadIdx.x;
+        TV total = TV(0);
+        for (int base = begin; base < end; base += kCapacity) {
+            TV part = TV(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = base + tid + BS * u;
+                if (k < end) part += TV(CIEHIP_STREAM_LOAD(entries + k)) * in[CIEHIP_STREAM_LOAD(columnIndices + k)];
+            }
+            part = groupSum<kWave>(part);
+            if ((tid & (kWave - 1)) == 0) stage[tid / kWave] = part;
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int w = 0; w < BS / kWave; ++w) total += stage[w];
+            }
+            __syncthreads();
+        }
+        return total;
+    }
+};
+
 // out = inScale * out + outScale * A * in. With contraction off this equals
 // each special case of CSROperator.cpp:78-110 bit for bit (1 * y, -1 * y and
 // 0 + x are exact), except inScale = outScale = -1, where the reference
 // computes A*in - out and this follows the documented contract, -A*in - out.
 template <class Walker>
 __global__ __launch_bounds__(Walker::kBlock) void csrProductKernel(
-    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
-    const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ in,
+    RowSource source, const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ in,
     typename Walker::Value* __restrict__ out, typename Walker::Value inScale, typename Walker::Value outScale) {
     using TV = typename Walker::Value;
-    Walker::run(rows, param, rowExtents, columnIndices, entries, in, [&](int row, TV sum) {
+    Walker::run(source, entries, in, [&](int row, TV sum) {
         out[row] = inScale == TV(0) ? outScale * sum : inScale * out[row] + outScale * sum;
     });
 }

@@ -41,8 +41,7 @@ namespace {
 // Partials: r.r in FP64, r.z in the recurrence precision TR.
 template <class Walker, class TR, bool kStart>
 __global__ __launch_bounds__(Walker::kBlock) void residualKernel(
-    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
-    const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
+    RowSource source, const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
     TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p, double* __restrict__ partials,
     const SolverState* __restrict__ state) {
     constexpr int BS = Walker::kBlock;
@@ -51,7 +50,7 @@ __global__ __launch_bounds__(Walker::kBlock) void residualKernel(
     if (state->done || (!kStart && !state->refine)) return;
     double rr = 0.0;
     TR rz = TR(0);
-    Walker::run(rows, param, rowExtents, columnIndices, entries, x, [&](int row, double sum) {
+    Walker::run(source, entries, x, [&](int row, double sum) {
         const double t = b[row] - sum;
         const TR residual = TR(t);
         const TR z = residual * inverse[row];
@@ -60,32 +59,31 @@ __global__ __launch_bounds__(Walker::kBlock) void residualKernel(
         rr += t * t;
         rz += residual * z;
     });
-    const int chunk = xcdChunk();
+    const int slot = Walker::slot();
     rr = blockSum<BS>(rr, ldsRR);
     rz = blockSum<BS>(rz, ldsRZ);
     if (threadIdx.x == 0) {
-        partials[2 * chunk] = rr;
-        partials[2 * chunk + 1] = double(rz);
+        partials[2 * slot] = rr;
+        partials[2 * slot + 1] = double(rz);
     }
 }
 
 // q = A p, partial p.q (ConjugateGradients_impl.hpp:136-143).
 template <class Walker>
 __global__ __launch_bounds__(Walker::kBlock) void productKernel(
-    int rows, int param, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columnIndices,
-    const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ p,
+    RowSource source, const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ p,
     typename Walker::Value* __restrict__ q, double* __restrict__ partials, const SolverState* __restrict__ state) {
     using T = typename Walker::Value;
     constexpr int BS = Walker::kBlock;
     __shared__ T lds[BS / kWave];
     if (state->done) return;
     T pq = T(0);
-    Walker::run(rows, param, rowExtents, columnIndices, entries, p, [&](int row, T sum) {
+    Walker::run(source, entries, p, [&](int row, T sum) {
         q[row] = sum;
         pq += p[row] * sum;
     });
     pq = blockSum<BS>(pq, lds);
-    if (threadIdx.x == 0) partials[2 * xcdChunk()] = double(pq);
+    if (threadIdx.x == 0) partials[2 * Walker::slot()] = double(pq);
 }
 
 // x += alpha p; r -= alpha q; z = D^-1 r; partials r.r and r.z (:154-163).
@@ -134,27 +132,47 @@ enum Phase : int { kPhaseStart = 0, kPhaseAlpha = 1, kPhaseUpdate = 2, kPhaseUpd
 // Single block: sums `count` block partials in block order and advances the
 // solver state. TR is the precision of the recurrence's dot products; the
 // residual phases carry r.r in FP64.
+constexpr int kFinishBlock = 1024;
+constexpr int kFinishUnroll = 8;  // partial loads in flight per thread
+
 template <class TR, int kPhase>
-__global__ __launch_bounds__(kReduceBlock) void finishKernel(int count, const double* __restrict__ partials,
+__global__ __launch_bounds__(kFinishBlock) void finishKernel(int count, const double* __restrict__ partials,
                                                              SolverState* __restrict__ state) {
-    __shared__ double lds64[kReduceBlock / kWave];
-    __shared__ TR ldsR[kReduceBlock / kWave];
+    __shared__ double lds64[kFinishBlock / kWave];
+    __shared__ TR ldsR[kFinishBlock / kWave];
     SolverState& s = *state;
     if (s.done) return;
     if (kPhase == kPhaseRefine && !s.refine) return;
     constexpr bool kFirst64 = kPhase == kPhaseStart || kPhase == kPhaseRefine;
+    constexpr bool kSecond = kPhase != kPhaseAlpha;
     double first64 = 0.0;
     TR firstR = TR(0), second = TR(0);
-    for (int i = threadIdx.x; i < count; i += kReduceBlock) {
-        if (kFirst64) first64 += partials[2 * i];
-        else firstR += TR(partials[2 * i]);
-        if (kPhase != kPhaseAlpha) second += TR(partials[2 * i + 1]);
+    // Thread i adds partials i, i + kFinishBlock, ... in increasing order.
+    for (int base = threadIdx.x; base < count; base += kFinishBlock * kFinishUnroll) {
+        double a[kFinishUnroll], b[kFinishUnroll];
+#pragma unroll
+        for (int u = 0; u < kFinishUnroll; ++u) {
+            const int i = base + u * kFinishBlock;
+            a[u] = 0.0;
+            b[u] = 0.0;
+            if (i < count) {
+                a[u] = partials[2 * i];
+                if (kSecond) b[u] = partials[2 * i + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFinishUnroll; ++u) {
+            if (base + u * kFinishBlock >= count) break;
+            if (kFirst64) first64 += a[u];
+            else firstR += TR(a[u]);
+            if (kSecond) second += TR(b[u]);
+        }
     }
     double A64 = 0.0;
     TR AR = TR(0), B = TR(0);
-    if (kFirst64) A64 = blockSum<kReduceBlock>(first64, lds64);
-    else AR = blockSum<kReduceBlock>(firstR, ldsR);
-    if (kPhase != kPhaseAlpha) B = blockSum<kReduceBlock>(second, ldsR);
+    if (kFirst64) A64 = blockSum<kFinishBlock>(first64, lds64);
+    else AR = blockSum<kFinishBlock>(firstR, ldsR);
+    if (kSecond) B = blockSum<kFinishBlock>(second, ldsR);
     if (threadIdx.x != 0) return;
     if (kPhase == kPhaseStart) {
         // ConjugateGradients_impl.hpp:93-132
@@ -237,7 +255,7 @@ size_t alignUp(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
 template <class TR, int kPhase>
 void launchFinish(cie_hip_context ctx, int count) {
-    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kReduceBlock), 0, ctx->stream, count,
+    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kFinishBlock), 0, ctx->stream, count,
                        ctx->scratch.partials, ctx->scratch.state);
 }
 
@@ -247,23 +265,26 @@ int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, 
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
     int grid = 0;
+    int status = CIE_HIP_OK;
     const bool ok = withWalker<double, double>(launch, [&](auto walker) {
         using Walker = decltype(walker);
+        RowSource source;
         if (start) {
-            const GridShape shape = gridFor(residualKernel<Walker, TR, true>, launch, m->rows, ctx->computeUnits);
-            grid = shape.grid;
-            hipLaunchKernelGGL((residualKernel<Walker, TR, true>), dim3(shape.grid), dim3(Walker::kBlock), 0,
-                               ctx->stream, int(m->rows), shape.param, m->rowExtents, m->columnIndices, m->entries64,
-                               x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
+            status = prepareLaunch(m, launch, residualKernel<Walker, TR, true>, source, grid);
+            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+            if (status != CIE_HIP_OK) return;
+            hipLaunchKernelGGL((residualKernel<Walker, TR, true>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
+                               source, m->entries64, x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
         } else {
-            const GridShape shape = gridFor(residualKernel<Walker, TR, false>, launch, m->rows, ctx->computeUnits);
-            grid = shape.grid;
-            hipLaunchKernelGGL((residualKernel<Walker, TR, false>), dim3(shape.grid), dim3(Walker::kBlock), 0,
-                               ctx->stream, int(m->rows), shape.param, m->rowExtents, m->columnIndices, m->entries64,
-                               x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
+            status = prepareLaunch(m, launch, residualKernel<Walker, TR, false>, source, grid);
+            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+            if (status != CIE_HIP_OK) return;
+            hipLaunchKernelGGL((residualKernel<Walker, TR, false>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
+                               source, m->entries64, x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
         }
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    if (status != CIE_HIP_OK) return status;
     if (start) launchFinish<TR, kPhaseStart>(ctx, grid);
     else launchFinish<TR, kPhaseRefine>(ctx, grid);
     return CIE_HIP_OK;
@@ -302,17 +323,20 @@ int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, 
                       ProductProfile& profile) {
     cie_hip_context ctx = m->context;
     int grid = 0;
-    const int mark = profile.begin(ctx);
+    int status = CIE_HIP_OK;
     const bool ok = withWalker<T, T>(launch, [&](auto walker) {
         using Walker = decltype(walker);
-        const GridShape shape = gridFor(productKernel<Walker>, launch, m->rows, ctx->computeUnits);
-        grid = shape.grid;
-        hipLaunchKernelGGL((productKernel<Walker>), dim3(shape.grid), dim3(Walker::kBlock), 0, ctx->stream,
-                           int(m->rows), shape.param, m->rowExtents, m->columnIndices, entries, p, q,
-                           ctx->scratch.partials, ctx->scratch.state);
+        RowSource source;
+        status = prepareLaunch(m, launch, productKernel<Walker>, source, grid);
+        if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+        if (status != CIE_HIP_OK) return;
+        const int mark = profile.begin(ctx);
+        hipLaunchKernelGGL((productKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, source, entries,
+                           p, q, ctx->scratch.partials, ctx->scratch.state);
+        profile.end(ctx, mark);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
-    profile.end(ctx, mark);
+    if (status != CIE_HIP_OK) return status;
     launchFinish<T, kPhaseAlpha>(ctx, grid);
     return CIE_HIP_OK;
 }

@@ -36,6 +36,17 @@ def _torch():
     return torch
 
 
+_KERNEL_NAMES = {_lib.SPMV_VECTOR: "vector", _lib.SPMV_STAGED: "staged", _lib.SPMV_TILED: "tiled"}
+
+
+def _kernel_code(name):
+    """C-ABI code of an SpMV kernel name ("vector", "staged", "tiled")."""
+    for code, known in _KERNEL_NAMES.items():
+        if known == name:
+            return code
+    raise ValueError(f"unknown SpMV kernel {name!r}")
+
+
 class Context:
     """One device + one HIP stream (cie_hip_context)."""
 
@@ -106,15 +117,15 @@ class CSRMatrix:
         """(kernel, width, unroll) of the SpMV for `precision` (64 or 32)."""
         kernel, width, unroll = c_int(), c_int(), c_int()
         check(library().cie_hip_csr_get_launch(self.handle, precision, byref(kernel), byref(width), byref(unroll)))
-        return ("staged" if kernel.value == _lib.SPMV_STAGED else "vector"), width.value, unroll.value
+        return _KERNEL_NAMES[kernel.value], width.value, unroll.value
 
     def set_launch(self, precision, kernel, width, unroll):
-        code = _lib.SPMV_STAGED if kernel == "staged" else _lib.SPMV_VECTOR
+        code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
     def autotune(self, precision=64, repetitions=10, kernels=("staged",)):
-        """Time the candidate launches of `kernels` ("staged", "vector") and keep the fastest."""
-        mask = sum(1 << (_lib.SPMV_STAGED if k == "staged" else _lib.SPMV_VECTOR) for k in kernels)
+        """Time the candidate launches of `kernels` ("staged", "vector", "tiled") and keep the fastest."""
+        mask = sum(1 << _kernel_code(k) for k in kernels)
         best = c_double()
         check(library().cie_hip_csr_autotune(self.handle, precision, mask, repetitions, byref(best)),
               "cie_hip_csr_autotune")

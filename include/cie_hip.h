@@ -118,7 +118,7 @@ int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64
  * the sparsity pattern is unchanged. */
 int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 
-/* SpMV launch shape, per value precision (64 or 32). Two kernels:
+/* SpMV launch shape, per value precision (64 or 32). Three kernels:
  *   CIE_HIP_SPMV_STAGED: one wavefront per 64-row tile; coalesced entry
  *     streams, products staged in LDS, each row summed in CSR entry order by
  *     one lane -- rows bit-identical to the sequential CPU loop of
@@ -127,12 +127,18 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
  *   CIE_HIP_SPMV_VECTOR: `width` lanes per row (1..64, power of two) with
  *     strided FMAs and a butterfly sum; unroll 1, 2, 4 or 8. The
  *     SYCLCSROperator layout (SYCLCSROperator.cpp:26-27,71-80) is width = 64.
- * Both are bitwise reproducible run to run for a fixed launch. The default is
+ *   CIE_HIP_SPMV_TILED: one block of width = 256 threads per tile of whole
+ *     rows holding at most 256 * unroll entries (unroll 4, 8 or 16): every
+ *     lane streams entries, products are staged in LDS and each row is
+ *     summed by a power-of-two group of lanes. Tiles are cut once per matrix
+ *     and capacity, on the host, on first use.
+ * All are bitwise reproducible run to run for a fixed launch. The default is
  * STAGED/1024/8 (VECTOR if a row exceeds 8192 entries). cie_hip_csr_autotune
  * times the candidates of the kernels in `kernels` (a mask of 1 << kernel)
  * on the device and keeps the fastest. */
 #define CIE_HIP_SPMV_VECTOR 0
 #define CIE_HIP_SPMV_STAGED 1
+#define CIE_HIP_SPMV_TILED 2
 int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll);
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll);
 int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds);

@@ -83,15 +83,19 @@ def test_fused_pcg_output_scaling_and_warm_start(hip, torch_gpu, in_scale, out_s
     assert rel(x.cpu().numpy(), x_ref) < 1e-9
 
 
+@pytest.mark.parametrize("kernel", ["autotuned", "tiled"])
 @pytest.mark.parametrize("dim,order,n", [(3, 2, 10), (2, 4, 12), (3, 4, 4)])
-def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n):
+def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n, kernel):
     count = n ** dim
     k = np.exp(systems.normal(2, 1.0, count))
     csr = systems.lagrange(dim, order, n, conductivity=k)
     b = systems.uniform(1, csr.rows)
     x_ref, s_ref, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-10)
     matrix = hip.CSRMatrix(csr)
-    matrix.autotune(64, 3)
+    if kernel == "tiled":
+        matrix.set_launch(64, "tiled", 256, 8)
+    else:
+        matrix.autotune(64, 3, kernels=("staged", "vector", "tiled"))
     solver = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
     solver.product(0, to_dev(torch_gpu, b), 1, x)
@@ -100,14 +104,17 @@ def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n):
     assert solver.info.trueRelativeResidual < 2e-10
 
 
-@pytest.mark.parametrize("restart", [False, True])
-def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart):
+@pytest.mark.parametrize("restart,tiled", [(False, False), (True, False), (False, True)])
+def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart, tiled):
     count = 8 ** 3
     k = np.exp(systems.normal(2, 1.0, count))
     csr = systems.lagrange(3, 4, 8, conductivity=k)
     b = systems.uniform(1, csr.rows)
     x_ref, _, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-14, max_iterations=20000)
     matrix = hip.CSRMatrix(csr)
+    if tiled:
+        matrix.set_launch(64, "tiled", 256, 16)
+        matrix.set_launch(32, "tiled", 256, 8)
     solver = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), restart=restart)
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
     solver.product(0, to_dev(torch_gpu, b), 1, x)
@@ -160,8 +167,8 @@ def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu):
     solution agrees with the FP64 one to the conditioning-limited level."""
     w = systems.q2_laplace_3d(64)
     matrix = hip.CSRMatrix(w.matrix)
-    matrix.autotune(64, 3)
-    matrix.autotune(32, 3)
+    matrix.autotune(64, 3, kernels=("staged", "vector", "tiled"))
+    matrix.autotune(32, 3, kernels=("staged", "vector", "tiled"))
     b = to_dev(torch_gpu, w.rhs)
     fp64 = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
     x64 = torch_gpu.zeros_like(b)

@@ -55,7 +55,7 @@ SCALES = [(0, 1), (0, -1), (0, 2.5), (1, 1), (1, -1), (1, 0.5), (-1, 1), (-1, 3.
 
 LAUNCHES = [("staged", 512, 4), ("staged", 512, 8), ("staged", 1024, 4), ("staged", 1024, 8), ("vector", 1, 1),
             ("vector", 2, 4), ("vector", 4, 2), ("vector", 8, 8), ("vector", 16, 4), ("vector", 32, 1),
-            ("vector", 64, 2)]
+            ("vector", 64, 2), ("tiled", 256, 4), ("tiled", 256, 8), ("tiled", 256, 16)]
 
 
 @pytest.mark.parametrize("kernel,width,unroll", LAUNCHES)
@@ -84,6 +84,39 @@ def test_spmv_random_against_oracle(hip, torch_gpu, kernel, width, unroll):
         assert np.all(np.abs(y32.cpu().numpy() - expected) <= bound32)
 
 
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll):
+    """Block tiles: rows longer than a tile (summed in passes), tiles of exactly 256 short rows, empty rows."""
+    rng = np.random.default_rng(unroll)
+    rows, cols = 2000, 16000
+    lengths = rng.integers(1, 4, rows)
+    lengths[rng.choice(rows, 40, replace=False)] = 0
+    lengths[[0, 700, 701, 1999]] = [5000, 256 * unroll + 1, 3 * 256 * unroll, 4000]
+    ext = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    columns = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lengths]).astype(np.int32)
+    csr = systems.CSR(rows, cols, ext, columns, rng.standard_normal(int(ext[-1])))
+    matrix = hip.CSRMatrix(csr)
+    matrix.set_launch(64, "tiled", 256, unroll)
+    matrix.set_launch(32, "tiled", 256, unroll)
+    assert matrix.launch(64) == ("tiled", 256, unroll)
+    x = rng.standard_normal(cols)
+    y0 = rng.standard_normal(rows)
+    for in_scale, out_scale in [(0, 1), (1, -1), (0.5, 2.0)]:
+        expected = oracle.csr_product(csr, in_scale, x, out_scale, y0.copy())
+        space = hip.HIPSpace(torch_gpu.float64)
+        results = []
+        for _ in range(2):
+            y = to_dev(torch_gpu, y0)
+            hip.HIPCSROperator(matrix, space).product(in_scale, to_dev(torch_gpu, x), out_scale, y)
+            results.append(y.cpu().numpy())
+        assert np.array_equal(results[0], results[1])  # bitwise run to run
+        assert np.all(np.abs(results[0] - expected) <= spmv_bound(csr, x, y0, in_scale, out_scale, EPS64))
+        space32 = hip.HIPSpace(torch_gpu.float32)
+        y32 = to_dev(torch_gpu, y0, torch_gpu.float32)
+        hip.HIPCSROperator(matrix, space32).product(in_scale, to_dev(torch_gpu, x, torch_gpu.float32), out_scale, y32)
+        assert np.all(np.abs(y32.cpu().numpy() - expected) <= 4 * spmv_bound(csr, x, y0, in_scale, out_scale, EPS32))
+
+
 def test_spmv_deterministic_and_autotuned(hip, torch_gpu):
     w = systems.q2_laplace_3d(16)
     matrix = hip.CSRMatrix(w.matrix)
@@ -109,6 +142,15 @@ def test_spmv_deterministic_and_autotuned(hip, torch_gpu):
     op.product(0, x, 1, second)
     assert torch_gpu.equal(first, second)
     bound = spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64)
+    assert np.all(np.abs(first.cpu().numpy() - expected) <= bound)
+    # block tiles: the same
+    matrix.autotune(64, repetitions=3, kernels=("tiled",))
+    assert matrix.launch(64)[0] == "tiled"
+    first = space.makeVector(w.matrix.rows)
+    op.product(0, x, 1, first)
+    second = space.makeVector(w.matrix.rows)
+    op.product(0, x, 1, second)
+    assert torch_gpu.equal(first, second)
     assert np.all(np.abs(first.cpu().numpy() - expected) <= bound)
 
 
