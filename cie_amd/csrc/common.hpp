@@ -13,7 +13,8 @@ namespace ciehip {
 
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kReduceBlock = 256;  // threads per block of the vector/reduction kernels
-constexpr int kMaxPartials = 4096; // block partials one grid reduction may produce
+constexpr int kMaxPartials = 4096; // partial slots allocated with a context (grown on demand)
+constexpr int kLevelBlocks = 256;  // first-level blocks when a reduction has more than kMaxPartials partials
 constexpr int kXcds = 8;           // MI355X accelerator complex dies (blocks are dealt round-robin)
 
 // Error state: the C ABI never throws; each failure sets a thread-local message.
@@ -64,6 +65,7 @@ enum SolveEnd : int32_t {
 struct Scratch {
     double* partials = nullptr;   // partialSlots * 4 doubles (grown on demand)
     int partialSlots = 0;
+    double* level = nullptr;      // kLevelBlocks * 2 doubles: first level of a large partial reduction
     double* scalars = nullptr;    // 64 doubles of device scalar results
     double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
     SolverState* state = nullptr; // one solver state

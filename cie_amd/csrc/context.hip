@@ -88,6 +88,7 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     auto& s = ctx->scratch;
     s.partialSlots = ciehip::kMaxPartials;
     bool ok = hipMalloc(&s.partials, sizeof(double) * ciehip::kMaxPartials * 4) == hipSuccess
+           && hipMalloc(&s.level, sizeof(double) * ciehip::kLevelBlocks * 2) == hipSuccess
            && hipMalloc(&s.scalars, sizeof(double) * 64) == hipSuccess
            && hipMalloc(&s.state, sizeof(ciehip::SolverState)) == hipSuccess
            && hipHostMalloc(&s.pinned, sizeof(double) * 64) == hipSuccess
@@ -106,6 +107,7 @@ int cie_hip_context_destroy(cie_hip_context context) {
     if (context->stream) hipStreamSynchronize(context->stream);
     auto& s = context->scratch;
     if (s.partials) hipFree(s.partials);
+    if (s.level) hipFree(s.level);
     if (s.scalars) hipFree(s.scalars);
     if (s.state) hipFree(s.state);
     if (s.pinned) hipHostFree(s.pinned);

@@ -112,10 +112,13 @@ int residentBlocks(const void* kernel, int block);
 // one resident round of blocks: compute units x blocks per unit (rounded down
 // to a multiple of 8 so each XCD gets a contiguous run), but no more blocks
 // than there is work for; param is each block's share. The tiled walker gets
-// one block per tile (the tiles are cut on first use). A grid reduction over
-// the launch writes one partial per block (Walker::slot()).
+// one block per tile (the tiles are cut on first use), or one resident round
+// with `residentRound` (for launches that usually exit at once, where tens of
+// thousands of empty blocks would cost more than the loop). A grid reduction
+// over the launch writes one partial per block (Walker::slot()).
 template <class Kernel>
-int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSource& source, int& grid) {
+int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSource& source, int& grid,
+                  bool residentRound = false) {
     const int perUnit = residentBlocks(reinterpret_cast<const void*>(kernel), launch.block);
     int cap = std::max(1, m->context->computeUnits) * perUnit;
     if (cap >= kXcds) cap -= cap % kXcds;
@@ -134,7 +137,7 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
         source.tiles = set->tiles;
         source.tileCount = set->count;
         source.param = 1;
-        grid = std::max(1, set->count);  // one block per tile
+        grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
     } else if (launch.kernel == CIE_HIP_SPMV_STAGED) {
         const int64_t tiles = std::max<int64_t>(1, (rows + kWave - 1) / kWave);
         const int64_t wavesPerBlock = launch.block / kWave;

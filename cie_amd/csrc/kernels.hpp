@@ -293,7 +293,8 @@ struct StagedRows {
 // lane g of a group adds products g, g + G, ... in order, then an xor butterfly
 // combines the group. The order depends only on the tile cut, so results are
 // bitwise reproducible. A row longer than a tile is a tile of its own, summed
-// in passes of BS * U.
+// in passes of BS * U. With fewer blocks than tiles (a launch sized to one
+// resident round), block b handles tiles b, b + grid, ... in turn.
 template <class TV, class TM, int U, int BS>
 struct BlockTiles {
     using Value = TV;
@@ -309,9 +310,17 @@ struct BlockTiles {
                                                const TV* __restrict__ in, Epilogue&& epilogue) {
         __shared__ TV stage[kCapacity];
         __shared__ int32_t extents[BS + 1];
+        for (int t = blockIdx.x; t < source.tileCount; t += gridDim.x) {
+            tile(t, source, entries, in, epilogue, stage, extents);
+            if (t + int(gridDim.x) < source.tileCount) __syncthreads();  // stage and extents are reused
+        }
+    }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void tile(int t, const RowSource& source, const TM* __restrict__ entries,
+                                                const TV* __restrict__ in, Epilogue& epilogue, TV* stage,
+                                                int32_t* extents) {
         const int32_t* __restrict__ columnIndices = source.columnIndices;
-        const int t = blockIdx.x;
-        if (t >= source.tileCount) return;
         const int tid = threadIdx.x;
         const int2 head = source.tiles[t];
         const int2 next = source.tiles[t + 1];

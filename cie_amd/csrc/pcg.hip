@@ -135,6 +135,35 @@ enum Phase : int { kPhaseStart = 0, kPhaseAlpha = 1, kPhaseUpdate = 2, kPhaseUpd
 constexpr int kFinishBlock = 1024;
 constexpr int kFinishUnroll = 8;  // partial loads in flight per thread
 
+// First level of a reduction over more than kMaxPartials partials (one per
+// tile of a tiled SpMV): block b sums slots [b * per, min(count, b * per + per))
+// in a fixed order, both values of a slot, in FP64. One compute unit cannot
+// read tens of thousands of partials quickly; kLevelBlocks of them can.
+template <int kPhase>
+__global__ __launch_bounds__(kReduceBlock) void partialLevelKernel(int count, int per,
+                                                                   const double* __restrict__ partials,
+                                                                   double* __restrict__ level,
+                                                                   const SolverState* __restrict__ state) {
+    __shared__ double ldsA[kReduceBlock / kWave];
+    __shared__ double ldsB[kReduceBlock / kWave];
+    if (state->done) return;
+    if (kPhase == kPhaseRefine && !state->refine) return;
+    constexpr bool kSecond = kPhase != kPhaseAlpha;
+    const int begin = int(blockIdx.x) * per;
+    const int end = min(count, begin + per);
+    double a = 0.0, b = 0.0;
+    for (int i = begin + int(threadIdx.x); i < end; i += kReduceBlock) {
+        a += partials[2 * i];
+        if (kSecond) b += partials[2 * i + 1];
+    }
+    a = blockSum<kReduceBlock>(a, ldsA);
+    if (kSecond) b = blockSum<kReduceBlock>(b, ldsB);
+    if (threadIdx.x == 0) {
+        level[2 * blockIdx.x] = a;
+        level[2 * blockIdx.x + 1] = b;
+    }
+}
+
 template <class TR, int kPhase>
 __global__ __launch_bounds__(kFinishBlock) void finishKernel(int count, const double* __restrict__ partials,
                                                              SolverState* __restrict__ state) {
@@ -255,8 +284,17 @@ size_t alignUp(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
 template <class TR, int kPhase>
 void launchFinish(cie_hip_context ctx, int count) {
-    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kFinishBlock), 0, ctx->stream, count,
-                       ctx->scratch.partials, ctx->scratch.state);
+    const double* partials = ctx->scratch.partials;
+    if (count > kMaxPartials) {
+        const int per = (count + kLevelBlocks - 1) / kLevelBlocks;
+        const int blocks = (count + per - 1) / per;
+        hipLaunchKernelGGL((partialLevelKernel<kPhase>), dim3(blocks), dim3(kReduceBlock), 0, ctx->stream, count, per,
+                           ctx->scratch.partials, ctx->scratch.level, ctx->scratch.state);
+        partials = ctx->scratch.level;
+        count = blocks;
+    }
+    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kFinishBlock), 0, ctx->stream, count, partials,
+                       ctx->scratch.state);
 }
 
 // r = b - A x and its dot products, then the start or refinement finish.
@@ -276,7 +314,7 @@ int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, 
             hipLaunchKernelGGL((residualKernel<Walker, TR, true>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
                                source, m->entries64, x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
         } else {
-            status = prepareLaunch(m, launch, residualKernel<Walker, TR, false>, source, grid);
+            status = prepareLaunch(m, launch, residualKernel<Walker, TR, false>, source, grid, true);
             if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
             if (status != CIE_HIP_OK) return;
             hipLaunchKernelGGL((residualKernel<Walker, TR, false>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
