@@ -30,9 +30,15 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 
 
-def spmv_bytes(rows, nnz, value_bytes):
-    """Algorithmic bytes of one SpMV y = A x: entries + columns, row extents, x read once, y written once."""
-    return (value_bytes + 4) * nnz + 4 * (rows + 1) + 2 * value_bytes * rows
+def spmv_bytes(rows, nnz, value_bytes, index_bytes=4):
+    """Algorithmic bytes of one SpMV y = A x: entries + columns, row extents, x read once, y written once.
+    index_bytes is 4 for int32 columns, 2 when the tiled kernel reads the narrow 16-bit column plane."""
+    return (value_bytes + index_bytes) * nnz + 4 * (rows + 1) + 2 * value_bytes * rows
+
+
+def index_bytes(matrix, precision):
+    """Bytes per column index the selected SpMV launch reads."""
+    return matrix.column_bytes() if matrix.launch(precision)[0] == "tiled" else 4
 
 
 def time_spmv(torch, linalg, matrix, precision, repetitions=50):
@@ -164,8 +170,12 @@ def main():
 
     spmv64 = time_spmv(torch, linalg, matrix, 64)
     spmv32 = time_spmv(torch, linalg, matrix, 32)
-    bytes64 = spmv_bytes(rows, nnz, 8)
-    bytes32 = spmv_bytes(rows, nnz, 4)
+    # Bytes of the format each launch reads (narrow column plane: 2 B per index), and of plain
+    # int32 CSR (the effective rate an int32-index kernel would need for the same time).
+    bytes64 = spmv_bytes(rows, nnz, 8, index_bytes(matrix, 64))
+    bytes32 = spmv_bytes(rows, nnz, 4, index_bytes(matrix, 32))
+    csr64 = spmv_bytes(rows, nnz, 8)
+    csr32 = spmv_bytes(rows, nnz, 4)
 
     m = mixed_infos[-1]
     f = fp64_infos[-1]
@@ -205,14 +215,18 @@ def main():
         "speedup_mixed_vs_fp64": fp64_wall / mixed_wall,
         "spmv": {
             "fp64_seconds": spmv64, "fp64_GBps": bytes64 / spmv64 / 1e9, "fp64_bytes": bytes64,
+            "fp64_int32_csr_GBps": csr64 / spmv64 / 1e9,
             "fp32_seconds": spmv32, "fp32_GBps": bytes32 / spmv32 / 1e9, "fp32_bytes": bytes32,
+            "fp32_int32_csr_GBps": csr32 / spmv32 / 1e9,
             "launch_fp64": list(matrix.launch(64)), "launch_fp32": list(matrix.launch(32)),
+            "index_bytes_fp64": index_bytes(matrix, 64), "index_bytes_fp32": index_bytes(matrix, 32),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(matrix.launch(32)),
             "kernel": f"productKernel {list(matrix.launch(32))} (q = A p, FP32 plane, fused p.q) inside the mixed solve",
             "bytes_per_launch": bytes32, "mean_launch_seconds": product_seconds,
+            "int32_csr_bytes_per_launch": csr32, "int32_csr_equivalent_GBps": csr32 / product_seconds / 1e9,
         },
     }
     if args.cpu_iterations > 0:

@@ -121,6 +121,7 @@ _SIGNATURES = {
     "cie_hip_csr_destroy": (c_int, [_P]),
     "cie_hip_csr_shape": (c_int, [_P, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
     "cie_hip_csr_update_entries": (c_int, [_P, _P]),
+    "cie_hip_csr_column_bytes": (c_int, [_P, POINTER(c_int)]),
     "cie_hip_csr_set_launch": (c_int, [_P, c_int, c_int, c_int, c_int]),
     "cie_hip_csr_get_launch": (c_int, [_P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "cie_hip_csr_autotune": (c_int, [_P, c_int, c_int, c_int, POINTER(c_double)]),

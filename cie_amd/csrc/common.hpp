@@ -118,6 +118,10 @@ struct cie_hip_csr_s {
     size_t workspaceBytes = 0;
     std::vector<int32_t> hostExtents;  // host copy of rowExtents (tile cuts are made on the host)
     std::vector<TileSet> tileSets;
+    int narrowState = 0;               // narrow column plane: 0 not built yet, 1 built, -1 does not fit
+    int narrowRuns = 0;
+    uint16_t* narrowColumns = nullptr;
+    int32_t* narrowBases = nullptr;
 };
 
 namespace ciehip {
@@ -127,6 +131,9 @@ int matrixWorkspace(cie_hip_csr matrix, size_t bytes, void** pointer);
 int ensureInverseDiagonal(cie_hip_csr matrix, int precision);
 // The block tiles of `capacity` entries (built and uploaded on first use).
 int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
+// Builds the narrow (16-bit) column plane on first use if every run of
+// entries fits; matrix->narrowState tells whether it did.
+int ensureNarrowColumns(cie_hip_csr matrix);
 // Grows the context's partials to at least `slots` slots of 4 doubles. A
 // growth waits for the stream first (queued kernels may use the old buffer).
 int ensurePartials(cie_hip_context context, int slots);

@@ -12,6 +12,49 @@ using namespace ciehip;
 
 namespace {
 
+// Smallest column of each narrow run (one block per run) and the largest
+// column span of any run (an atomic max of integers: order-independent).
+__global__ __launch_bounds__(kReduceBlock) void narrowBoundsKernel(int64_t nnz, const int32_t* __restrict__ columns,
+                                                                   int32_t* __restrict__ bases,
+                                                                   unsigned* __restrict__ maxSpan) {
+    __shared__ int ldsLow[kReduceBlock / kWave];
+    __shared__ int ldsHigh[kReduceBlock / kWave];
+    const int64_t begin = int64_t(blockIdx.x) << kNarrowShift;
+    const int64_t end = min(nnz, begin + kNarrowRun);
+    int low = INT32_MAX, high = INT32_MIN;
+    for (int64_t k = begin + threadIdx.x; k < end; k += kReduceBlock) {
+        const int c = columns[k];
+        low = min(low, c);
+        high = max(high, c);
+    }
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset /= 2) {
+        low = min(low, __shfl_xor(low, offset, kWave));
+        high = max(high, __shfl_xor(high, offset, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        ldsLow[threadIdx.x / kWave] = low;
+        ldsHigh[threadIdx.x / kWave] = high;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kReduceBlock / kWave; ++w) {
+            low = min(low, ldsLow[w]);
+            high = max(high, ldsHigh[w]);
+        }
+        bases[blockIdx.x] = low;
+        atomicMax(maxSpan, unsigned(high - low));
+    }
+}
+
+__global__ __launch_bounds__(kReduceBlock) void narrowColumnsKernel(int64_t nnz, const int32_t* __restrict__ columns,
+                                                                    const int32_t* __restrict__ bases,
+                                                                    uint16_t* __restrict__ narrow) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride)
+        narrow[k] = uint16_t(columns[k] - bases[k >> kNarrowShift]);
+}
+
 template <class T>
 int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
                   const T* entries) {
@@ -50,6 +93,8 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->workspace) hipFree(m->workspace);
     for (TileSet& set : m->tileSets)
         if (set.tiles) hipFree(set.tiles);
+    if (m->narrowColumns) hipFree(m->narrowColumns);
+    if (m->narrowBases) hipFree(m->narrowBases);
     delete m;
 }
 
@@ -163,6 +208,59 @@ int ensureTiles(cie_hip_csr m, int capacity, const TileSet** tiles) {
     }
     m->tileSets.push_back(set);
     *tiles = &m->tileSets.back();
+    return CIE_HIP_OK;
+}
+
+int ensureNarrowColumns(cie_hip_csr m) {
+    if (m->narrowState != 0) return CIE_HIP_OK;
+#ifdef CIEHIP_NO_NARROW_COLUMNS
+    m->narrowState = -1;  // build option: always read the int32 columns
+    return CIE_HIP_OK;
+#endif
+    if (!m->nnz) {
+        m->narrowState = -1;
+        return CIE_HIP_OK;
+    }
+    cie_hip_context ctx = m->context;
+    const int64_t runs = (m->nnz + kNarrowRun - 1) / kNarrowRun;
+    int32_t* bases = nullptr;
+    uint16_t* narrow = nullptr;
+    CIE_HIP_CHECK(hipMalloc(&bases, sizeof(int32_t) * runs));
+    unsigned* span = reinterpret_cast<unsigned*>(ctx->scratch.scalars + 8);
+    unsigned* pinned = reinterpret_cast<unsigned*>(ctx->scratch.pinned + 8);
+    bool fits = false;
+    hipError_t error = hipMemsetAsync(span, 0, sizeof(unsigned), ctx->stream);
+    if (error == hipSuccess) {
+        hipLaunchKernelGGL(narrowBoundsKernel, dim3(unsigned(runs)), dim3(kReduceBlock), 0, ctx->stream, m->nnz,
+                           m->columnIndices, bases, span);
+        error = hipGetLastError();
+    }
+    if (error == hipSuccess) error = hipMemcpyAsync(pinned, span, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream);
+    if (error == hipSuccess) error = hipStreamSynchronize(ctx->stream);
+    if (error == hipSuccess) fits = *pinned <= 0xffffu;
+    if (error == hipSuccess && fits) {
+        error = hipMalloc(&narrow, sizeof(uint16_t) * size_t(m->nnz));
+        if (error == hipSuccess) {
+            hipLaunchKernelGGL(narrowColumnsKernel, dim3(vectorGrid(m->nnz)), dim3(kReduceBlock), 0, ctx->stream,
+                               m->nnz, m->columnIndices, bases, narrow);
+            error = hipGetLastError();
+        }
+        if (error == hipSuccess) error = hipStreamSynchronize(ctx->stream);
+    }
+    if (error != hipSuccess) {
+        (void)hipFree(bases);
+        if (narrow) (void)hipFree(narrow);
+        return setError(CIE_HIP_ERROR_HIP, std::string("narrow column plane: ") + hipGetErrorString(error));
+    }
+    if (!fits) {
+        (void)hipFree(bases);
+        m->narrowState = -1;
+        return CIE_HIP_OK;
+    }
+    m->narrowBases = bases;
+    m->narrowColumns = narrow;
+    m->narrowRuns = int(runs);
+    m->narrowState = 1;
     return CIE_HIP_OK;
 }
 
@@ -326,6 +424,15 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     launch.width = width;
     launch.unroll = unroll;
     launch.block = kSpmvBlock;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_column_bytes(cie_hip_csr matrix, int* bytes) {
+    if (!matrix || !bytes) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_column_bytes: null argument");
+    CIE_HIP_CHECK(hipSetDevice(matrix->context->device));
+    const int status = ensureNarrowColumns(matrix);
+    if (status != CIE_HIP_OK) return status;
+    *bytes = matrix->narrowState > 0 ? int(sizeof(uint16_t)) : int(sizeof(int32_t));
     return CIE_HIP_OK;
 }
 

@@ -128,7 +128,9 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     source.columnIndices = m->columnIndices;
     source.tiles = nullptr;
     source.tileCount = 0;
-    source.pad = 0;
+    source.narrowRuns = 0;
+    source.narrowColumns = nullptr;
+    source.narrowBases = nullptr;
     const int64_t rows = m->rows;
     if (launch.kernel == CIE_HIP_SPMV_TILED) {
         const TileSet* set = nullptr;
@@ -136,6 +138,13 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
         if (status != CIE_HIP_OK) return status;
         source.tiles = set->tiles;
         source.tileCount = set->count;
+        const int narrow = ensureNarrowColumns(m);
+        if (narrow != CIE_HIP_OK) return narrow;
+        if (m->narrowState > 0) {
+            source.narrowRuns = m->narrowRuns;
+            source.narrowColumns = m->narrowColumns;
+            source.narrowBases = m->narrowBases;
+        }
         source.param = 1;
         grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
     } else if (launch.kernel == CIE_HIP_SPMV_STAGED) {

@@ -16,6 +16,14 @@
 
 namespace ciehip {
 
+// Narrow column plane: the columns of each run of kNarrowRun consecutive
+// entries are stored as 16-bit offsets from the run's smallest column, when
+// every run's columns span at most 65535 (banded FEM matrices in their natural
+// node order do: the 3D Q2 64^3 matrix spans at most 65,160 per run). A block
+// tile of at most kNarrowRun entries touches at most two runs.
+constexpr int kNarrowShift = 12;
+constexpr int kNarrowRun = 1 << kNarrowShift;
+
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
@@ -134,7 +142,9 @@ struct RowSource {
     const int32_t* __restrict__ columnIndices;
     const int2* __restrict__ tiles;
     int tileCount;
-    int pad;
+    int narrowRuns;                            // runs of the narrow column plane (0: none)
+    const uint16_t* __restrict__ narrowColumns;  // column - narrowBases[entry >> kNarrowShift]
+    const int32_t* __restrict__ narrowBases;     // smallest column of each run of kNarrowRun entries
 };
 
 // L lanes per row, BS / L rows per block step; rows split into one contiguous
@@ -301,6 +311,7 @@ struct BlockTiles {
     using Matrix = TM;
     static constexpr int kBlock = BS;
     static constexpr int kCapacity = BS * U;
+    static_assert(kCapacity <= kNarrowRun, "a tile must touch at most two narrow runs");
 
     // Index of this block's partial in a grid reduction.
     __device__ __forceinline__ static int slot() { return blockIdx.x; }
@@ -330,16 +341,48 @@ struct BlockTiles {
             if (tid == 0) epilogue(first, sum);
             return;
         }
+        // Every load is issued unconditionally (indices clamped into the tile,
+        // so lanes past its end re-read its last entry) and masked afterwards:
+        // a load consumed inside a per-entry branch would make the compiler wait
+        // for it there, one entry at a time.
         int column[U];
         TM entry[U];
+        const int last = max(begin, end - 1);  // an empty tile loads nothing below
+        if (end > begin && source.narrowRuns) {
+            const int run = begin >> kNarrowShift;
+            const int base0 = source.narrowBases[run];
+            const int base1 = source.narrowBases[min(run + 1, source.narrowRuns - 1)];
+            uint32_t offset[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = begin + tid + BS * u;
-            column[u] = 0;  // a tile holds at least one row; gathering in[0] is harmless
-            entry[u] = TM(0);
-            if (k < end) {
+            for (int u = 0; u < U; ++u) {
+                const int k = min(begin + tid + BS * u, last);
+                offset[u] = CIEHIP_STREAM_LOAD(source.narrowColumns + k);
+                entry[u] = CIEHIP_STREAM_LOAD(entries + k);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = begin + tid + BS * u;
+                const bool live = k < end;
+                column[u] = live ? ((k >> kNarrowShift) == run ? base0 : base1) + int(offset[u]) : base0;
+                entry[u] = live ? entry[u] : TM(0);
+            }
+        } else if (end > begin) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(begin + tid + BS * u, last);
                 column[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
                 entry[u] = CIEHIP_STREAM_LOAD(entries + k);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool live = begin + tid + BS * u < end;
+                entry[u] = live ? entry[u] : TM(0);  // the re-read column is a real one: harmless to gather
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                column[u] = 0;
+                entry[u] = TM(0);
             }
         }
         if (tid < rows) extents[tid] = source.rowExtents[first + tid] - begin;
@@ -371,12 +414,21 @@ struct BlockTiles {
         const int tid = threadIdx.x;
         TV total = TV(0);
         for (int base = begin; base < end; base += kCapacity) {
+            int column[U];
+            TM entry[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {  // clamped, unconditional loads (see tile())
+                const int k = min(base + tid + BS * u, end - 1);
+                column[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
+                entry[u] = CIEHIP_STREAM_LOAD(entries + k);
+            }
+            TV gathered[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) gathered[u] = in[column[u]];
             TV part = TV(0);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int k = base + tid + BS * u;
-                if (k < end) part += TV(CIEHIP_STREAM_LOAD(entries + k)) * in[CIEHIP_STREAM_LOAD(columnIndices + k)];
-            }
+            for (int u = 0; u < U; ++u)
+                if (base + tid + BS * u < end) part += TV(entry[u]) * gathered[u];
             part = groupSum<kWave>(part);
             if ((tid & (kWave - 1)) == 0) stage[tid / kWave] = part;
             __syncthreads();

@@ -119,6 +119,12 @@ class CSRMatrix:
         check(library().cie_hip_csr_get_launch(self.handle, precision, byref(kernel), byref(width), byref(unroll)))
         return _KERNEL_NAMES[kernel.value], width.value, unroll.value
 
+    def column_bytes(self):
+        """Bytes per column index the tiled SpMV reads: 2 (narrow 16-bit plane) or 4."""
+        value = c_int()
+        check(library().cie_hip_csr_column_bytes(self.handle, byref(value)), "cie_hip_csr_column_bytes")
+        return value.value
+
     def set_launch(self, precision, kernel, width, unroll):
         code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")

@@ -139,6 +139,13 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 #define CIE_HIP_SPMV_VECTOR 0
 #define CIE_HIP_SPMV_STAGED 1
 #define CIE_HIP_SPMV_TILED 2
+/* The tiled kernel reads a narrow copy of the column indices when every run of
+ * 4096 consecutive entries spans at most 65535 columns (banded FEM matrices in
+ * their natural node order do): 16-bit offsets from each run's smallest
+ * column, built on the device on first use. The int32 columns stay the
+ * matrix's indices for every other kernel. *bytes receives the bytes per
+ * column index the tiled kernel reads: 2 (narrow copy) or 4. */
+int cie_hip_csr_column_bytes(cie_hip_csr matrix, int* bytes);
 int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll);
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll);
 int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds);

@@ -85,10 +85,12 @@ def test_spmv_random_against_oracle(hip, torch_gpu, kernel, width, unroll):
 
 
 @pytest.mark.parametrize("unroll", [4, 8, 16])
-def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll):
-    """Block tiles: rows longer than a tile (summed in passes), tiles of exactly 256 short rows, empty rows."""
+@pytest.mark.parametrize("cols", [16000, 200000])
+def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll, cols):
+    """Block tiles: rows longer than a tile (summed in passes), tiles of exactly 256 short rows, empty rows;
+    with 16000 columns the tiled kernel reads the narrow 16-bit column plane, with 200000 the int32 one."""
     rng = np.random.default_rng(unroll)
-    rows, cols = 2000, 16000
+    rows = 2000
     lengths = rng.integers(1, 4, rows)
     lengths[rng.choice(rows, 40, replace=False)] = 0
     lengths[[0, 700, 701, 1999]] = [5000, 256 * unroll + 1, 3 * 256 * unroll, 4000]
@@ -99,6 +101,7 @@ def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll):
     matrix.set_launch(64, "tiled", 256, unroll)
     matrix.set_launch(32, "tiled", 256, unroll)
     assert matrix.launch(64) == ("tiled", 256, unroll)
+    assert matrix.column_bytes() == (2 if cols < 65536 else 4)
     x = rng.standard_normal(cols)
     y0 = rng.standard_normal(rows)
     for in_scale, out_scale in [(0, 1), (1, -1), (0.5, 2.0)]:
@@ -171,6 +174,14 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
     hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
     rel = np.linalg.norm(y32.cpu().numpy() - expected) / np.linalg.norm(expected)
     assert rel < 1e-5
+    # block tiles over the narrow 16-bit column plane (every 4096-entry run spans <= 65,160 columns)
+    assert matrix.column_bytes() == 2
+    bound = spmv_bound(w.matrix, w.rhs, np.zeros(w.matrix.rows), 0, 1, EPS64)
+    for unroll in (8, 16):
+        matrix.set_launch(64, "tiled", 256, unroll)
+        y = space.makeVector(w.matrix.rows)
+        hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
+        assert np.all(np.abs(y.cpu().numpy() - expected) <= bound)
 
 
 def test_spmv_shape_error(hip, torch_gpu):
