@@ -14,7 +14,8 @@ namespace ciehip {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kReduceBlock = 256;  // threads per block of the vector/reduction kernels
 constexpr int kMaxPartials = 4096; // partial slots allocated with a context (grown on demand)
-constexpr int kLevelBlocks = 256;  // first-level blocks when a reduction has more than kMaxPartials partials
+constexpr int kLevelBlocks = 256;  // a reduction over more partials than this gets a first-level sum
+constexpr int kVectorPartials = 512;  // partial slots of the solver's update kernel
 constexpr int kXcds = 8;           // MI355X accelerator complex dies (blocks are dealt round-robin)
 
 // Error state: the C ABI never throws; each failure sets a thread-local message.
@@ -33,7 +34,7 @@ int setError(int status, const std::string& message);
 // Device-side scalar state of one solve. Lives in device memory; kernels read
 // and update it so that a whole PCG iteration needs no host round trip.
 struct SolverState {
-    double rho;            // r^T z of the current iterate
+    double rhos[2];        // r^T z of the current iterate, in the slot of the iteration's parity
     double alpha;          // search scale p^T r / p^T A p
     double beta;           // direction update scale
     double rr;             // r^T r
@@ -48,9 +49,8 @@ struct SolverState {
     uint64_t maxIterations;
     uint64_t refinements;  // mixed: FP64 refinement steps taken
     int32_t done;          // 0 running, else a SolveEnd value
-    int32_t refine;        // mixed: 1 if the next step recomputes the true residual
+    int32_t refined[2];    // mixed: 1 if the trip of that parity recomputed the true residual
     int32_t restart;       // mixed: 1 resets the search direction at each refinement
-    int32_t pad;
 };
 
 enum SolveEnd : int32_t {
@@ -66,6 +66,7 @@ struct Scratch {
     double* partials = nullptr;   // partialSlots * 4 doubles (grown on demand)
     int partialSlots = 0;
     double* level = nullptr;      // kLevelBlocks * 2 doubles: first level of a large partial reduction
+    double* vectorPartials = nullptr;  // kVectorPartials * 2 doubles: partials of the solver's vector kernels
     double* scalars = nullptr;    // 64 doubles of device scalar results
     double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
     SolverState* state = nullptr; // one solver state

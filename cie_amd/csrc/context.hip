@@ -89,6 +89,7 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     s.partialSlots = ciehip::kMaxPartials;
     bool ok = hipMalloc(&s.partials, sizeof(double) * ciehip::kMaxPartials * 4) == hipSuccess
            && hipMalloc(&s.level, sizeof(double) * ciehip::kLevelBlocks * 2) == hipSuccess
+           && hipMalloc(&s.vectorPartials, sizeof(double) * ciehip::kVectorPartials * 2) == hipSuccess
            && hipMalloc(&s.scalars, sizeof(double) * 64) == hipSuccess
            && hipMalloc(&s.state, sizeof(ciehip::SolverState)) == hipSuccess
            && hipHostMalloc(&s.pinned, sizeof(double) * 64) == hipSuccess
@@ -108,6 +109,7 @@ int cie_hip_context_destroy(cie_hip_context context) {
     auto& s = context->scratch;
     if (s.partials) hipFree(s.partials);
     if (s.level) hipFree(s.level);
+    if (s.vectorPartials) hipFree(s.vectorPartials);
     if (s.scalars) hipFree(s.scalars);
     if (s.state) hipFree(s.state);
     if (s.pinned) hipHostFree(s.pinned);

@@ -7,12 +7,20 @@
 //   loop: q = A p; alpha = rho / p.q; x += alpha p; r -= alpha q; rr = r.r;
 //         z = D^-1 r; rho' = r.z; beta = rho'/rho; p = beta p + z;
 //         abs = sqrt(rr), rel = abs / ||r0||; stop on Comparison::less.
-// Each loop trip is three streaming kernels, each followed (where it reduces)
-// by a one-block finish kernel that keeps the scalars on the device:
-//   product:   q = A p and p.q            -> finish: alpha, p.q <= 0 check
-//   update:    x, r updates, z, r.r, r.z  -> finish: beta, convergence test
-//   direction: p = beta p + D^-1 r
-// The host only polls the device state once per chunk of iterations.
+// One loop trip is three streaming kernels (four in the mixed form), and the
+// scalars never leave the device:
+//   product:   q = A p, one p.q partial per block (then, for tens of
+//              thousands of tiles, a first-level sum of those partials);
+//   update:    every block sums the p.q partials itself, in one fixed order,
+//              so all blocks hold the same alpha; x += alpha p, r -= alpha q,
+//              one partial of r.r and r.z per block;
+//   direction: every block sums the r.r and r.z partials the same way, forms
+//              beta and the convergence test, and updates p = beta p + D^-1 r.
+// Block 0 of a kernel records its scalars in the solver state; no block reads
+// a state field that the same kernel writes (rho lives in two slots, picked
+// by the host's iteration parity, so rho is read from one slot and written to
+// the other), so no cross-block synchronisation is needed. The host only polls
+// the device state once per chunk of iterations.
 //
 // The mixed form keeps A, r, p, q, z and their dot products in FP32 and the
 // solution x in FP64. Whenever the FP32 residual has fallen by the refinement
@@ -20,6 +28,8 @@
 // is near), a refinement step recomputes r = b - A x with the FP64 matrix,
 // tests convergence on that FP64 residual, and restarts the FP32 recurrence
 // from it; the search direction is kept (restart = 0) or reset (restart = 1).
+// The refinement kernel runs every trip: its blocks take the decision from
+// the update's partials and return at once when there is nothing to do.
 #include "dispatch.hpp"
 #include "kernels.hpp"
 
@@ -30,24 +40,48 @@ using namespace ciehip;
 
 namespace {
 
+constexpr int kVectorThreads = 1024;  // update and direction blocks
+constexpr int kVectorBlocksMax = kVectorPartials;  // at most this many update partials
+constexpr int kFinishBlock = 1024;
+
+// Blocks of the update and direction kernels for n rows (about 4 rows per thread).
+int vectorBlocks(int64_t n) {
+    const int64_t blocks = (n + int64_t(kVectorThreads) * 4 - 1) / (int64_t(kVectorThreads) * 4);
+    return int(std::max<int64_t>(1, std::min<int64_t>(kVectorBlocksMax, blocks)));
+}
+
+// Sum over the block of `count` partials (value `which` of each slot of two),
+// each converted to TR, in one fixed order; every thread returns the same bits.
+template <int BS, class TR>
+__device__ __forceinline__ TR sumPartialsAll(const double* __restrict__ partials, int count, int which, TR* lds) {
+    TR value = TR(0);
+    for (int i = threadIdx.x; i < count; i += BS) value += TR(partials[2 * i + which]);
+    value = groupSum<kWave>(value);
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = value;
+    __syncthreads();
+    TR total = TR(0);
+#pragma unroll
+    for (int w = 0; w < BS / kWave; ++w) total += lds[w];
+    __syncthreads();  // lds is reused by the caller
+    return total;
+}
+
+__device__ __forceinline__ bool firstThread() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
 // ---------------------------------------------------------------------------
-// Kernels. The row-walking and elementwise kernels only write one partial
-// per block (two values: slots 2b and 2b + 1); a single-block finish kernel
-// then sums the partials in block order and updates the device state. Kernel
-// boundaries order the two, so no cross-block fence is needed.
+// Kernels
 // ---------------------------------------------------------------------------
 
-// r = b - A x (FP64 matrix and x), z = D^-1 r, start: p = z.
-// Partials: r.r in FP64, r.z in the recurrence precision TR.
-template <class Walker, class TR, bool kStart>
-__global__ __launch_bounds__(Walker::kBlock) void residualKernel(
+// Start: r = b - A x (FP64 matrix and x), z = D^-1 r, p = z; partials r.r
+// (FP64) and r.z (recurrence precision TR) per block.
+template <class Walker, class TR>
+__global__ __launch_bounds__(Walker::kBlock) void startKernel(
     RowSource source, const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
-    TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p, double* __restrict__ partials,
-    const SolverState* __restrict__ state) {
+    TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p, double* __restrict__ partials) {
     constexpr int BS = Walker::kBlock;
     __shared__ double ldsRR[BS / kWave];
     __shared__ TR ldsRZ[BS / kWave];
-    if (state->done || (!kStart && !state->refine)) return;
     double rr = 0.0;
     TR rz = TR(0);
     Walker::run(source, entries, x, [&](int row, double sum) {
@@ -55,7 +89,7 @@ __global__ __launch_bounds__(Walker::kBlock) void residualKernel(
         const TR residual = TR(t);
         const TR z = residual * inverse[row];
         r[row] = residual;
-        if (kStart) p[row] = z;
+        p[row] = z;
         rr += t * t;
         rz += residual * z;
     });
@@ -86,16 +120,51 @@ __global__ __launch_bounds__(Walker::kBlock) void productKernel(
     if (threadIdx.x == 0) partials[2 * Walker::slot()] = double(pq);
 }
 
-// x += alpha p; r -= alpha q; z = D^-1 r; partials r.r and r.z (:154-163).
-template <class T, class TX>
-__global__ __launch_bounds__(kReduceBlock) void updateKernel(int64_t n, TX* __restrict__ x, T* __restrict__ r,
-                                                             const T* __restrict__ p, const T* __restrict__ q,
-                                                             const T* __restrict__ inverse, double* __restrict__ partials,
-                                                             const SolverState* __restrict__ state) {
-    __shared__ T lds[kReduceBlock / kWave];
-    __shared__ T lds2[kReduceBlock / kWave];
+// First level of a sum over more than kLevelBlocks partials (one per tile of
+// a tiled SpMV): block b sums slots [b * per, min(count, b * per + per)) in a
+// fixed order, both values of a slot, in FP64. One compute unit cannot read
+// tens of thousands of partials quickly; kLevelBlocks of them can.
+__global__ __launch_bounds__(kReduceBlock) void partialLevelKernel(int count, int per,
+                                                                   const double* __restrict__ partials,
+                                                                   double* __restrict__ level,
+                                                                   const SolverState* __restrict__ state) {
+    __shared__ double ldsA[kReduceBlock / kWave];
+    __shared__ double ldsB[kReduceBlock / kWave];
     if (state->done) return;
-    const T alpha = T(state->alpha);
+    const int begin = int(blockIdx.x) * per;
+    const int end = min(count, begin + per);
+    double a = 0.0, b = 0.0;
+    for (int i = begin + int(threadIdx.x); i < end; i += kReduceBlock) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    a = blockSum<kReduceBlock>(a, ldsA);
+    b = blockSum<kReduceBlock>(b, ldsB);
+    if (threadIdx.x == 0) {
+        level[2 * blockIdx.x] = a;
+        level[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// alpha = rho / p.q (:139-151; p.q <= 0 ends the solve), then x += alpha p;
+// r -= alpha q; z = D^-1 r; partials r.r and r.z (:154-163).
+template <class T, class TX>
+__global__ __launch_bounds__(kVectorThreads) void updateKernel(int slot, int64_t n, TX* __restrict__ x,
+                                                               T* __restrict__ r, const T* __restrict__ p,
+                                                               const T* __restrict__ q, const T* __restrict__ inverse,
+                                                               int productCount, const double* __restrict__ productPartials,
+                                                               double* __restrict__ partials,
+                                                               SolverState* __restrict__ state) {
+    __shared__ T lds[kVectorThreads / kWave];
+    __shared__ T lds2[kVectorThreads / kWave];
+    if (state->done) return;
+    const T pq = sumPartialsAll<kVectorThreads, T>(productPartials, productCount, 0, lds);
+    if (!(T(0) < pq)) {
+        if (firstThread()) state->done = kNotSpd;
+        return;
+    }
+    const T alpha = T(T(state->rhos[slot]) / pq);
+    if (firstThread()) state->alpha = double(alpha);
     const TX alphaX = TX(alpha);
     T rr = T(0), rz = T(0);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -107,172 +176,161 @@ __global__ __launch_bounds__(kReduceBlock) void updateKernel(int64_t n, TX* __re
         rr += residual * residual;
         rz += residual * z;
     }
-    rr = blockSum<kReduceBlock>(rr, lds);
-    rz = blockSum<kReduceBlock>(rz, lds2);
+    rr = blockSum<kVectorThreads>(rr, lds);
+    rz = blockSum<kVectorThreads>(rz, lds2);
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = double(rr);
         partials[2 * blockIdx.x + 1] = double(rz);
     }
 }
 
-// p = beta p + D^-1 r (ConjugateGradients_impl.hpp:168-170: scale, then add).
-template <class T>
-__global__ __launch_bounds__(kReduceBlock) void directionKernel(int64_t n, T* __restrict__ p, const T* __restrict__ r,
-                                                                const T* __restrict__ inverse,
-                                                                const SolverState* state) {
-    if (state->done) return;
-    const T beta = T(state->beta);
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-        p[i] = p[i] * beta + r[i] * inverse[i];
+// Mixed form: does this trip refine? (FP32 r.r fell by the refinement factor
+// since the last refinement, looks converged, or the iteration cap is near.)
+__device__ __forceinline__ bool refineDue(float rr, const SolverState& s) {
+    const double absolute = sqrt(double(rr));
+    const double relative = absolute / s.initialNorm;
+    return double(rr) <= s.updateThreshold * s.updateNorm || floatLess(absolute, s.absoluteTolerance) ||
+           floatLess(relative, s.relativeTolerance) || s.iteration + 1 >= s.maxIterations;
 }
 
-enum Phase : int { kPhaseStart = 0, kPhaseAlpha = 1, kPhaseUpdate = 2, kPhaseUpdateMixed = 3, kPhaseRefine = 4 };
-
-// Single block: sums `count` block partials in block order and advances the
-// solver state. TR is the precision of the recurrence's dot products; the
-// residual phases carry r.r in FP64.
-constexpr int kFinishBlock = 1024;
-constexpr int kFinishUnroll = 8;  // partial loads in flight per thread
-
-// First level of a reduction over more than kMaxPartials partials (one per
-// tile of a tiled SpMV): block b sums slots [b * per, min(count, b * per + per))
-// in a fixed order, both values of a slot, in FP64. One compute unit cannot
-// read tens of thousands of partials quickly; kLevelBlocks of them can.
-template <int kPhase>
-__global__ __launch_bounds__(kReduceBlock) void partialLevelKernel(int count, int per,
-                                                                   const double* __restrict__ partials,
-                                                                   double* __restrict__ level,
-                                                                   const SolverState* __restrict__ state) {
-    __shared__ double ldsA[kReduceBlock / kWave];
-    __shared__ double ldsB[kReduceBlock / kWave];
+// Mixed form, every trip: the refinement decision from the update's partials
+// (recorded for the direction kernel); if due, r = b - A x with the FP64
+// matrix and x, and partials of the FP64 r.r and the FP32 r.z.
+template <class Walker>
+__global__ __launch_bounds__(Walker::kBlock) void refineKernel(
+    int slot, int updateCount, const double* __restrict__ updatePartials, RowSource source,
+    const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
+    float* __restrict__ r, const float* __restrict__ inverse, double* __restrict__ partials,
+    SolverState* __restrict__ state) {
+    constexpr int BS = Walker::kBlock;
+    __shared__ double ldsRR[BS / kWave];
+    __shared__ float ldsRZ[BS / kWave];
     if (state->done) return;
-    if (kPhase == kPhaseRefine && !state->refine) return;
-    constexpr bool kSecond = kPhase != kPhaseAlpha;
-    const int begin = int(blockIdx.x) * per;
-    const int end = min(count, begin + per);
-    double a = 0.0, b = 0.0;
-    for (int i = begin + int(threadIdx.x); i < end; i += kReduceBlock) {
-        a += partials[2 * i];
-        if (kSecond) b += partials[2 * i + 1];
-    }
-    a = blockSum<kReduceBlock>(a, ldsA);
-    if (kSecond) b = blockSum<kReduceBlock>(b, ldsB);
+    const bool due = refineDue(sumPartialsAll<BS, float>(updatePartials, updateCount, 0, ldsRZ), *state);
+    if (firstThread()) state->refined[slot] = due ? 1 : 0;
+    if (!due) return;
+    double rr = 0.0;
+    float rz = 0.0f;
+    Walker::run(source, entries, x, [&](int row, double sum) {
+        const double t = b[row] - sum;
+        const float residual = float(t);
+        const float z = residual * inverse[row];
+        r[row] = residual;
+        rr += t * t;
+        rz += residual * z;
+    });
+    const int own = Walker::slot();
+    rr = blockSum<BS>(rr, ldsRR);
+    rz = blockSum<BS>(rz, ldsRZ);
     if (threadIdx.x == 0) {
-        level[2 * blockIdx.x] = a;
-        level[2 * blockIdx.x + 1] = b;
+        partials[2 * own] = rr;
+        partials[2 * own + 1] = double(rz);
     }
 }
 
-template <class TR, int kPhase>
-__global__ __launch_bounds__(kFinishBlock) void finishKernel(int count, const double* __restrict__ partials,
-                                                             SolverState* __restrict__ state) {
-    __shared__ double lds64[kFinishBlock / kWave];
-    __shared__ TR ldsR[kFinishBlock / kWave];
+// beta, the convergence test and the iteration count (:158-185), then
+// p = beta p + D^-1 r (:168-170: scale, then add). kMixed: the FP32 recurrence
+// step, or after a refinement the FP64 test on the true residual and the
+// restart of the recurrence from it.
+template <class T, bool kMixed>
+__global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int64_t n, T* __restrict__ p,
+                                                                  const T* __restrict__ r, const T* __restrict__ inverse,
+                                                                  int updateCount, const double* __restrict__ updatePartials,
+                                                                  int refineCount, const double* __restrict__ refinePartials,
+                                                                  SolverState* __restrict__ state) {
+    __shared__ T ldsT[kVectorThreads / kWave];
+    __shared__ double lds64[kVectorThreads / kWave];
     SolverState& s = *state;
     if (s.done) return;
-    if (kPhase == kPhaseRefine && !s.refine) return;
-    constexpr bool kFirst64 = kPhase == kPhaseStart || kPhase == kPhaseRefine;
-    constexpr bool kSecond = kPhase != kPhaseAlpha;
-    double first64 = 0.0;
-    TR firstR = TR(0), second = TR(0);
-    // Thread i adds partials i, i + kFinishBlock, ... in increasing order.
-    for (int base = threadIdx.x; base < count; base += kFinishBlock * kFinishUnroll) {
-        double a[kFinishUnroll], b[kFinishUnroll];
-#pragma unroll
-        for (int u = 0; u < kFinishUnroll; ++u) {
-            const int i = base + u * kFinishBlock;
-            a[u] = 0.0;
-            b[u] = 0.0;
-            if (i < count) {
-                a[u] = partials[2 * i];
-                if (kSecond) b[u] = partials[2 * i + 1];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kFinishUnroll; ++u) {
-            if (base + u * kFinishBlock >= count) break;
-            if (kFirst64) first64 += a[u];
-            else firstR += TR(a[u]);
-            if (kSecond) second += TR(b[u]);
-        }
-    }
-    double A64 = 0.0;
-    TR AR = TR(0), B = TR(0);
-    if (kFirst64) A64 = blockSum<kFinishBlock>(first64, lds64);
-    else AR = blockSum<kFinishBlock>(firstR, ldsR);
-    if (kSecond) B = blockSum<kFinishBlock>(second, ldsR);
-    if (threadIdx.x != 0) return;
-    if (kPhase == kPhaseStart) {
-        // ConjugateGradients_impl.hpp:93-132
-        const double initial = sqrt(A64);
-        s.rr = A64;
-        s.initialNorm = initial;
-        s.absolute = initial;
-        s.relative = 1.0;
-        s.updateNorm = A64;
-        if (floatLess(initial, s.absoluteTolerance) || floatLess(1.0, s.relativeTolerance)) {
-            s.done = kConverged;
-        } else if (!(TR(0) < B)) {
-            s.done = kBreakdown;
-        } else {
-            s.rho = double(B);
-        }
-    } else if (kPhase == kPhaseAlpha) {
-        // :139-151
-        if (!(TR(0) < AR)) s.done = kNotSpd;
-        else s.alpha = double(TR(s.rho) / AR);
-    } else if (kPhase == kPhaseUpdate) {
-        // :158-185
-        const double absolute = sqrt(double(AR));
+    const double rhoPrevious = s.rhos[slot];
+    bool converged = false;
+    double beta = 0.0;
+    if constexpr (!kMixed) {
+        const T rr = sumPartialsAll<kVectorThreads, T>(updatePartials, updateCount, 0, ldsT);
+        const T rz = sumPartialsAll<kVectorThreads, T>(updatePartials, updateCount, 1, ldsT);
+        const double absolute = sqrt(double(rr));
         const double relative = absolute / s.initialNorm;
-        const double previous = s.rho;
-        s.rho = double(B);
-        s.beta = double(B) / previous;
-        s.rr = double(AR);
-        s.absolute = absolute;
-        s.relative = relative;
-        if (floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance)) {
-            s.done = kConverged;
-        } else {
-            s.iteration += 1;
-            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
-        }
-    } else if (kPhase == kPhaseUpdateMixed) {
-        // FP32 recurrence: continue, or ask for an FP64 refinement of the residual.
-        const double absolute = sqrt(double(AR));
-        const double relative = absolute / s.initialNorm;
-        const bool refine = double(AR) <= s.updateThreshold * s.updateNorm ||
-                            floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance) ||
-                            s.iteration + 1 >= s.maxIterations;
-        if (refine) {
-            s.refine = 1;
-        } else {
-            const TR previous = TR(s.rho);
-            s.rho = double(B);
-            s.beta = double(B / previous);
-            s.rr = double(AR);
+        converged = floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance);
+        beta = double(rz) / rhoPrevious;
+        if (firstThread()) {
+            s.rhos[slot ^ 1] = double(rz);
+            s.beta = beta;
+            s.rr = double(rr);
             s.absolute = absolute;
             s.relative = relative;
-            s.iteration += 1;
+            if (converged) {
+                s.done = kConverged;
+            } else {
+                s.iteration += 1;
+                if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
+            }
+        }
+    } else if (s.refined[slot]) {
+        const double rr = sumPartialsAll<kVectorThreads, double>(refinePartials, refineCount, 0, lds64);
+        const float rz = sumPartialsAll<kVectorThreads, float>(refinePartials, refineCount, 1, ldsT);
+        const double absolute = sqrt(rr);
+        const double relative = absolute / s.initialNorm;
+        converged = floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance);
+        beta = s.restart ? 0.0 : double(rz / float(rhoPrevious));
+        if (firstThread()) {
+            s.refinements += 1;
+            s.rr = rr;
+            s.absolute = absolute;
+            s.relative = relative;
+            if (converged) {
+                s.done = kConverged;
+            } else {
+                s.rhos[slot ^ 1] = double(rz);
+                s.beta = beta;
+                s.updateNorm = rr;
+                s.iteration += 1;
+                if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
+            }
         }
     } else {
-        // FP64 refinement: true residual, FP64 convergence test, restart the FP32 recurrence.
-        s.refine = 0;
-        s.refinements += 1;
-        s.rr = A64;
-        s.absolute = sqrt(A64);
-        s.relative = s.absolute / s.initialNorm;
-        if (floatLess(s.absolute, s.absoluteTolerance) || floatLess(s.relative, s.relativeTolerance)) {
-            s.done = kConverged;
-        } else {
-            const TR previous = TR(s.rho);
-            s.rho = double(B);
-            s.beta = s.restart ? 0.0 : double(B / previous);
-            s.updateNorm = A64;
+        const float rr = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 0, ldsT);
+        const float rz = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 1, ldsT);
+        beta = double(rz / float(rhoPrevious));
+        if (firstThread()) {
+            const double absolute = sqrt(double(rr));
+            s.rhos[slot ^ 1] = double(rz);
+            s.beta = beta;
+            s.rr = double(rr);
+            s.absolute = absolute;
+            s.relative = absolute / s.initialNorm;
             s.iteration += 1;
-            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
         }
+    }
+    if (converged) return;
+    const T scale = T(beta);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = p[i] * scale + r[i] * inverse[i];
+}
+
+// Single block, once per solve: the start sums (r.r in FP64, r.z in TR) and
+// the early exit of ConjugateGradients_impl.hpp:93-132.
+template <class TR>
+__global__ __launch_bounds__(kFinishBlock) void startFinishKernel(int count, const double* __restrict__ partials,
+                                                                  SolverState* __restrict__ state) {
+    __shared__ double lds64[kFinishBlock / kWave];
+    __shared__ TR ldsR[kFinishBlock / kWave];
+    const double rr = sumPartialsAll<kFinishBlock, double>(partials, count, 0, lds64);
+    const TR rz = sumPartialsAll<kFinishBlock, TR>(partials, count, 1, ldsR);
+    if (threadIdx.x != 0) return;
+    SolverState& s = *state;
+    const double initial = sqrt(rr);
+    s.rr = rr;
+    s.initialNorm = initial;
+    s.absolute = initial;
+    s.relative = 1.0;
+    s.updateNorm = rr;
+    if (floatLess(initial, s.absoluteTolerance) || floatLess(1.0, s.relativeTolerance)) {
+        s.done = kConverged;
+    } else if (!(TR(0) < rz)) {
+        s.done = kBreakdown;
+    } else {
+        s.rhos[0] = double(rz);
     }
 }
 
@@ -282,24 +340,25 @@ __global__ __launch_bounds__(kFinishBlock) void finishKernel(int count, const do
 
 size_t alignUp(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
-template <class TR, int kPhase>
-void launchFinish(cie_hip_context ctx, int count) {
-    const double* partials = ctx->scratch.partials;
-    if (count > kMaxPartials) {
-        const int per = (count + kLevelBlocks - 1) / kLevelBlocks;
-        const int blocks = (count + per - 1) / per;
-        hipLaunchKernelGGL((partialLevelKernel<kPhase>), dim3(blocks), dim3(kReduceBlock), 0, ctx->stream, count, per,
-                           ctx->scratch.partials, ctx->scratch.level, ctx->scratch.state);
-        partials = ctx->scratch.level;
-        count = blocks;
-    }
-    hipLaunchKernelGGL((finishKernel<TR, kPhase>), dim3(1), dim3(kFinishBlock), 0, ctx->stream, count, partials,
-                       ctx->scratch.state);
+// Partials of a product or start launch as its consumer reads them: at most
+// kLevelBlocks slots (a first-level sum is launched when there are more).
+struct PartialSource {
+    const double* partials;
+    int count;
+};
+
+PartialSource levelPartials(cie_hip_context ctx, int count) {
+    if (count <= kLevelBlocks) return {ctx->scratch.partials, count};
+    const int per = (count + kLevelBlocks - 1) / kLevelBlocks;
+    const int blocks = (count + per - 1) / per;
+    hipLaunchKernelGGL(partialLevelKernel, dim3(blocks), dim3(kReduceBlock), 0, ctx->stream, count, per,
+                       ctx->scratch.partials, ctx->scratch.level, ctx->scratch.state);
+    return {ctx->scratch.level, blocks};
 }
 
-// r = b - A x and its dot products, then the start or refinement finish.
+// r = b - A x, z, p = z and the start sums.
 template <class TR>
-int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, TR* r, const TR* inverse, TR* p) {
+int launchStart(cie_hip_csr m, const double* x, const double* b, TR* r, const TR* inverse, TR* p) {
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
     int grid = 0;
@@ -307,29 +366,45 @@ int launchResidual(cie_hip_csr m, bool start, const double* x, const double* b, 
     const bool ok = withWalker<double, double>(launch, [&](auto walker) {
         using Walker = decltype(walker);
         RowSource source;
-        if (start) {
-            status = prepareLaunch(m, launch, residualKernel<Walker, TR, true>, source, grid);
-            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
-            if (status != CIE_HIP_OK) return;
-            hipLaunchKernelGGL((residualKernel<Walker, TR, true>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
-                               source, m->entries64, x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
-        } else {
-            status = prepareLaunch(m, launch, residualKernel<Walker, TR, false>, source, grid, true);
-            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
-            if (status != CIE_HIP_OK) return;
-            hipLaunchKernelGGL((residualKernel<Walker, TR, false>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream,
-                               source, m->entries64, x, b, r, inverse, p, ctx->scratch.partials, ctx->scratch.state);
-        }
+        status = prepareLaunch(m, launch, startKernel<Walker, TR>, source, grid);
+        if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+        if (status != CIE_HIP_OK) return;
+        hipLaunchKernelGGL((startKernel<Walker, TR>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, source,
+                           m->entries64, x, b, r, inverse, p, ctx->scratch.partials);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     if (status != CIE_HIP_OK) return status;
-    if (start) launchFinish<TR, kPhaseStart>(ctx, grid);
-    else launchFinish<TR, kPhaseRefine>(ctx, grid);
+    const PartialSource sums = levelPartials(ctx, grid);
+    hipLaunchKernelGGL((startFinishKernel<TR>), dim3(1), dim3(kFinishBlock), 0, ctx->stream, sums.count, sums.partials,
+                       ctx->scratch.state);
     return CIE_HIP_OK;
 }
 
-// Brackets the first kProfileEvents loop SpMV kernels (not their finish
-// kernels) with events on the solver stream (profiling on).
+// Mixed form: the refinement kernel (resident-round grid: it usually exits at
+// once). Returns its partial count in `count`.
+int launchRefine(cie_hip_csr m, int slot, int updateCount, const double* x, const double* b, float* r,
+                 int* count) {
+    cie_hip_context ctx = m->context;
+    const CsrLaunch& launch = m->launch64;
+    int grid = 0;
+    int status = CIE_HIP_OK;
+    const bool ok = withWalker<double, double>(launch, [&](auto walker) {
+        using Walker = decltype(walker);
+        RowSource source;
+        status = prepareLaunch(m, launch, refineKernel<Walker>, source, grid, true);
+        if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+        if (status != CIE_HIP_OK) return;
+        hipLaunchKernelGGL((refineKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, slot, updateCount,
+                           ctx->scratch.vectorPartials, source, m->entries64, x, b, r, m->inverseDiagonal32,
+                           ctx->scratch.partials, ctx->scratch.state);
+    });
+    if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
+    *count = grid;
+    return status;
+}
+
+// Brackets the first kProfileEvents loop SpMV kernels with events on the
+// solver stream (profiling on).
 struct ProductProfile {
     int used = 0;
     int begin(cie_hip_context ctx) {
@@ -355,10 +430,11 @@ struct ProductProfile {
     }
 };
 
-// q = A p with its p.q, then the alpha finish.
+// q = A p with its p.q partials (first-level sum included); returns where the
+// update kernel reads them.
 template <class T>
-int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q,
-                      ProductProfile& profile) {
+int launchProduct(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q,
+                  ProductProfile& profile, PartialSource* sums) {
     cie_hip_context ctx = m->context;
     int grid = 0;
     int status = CIE_HIP_OK;
@@ -375,7 +451,7 @@ int launchProductStep(cie_hip_csr m, const T* entries, const CsrLaunch& launch, 
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     if (status != CIE_HIP_OK) return status;
-    launchFinish<T, kPhaseAlpha>(ctx, grid);
+    *sums = levelPartials(ctx, grid);
     return CIE_HIP_OK;
 }
 
@@ -532,41 +608,50 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     CIE_HIP_CHECK(hipEventCreate(&timer.marks[1]));
     CIE_HIP_CHECK(hipEventRecord(timer.start, ctx->stream));
 
-    const dim3 vgrid(vectorGrid(n)), vblock(kReduceBlock);
+    const dim3 vgrid(vectorBlocks(n)), vblock(kVectorThreads);
+    const int updateCount = int(vgrid.x);
     double seconds = 0.0;
     ProductProfile profile;
+    int trip = 0;  // iterations enqueued; its parity picks the rho slot
     if (!q.mixed) {
         double* r64 = static_cast<double*>(r);
         double* p64 = static_cast<double*>(p);
         double* q64 = static_cast<double*>(qv);
-        status = launchResidual<double>(m, true, solution, q.b, r64, m->inverseDiagonal64, p64);
+        status = launchStart<double>(m, solution, q.b, r64, m->inverseDiagonal64, p64);
         if (status) return status;
         status = runLoop(ctx, timer, [&]() -> int {
-            int s = launchProductStep<double>(m, m->entries64, m->launch64, p64, q64, profile);
+            const int slot = trip++ & 1;
+            PartialSource sums{};
+            int s = launchProduct<double>(m, m->entries64, m->launch64, p64, q64, profile, &sums);
             if (s) return s;
-            hipLaunchKernelGGL((updateKernel<double, double>), vgrid, vblock, 0, ctx->stream, n, solution, r64, p64,
-                               q64, m->inverseDiagonal64, ctx->scratch.partials, ctx->scratch.state);
-            launchFinish<double, kPhaseUpdate>(ctx, int(vgrid.x));
-            hipLaunchKernelGGL((directionKernel<double>), vgrid, vblock, 0, ctx->stream, n, p64, r64,
-                               m->inverseDiagonal64, ctx->scratch.state);
+            hipLaunchKernelGGL((updateKernel<double, double>), vgrid, vblock, 0, ctx->stream, slot, n, solution, r64,
+                               p64, q64, m->inverseDiagonal64, sums.count, sums.partials, ctx->scratch.vectorPartials,
+                               ctx->scratch.state);
+            hipLaunchKernelGGL((directionKernel<double, false>), vgrid, vblock, 0, ctx->stream, slot, n, p64, r64,
+                               m->inverseDiagonal64, updateCount, ctx->scratch.vectorPartials, 0,
+                               static_cast<const double*>(nullptr), ctx->scratch.state);
             return CIE_HIP_OK;
         }, &seconds);
     } else {
         float* r32 = static_cast<float*>(r);
         float* p32 = static_cast<float*>(p);
         float* q32 = static_cast<float*>(qv);
-        status = launchResidual<float>(m, true, solution, q.b, r32, m->inverseDiagonal32, p32);
+        status = launchStart<float>(m, solution, q.b, r32, m->inverseDiagonal32, p32);
         if (status) return status;
         status = runLoop(ctx, timer, [&]() -> int {
-            int s = launchProductStep<float>(m, m->entries32, m->launch32, p32, q32, profile);
+            const int slot = trip++ & 1;
+            PartialSource sums{};
+            int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, &sums);
             if (s) return s;
-            hipLaunchKernelGGL((updateKernel<float, double>), vgrid, vblock, 0, ctx->stream, n, solution, r32, p32,
-                               q32, m->inverseDiagonal32, ctx->scratch.partials, ctx->scratch.state);
-            launchFinish<float, kPhaseUpdateMixed>(ctx, int(vgrid.x));
-            s = launchResidual<float>(m, false, solution, q.b, r32, m->inverseDiagonal32, p32);
+            hipLaunchKernelGGL((updateKernel<float, double>), vgrid, vblock, 0, ctx->stream, slot, n, solution, r32,
+                               p32, q32, m->inverseDiagonal32, sums.count, sums.partials, ctx->scratch.vectorPartials,
+                               ctx->scratch.state);
+            int refineCount = 0;
+            s = launchRefine(m, slot, updateCount, solution, q.b, r32, &refineCount);
             if (s) return s;
-            hipLaunchKernelGGL((directionKernel<float>), vgrid, vblock, 0, ctx->stream, n, p32, r32,
-                               m->inverseDiagonal32, ctx->scratch.state);
+            hipLaunchKernelGGL((directionKernel<float, true>), vgrid, vblock, 0, ctx->stream, slot, n, p32, r32,
+                               m->inverseDiagonal32, updateCount, ctx->scratch.vectorPartials, refineCount,
+                               ctx->scratch.partials, ctx->scratch.state);
             return CIE_HIP_OK;
         }, &seconds);
     }
