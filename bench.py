@@ -247,11 +247,12 @@ def measured_traffic(launch):
         return None
     kernel, width, unroll = launch
     if kernel == "tiled":
-        name = f"productKernel<BlockTiles<float, float, {unroll}, {width}> >"
+        walker = f"BlockTiles<float, float, {unroll}, {width}> >"
     else:
-        walker = "StagedRows" if kernel == "staged" else "VectorRows"
-        name = f"productKernel<{walker}<float, float, {width}, {unroll}, 256> >"
-    entry = json.loads(path.read_text()).get("kernels", {}).get(name)
+        walker = f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<float, float, {width}, {unroll}, 256> >"
+    kernels = json.loads(path.read_text()).get("kernels", {})
+    # the in-solve product kernel; else the standalone SpMV of the same shape (the autotuner times every shape)
+    entry = kernels.get(f"productKernel<{walker}") or kernels.get(f"csrProductKernel<{walker}")
     return None if entry is None else entry["traffic_bytes"]
 
 

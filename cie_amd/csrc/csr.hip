@@ -364,7 +364,7 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
     }
     if (hipStreamSynchronize(context->stream) != hipSuccess)
         return bail(CIE_HIP_ERROR_HIP, "cie_hip_csr_create: stream synchronisation failed");
-    m->launch64 = m->launch32 = defaultLaunch(m->meanRowLength, m->maxRowLength, context->computeUnits, rows);
+    m->launch64 = m->launch32 = defaultLaunch();
     *matrix = m;
     return CIE_HIP_OK;
 }

@@ -11,7 +11,7 @@
 namespace ciehip {
 
 constexpr int kSpmvBlock = 256;
-constexpr int kStagedMaxRow = 8192;  // longest row the staged kernel is chosen for by default
+constexpr int kStagedMaxRow = 8192;  // longest row the autotuner tries the staged kernel for
 
 // Lanes per row x entries per lane in flight.
 #define CIEHIP_SHAPES(X)                                              \
@@ -162,27 +162,15 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     return CIE_HIP_OK;
 }
 
-// Starting point before autotuning: the staged kernel (bit-identical rows),
-// unless some row is too long for one lane to sum it, then vector rows with
-// about one pass of L * U entries per row.
-inline CsrLaunch defaultLaunch(double meanRowLength, int maxRowLength, int computeUnits, int64_t rows) {
+// Starting point before autotuning: block tiles of 2048 entries (handles
+// rows of any length). The staged kernel (rows bit-identical to the CPU loop)
+// and vector rows are selected with cie_hip_csr_set_launch or the autotuner.
+inline CsrLaunch defaultLaunch() {
     CsrLaunch launch;
+    launch.kernel = CIE_HIP_SPMV_TILED;
+    launch.width = kSpmvBlock;
+    launch.unroll = 8;
     launch.block = kSpmvBlock;
-    if (maxRowLength <= kStagedMaxRow) {
-        launch.kernel = CIE_HIP_SPMV_STAGED;
-        launch.width = 1024;
-        launch.unroll = 8;
-    } else {
-        launch.kernel = CIE_HIP_SPMV_VECTOR;
-        int unroll = 4;
-        int lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
-        if (lanes < 4) {
-            unroll = 2;
-            lanes = nextPow2(int(std::ceil(std::max(1.0, meanRowLength) / unroll)));
-        }
-        launch.width = std::min(64, std::max(1, lanes));
-        launch.unroll = unroll;
-    }
     return launch;
 }
 

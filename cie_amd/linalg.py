@@ -129,7 +129,7 @@ class CSRMatrix:
         code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
-    def autotune(self, precision=64, repetitions=10, kernels=("staged",)):
+    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "staged", "vector")):
         """Time the candidate launches of `kernels` ("staged", "vector", "tiled") and keep the fastest."""
         mask = sum(1 << _kernel_code(k) for k in kernels)
         best = c_double()

@@ -23,11 +23,10 @@ CALIBRATION_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def _short(name):
+    """Kernel name without namespaces, return type and parameter list."""
     name = re.sub(r"\(anonymous namespace\)::", "", name)
-    name = re.sub(r"\(int, int, .*", "", name)
-    name = re.sub(r"\(long, .*", "", name)
     name = name.replace("ciehip::", "").replace("void ", "")
-    return name[:100]
+    return name.split("(", 1)[0][:100]
 
 
 def kernel_stats(path, out, command):

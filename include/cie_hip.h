@@ -133,7 +133,7 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
  *     summed by a power-of-two group of lanes. Tiles are cut once per matrix
  *     and capacity, on the host, on first use.
  * All are bitwise reproducible run to run for a fixed launch. The default is
- * STAGED/1024/8 (VECTOR if a row exceeds 8192 entries). cie_hip_csr_autotune
+ * TILED/256/8; STAGED gives rows bit-identical to the CPU loop. cie_hip_csr_autotune
  * times the candidates of the kernels in `kernels` (a mask of 1 << kernel)
  * on the device and keeps the fastest. */
 #define CIE_HIP_SPMV_VECTOR 0

@@ -123,8 +123,9 @@ def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll, cols):
 def test_spmv_deterministic_and_autotuned(hip, torch_gpu):
     w = systems.q2_laplace_3d(16)
     matrix = hip.CSRMatrix(w.matrix)
-    best = matrix.autotune(64, repetitions=5)
+    best = matrix.autotune(64, repetitions=5, kernels=("staged",))
     assert best > 0
+    assert matrix.launch(64)[0] == "staged"
     space = hip.HIPSpace()
     op = hip.HIPCSROperator(matrix, space)
     x = to_dev(torch_gpu, w.rhs)
@@ -162,12 +163,14 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
     w = systems.q2_laplace_3d(64)
     assert w.matrix.nnz == 127263527
     matrix = hip.CSRMatrix(w.matrix)
+    assert matrix.launch(64) == ("tiled", 256, 8)  # the default launch
+    matrix.set_launch(64, "staged", 1024, 8)
+    matrix.set_launch(32, "staged", 1024, 8)
     expected = oracle.csr_product(w.matrix, 0.0, w.rhs, 1.0, np.zeros(w.matrix.rows), threads=8)
     space = hip.HIPSpace()
     y = space.makeVector(w.matrix.rows)
     hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
     got = y.cpu().numpy()
-    assert matrix.launch(64)[0] == "staged"
     assert np.array_equal(got, expected)  # bit-identical to the sequential CPU rows
     space32 = hip.HIPSpace(torch_gpu.float32)
     y32 = space32.makeVector(w.matrix.rows)
