@@ -4,22 +4,31 @@
 //
 // Layout: the systems are the diagonal blocks of one CSR matrix (rows of
 // system s are [offsets[s], offsets[s+1]) and its columns stay inside that
-// range). Rows are cut into tiles of up to 64 rows that never straddle two
-// systems; each tile is processed by one wavefront with the staged row sum
-// (rows bit-identical to a sequential CPU loop), and writes its share of the
-// system's dot products. Every kernel covers all systems in one launch,
-// walking the tile list grid-stride; tiles of converged systems are skipped
-// (per-system convergence mask). Per-system scalars (rho, alpha, beta,
-// residuals, flags) live on the device and are finished by one block per
-// system that sums that system's tile partials in tile order. The chain of
-// kernels of one iteration is captured once in a HIP graph and replayed until
-// every system is done.
+// range). Two work lists are cut once per batch, on the host:
+//   * block tiles for the SpMV kernels (whole rows of one system, at most
+//     kTileEntries entries and rowsPerTile[s] rows: the same block-tile
+//     kernel as the single-system SpMV, one workgroup per tile);
+//   * chunks of at most kChunkRows rows of one system for the vector kernels
+//     (one row per thread).
+// Each tile or chunk writes its share of its system's dot products; one block
+// per system then sums that system's partials in order and advances its
+// scalars (rho, alpha, beta, residuals, flags), kept on the device.
 //
-// Tile height (rows per wavefront: 16, 32 or 64) is chosen per system by
-// cie_hip_batch_autotune from timed runs of the system's own tiles.
+// Convergence mask: after every trip a one-block kernel lists the systems
+// still running with the prefix sums of their tile and chunk counts, and a
+// second kernel expands that into the list of their tiles and chunks; the work
+// kernels walk only those items, so finished systems cost nothing. The mixed
+// form builds a third list, of the tiles of the systems that refine this trip,
+// for its FP64 residual kernel. Grids are fixed (the chain of one trip is
+// captured once in a HIP graph and replayed until every system is done); the
+// item counts are read on the device.
+//
+// Rows per tile (32, 64, 128 or 256) are chosen per system by
+// cie_hip_batch_autotune from timed SpMV runs of the system's own tiles.
 #include "dispatch.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 #include <vector>
@@ -28,18 +37,16 @@ using namespace ciehip;
 
 namespace {
 
-constexpr int kBatchBlock = 256;
-constexpr int kBatchWaves = kBatchBlock / kWave;
-constexpr int kBatchStage = 1024;
-constexpr int kBatchUnroll = 8;
+constexpr int kTileBlock = 256;
+constexpr int kTileUnroll = 8;
+constexpr int kTileEntries = kTileBlock * kTileUnroll;  // entries per tile at most
+constexpr int kChunkRows = 1024;                         // rows per vector chunk (one per thread)
+constexpr int kCompactThreads = 1024;
 constexpr int kGraphIterations = 8;
+constexpr int kDefaultTileRows = 256;
 
-struct Tile {
-    int32_t first;
-    int32_t count;
-    int32_t system;
-    int32_t pad;
-};
+template <class T>
+using BatchTiles = BlockTiles<T, T, kTileUnroll, kTileBlock>;
 
 struct SystemState {
     double rho, alpha, beta, rr, initialNorm, absolute, relative, updateNorm;
@@ -53,143 +60,191 @@ struct BatchSettings {
     int32_t restart, mixed;
 };
 
+// A work list: the selected systems (in system order), the prefix sums of
+// their tile and chunk counts (count + 1 entries each; the last is the total),
+// and the expanded items {tile or chunk, system}.
+struct ActiveWork {
+    int32_t* systems;
+    int32_t* tilePrefix;
+    int32_t* chunkPrefix;
+    int32_t* count;  // [0] systems, [1] tiles, [2] chunks
+    int2* tileList;
+    int2* chunkList;
+};
+
+enum Selection : int { kRunning = 0, kRefining = 1 };
+
+// Where the work of a batch lives on the device.
+struct BatchLayout {
+    RowSource source;                 // block tiles of all systems (tiles / tileCount)
+    const int32_t* systemTiles;       // systems + 1: first tile of each system
+    const int32_t* systemChunks;      // systems + 1: first chunk of each system
+    const int32_t* chunkRows;         // chunks + 1: first row of each chunk
+};
+
 enum Phase : int { kPhaseStart = 0, kPhaseAlpha = 1, kPhaseUpdate = 2, kPhaseUpdateMixed = 3, kPhaseRefine = 4 };
 
+// Largest k < count with prefix[k] <= item (the prefix is increasing).
+__device__ __forceinline__ int locate(int item, const int32_t* __restrict__ prefix, int count) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
-// Tile kernels (one wavefront per tile, grid-stride over the tile list)
+// Work kernels: item = blockIdx.x, blockIdx.x + gridDim.x, ... over the active
+// tiles (tile kernels) or chunks (vector kernels); with one block per item
+// the loop runs at most once.
 // ---------------------------------------------------------------------------
 
-#define CIEHIP_FOR_EACH_TILE(tileBegin, tileEnd)                                                        \
-    const int wave = threadIdx.x / kWave;                                                              \
-    const int lane = threadIdx.x % kWave;                                                              \
-    for (int t = (tileBegin) + int(blockIdx.x) * kBatchWaves + wave; t < (tileEnd); t += int(gridDim.x) * kBatchWaves)
-
-// r = b - A x (FP64), z = D^-1 r; start: p = z. Tile partials: r.r (FP64), r.z (TR).
+// r = b - A x (FP64), z = D^-1 r; start: p = z. Tile partials r.r (FP64), r.z (TR).
 template <class TR, bool kStart>
-__global__ __launch_bounds__(kBatchBlock) void batchResidualKernel(
-    int tileBegin, int tileEnd, const Tile* __restrict__ tiles, const int32_t* __restrict__ rowExtents,
-    const int32_t* __restrict__ columnIndices, const double* __restrict__ entries, const double* __restrict__ x,
+__global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
+    BatchLayout layout, ActiveWork work, const double* __restrict__ entries, const double* __restrict__ x,
     const double* __restrict__ b, TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p,
     double* __restrict__ partials, const SystemState* __restrict__ state) {
-    __shared__ double stage[kBatchWaves][kBatchStage];
-    CIEHIP_FOR_EACH_TILE(tileBegin, tileEnd) {
-        const Tile tile = tiles[t];
-        const SystemState& s = state[tile.system];
-        if (s.done || (!kStart && !s.refine)) continue;
-        const double sum = stagedTileSum<double, double, kBatchStage, kBatchUnroll>(
-            tile.first, tile.count, lane, rowExtents, columnIndices, entries, x, stage[wave]);
+    __shared__ double stage[kTileEntries];
+    __shared__ int32_t extents[kTileBlock + 1];
+    __shared__ double ldsRR[kTileBlock / kWave];
+    __shared__ TR ldsRZ[kTileBlock / kWave];
+    const int items = work.count[1];
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int t = work.tileList[item].x;
         double rr = 0.0;
         TR rz = TR(0);
-        if (lane < tile.count) {
-            const int row = tile.first + lane;
+        auto epilogue = [&](int row, double sum) {
             const double residual64 = b[row] - sum;
             const TR residual = TR(residual64);
             const TR z = residual * inverse[row];
             r[row] = residual;
             if (kStart) p[row] = z;
-            rr = residual64 * residual64;
-            rz = residual * z;
-        }
-        rr = groupSum<kWave>(rr);
-        rz = groupSum<kWave>(rz);
-        if (lane == 0) {
+            rr += residual64 * residual64;
+            rz += residual * z;
+        };
+        BlockTiles<double, double, kTileUnroll, kTileBlock>::tile(t, layout.source, entries, x, epilogue, stage, extents);
+        rr = blockSum<kTileBlock>(rr, ldsRR);
+        rz = blockSum<kTileBlock>(rz, ldsRZ);
+        if (threadIdx.x == 0) {
             partials[2 * t] = rr;
             partials[2 * t + 1] = double(rz);
         }
+        __syncthreads();  // stage and extents are reused by the next item
     }
 }
 
 // q = A p; tile partial p.q.
 template <class T>
-__global__ __launch_bounds__(kBatchBlock) void batchProductKernel(
-    int tileBegin, int tileEnd, const Tile* __restrict__ tiles, const int32_t* __restrict__ rowExtents,
-    const int32_t* __restrict__ columnIndices, const T* __restrict__ entries, const T* __restrict__ p,
-    T* __restrict__ q, double* __restrict__ partials, const SystemState* __restrict__ state) {
-    __shared__ T stage[kBatchWaves][kBatchStage];
-    CIEHIP_FOR_EACH_TILE(tileBegin, tileEnd) {
-        const Tile tile = tiles[t];
-        if (state[tile.system].done) continue;
-        const T sum = stagedTileSum<T, T, kBatchStage, kBatchUnroll>(tile.first, tile.count, lane, rowExtents,
-                                                                    columnIndices, entries, p, stage[wave]);
+__global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout layout, ActiveWork work,
+                                                                 const T* __restrict__ entries, const T* __restrict__ p,
+                                                                 T* __restrict__ q, double* __restrict__ partials) {
+    __shared__ T stage[kTileEntries];
+    __shared__ int32_t extents[kTileBlock + 1];
+    __shared__ T lds[kTileBlock / kWave];
+    const int items = work.count[1];
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int t = work.tileList[item].x;
         T pq = T(0);
-        if (lane < tile.count) {
-            const int row = tile.first + lane;
+        auto epilogue = [&](int row, T sum) {
             q[row] = sum;
-            pq = p[row] * sum;
-        }
-        pq = groupSum<kWave>(pq);
-        if (lane == 0) partials[2 * t] = double(pq);
+            pq += p[row] * sum;
+        };
+        BatchTiles<T>::tile(t, layout.source, entries, p, epilogue, stage, extents);
+        pq = blockSum<kTileBlock>(pq, lds);
+        if (threadIdx.x == 0) partials[2 * t] = double(pq);
+        __syncthreads();
     }
 }
 
-// x += alpha p; r -= alpha q; z = D^-1 r; tile partials r.r and r.z.
+// The same product over an explicit tile range (the autotuner's timing runs).
+template <class T>
+__global__ __launch_bounds__(kTileBlock) void batchTileRangeKernel(RowSource source, int firstTile,
+                                                                   const T* __restrict__ entries,
+                                                                   const T* __restrict__ p, T* __restrict__ q) {
+    __shared__ T stage[kTileEntries];
+    __shared__ int32_t extents[kTileBlock + 1];
+    auto epilogue = [&](int row, T sum) { q[row] = sum; };
+    BatchTiles<T>::tile(firstTile + int(blockIdx.x), source, entries, p, epilogue, stage, extents);
+}
+
+// x += alpha p; r -= alpha q; z = D^-1 r; chunk partials r.r and r.z.
 template <class T, class TX>
-__global__ __launch_bounds__(kBatchBlock) void batchUpdateKernel(
-    int tileBegin, int tileEnd, const Tile* __restrict__ tiles, TX* __restrict__ x, T* __restrict__ r,
-    const T* __restrict__ p, const T* __restrict__ q, const T* __restrict__ inverse, double* __restrict__ partials,
-    const SystemState* __restrict__ state) {
-    CIEHIP_FOR_EACH_TILE(tileBegin, tileEnd) {
-        const Tile tile = tiles[t];
-        const SystemState& s = state[tile.system];
-        if (s.done) continue;
-        const T alpha = T(s.alpha);
+__global__ __launch_bounds__(kChunkRows) void batchUpdateKernel(BatchLayout layout, ActiveWork work,
+                                                                TX* __restrict__ x, T* __restrict__ r,
+                                                                const T* __restrict__ p, const T* __restrict__ q,
+                                                                const T* __restrict__ inverse,
+                                                                double* __restrict__ partials,
+                                                                const SystemState* __restrict__ state) {
+    __shared__ T lds[kChunkRows / kWave];
+    __shared__ T lds2[kChunkRows / kWave];
+    const int items = work.count[2];
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int2 chunk = work.chunkList[item];
+        const int c = chunk.x;
+        const SystemState& s = state[chunk.y];
         T rr = T(0), rz = T(0);
-        if (lane < tile.count) {
-            const int row = tile.first + lane;
-            x[row] = x[row] + TX(alpha) * TX(p[row]);
-            const T residual = r[row] + (-alpha) * q[row];
-            r[row] = residual;
-            const T z = residual * inverse[row];
-            rr = residual * residual;
-            rz = residual * z;
+        if (!s.done) {  // NOT_SPD may end a system between the product and this kernel
+            const T alpha = T(s.alpha);
+            const int row = layout.chunkRows[c] + int(threadIdx.x);
+            if (row < layout.chunkRows[c + 1]) {
+                x[row] = x[row] + TX(alpha) * TX(p[row]);
+                const T residual = r[row] + (-alpha) * q[row];
+                r[row] = residual;
+                const T z = residual * inverse[row];
+                rr = residual * residual;
+                rz = residual * z;
+            }
         }
-        rr = groupSum<kWave>(rr);
-        rz = groupSum<kWave>(rz);
-        if (lane == 0) {
-            partials[2 * t] = double(rr);
-            partials[2 * t + 1] = double(rz);
+        rr = blockSum<kChunkRows>(rr, lds);
+        rz = blockSum<kChunkRows>(rz, lds2);
+        if (threadIdx.x == 0) {
+            partials[2 * c] = double(rr);
+            partials[2 * c + 1] = double(rz);
         }
     }
 }
 
 // p = beta p + D^-1 r
 template <class T>
-__global__ __launch_bounds__(kBatchBlock) void batchDirectionKernel(int tileBegin, int tileEnd,
-                                                                    const Tile* __restrict__ tiles, T* __restrict__ p,
-                                                                    const T* __restrict__ r,
-                                                                    const T* __restrict__ inverse,
-                                                                    const SystemState* __restrict__ state) {
-    CIEHIP_FOR_EACH_TILE(tileBegin, tileEnd) {
-        const Tile tile = tiles[t];
-        const SystemState& s = state[tile.system];
+__global__ __launch_bounds__(kChunkRows) void batchDirectionKernel(BatchLayout layout, ActiveWork work,
+                                                                   T* __restrict__ p, const T* __restrict__ r,
+                                                                   const T* __restrict__ inverse,
+                                                                   const SystemState* __restrict__ state) {
+    const int items = work.count[2];
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int2 chunk = work.chunkList[item];
+        const SystemState& s = state[chunk.y];
         if (s.done) continue;
+        const int c = chunk.x;
         const T beta = T(s.beta);
-        if (lane < tile.count) {
-            const int row = tile.first + lane;
-            p[row] = p[row] * beta + r[row] * inverse[row];
-        }
+        const int row = layout.chunkRows[c] + int(threadIdx.x);
+        if (row < layout.chunkRows[c + 1]) p[row] = p[row] * beta + r[row] * inverse[row];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Per-system finish: one block per system sums its tile partials in order.
+// Per-system finish: one block per system sums its partials in order (tile
+// partials after the SpMV kernels, chunk partials after the update).
 // ---------------------------------------------------------------------------
 
 template <class TR, int kPhase>
-__global__ __launch_bounds__(kBatchBlock) void batchFinishKernel(const int32_t* __restrict__ systemTiles,
-                                                                 const double* __restrict__ partials,
-                                                                 SystemState* __restrict__ state,
-                                                                 BatchSettings settings, int* active) {
-    __shared__ double lds64[kBatchBlock / kWave];
-    __shared__ TR ldsR[kBatchBlock / kWave];
+__global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* __restrict__ systemItems,
+                                                                const double* __restrict__ partials,
+                                                                SystemState* __restrict__ state,
+                                                                BatchSettings settings, int* active) {
+    __shared__ double lds64[kTileBlock / kWave];
+    __shared__ TR ldsR[kTileBlock / kWave];
     SystemState& s = state[blockIdx.x];
     if (s.done) return;
     if (kPhase == kPhaseRefine && !s.refine) return;
-    const int begin = systemTiles[blockIdx.x], end = systemTiles[blockIdx.x + 1];
+    const int begin = systemItems[blockIdx.x], end = systemItems[blockIdx.x + 1];
     // first value: r.r (FP64 in the residual phases, TR otherwise) or p.q; second: r.z (TR)
     double first64 = 0.0;
     TR firstR = TR(0), second = TR(0);
-    for (int t = begin + threadIdx.x; t < end; t += kBatchBlock) {
+    for (int t = begin + threadIdx.x; t < end; t += kTileBlock) {
         first64 += partials[2 * t];
         firstR += TR(partials[2 * t]);
         second += TR(partials[2 * t + 1]);
@@ -197,9 +252,9 @@ __global__ __launch_bounds__(kBatchBlock) void batchFinishKernel(const int32_t* 
     constexpr bool kFirst64 = kPhase == kPhaseStart || kPhase == kPhaseRefine;
     double A64 = 0.0;
     TR AR = TR(0), B = TR(0);
-    if (kFirst64) A64 = blockSum<kBatchBlock>(first64, lds64);
-    else AR = blockSum<kBatchBlock>(firstR, ldsR);
-    if (kPhase != kPhaseAlpha) B = blockSum<kBatchBlock>(second, ldsR);
+    if (kFirst64) A64 = blockSum<kTileBlock>(first64, lds64);
+    else AR = blockSum<kTileBlock>(firstR, ldsR);
+    if (kPhase != kPhaseAlpha) B = blockSum<kTileBlock>(second, ldsR);
     if (threadIdx.x != 0) return;
     bool finished = false;
     if (kPhase == kPhaseStart) {
@@ -278,6 +333,79 @@ __global__ __launch_bounds__(kBatchBlock) void batchFinishKernel(const int32_t* 
     if (finished) atomicSub(active, 1);
 }
 
+// Lists the selected systems (running, or running and refining this trip)
+// and the prefix sums of their tile and chunk counts (one block; thread i
+// handles a contiguous run of systems).
+template <int kSelect>
+__global__ __launch_bounds__(kCompactThreads) void batchCompactKernel(int systems, const SystemState* __restrict__ state,
+                                                                      const int32_t* __restrict__ systemTiles,
+                                                                      const int32_t* __restrict__ systemChunks,
+                                                                      ActiveWork work) {
+    __shared__ int32_t scanRun[kCompactThreads], scanTiles[kCompactThreads], scanChunks[kCompactThreads];
+    const int tid = threadIdx.x;
+    const int per = (systems + kCompactThreads - 1) / kCompactThreads;
+    const int lo = min(systems, tid * per), hi = min(systems, lo + per);
+    int run = 0, tiles = 0, chunks = 0;
+    auto selected = [&](int s) { return !state[s].done && (kSelect == kRunning || state[s].refine); };
+    for (int s = lo; s < hi; ++s) {
+        if (!selected(s)) continue;
+        run += 1;
+        tiles += systemTiles[s + 1] - systemTiles[s];
+        chunks += systemChunks[s + 1] - systemChunks[s];
+    }
+    scanRun[tid] = run;
+    scanTiles[tid] = tiles;
+    scanChunks[tid] = chunks;
+    __syncthreads();
+    // inclusive Hillis-Steele scan
+    for (int offset = 1; offset < kCompactThreads; offset *= 2) {
+        const int a = tid >= offset ? scanRun[tid - offset] : 0;
+        const int b = tid >= offset ? scanTiles[tid - offset] : 0;
+        const int c = tid >= offset ? scanChunks[tid - offset] : 0;
+        __syncthreads();
+        scanRun[tid] += a;
+        scanTiles[tid] += b;
+        scanChunks[tid] += c;
+        __syncthreads();
+    }
+    int k = scanRun[tid] - run, tileAt = scanTiles[tid] - tiles, chunkAt = scanChunks[tid] - chunks;
+    for (int s = lo; s < hi; ++s) {
+        if (!selected(s)) continue;
+        work.systems[k] = s;
+        work.tilePrefix[k] = tileAt;
+        work.chunkPrefix[k] = chunkAt;
+        tileAt += systemTiles[s + 1] - systemTiles[s];
+        chunkAt += systemChunks[s + 1] - systemChunks[s];
+        k += 1;
+    }
+    if (tid == kCompactThreads - 1) {
+        work.count[0] = scanRun[tid];
+        work.count[1] = scanTiles[tid];
+        work.count[2] = kSelect == kRunning ? scanChunks[tid] : 0;
+        work.tilePrefix[scanRun[tid]] = scanTiles[tid];
+        work.chunkPrefix[scanRun[tid]] = scanChunks[tid];
+    }
+}
+
+// Expands the selected systems into their items: thread i writes tile item i
+// and chunk item i, {index, system}, by binary search over the prefix sums.
+__global__ __launch_bounds__(kTileBlock) void batchExpandKernel(const int32_t* __restrict__ systemTiles,
+                                                                const int32_t* __restrict__ systemChunks,
+                                                                ActiveWork work) {
+    const int i = int(blockIdx.x) * kTileBlock + int(threadIdx.x);
+    const int selected = work.count[0];
+    if (i < work.count[1]) {
+        const int k = locate(i, work.tilePrefix, selected);
+        const int s = work.systems[k];
+        work.tileList[i] = make_int2(systemTiles[s] + (i - work.tilePrefix[k]), s);
+    }
+    if (i < work.count[2]) {
+        const int k = locate(i, work.chunkPrefix, selected);
+        const int s = work.systems[k];
+        work.chunkList[i] = make_int2(systemChunks[s] + (i - work.chunkPrefix[k]), s);
+    }
+}
+
 __global__ void batchCheckKernel(int systems, const int64_t* __restrict__ offsets, const int32_t* __restrict__ rowExtents,
                                  const int32_t* __restrict__ columnIndices, int* __restrict__ bad) {
     const int s = blockIdx.x;
@@ -300,11 +428,17 @@ struct cie_hip_batch_s {
     int systems = 0;
     std::vector<int64_t> offsets;       // host, systems + 1
     std::vector<int32_t> tileRows;      // rows per tile of each system
-    Tile* tiles = nullptr;
+    int2* tiles = nullptr;              // device, tileCount + 1: {first row, first entry}
     int32_t* systemTiles = nullptr;     // device, systems + 1
     int tileCount = 0;
+    int32_t* chunkRows = nullptr;       // device, chunkCount + 1
+    int32_t* systemChunks = nullptr;    // device, systems + 1
+    int chunkCount = 0;
+    ActiveWork running{};               // device work list of the running systems
+    ActiveWork refining{};              // device work list of the systems refining this trip (mixed)
     SystemState* state = nullptr;
-    double* partials = nullptr;
+    double* tilePartials = nullptr;     // 2 per tile
+    double* chunkPartials = nullptr;    // 2 per chunk
     int* active = nullptr;
     int* pinnedActive = nullptr;
     SystemState* pinnedStates = nullptr;
@@ -322,38 +456,100 @@ void releaseGraph(cie_hip_batch batch) {
     batch->graph = nullptr;
 }
 
-int buildTiles(cie_hip_batch batch) {
-    std::vector<Tile> tiles;
+template <class T>
+void freeDevice(T*& pointer) {
+    if (pointer) (void)hipFree(pointer);
+    pointer = nullptr;
+}
+
+// Cuts every system into block tiles (<= kTileEntries entries, <= tileRows[s]
+// rows; a longer row is a tile of its own) and into chunks of kChunkRows rows.
+int buildWork(cie_hip_batch batch) {
+    const int32_t* extents = batch->matrix->hostExtents.data();
+    std::vector<int2> tiles;
     std::vector<int32_t> systemTiles(size_t(batch->systems) + 1, 0);
+    std::vector<int32_t> chunkRows;
+    std::vector<int32_t> systemChunks(size_t(batch->systems) + 1, 0);
     for (int s = 0; s < batch->systems; ++s) {
         systemTiles[s] = int32_t(tiles.size());
-        const int rows = batch->tileRows[s];
-        for (int64_t first = batch->offsets[s]; first < batch->offsets[s + 1]; first += rows)
-            tiles.push_back({int32_t(first), int32_t(std::min<int64_t>(rows, batch->offsets[s + 1] - first)), s, 0});
+        systemChunks[s] = int32_t(chunkRows.size());
+        const int lo = int(batch->offsets[s]), hi = int(batch->offsets[s + 1]);
+        const int cap = batch->tileRows[s];
+        int row = lo;
+        while (row < hi) {
+            const int first = row;
+            if (extents[row + 1] - extents[row] > kTileEntries) {
+                ++row;
+            } else {
+                while (row < hi && row - first < cap && extents[row + 1] - extents[first] <= kTileEntries) ++row;
+            }
+            tiles.push_back(make_int2(first, extents[first]));
+        }
+        for (int first = lo; first < hi; first += kChunkRows) chunkRows.push_back(first);
     }
+    const int rows = int(batch->offsets[batch->systems]);
     systemTiles[batch->systems] = int32_t(tiles.size());
-    if (batch->tiles) CIE_HIP_CHECK(hipFree(batch->tiles));
-    if (batch->partials) CIE_HIP_CHECK(hipFree(batch->partials));
-    batch->tiles = nullptr;
-    batch->partials = nullptr;
+    systemChunks[batch->systems] = int32_t(chunkRows.size());
     batch->tileCount = int(tiles.size());
-    CIE_HIP_CHECK(hipMalloc(&batch->tiles, sizeof(Tile) * std::max<size_t>(1, tiles.size())));
-    CIE_HIP_CHECK(hipMalloc(&batch->partials, 2 * sizeof(double) * std::max<size_t>(1, tiles.size())));
-    if (!tiles.empty())
-        CIE_HIP_CHECK(hipMemcpy(batch->tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    batch->chunkCount = int(chunkRows.size());
+    tiles.push_back(make_int2(rows, extents[rows]));
+    chunkRows.push_back(rows);
+    freeDevice(batch->tiles);
+    freeDevice(batch->chunkRows);
+    freeDevice(batch->tilePartials);
+    freeDevice(batch->chunkPartials);
+    for (ActiveWork* work : {&batch->running, &batch->refining}) {
+        freeDevice(work->tileList);
+        freeDevice(work->chunkList);
+        CIE_HIP_CHECK(hipMalloc(&work->tileList, sizeof(int2) * std::max(1, batch->tileCount)));
+        CIE_HIP_CHECK(hipMalloc(&work->chunkList, sizeof(int2) * std::max(1, batch->chunkCount)));
+    }
+    CIE_HIP_CHECK(hipMalloc(&batch->tiles, sizeof(int2) * tiles.size()));
+    CIE_HIP_CHECK(hipMalloc(&batch->chunkRows, sizeof(int32_t) * chunkRows.size()));
+    CIE_HIP_CHECK(hipMalloc(&batch->tilePartials, 2 * sizeof(double) * std::max(1, batch->tileCount)));
+    CIE_HIP_CHECK(hipMalloc(&batch->chunkPartials, 2 * sizeof(double) * std::max(1, batch->chunkCount)));
+    CIE_HIP_CHECK(hipMemcpy(batch->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+    CIE_HIP_CHECK(hipMemcpy(batch->chunkRows, chunkRows.data(), sizeof(int32_t) * chunkRows.size(),
+                            hipMemcpyHostToDevice));
     CIE_HIP_CHECK(hipMemcpy(batch->systemTiles, systemTiles.data(), sizeof(int32_t) * systemTiles.size(),
+                            hipMemcpyHostToDevice));
+    CIE_HIP_CHECK(hipMemcpy(batch->systemChunks, systemChunks.data(), sizeof(int32_t) * systemChunks.size(),
                             hipMemcpyHostToDevice));
     releaseGraph(batch);
     return CIE_HIP_OK;
 }
 
-// One resident round of blocks of `kernel` (grid-stride covers the rest).
+int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
+    cie_hip_csr m = batch->matrix;
+    const int status = ensureNarrowColumns(m);
+    if (status != CIE_HIP_OK) return status;
+    RowSource& source = layout->source;
+    source.rows = int(m->rows);
+    source.param = 1;
+    source.rowExtents = m->rowExtents;
+    source.columnIndices = m->columnIndices;
+    source.tiles = batch->tiles;
+    source.tileCount = batch->tileCount;
+    source.narrowRuns = m->narrowState > 0 ? m->narrowRuns : 0;
+    source.narrowColumns = m->narrowState > 0 ? m->narrowColumns : nullptr;
+    source.narrowBases = m->narrowState > 0 ? m->narrowBases : nullptr;
+    layout->systemTiles = batch->systemTiles;
+    layout->systemChunks = batch->systemChunks;
+    layout->chunkRows = batch->chunkRows;
+    return CIE_HIP_OK;
+}
+
+
+// Blocks of a work kernel. The main kernels get one block per item (blocks
+// past the active count exit at once; the hardware balances the rest as
+// blocks retire). The mixed form's residual kernel, which walks the usually
+// empty list of refining systems, gets one resident round walking its items
+// grid-stride, so an empty list costs a few hundred empty blocks.
 template <class Kernel>
-dim3 tileGrid(Kernel kernel, cie_hip_context ctx, int tiles) {
-    const int wanted = (tiles + kBatchWaves - 1) / kBatchWaves;
-    const int resident = std::max(1, ctx->computeUnits) *
-                         residentBlocks(reinterpret_cast<const void*>(kernel), kBatchBlock);
-    return dim3(std::max(1, std::min(wanted, resident)));
+dim3 workGrid(Kernel kernel, cie_hip_context ctx, int block, int items, bool residentRound = false) {
+    if (!residentRound) return dim3(std::max(1, items));
+    const int resident = std::max(1, ctx->computeUnits) * residentBlocks(reinterpret_cast<const void*>(kernel), block);
+    return dim3(std::max(1, std::min(items, resident)));
 }
 
 void releaseBatch(cie_hip_batch batch) {
@@ -363,51 +559,75 @@ void releaseBatch(cie_hip_batch batch) {
         (void)hipStreamSynchronize(batch->context->stream);
     }
     releaseGraph(batch);
-    if (batch->tiles) (void)hipFree(batch->tiles);
-    if (batch->systemTiles) (void)hipFree(batch->systemTiles);
-    if (batch->state) (void)hipFree(batch->state);
-    if (batch->partials) (void)hipFree(batch->partials);
-    if (batch->active) (void)hipFree(batch->active);
+    freeDevice(batch->tiles);
+    freeDevice(batch->systemTiles);
+    freeDevice(batch->chunkRows);
+    freeDevice(batch->systemChunks);
+    for (ActiveWork* work : {&batch->running, &batch->refining}) {
+        freeDevice(work->systems);
+        freeDevice(work->tilePrefix);
+        freeDevice(work->chunkPrefix);
+        freeDevice(work->count);
+        freeDevice(work->tileList);
+        freeDevice(work->chunkList);
+    }
+    freeDevice(batch->state);
+    freeDevice(batch->tilePartials);
+    freeDevice(batch->chunkPartials);
+    freeDevice(batch->active);
     if (batch->pinnedActive) (void)hipHostFree(batch->pinnedActive);
     if (batch->pinnedStates) (void)hipHostFree(batch->pinnedStates);
-    if (batch->workspace) (void)hipFree(batch->workspace);
+    freeDevice(batch->workspace);
     delete batch;
+}
+
+// Rebuilds a work list from the system states.
+template <int kSelect>
+void launchSelect(cie_hip_batch batch, const ActiveWork& work) {
+    const hipStream_t stream = batch->context->stream;
+    hipLaunchKernelGGL((batchCompactKernel<kSelect>), dim3(1), dim3(kCompactThreads), 0, stream, batch->systems,
+                       batch->state, batch->systemTiles, batch->systemChunks, work);
+    const int items = std::max(batch->tileCount, kSelect == kRunning ? batch->chunkCount : 0);
+    hipLaunchKernelGGL(batchExpandKernel, dim3(std::max(1, (items + kTileBlock - 1) / kTileBlock)), dim3(kTileBlock), 0,
+                       stream, batch->systemTiles, batch->systemChunks, work);
 }
 
 // Enqueues the kernels of one iteration on the context stream.
 template <class TR>
-void enqueueIteration(cie_hip_batch batch, const double* b, double* x, TR* r, TR* p, TR* q, const TR* inverse,
-                      const BatchSettings& settings, bool mixed) {
+void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
+                      const TR* inverse, const BatchSettings& settings, bool mixed) {
     cie_hip_context ctx = batch->context;
     cie_hip_csr m = batch->matrix;
     const hipStream_t stream = ctx->stream;
-    const int tiles = batch->tileCount;
-    const dim3 block(kBatchBlock), systems(batch->systems);
+    const ActiveWork& work = batch->running;
+    const dim3 tileBlock(kTileBlock), chunkBlock(kChunkRows), systems(batch->systems);
     const TR* entries;
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
-    hipLaunchKernelGGL((batchProductKernel<TR>), tileGrid(batchProductKernel<TR>, ctx, tiles), block, 0, stream, 0,
-                       tiles, batch->tiles, m->rowExtents, m->columnIndices, entries, p, q, batch->partials,
-                       batch->state);
-    hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, block, 0, stream, batch->systemTiles,
-                       batch->partials, batch->state, settings, batch->active);
-    hipLaunchKernelGGL((batchUpdateKernel<TR, double>), tileGrid(batchUpdateKernel<TR, double>, ctx, tiles), block, 0,
-                       stream, 0, tiles, batch->tiles, x, r, p, q, inverse, batch->partials, batch->state);
+    hipLaunchKernelGGL((batchProductKernel<TR>), workGrid(batchProductKernel<TR>, ctx, kTileBlock, batch->tileCount),
+                       tileBlock, 0, stream, layout, work, entries, p, q, batch->tilePartials);
+    hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
+                       batch->tilePartials, batch->state, settings, batch->active);
+    hipLaunchKernelGGL((batchUpdateKernel<TR, double>),
+                       workGrid(batchUpdateKernel<TR, double>, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
+                       stream, layout, work, x, r, p, q, inverse, batch->chunkPartials, batch->state);
     if (mixed) {
-        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdateMixed>), systems, block, 0, stream, batch->systemTiles,
-                           batch->partials, batch->state, settings, batch->active);
-        hipLaunchKernelGGL((batchResidualKernel<TR, false>), tileGrid(batchResidualKernel<TR, false>, ctx, tiles),
-                           block, 0, stream, 0, batch->tileCount, batch->tiles,
-                           m->rowExtents, m->columnIndices, m->entries64, x, b, r, inverse, p, batch->partials,
+        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdateMixed>), systems, tileBlock, 0, stream,
+                           batch->systemChunks, batch->chunkPartials, batch->state, settings, batch->active);
+        launchSelect<kRefining>(batch, batch->refining);
+        hipLaunchKernelGGL((batchResidualKernel<TR, false>),
+                           workGrid(batchResidualKernel<TR, false>, ctx, kTileBlock, batch->tileCount, true), tileBlock, 0,
+                           stream, layout, batch->refining, m->entries64, x, b, r, inverse, p, batch->tilePartials,
                            batch->state);
-        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseRefine>), systems, block, 0, stream, batch->systemTiles,
-                           batch->partials, batch->state, settings, batch->active);
+        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseRefine>), systems, tileBlock, 0, stream, batch->systemTiles,
+                           batch->tilePartials, batch->state, settings, batch->active);
     } else {
-        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdate>), systems, block, 0, stream, batch->systemTiles,
-                           batch->partials, batch->state, settings, batch->active);
+        hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdate>), systems, tileBlock, 0, stream, batch->systemChunks,
+                           batch->chunkPartials, batch->state, settings, batch->active);
     }
-    hipLaunchKernelGGL((batchDirectionKernel<TR>), tileGrid(batchDirectionKernel<TR>, ctx, tiles), block, 0, stream, 0, batch->tileCount, batch->tiles, p, r,
-                       inverse, batch->state);
+    hipLaunchKernelGGL((batchDirectionKernel<TR>), workGrid(batchDirectionKernel<TR>, ctx, kChunkRows, batch->chunkCount),
+                       chunkBlock, 0, stream, layout, work, p, r, inverse, batch->state);
+    launchSelect<kRunning>(batch, work);
 }
 
 template <class TR>
@@ -418,13 +638,15 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     const int64_t n = m->rows;
     int status = ensureInverseDiagonal(m, mixed ? 32 : 64);
     if (status) return status;
+    BatchLayout layout{};
+    status = batchLayout(batch, &layout);
+    if (status) return status;
     const TR* inverse;
     if constexpr (std::is_same_v<TR, float>) inverse = m->inverseDiagonal32;
     else inverse = m->inverseDiagonal64;
     const size_t vector = (sizeof(TR) * size_t(std::max<int64_t>(1, n)) + 255) / 256 * 256;
     if (batch->workspaceBytes < 3 * vector) {
-        if (batch->workspace) CIE_HIP_CHECK(hipFree(batch->workspace));
-        batch->workspace = nullptr;
+        freeDevice(batch->workspace);
         CIE_HIP_CHECK(hipMalloc(&batch->workspace, 3 * vector));
         batch->workspaceBytes = 3 * vector;
         releaseGraph(batch);
@@ -452,12 +674,14 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
         (void)hipEventDestroy(marks[1]);
     };
     CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
-    const dim3 grid(tileGrid(batchResidualKernel<TR, true>, ctx, batch->tileCount)), block(kBatchBlock), systems(batch->systems);
-    hipLaunchKernelGGL((batchResidualKernel<TR, true>), grid, block, 0, ctx->stream, 0, batch->tileCount, batch->tiles,
-                       m->rowExtents, m->columnIndices, m->entries64, x, b, r, inverse, p, batch->partials,
-                       batch->state);
-    hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseStart>), systems, block, 0, ctx->stream, batch->systemTiles,
-                       batch->partials, batch->state, settings, batch->active);
+    const ActiveWork& work = batch->running;
+    launchSelect<kRunning>(batch, work);  // every system runs
+    hipLaunchKernelGGL((batchResidualKernel<TR, true>),
+                       workGrid(batchResidualKernel<TR, true>, ctx, kTileBlock, batch->tileCount), dim3(kTileBlock), 0,
+                       ctx->stream, layout, work, m->entries64, x, b, r, inverse, p, batch->tilePartials, batch->state);
+    hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseStart>), dim3(batch->systems), dim3(kTileBlock), 0, ctx->stream,
+                       batch->systemTiles, batch->tilePartials, batch->state, settings, batch->active);
+    launchSelect<kRunning>(batch, work);
     CIE_HIP_CHECK(hipGetLastError());
 
     if (useGraph) {
@@ -468,7 +692,8 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             releaseGraph(batch);
             hipGraph_t graph = nullptr;
             CIE_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < kGraphIterations; ++i) enqueueIteration<TR>(batch, b, x, r, p, q, inverse, settings, mixed);
+            for (int i = 0; i < kGraphIterations; ++i)
+                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed);
             const hipError_t captured = hipStreamEndCapture(ctx->stream, &graph);
             if (captured != hipSuccess) {
                 cleanup();
@@ -495,7 +720,8 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
         if (useGraph) {
             CIE_HIP_CHECK(hipGraphLaunch(batch->graph, ctx->stream));
         } else {
-            for (int i = 0; i < kGraphIterations; ++i) enqueueIteration<TR>(batch, b, x, r, p, q, inverse, settings, mixed);
+            for (int i = 0; i < kGraphIterations; ++i)
+                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed);
             CIE_HIP_CHECK(hipGetLastError());
         }
         ++count;
@@ -544,13 +770,21 @@ int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t sy
     b->matrix = matrix;
     b->systems = systems;
     b->offsets.assign(offsets, offsets + systems + 1);
-    b->tileRows.assign(size_t(systems), kWave);
+    b->tileRows.assign(size_t(systems), kDefaultTileRows);
     auto bail = [&](int status, const std::string& message) {
         releaseBatch(b);
         return setError(status, message);
     };
     int64_t* deviceOffsets = nullptr;
-    if (hipMalloc(&b->systemTiles, sizeof(int32_t) * (systems + 1)) != hipSuccess ||
+    const size_t perSystem = sizeof(int32_t) * (size_t(systems) + 1);
+    bool lists = true;
+    for (ActiveWork* work : {&b->running, &b->refining})
+        lists = lists && hipMalloc(&work->systems, perSystem) == hipSuccess &&
+                hipMalloc(&work->tilePrefix, perSystem) == hipSuccess &&
+                hipMalloc(&work->chunkPrefix, perSystem) == hipSuccess &&
+                hipMalloc(&work->count, sizeof(int32_t) * 4) == hipSuccess;
+    if (!lists || hipMalloc(&b->systemTiles, perSystem) != hipSuccess ||
+        hipMalloc(&b->systemChunks, perSystem) != hipSuccess ||
         hipMalloc(&b->state, sizeof(SystemState) * systems) != hipSuccess ||
         hipMalloc(&b->active, sizeof(int) * 2) != hipSuccess ||
         hipHostMalloc(&b->pinnedActive, sizeof(int) * 4) != hipSuccess ||
@@ -570,7 +804,7 @@ int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t sy
     (void)hipFree(deviceOffsets);
     if (!ok) return bail(CIE_HIP_ERROR_HIP, "cie_hip_batch_create: block check failed to run");
     if (b->pinnedActive[0]) return bail(CIE_HIP_ERROR_SHAPE, "cie_hip_batch_create: the matrix is not block diagonal");
-    const int status = buildTiles(b);
+    const int status = buildWork(b);
     if (status) return bail(status, cie_hip_last_error());
     *batch = b;
     return CIE_HIP_OK;
@@ -584,10 +818,11 @@ int cie_hip_batch_destroy(cie_hip_batch batch) {
 int cie_hip_batch_set_tile_rows(cie_hip_batch batch, const int32_t* rowsPerTile) {
     if (!batch || !rowsPerTile) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_tile_rows: null argument");
     for (int s = 0; s < batch->systems; ++s)
-        if (rowsPerTile[s] != 16 && rowsPerTile[s] != 32 && rowsPerTile[s] != 64)
-            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_tile_rows: rows per tile must be 16, 32 or 64");
+        if (rowsPerTile[s] != 32 && rowsPerTile[s] != 64 && rowsPerTile[s] != 128 && rowsPerTile[s] != 256)
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT,
+                            "cie_hip_batch_set_tile_rows: rows per tile must be 32, 64, 128 or 256");
     batch->tileRows.assign(rowsPerTile, rowsPerTile + batch->systems);
-    return buildTiles(batch);
+    return buildWork(batch);
 }
 
 int cie_hip_batch_get_tile_rows(cie_hip_batch batch, int32_t* rowsPerTile) {
@@ -606,39 +841,36 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     const size_t valueSize = precision == 32 ? sizeof(float) : sizeof(double);
     void* buffer = nullptr;
     CIE_HIP_CHECK(hipMalloc(&buffer, 2 * valueSize * size_t(std::max<int64_t>(1, m->rows))));
-    std::vector<SystemState> running(size_t(batch->systems));
-    CIE_HIP_CHECK(hipMemcpy(batch->state, running.data(), sizeof(SystemState) * running.size(), hipMemcpyHostToDevice));
     CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, 2 * valueSize * size_t(std::max<int64_t>(1, m->rows)), ctx->stream));
     hipEvent_t start, stop;
     CIE_HIP_CHECK(hipEventCreate(&start));
     CIE_HIP_CHECK(hipEventCreate(&stop));
     std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
-    std::vector<int32_t> choice(size_t(batch->systems), kWave);
+    std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
     int status = CIE_HIP_OK;
-    for (int rows : {16, 32, 64}) {
-        std::vector<int32_t> uniform(size_t(batch->systems), rows);
-        batch->tileRows = uniform;
-        status = buildTiles(batch);
+    for (int rows : {32, 64, 128, 256}) {
+        batch->tileRows.assign(size_t(batch->systems), rows);
+        status = buildWork(batch);
+        if (status) break;
+        BatchLayout layout{};
+        status = batchLayout(batch, &layout);
         if (status) break;
         std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
         CIE_HIP_CHECK(hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
                                 hipMemcpyDeviceToHost));
         for (int s = 0; s < batch->systems && status == CIE_HIP_OK; ++s) {
-            const int t0 = systemTiles[s], t1 = systemTiles[s + 1];
-            const dim3 grid(precision == 32 ? tileGrid(batchProductKernel<float>, ctx, t1 - t0) : tileGrid(batchProductKernel<double>, ctx, t1 - t0)), block(kBatchBlock);
+            const int t0 = systemTiles[s], tiles = systemTiles[s + 1] - t0;
             for (int pass = 0; pass < 2; ++pass) {
                 (void)hipEventRecord(start, ctx->stream);
                 for (int r = 0; r < (pass ? repetitions : 1); ++r) {
                     if (precision == 32) {
                         float* in = static_cast<float*>(buffer);
-                        hipLaunchKernelGGL((batchProductKernel<float>), grid, block, 0, ctx->stream, t0, t1,
-                                           batch->tiles, m->rowExtents, m->columnIndices, m->entries32, in,
-                                           in + m->rows, batch->partials, batch->state);
+                        hipLaunchKernelGGL((batchTileRangeKernel<float>), dim3(tiles), dim3(kTileBlock), 0, ctx->stream,
+                                           layout.source, t0, m->entries32, in, in + m->rows);
                     } else {
                         double* in = static_cast<double*>(buffer);
-                        hipLaunchKernelGGL((batchProductKernel<double>), grid, block, 0, ctx->stream, t0, t1,
-                                           batch->tiles, m->rowExtents, m->columnIndices, m->entries64, in,
-                                           in + m->rows, batch->partials, batch->state);
+                        hipLaunchKernelGGL((batchTileRangeKernel<double>), dim3(tiles), dim3(kTileBlock), 0,
+                                           ctx->stream, layout.source, t0, m->entries64, in, in + m->rows);
                     }
                 }
                 if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
@@ -659,7 +891,7 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     (void)hipFree(buffer);
     if (status) return status;
     batch->tileRows = choice;
-    return buildTiles(batch);
+    return buildWork(batch);
 }
 
 int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const double* b, double* x,

@@ -220,9 +220,12 @@ int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScal
  * all systems advance together, one kernel launch per step for the whole
  * batch, with per-system scalars and convergence masks on the device; one
  * iteration's kernels are captured in a HIP graph (useGraph != 0) and
- * replayed until every system is done. Rows are processed in tiles of 16, 32
- * or 64 rows per wavefront (never straddling two systems), chosen per system
- * by cie_hip_batch_autotune from timed runs (default 64). */
+ * replayed until every system is done. The SpMV walks block tiles (one
+ * workgroup per tile of whole rows of one system, at most 2048 entries and
+ * rowsPerTile[s] rows: 32, 64, 128 or 256, chosen per system by
+ * cie_hip_batch_autotune from timed runs; default 256); the vector steps walk
+ * chunks of 1024 rows of one system. After every step the finished systems are
+ * dropped from the work lists on the device. */
 typedef struct cie_hip_batch_s* cie_hip_batch;
 int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t systems, const int64_t* offsets,
                          cie_hip_batch* batch);

@@ -56,7 +56,7 @@ def test_batched_graph_replays_match_eager(hip, torch_gpu):
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
     batch.autotune(32, 2)
-    assert set(batch.tile_rows().tolist()) <= {16, 32, 64}
+    assert set(batch.tile_rows().tolist()) <= {32, 64, 128, 256}
     b = torch_gpu.from_numpy(w.rhs).cuda()
     results = []
     for graph in (False, True, True, True):   # the graph is captured once and replayed on later calls
@@ -91,4 +91,24 @@ def test_full_batch_256_uneven_systems(hip, torch_gpu):
     assert np.all(residuals < 1e-10)
     print(f"batch: {w.matrix.rows} rows, {w.matrix.nnz} nnz, {summary.seconds:.3f} s, max iterations "
           f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, tile rows "
-          f"{np.bincount(batch.tile_rows(), minlength=65)[[16, 32, 64]].tolist()}")
+          f"{np.bincount(batch.tile_rows(), minlength=257)[[32, 64, 128, 256]].tolist()}")
+
+
+def test_batched_tile_rows(hip, torch_gpu):
+    """Rows per block tile change the sum order only: same iteration counts (within one), same solutions."""
+    w = small_batch(seed=10)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    results = []
+    for rows in (32, 256):
+        batch.set_tile_rows(np.full(batch.systems, rows, dtype=np.int32))
+        assert set(batch.tile_rows().tolist()) == {rows}
+        x = torch_gpu.zeros_like(b)
+        stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=False)
+        assert np.all(status == 1)
+        results.append((x.cpu().numpy(), np.array([s.iterationCount for s in stats])))
+    assert np.all(np.abs(results[0][1].astype(int) - results[1][1].astype(int)) <= 1)
+    assert np.linalg.norm(results[0][0] - results[1][0]) <= 1e-9 * np.linalg.norm(results[1][0])
+    with pytest.raises(hip._lib.CieHipError):
+        batch.set_tile_rows(np.full(batch.systems, 48, dtype=np.int32))
