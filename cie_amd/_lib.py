@@ -161,6 +161,11 @@ _SIGNATURES = {
     # adjoint
     "cie_hip_element_gradient": (c_int, [_P, c_int64, c_int32, _P, _P, _P, _P, _P, _P]),
     "cie_hip_csr_add_element_matrices": (c_int, [_P, c_int64, c_int32, _P, _P, _P]),
+    # parameterised stiffness assembly
+    "cie_hip_assembly_create": (c_int, [_P, c_int64, c_int32, _P, POINTER(_P)]),
+    "cie_hip_assembly_destroy": (c_int, [_P]),
+    "cie_hip_assemble": (c_int, [_P, _P, c_int, _P]),
+    "cie_hip_assemble_vector": (c_int, [_P, _P, c_int, _P, _P]),
     # structured systems (include/cie_hip_systems.h)
     "cie_sys_last_error": (c_char_p, []),
     "cie_sys_splitmix64": (c_int, [c_uint64, c_int64, _P]),

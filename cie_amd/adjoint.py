@@ -8,8 +8,9 @@ and J(g) = 0.5 ||u(g) - u_target||^2 with K(k) u = f:
     K lambda = u - u_target            (adjoint; K is symmetric)
     dJ/dg_e  = -k_e lambda_e^T K_e0 u_e
 
-Both solves run the fused FP64 PCG through the C ABI; the element gather runs
-in cie_hip_element_gradient. ``Objective`` wraps this as a
+K(exp(g)) is assembled on the device (cie_hip_assemble), both solves run the
+fused FP64 PCG through the C ABI, and the element gather runs in
+cie_hip_element_gradient. ``Objective`` wraps this as a
 torch.autograd.Function; ``finite_difference`` is the central-difference check
 used by the tests.
 """
@@ -52,14 +53,22 @@ class LagrangeProblem:
         self.tolerance = tolerance
         self.matrix = None
         self.host_matrix = None
+        self.assembly = None
 
     def assemble(self, g):
-        """K(exp(g)) on the host generator, uploaded with FP64 and FP32 planes."""
+        """K(exp(g)) on the device: the pattern once from the structured generator, the values by
+        the device assembly (bitwise the generator's values for the same k)."""
+        torch = _torch()
+        if torch.is_tensor(g):
+            g = g.detach().cpu().numpy()
         k = np.exp(np.asarray(g, dtype=np.float64))
-        self.host_matrix = systems.lagrange(self.dim, self.order, self.elements, conductivity=k, threads=self.threads)
-        if self.matrix is not None:
-            self.matrix.close()
-        self.matrix = linalg.CSRMatrix(self.host_matrix, self.context)
+        if k.shape != (self.count,):
+            raise ValueError(f"expected {self.count} element values, got {k.shape}")
+        if self.matrix is None:
+            self.host_matrix = systems.lagrange(self.dim, self.order, self.elements, threads=self.threads)
+            self.matrix = linalg.CSRMatrix(self.host_matrix, self.context)
+            self.assembly = linalg.Assembly(self.matrix, self.dofs_host)
+        self.assembly.assemble(self.element_matrix, torch.from_numpy(k).to(self.device))
         return self.matrix
 
     def solve(self, b, x0=None, tolerance=None, absolute=0.0):
@@ -136,8 +145,7 @@ def finite_difference(problem, g, u, elements, h=1e-6, tolerance=1e-13):
                                             _dptr(r0), problem.rows), "r0")
     problem.context.synchronize()
     u_host = u.cpu().numpy()
-    base_entries = torch.from_numpy(problem.host_matrix.entries).to(device)
-    csr = problem.host_matrix
+    csr = problem.host_matrix  # the pattern
     results = []
     for element in elements:
         d = problem.dofs_host[element]
@@ -161,7 +169,7 @@ def finite_difference(problem, g, u, elements, h=1e-6, tolerance=1e-13):
             rhs[rows] -= torch.from_numpy(c * local[keep]).to(device)
             delta, _ = problem.solve(rhs, tolerance=tolerance)
             deltas[sign] = delta
-            matrix.update_entries(base_entries)  # restore K(g) exactly
+            problem.assemble(g)  # restore K(g) exactly (the assembly is deterministic)
         plus, minus = deltas[+1], deltas[-1]
         difference = torch.dot(plus - minus, u - problem.target + 0.5 * (plus + minus)).item()
         results.append(difference / (2.0 * h))

@@ -445,6 +445,73 @@ class MixedPCG(IterativeSolver):
         self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
 
 
+class Assembly:
+    """K = sum_e scale[e] P_e^T K_e0 P_e on the pattern of `matrix`, assembled on the device
+    (cie_hip_assembly; the element scatter of fem/packages/graph/impl/Assembler_impl.hpp:156-194)."""
+
+    def __init__(self, matrix, dofs):
+        self.matrix = matrix
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        if dofs.ndim != 2:
+            raise ValueError("dofs must be elements x nodes per element")
+        self.elements, self.nodes = dofs.shape
+        self.handle = c_void_p()
+        check(library().cie_hip_assembly_create(matrix.handle, self.elements, self.nodes,
+                                                dofs.ctypes.data_as(c_void_p), byref(self.handle)),
+              "cie_hip_assembly_create")
+
+    def assemble(self, element_matrices, scale=None):
+        """Overwrite the matrix values with sum_e scale[e] K_e. element_matrices (device tensor) is one
+        nodes x nodes matrix shared by all elements or elements x nodes x nodes; scale (elements) or None."""
+        self._check_values(element_matrices)
+        shared = element_matrices.numel() == self.nodes * self.nodes and element_matrices.dim() <= 2
+        if not shared and element_matrices.numel() != self.elements * self.nodes * self.nodes:
+            raise ValueError("element matrices have the wrong size")
+        self._check_scale(scale)
+        _sync_torch()
+        check(library().cie_hip_assemble(self.handle, _dptr(element_matrices), 0 if shared else 1,
+                                         None if scale is None else _dptr(scale)), "cie_hip_assemble")
+
+    def assemble_vector(self, element_vectors, scale=None, out=None):
+        """sum_e scale[e] P_e^T v_e as a device vector; element_vectors: one nodes vector or elements x nodes."""
+        torch = _torch()
+        self._check_values(element_vectors)
+        shared = element_vectors.numel() == self.nodes and element_vectors.dim() == 1
+        if not shared and element_vectors.numel() != self.elements * self.nodes:
+            raise ValueError("element vectors have the wrong size")
+        self._check_scale(scale)
+        if out is None:
+            out = torch.empty(self.matrix.rows, dtype=torch.float64, device=element_vectors.device)
+        elif out.numel() != self.matrix.rows or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 vector of one value per row")
+        _sync_torch()
+        check(library().cie_hip_assemble_vector(self.handle, _dptr(element_vectors), 0 if shared else 1,
+                                                None if scale is None else _dptr(scale), _dptr(out)),
+              "cie_hip_assemble_vector")
+        return out
+
+    @staticmethod
+    def _check_values(values):
+        if values.dtype != _torch().float64 or not values.is_contiguous() or values.device.type != "cuda":
+            raise ValueError("element values must be a contiguous float64 device tensor")
+
+    def _check_scale(self, scale):
+        if scale is not None and (scale.numel() != self.elements or scale.dtype != _torch().float64
+                                  or not scale.is_contiguous() or scale.device.type != "cuda"):
+            raise ValueError("scale needs one contiguous float64 value per element")
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_assembly_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Many independent systems on the diagonal blocks of one CSRMatrix (cie_hip_batch)."""
 

@@ -266,6 +266,34 @@ int cie_hip_element_gradient(cie_hip_context context, int64_t elements, int32_t 
 int cie_hip_csr_add_element_matrices(cie_hip_csr matrix, int64_t count, int32_t nodesPerElement, const int64_t* slots,
                                      const double* elementMatrix, const double* factors);
 
+/* ---- Parameterised stiffness assembly ------------------------------------ */
+/* K = sum_e scale[e] P_e^T K_e0 P_e on the pattern of `matrix`: the element
+ * scatter of fem/packages/graph/impl/Assembler_impl.hpp:156-194 for an
+ * integrand linear in the cell modulus (LinearIsotropicStiffnessIntegrand_impl.hpp:74),
+ * done on the device as a row gather: no atomics, every slot summed in element
+ * order from zero, so the values are bitwise those of a sequential scatter in
+ * element order (and of cie_sys_lagrange_fill). The plan is made once per
+ * pattern: dofs (host) holds elements x nodesPerElement row indices, -1 for
+ * eliminated nodes; every kept node pair must be in the pattern
+ * (CIE_HIP_ERROR_SHAPE otherwise). */
+typedef struct cie_hip_assembly_s* cie_hip_assembly;
+int cie_hip_assembly_create(cie_hip_csr matrix, int64_t elements, int32_t nodesPerElement, const int32_t* dofs,
+                            cie_hip_assembly* plan);
+int cie_hip_assembly_destroy(cie_hip_assembly plan);
+/* Overwrites the FP64 entries (and the FP32 plane) of the plan's matrix with
+ * sum_e scale[e] K_e[a][b]; scale NULL means 1. perElement 0: elementMatrices
+ * is one nodesPerElement^2 matrix K_e0 shared by all elements (row-major);
+ * perElement 1: elements x nodesPerElement^2, one matrix per element (the
+ * contributions of Assembler::addContribution). Device arrays. Drops the
+ * cached Jacobi diagonal. The matrix must own its FP64 entries. */
+int cie_hip_assemble(cie_hip_assembly plan, const double* elementMatrices, int perElement, const double* scale);
+/* out[rows] = sum_e scale[e] P_e^T v_e, elements in increasing order: the
+ * vector overload of Assembler::addContribution (Assembler_impl.hpp:197-220).
+ * perElement 0: one nodesPerElement vector; 1: elements x nodesPerElement.
+ * Device arrays. */
+int cie_hip_assemble_vector(cie_hip_assembly plan, const double* elementVectors, int perElement, const double* scale,
+                            double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,8 @@ def lib():
         l.oracle_pcg.argtypes = [c_int64, P, P, P, c_double, P, c_double, P, c_uint64, c_double, c_double, c_int,
                                  c_int, POINTER(Statistics), P, c_int64, POINTER(c_double)]
         l.oracle_element_gradient.argtypes = [c_int64, c_int32, P, P, P, P, P]
+        l.oracle_assemble.argtypes = [c_int64, P, P, c_int64, c_int32, P, P, c_int64, P, P]
+        l.oracle_assemble_vector.argtypes = [c_int64, c_int64, c_int32, P, P, c_int64, P, P]
         _lib = l
     return _lib
 
@@ -127,4 +129,39 @@ def element_gradient(dofs, element_matrix, lam, u):
     lam, u = _f64(lam), _f64(u)
     out = np.empty(dofs.shape[0])
     lib().oracle_element_gradient(dofs.shape[0], dofs.shape[1], _p(dofs), _p(ke), _p(lam), _p(u), _p(out))
+    return out
+
+
+def assemble(csr, dofs, matrices, scale=None):
+    """Entries of sum_e scale_e P_e^T K_e P_e on the pattern of `csr`, elements in order.
+
+    matrices: one nodes x nodes matrix shared by all elements, or elements x nodes x nodes."""
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    elements, nodes = dofs.shape
+    matrices = _f64(matrices)
+    stride = 0 if matrices.size == nodes * nodes and matrices.ndim <= 2 else nodes * nodes
+    assert matrices.size == (nodes * nodes if stride == 0 else elements * nodes * nodes)
+    scale = None if scale is None else _f64(scale)
+    ext, cols = _i32(csr.row_extents), _i32(csr.column_indices)
+    out = np.empty(int(ext[-1]))
+    status = lib().oracle_assemble(csr.rows, _p(ext), _p(cols), elements, nodes, _p(dofs), _p(matrices), stride,
+                                   None if scale is None else _p(scale), _p(out))
+    if status:
+        raise OracleError(status, "a coupling is missing from the pattern")
+    return out
+
+
+def assemble_vector(rows, dofs, vectors, scale=None):
+    """sum_e scale_e P_e^T v_e; vectors: one shared nodes vector or elements x nodes."""
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    elements, nodes = dofs.shape
+    vectors = _f64(vectors)
+    stride = 0 if vectors.size == nodes and vectors.ndim == 1 else nodes
+    assert vectors.size == (nodes if stride == 0 else elements * nodes)
+    scale = None if scale is None else _f64(scale)
+    out = np.empty(rows)
+    status = lib().oracle_assemble_vector(rows, elements, nodes, _p(dofs), _p(vectors), stride,
+                                          None if scale is None else _p(scale), _p(out))
+    if status:
+        raise OracleError(status, "row index out of range")
     return out

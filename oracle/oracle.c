@@ -27,6 +27,15 @@
  *                           the Jacobi DiagonalOperator; convergence test of
  *                           cieutils/packages/maths/impl/Comparison_impl.hpp:32-64
  *                           (abs tol DBL_MIN, rel tol DBL_EPSILON).
+ *   oracle_assemble         fem/packages/graph/impl/Assembler_impl.hpp:156-194
+ *                           (Assembler::addContribution, Serial branch): for
+ *                           every element in order, every (local row, local
+ *                           column) pair is located by lower_bound in the CSR
+ *                           row and added into its slot; a pair missing from
+ *                           the pattern is an out-of-range error (status 2).
+ *                           The contribution is scale_e * K_e (scale NULL: K_e),
+ *                           the product of LinearIsotropicStiffnessIntegrand_impl.hpp:74.
+ *   oracle_assemble_vector  Assembler_impl.hpp:197-220 (the vector overload).
  *   oracle_element_gradient dJ/dk_e = -lambda_e^T K_e0 u_e (no reference
  *                           counterpart; K_e = k_e K_e0 as in
  *                           fem/.../LinearIsotropicStiffnessIntegrand_impl.hpp:74).
@@ -458,6 +467,60 @@ int oracle_element_gradient(int64_t elements, int32_t nodesPerElement, const int
             sum += lambda[d[a]] * row;
         }
         gradient[e] = -sum;
+    }
+    return 0;
+}
+
+/* Position of `column` in cols[begin, end) (sorted) by lower bound, or -1. */
+static int64_t find_slot(const int32_t* cols, int64_t begin, int64_t end, int32_t column) {
+    int64_t lo = begin, hi = end;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (cols[mid] < column) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < end && cols[lo] == column ? lo : -1;
+}
+
+/* entries = sum over elements e (in order) of scale_e * K_e scattered into the
+ * pattern; K_e = matrices + e * stride (stride 0: one shared matrix), nodes^2
+ * row-major; dofs -1 marks eliminated nodes, which are skipped. entries is
+ * zeroed first. Returns 2 if a kept pair is not in the pattern. */
+int oracle_assemble(int64_t rows, const int32_t* rowptr, const int32_t* cols, int64_t elements,
+                    int32_t nodesPerElement, const int32_t* dofs, const double* matrices, int64_t stride,
+                    const double* scale, double* entries) {
+    memset(entries, 0, sizeof(double) * (size_t)rowptr[rows]);
+    for (int64_t e = 0; e < elements; ++e) {
+        const int32_t* d = dofs + e * nodesPerElement;
+        const double* k = matrices + e * stride;
+        for (int a = 0; a < nodesPerElement; ++a) {
+            if (d[a] < 0) continue;
+            if (d[a] >= rows) return 2;
+            for (int b = 0; b < nodesPerElement; ++b) {
+                if (d[b] < 0) continue;
+                const int64_t slot = find_slot(cols, rowptr[d[a]], rowptr[d[a] + 1], d[b]);
+                if (slot < 0) return 2;
+                const double value = k[(int64_t)a * nodesPerElement + b];
+                entries[slot] += scale ? scale[e] * value : value;
+            }
+        }
+    }
+    return 0;
+}
+
+/* out = sum over elements e (in order) of scale_e * v_e scattered by the dof
+ * map; v_e = vectors + e * stride (stride 0: one shared vector). */
+int oracle_assemble_vector(int64_t rows, int64_t elements, int32_t nodesPerElement, const int32_t* dofs,
+                           const double* vectors, int64_t stride, const double* scale, double* out) {
+    memset(out, 0, sizeof(double) * (size_t)rows);
+    for (int64_t e = 0; e < elements; ++e) {
+        const int32_t* d = dofs + e * nodesPerElement;
+        const double* v = vectors + e * stride;
+        for (int a = 0; a < nodesPerElement; ++a) {
+            if (d[a] < 0) continue;
+            if (d[a] >= rows) return 2;
+            out[d[a]] += scale ? scale[e] * v[a] : v[a];
+        }
     }
     return 0;
 }

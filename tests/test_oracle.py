@@ -213,3 +213,76 @@ def test_element_gradient_against_dense_assembly():
         bumped[e] += 1.0
         delta = systems.lagrange(3, 2, elements, conductivity=bumped).to_scipy() - a
         assert np.isclose(-grad[e], lam @ (delta @ u), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 1, 6), (2, 2, 5), (2, 4, 3), (3, 1, 4), (3, 2, 3), (3, 4, 2)])
+def test_assemble_matches_structured_generator_bitwise(dim, order, n):
+    # The generator sums each nonzero row by row over the elements sharing both nodes (its own loop);
+    # the oracle scatters element by element. Both take the elements in increasing order.
+    count = systems.lagrange_size(dim, order, n)[2]
+    k = np.exp(systems.normal(3, 1.0, count))
+    expected = systems.lagrange(dim, order, n, conductivity=k)
+    dofs, ke = systems.element_dofs(dim, order, n), systems.element_matrix(dim, order, n)
+    assert np.array_equal(oracle.assemble(expected, dofs, ke, k), expected.entries)
+    assert np.array_equal(oracle.assemble(expected, dofs, ke), systems.lagrange(dim, order, n).entries)
+    # one matrix per element, all equal to k_e K_e0: the same values (k_e * K_e0[a][b] is the same product)
+    per_element = k[:, None, None] * ke[None, :, :]
+    assert np.array_equal(oracle.assemble(expected, dofs, per_element), expected.entries)
+
+
+def test_assemble_random_element_matrices_against_dense_sum():
+    dim, order, n = 2, 2, 4
+    pattern = systems.lagrange(dim, order, n)
+    dofs = systems.element_dofs(dim, order, n)
+    count, nodes = dofs.shape
+    rng = np.random.default_rng(8)
+    matrices = rng.standard_normal((count, nodes, nodes))
+    scale = rng.uniform(0.5, 2.0, count)
+    entries = oracle.assemble(pattern, dofs, matrices, scale)
+    dense = np.zeros((pattern.rows, pattern.rows))
+    for e in range(count):
+        keep = dofs[e] >= 0
+        rows = dofs[e][keep]
+        dense[np.ix_(rows, rows)] += scale[e] * matrices[e][np.ix_(keep, keep)]
+    got = systems.CSR(pattern.rows, pattern.rows, pattern.row_extents, pattern.column_indices, entries).to_scipy()
+    assert np.allclose(got.toarray(), dense, rtol=1e-13, atol=1e-13)
+    # every pattern slot is covered by some element (no stored zero that the dense sum does not produce)
+    assert np.count_nonzero(dense) == pattern.nnz
+
+
+def test_assemble_vector_against_dense_sum():
+    dim, order, n = 3, 2, 3
+    rows = systems.lagrange_size(dim, order, n)[0]
+    dofs = systems.element_dofs(dim, order, n)
+    count, nodes = dofs.shape
+    rng = np.random.default_rng(9)
+    vectors = rng.standard_normal((count, nodes))
+    scale = rng.uniform(0.5, 2.0, count)
+    expected = np.zeros(rows)
+    for e in range(count):
+        keep = dofs[e] >= 0
+        expected[dofs[e][keep]] += scale[e] * vectors[e][keep]
+    assert np.array_equal(oracle.assemble_vector(rows, dofs, vectors, scale), expected)  # same order, same sums
+    shared = rng.standard_normal(nodes)
+    dense = sum(np.bincount(dofs[e][dofs[e] >= 0], shared[dofs[e] >= 0], minlength=rows) for e in range(count))
+    assert np.allclose(oracle.assemble_vector(rows, dofs, shared), dense, rtol=1e-14, atol=1e-14)
+
+
+def test_assemble_errors():
+    dim, order, n = 2, 2, 3
+    full = systems.lagrange(dim, order, n)
+    dofs, ke = systems.element_dofs(dim, order, n), systems.element_matrix(dim, order, n)
+    keep = np.ones(full.nnz, dtype=bool)
+    keep[int(full.row_extents[1]) - 1] = False
+    extents = full.row_extents.copy()
+    extents[1:] -= 1
+    pruned = systems.CSR(full.rows, full.columns, extents, full.column_indices[keep], full.entries[keep])
+    with pytest.raises(oracle.OracleError) as error:
+        oracle.assemble(pruned, dofs, ke)
+    assert error.value.status == 2
+    bad = dofs.copy()
+    bad[0, 0] = full.rows
+    with pytest.raises(oracle.OracleError):
+        oracle.assemble(full, bad, ke)
+    with pytest.raises(oracle.OracleError):
+        oracle.assemble_vector(full.rows, bad, np.ones(dofs.shape[1]))
