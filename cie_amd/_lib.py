@@ -30,6 +30,7 @@ DEVICE = 1
 SPMV_VECTOR = 0
 SPMV_STAGED = 1
 SPMV_TILED = 2
+REORDER_NONE, REORDER_CUTHILL_MCKEE, REORDER_REVERSE_CUTHILL_MCKEE = 0, 1, 2
 PLANE_F64 = 1
 PLANE_F32 = 2
 
@@ -166,6 +167,12 @@ _SIGNATURES = {
     "cie_hip_assembly_destroy": (c_int, [_P]),
     "cie_hip_assemble": (c_int, [_P, _P, c_int, _P]),
     "cie_hip_assemble_vector": (c_int, [_P, _P, c_int, _P, _P]),
+    # reordering
+    "cie_hip_make_reordering": (c_int, [c_int64, _P, _P, c_int, _P]),
+    "cie_hip_csr_reorder": (c_int, [_P, _P, c_int, POINTER(_P)]),
+    "cie_hip_reorder_f64": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
+    "cie_hip_reorder_f32": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
+    "cie_hip_csr_copy_out": (c_int, [_P, _P, _P, _P]),
     # structured systems (include/cie_hip_systems.h)
     "cie_sys_last_error": (c_char_p, []),
     "cie_sys_splitmix64": (c_int, [c_uint64, c_int64, _P]),

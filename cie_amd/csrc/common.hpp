@@ -135,6 +135,10 @@ int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.
 int ensureNarrowColumns(cie_hip_csr matrix);
+// A library-owned matrix over device arrays allocated by the caller of this
+// function (ownership passes to the matrix; entries32 may be null).
+int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t* rowExtents, int32_t* columnIndices,
+                double* entries64, float* entries32, std::vector<int32_t> hostExtents, cie_hip_csr* matrix);
 // Grows the context's partials to at least `slots` slots of 4 doubles. A
 // growth waits for the stream first (queued kernels may use the old buffer).
 int ensurePartials(cie_hip_context context, int slots);

@@ -291,6 +291,25 @@ int ensureInverseDiagonal(cie_hip_csr m, int precision) {
     return CIE_HIP_OK;
 }
 
+int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t* rowExtents, int32_t* columnIndices,
+                double* entries64, float* entries32, std::vector<int32_t> hostExtents, cie_hip_csr* matrix) {
+    auto* m = new cie_hip_csr_s;
+    m->context = context;
+    m->rows = rows;
+    m->columns = columns;
+    m->nnz = hostExtents[size_t(rows)];
+    m->rowExtents = rowExtents;
+    m->columnIndices = columnIndices;
+    m->entries64 = entries64;
+    m->entries32 = entries32;
+    m->ownsIndices = m->ownsEntries64 = true;
+    rowStatistics(m, hostExtents.data());
+    m->hostExtents = std::move(hostExtents);
+    m->launch64 = m->launch32 = defaultLaunch();
+    *matrix = m;
+    return CIE_HIP_OK;
+}
+
 }  // namespace ciehip
 
 extern "C" {
@@ -366,6 +385,20 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
         return bail(CIE_HIP_ERROR_HIP, "cie_hip_csr_create: stream synchronisation failed");
     m->launch64 = m->launch32 = defaultLaunch();
     *matrix = m;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_copy_out(cie_hip_csr matrix, int32_t* rowExtents, int32_t* columnIndices, double* entries) {
+    if (!matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_copy_out: null matrix");
+    cie_hip_context ctx = matrix->context;
+    CIE_HIP_CHECK(hipSetDevice(ctx->device));
+    CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (rowExtents)
+        CIE_HIP_CHECK(hipMemcpy(rowExtents, matrix->rowExtents, sizeof(int32_t) * (matrix->rows + 1), hipMemcpyDeviceToHost));
+    if (columnIndices && matrix->nnz)
+        CIE_HIP_CHECK(hipMemcpy(columnIndices, matrix->columnIndices, sizeof(int32_t) * matrix->nnz, hipMemcpyDeviceToHost));
+    if (entries && matrix->nnz)
+        CIE_HIP_CHECK(hipMemcpy(entries, matrix->entries64, sizeof(double) * matrix->nnz, hipMemcpyDeviceToHost));
     return CIE_HIP_OK;
 }
 

@@ -1,0 +1,250 @@
+// Cuthill-McKee orderings of a CSR pattern, and the permutation of a device
+// CSR matrix and of device vectors by such an ordering
+// (linalg/packages/utilities/src/reorder.cpp).
+//
+// The ordering is host work, a breadth-first traversal (reorder.cpp:23-173):
+// it starts at the first row of least degree (row length); each visited row
+// queues its unvisited columns in column order, and that batch is sorted by
+// degree with std::sort, whose order for equal degrees is the C++ library's
+// (this file is built against the same GCC libstdc++ the reference builds
+// with). When the queue runs dry the first unvisited row of least degree
+// starts the next component; reverse Cuthill-McKee reverses the result.
+// map[new] = old.
+//
+// The matrix permutation (reorder.cpp:215-365) runs on the device: one
+// wavefront per row of the result gathers the old row, renumbers its columns
+// and places every entry at its rank among the row's new columns (rows of up
+// to kRankEntries entries are ranked out of LDS, longer ones out of L2). Both
+// value planes follow their entries, so the result is bitwise the reordered
+// input.
+#include "dispatch.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+using namespace ciehip;
+
+namespace {
+
+constexpr int kPermuteBlock = 256;
+constexpr int kPermuteWaves = kPermuteBlock / 64;
+constexpr int kRankEntries = 1024;
+
+__global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
+    int rows, const int32_t* __restrict__ source, const int32_t* __restrict__ target,
+    const int32_t* __restrict__ oldExtents, const int32_t* __restrict__ oldColumns,
+    const double* __restrict__ oldEntries64, const float* __restrict__ oldEntries32,
+    const int32_t* __restrict__ newExtents, int32_t* __restrict__ newColumns, double* __restrict__ newEntries64,
+    float* __restrict__ newEntries32) {
+    __shared__ int32_t staged[kPermuteWaves][kRankEntries];
+    const int wave = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+    const int row = int(blockIdx.x) * kPermuteWaves + wave;
+    if (row >= rows) return;  // whole wavefronts leave; no block barrier follows
+    const int old = source[row];
+    const int begin = oldExtents[old];
+    const int length = oldExtents[old + 1] - begin;
+    const int out = newExtents[row];
+    int32_t* lds = staged[wave];
+    const bool inLds = length <= kRankEntries;  // uniform over the wavefront
+    if (inLds) {
+        for (int k = lane; k < length; k += 64) lds[k] = target[oldColumns[begin + k]];
+        // The row is shared inside this wavefront only: order its LDS writes before the reads.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    for (int k = lane; k < length; k += 64) {
+        const int column = inLds ? lds[k] : target[oldColumns[begin + k]];
+        int rank = 0;
+        for (int j = 0; j < length; ++j) {
+            const int other = inLds ? lds[j] : target[oldColumns[begin + j]];
+            rank += (other < column) || (other == column && j < k);
+        }
+        newColumns[out + rank] = column;
+        newEntries64[out + rank] = oldEntries64[begin + k];
+        if (newEntries32) newEntries32[out + rank] = oldEntries32[begin + k];
+    }
+}
+
+// out[i] = in[map[i]] (gather) or out[map[i]] = in[i] (scatter); an index
+// outside [0, n) is skipped and flagged.
+template <typename T>
+__global__ __launch_bounds__(kReduceBlock) void permuteVectorKernel(int64_t n, const int32_t* __restrict__ map,
+                                                                    int scatter, const T* __restrict__ in,
+                                                                    T* __restrict__ out, int* __restrict__ invalid) {
+    for (int64_t i = int64_t(blockIdx.x) * kReduceBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kReduceBlock) {
+        const int32_t j = map[i];
+        if (j < 0 || int64_t(j) >= n) {
+            atomicOr(invalid, 1);
+            continue;
+        }
+        if (scatter) out[j] = in[i];
+        else out[i] = in[j];
+    }
+}
+
+template <typename T>
+int permuteVector(cie_hip_context ctx, const int32_t* map, int reverse, const T* in, T* out, int64_t n,
+                  const char* name) {
+    if (!ctx || n < 0 || (n && (!map || !in || !out)))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid arguments");
+    if (n && in == out) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, std::string(name) + ": in and out must differ");
+    if (!n) return CIE_HIP_OK;
+    CIE_HIP_CHECK(hipSetDevice(ctx->device));
+    int* invalid = reinterpret_cast<int*>(ctx->scratch.scalars + 24);
+    int* pinned = reinterpret_cast<int*>(ctx->scratch.pinned + 24);
+    CIE_HIP_CHECK(hipMemsetAsync(invalid, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL((permuteVectorKernel<T>), dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, n, map,
+                       reverse ? 1 : 0, in, out, invalid);
+    CIE_HIP_CHECK(hipGetLastError());
+    CIE_HIP_CHECK(hipMemcpyAsync(pinned, invalid, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (*pinned) return setError(CIE_HIP_ERROR_SHAPE, std::string(name) + ": map index out of range");
+    return CIE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cie_hip_make_reordering(int64_t rows, const int32_t* rowExtents, const int32_t* columnIndices, int strategy,
+                            int32_t* map) {
+    if (rows < 0 || (rows && (!rowExtents || !map)) || strategy < CIE_HIP_REORDER_NONE ||
+        strategy > CIE_HIP_REORDER_REVERSE_CUTHILL_MCKEE)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_make_reordering: invalid arguments");
+    if (rows > INT32_MAX - 1) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_make_reordering: too many rows");
+    for (int64_t r = 0; r < rows; ++r) map[r] = int32_t(r);
+    if (rows == 0) return CIE_HIP_OK;
+    // reorder.cpp:33-55: sorted extents, sentinel = entry count.
+    if (rowExtents[0] != 0) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_make_reordering: row extents must start at 0");
+    for (int64_t r = 0; r < rows; ++r)
+        if (rowExtents[r + 1] < rowExtents[r])
+            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_make_reordering: row extents must be sorted");
+    const int64_t nnz = rowExtents[rows];
+    if (strategy == CIE_HIP_REORDER_NONE || nnz == 0) return CIE_HIP_OK;
+    if (!columnIndices) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_make_reordering: null column indices");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (columnIndices[k] < 0 || columnIndices[k] >= rows)
+            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_make_reordering: column index out of range (square pattern expected)");
+
+    std::vector<int32_t> degree(static_cast<size_t>(rows));
+    for (int64_t r = 0; r < rows; ++r) degree[size_t(r)] = rowExtents[r + 1] - rowExtents[r];
+    // Rows by (degree, index): the first unvisited one is where a component starts.
+    std::vector<int32_t> starts(static_cast<size_t>(rows));
+    std::iota(starts.begin(), starts.end(), 0);
+    std::stable_sort(starts.begin(), starts.end(), [&](int32_t a, int32_t b) { return degree[a] < degree[b]; });
+    size_t nextStart = 0;
+    std::vector<char> seen(static_cast<size_t>(rows), 0);
+    std::vector<int32_t> queue;
+    queue.reserve(static_cast<size_t>(rows));
+    size_t head = 0;
+    int64_t placed = 0;
+    auto byDegree = [&](int32_t left, int32_t right) { return degree[left] < degree[right]; };
+    while (placed < rows) {
+        if (head == queue.size()) {
+            while (seen[size_t(starts[nextStart])]) ++nextStart;
+            const int32_t start = starts[nextStart];
+            seen[size_t(start)] = 1;
+            queue.push_back(start);
+        }
+        const int32_t row = queue[head++];
+        const size_t batch = queue.size();
+        for (int32_t k = rowExtents[row]; k < rowExtents[row + 1]; ++k) {
+            const int32_t column = columnIndices[k];
+            if (!seen[size_t(column)]) {
+                seen[size_t(column)] = 1;
+                queue.push_back(column);
+            }
+        }
+        std::sort(queue.begin() + std::ptrdiff_t(batch), queue.end(), byDegree);
+        map[placed++] = row;
+    }
+    if (strategy == CIE_HIP_REORDER_REVERSE_CUTHILL_MCKEE) std::reverse(map, map + rows);
+    return CIE_HIP_OK;
+}
+
+int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie_hip_csr* reordered) {
+    if (!reordered) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_reorder: null output");
+    *reordered = nullptr;
+    if (!matrix || (matrix->rows && !map))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_reorder: invalid arguments");
+    if (matrix->rows != matrix->columns)
+        return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_reorder: the matrix must be square");
+    const int64_t rows = matrix->rows, nnz = matrix->nnz;
+    // source[new] = old row, target[old] = new index (reorder.cpp:273-274).
+    std::vector<int32_t> inverse(size_t(rows), -1);
+    for (int64_t i = 0; i < rows; ++i) {
+        if (map[i] < 0 || map[i] >= rows || inverse[size_t(map[i])] >= 0)
+            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_reorder: the map is not a permutation");
+        inverse[size_t(map[i])] = int32_t(i);
+    }
+    const std::vector<int32_t> forward(map, map + rows);
+    const std::vector<int32_t>& source = reverse ? inverse : forward;
+    const std::vector<int32_t>& target = reverse ? forward : inverse;
+    const std::vector<int32_t>& oldExtents = matrix->hostExtents;
+    std::vector<int32_t> newExtents(size_t(rows) + 1, 0);
+    for (int64_t i = 0; i < rows; ++i) {
+        const int32_t old = source[size_t(i)];
+        newExtents[size_t(i) + 1] = newExtents[size_t(i)] + (oldExtents[size_t(old) + 1] - oldExtents[size_t(old)]);
+    }
+    cie_hip_context ctx = matrix->context;
+    CIE_HIP_CHECK(hipSetDevice(ctx->device));
+    int32_t *dExtents = nullptr, *dColumns = nullptr, *dSource = nullptr, *dTarget = nullptr;
+    double* dEntries64 = nullptr;
+    float* dEntries32 = nullptr;
+    auto release = [&]() {
+        if (dExtents) (void)hipFree(dExtents);
+        if (dColumns) (void)hipFree(dColumns);
+        if (dEntries64) (void)hipFree(dEntries64);
+        if (dEntries32) (void)hipFree(dEntries32);
+    };
+    auto releaseMaps = [&]() {
+        if (dSource) (void)hipFree(dSource);
+        if (dTarget) (void)hipFree(dTarget);
+    };
+    const size_t count = size_t(std::max<int64_t>(1, nnz));
+    bool ok = hipMalloc(&dExtents, sizeof(int32_t) * (size_t(rows) + 1)) == hipSuccess &&
+              hipMalloc(&dColumns, sizeof(int32_t) * count) == hipSuccess &&
+              hipMalloc(&dEntries64, sizeof(double) * count) == hipSuccess &&
+              (!matrix->entries32 || hipMalloc(&dEntries32, sizeof(float) * count) == hipSuccess) &&
+              hipMalloc(&dSource, sizeof(int32_t) * size_t(std::max<int64_t>(1, rows))) == hipSuccess &&
+              hipMalloc(&dTarget, sizeof(int32_t) * size_t(std::max<int64_t>(1, rows))) == hipSuccess;
+    if (!ok) {
+        release();
+        releaseMaps();
+        return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_csr_reorder: device allocation failed");
+    }
+    ok = hipMemcpyAsync(dExtents, newExtents.data(), sizeof(int32_t) * (size_t(rows) + 1), hipMemcpyHostToDevice,
+                        ctx->stream) == hipSuccess &&
+         (!rows || (hipMemcpyAsync(dSource, source.data(), sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess &&
+                    hipMemcpyAsync(dTarget, target.data(), sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice,
+                                   ctx->stream) == hipSuccess));
+    if (ok && rows) {
+        hipLaunchKernelGGL(permuteRowsKernel, dim3(unsigned((rows + kPermuteWaves - 1) / kPermuteWaves)),
+                           dim3(kPermuteBlock), 0, ctx->stream, int(rows), dSource, dTarget, matrix->rowExtents,
+                           matrix->columnIndices, matrix->entries64, matrix->entries32, dExtents, dColumns,
+                           dEntries64, dEntries32);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    releaseMaps();
+    if (!ok) {
+        release();
+        return setError(CIE_HIP_ERROR_HIP, "cie_hip_csr_reorder: upload or permutation failed");
+    }
+    return adoptMatrix(ctx, rows, rows, dExtents, dColumns, dEntries64, dEntries32, std::move(newExtents), reordered);
+}
+
+int cie_hip_reorder_f64(cie_hip_context context, const int32_t* map, int reverse, const double* in, double* out,
+                        int64_t size) {
+    return permuteVector(context, map, reverse, in, out, size, "cie_hip_reorder_f64");
+}
+
+int cie_hip_reorder_f32(cie_hip_context context, const int32_t* map, int reverse, const float* in, float* out,
+                        int64_t size) {
+    return permuteVector(context, map, reverse, in, out, size, "cie_hip_reorder_f32");
+}
+
+}  // extern "C"

@@ -137,6 +137,39 @@ class CSRMatrix:
               "cie_hip_csr_autotune")
         return best.value
 
+    @classmethod
+    def _adopt(cls, handle, context, rows, columns, nnz):
+        matrix = cls.__new__(cls)
+        matrix.context, matrix.handle = context, handle
+        matrix.rows, matrix.columns, matrix.nnz = rows, columns, nnz
+        return matrix
+
+    def copy_out(self):
+        """The device arrays as a host CSR (row extents, column indices, FP64 entries)."""
+        from .systems import CSR
+
+        extents = np.empty(self.rows + 1, dtype=np.int32)
+        columns = np.empty(self.nnz, dtype=np.int32)
+        entries = np.empty(self.nnz, dtype=np.float64)
+        _sync_torch()
+        check(library().cie_hip_csr_copy_out(self.handle, extents.ctypes.data_as(c_void_p),
+                                             columns.ctypes.data_as(c_void_p), entries.ctypes.data_as(c_void_p)),
+              "cie_hip_csr_copy_out")
+        return CSR(self.rows, self.columns, extents, columns, entries)
+
+    def reorder(self, mapping, reverse=False):
+        """A new device matrix with rows and columns permuted (reorder / reverseReorder, reorder.cpp:215-365).
+
+        mapping[new] = old (reverse False) or mapping[old] = new (reverse True), a host permutation."""
+        mapping = np.ascontiguousarray(mapping, dtype=np.int32)
+        if mapping.shape != (self.rows,):
+            raise ValueError("the map needs one entry per row")
+        handle = c_void_p()
+        _sync_torch()
+        check(library().cie_hip_csr_reorder(self.handle, mapping.ctypes.data_as(c_void_p), 1 if reverse else 0,
+                                            byref(handle)), "cie_hip_csr_reorder")
+        return CSRMatrix._adopt(handle, self.context, self.rows, self.columns, self.nnz)
+
     def update_entries(self, entries):
         """Replace the FP64 entries (device tensor, same pattern); refreshes the FP32 plane."""
         _sync_torch()
@@ -443,6 +476,51 @@ class MixedPCG(IterativeSolver):
         self.info = info
         check(status, "cie_hip_pcg_mixed")
         self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+
+
+_REORDERINGS = {"none": _lib.REORDER_NONE, "cuthill-mckee": _lib.REORDER_CUTHILL_MCKEE,
+                "reverse-cuthill-mckee": _lib.REORDER_REVERSE_CUTHILL_MCKEE}
+
+
+def makeReordering(csr, strategy="cuthill-mckee"):
+    """map[new] = old for a host CSR pattern (makeReordering, reorder.cpp:23-173; the strategy names are the
+    driver's --reordering values, poisson2D/solver.cpp:490-493)."""
+    if strategy not in _REORDERINGS:
+        raise ValueError(f"unknown reordering strategy: {strategy}")
+    extents = np.ascontiguousarray(csr.row_extents, dtype=np.int32)
+    columns = np.ascontiguousarray(csr.column_indices, dtype=np.int32)
+    mapping = np.empty(int(csr.rows), dtype=np.int32)
+    check(library().cie_hip_make_reordering(int(csr.rows), extents.ctypes.data_as(c_void_p),
+                                            columns.ctypes.data_as(c_void_p), _REORDERINGS[strategy],
+                                            mapping.ctypes.data_as(c_void_p)), "cie_hip_make_reordering")
+    return mapping
+
+
+def reverseReorder(mapping):
+    """The inverse permutation (reverseReorder(map, buffer), reorder.cpp:176-198), on the host."""
+    mapping = np.ascontiguousarray(mapping, dtype=np.int32)
+    inverse = np.empty_like(mapping)
+    inverse[mapping] = np.arange(mapping.size, dtype=np.int32)
+    return inverse
+
+
+def reorder(mapping, vector, out=None, reverse=False, context=None):
+    """out[i] = vector[mapping[i]] (reverse: out[mapping[i]] = vector[i]) on device vectors
+    (reorder / reverseReorder of an array, reorder.cpp:368-426). mapping is a device int32 tensor."""
+    torch = _torch()
+    context = context or Context.default()
+    if mapping.dtype != torch.int32 or mapping.numel() != vector.numel() or not mapping.is_contiguous():
+        raise ValueError("mapping must be a contiguous int32 device tensor of the vector's size")
+    if out is None:
+        out = torch.empty_like(vector)
+    suffix = {torch.float64: "f64", torch.float32: "f32"}.get(vector.dtype)
+    if suffix is None or out.dtype != vector.dtype or out.numel() != vector.numel():
+        raise TypeError("reorder works on float64 or float32 vectors of equal size")
+    _sync_torch()
+    check(getattr(library(), f"cie_hip_reorder_{suffix}")(context.handle, _dptr(mapping), 1 if reverse else 0,
+                                                          _dptr(vector), _dptr(out), vector.numel()),
+          f"cie_hip_reorder_{suffix}")
+    return out
 
 
 class Assembly:

@@ -113,6 +113,10 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
                        const int32_t* rowExtents, const int32_t* columnIndices, const double* entries,
                        int location, int planes, cie_hip_csr* matrix);
 int cie_hip_csr_destroy(cie_hip_csr matrix);
+/* Copies the device arrays back to host memory (any pointer may be NULL):
+ * rowExtents[rows+1], columnIndices[nnz], FP64 entries[nnz] -- the arrays a
+ * CSRView exposes (CSRView.hpp:44-66). */
+int cie_hip_csr_copy_out(cie_hip_csr matrix, int32_t* rowExtents, int32_t* columnIndices, double* entries);
 int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64_t* nnz);
 /* Rewrite the FP64 entries (and the FP32 plane if present) from device memory;
  * the sparsity pattern is unchanged. */
@@ -293,6 +297,35 @@ int cie_hip_assemble(cie_hip_assembly plan, const double* elementMatrices, int p
  * Device arrays. */
 int cie_hip_assemble_vector(cie_hip_assembly plan, const double* elementVectors, int perElement, const double* scale,
                             double* out);
+
+/* ---- Reordering (linalg/packages/utilities/inc/reorder.hpp) -------------- */
+/* ReorderingStrategy (reorder.hpp:13-17). */
+#define CIE_HIP_REORDER_NONE 0
+#define CIE_HIP_REORDER_CUTHILL_MCKEE 1
+#define CIE_HIP_REORDER_REVERSE_CUTHILL_MCKEE 2
+/* makeReordering (reorder.cpp:23-173), host arrays: map[new] = old row.
+ * Breadth-first from the first row of least degree (row length); each row's
+ * unvisited columns are queued in column order and that batch is sorted by
+ * degree with std::sort; a new component starts at the first unvisited row of
+ * least degree; reverse Cuthill-McKee reverses the map. The pattern must be
+ * square (columns < rows). NONE, or a pattern without entries, gives the
+ * identity. */
+int cie_hip_make_reordering(int64_t rows, const int32_t* rowExtents, const int32_t* columnIndices, int strategy,
+                            int32_t* map);
+/* reorder<..., Reverse> on a CSR matrix (reorder.cpp:215-365), done on the
+ * device into a new library-owned matrix: row i of the result is row map[i]
+ * of `matrix` (reverse 0), or row j goes to map[j] (reverse 1, the map read as
+ * old -> new); columns are renumbered the same way and each row re-sorted.
+ * Both value planes follow. map is a host permutation of [0, rows). */
+int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie_hip_csr* reordered);
+/* reorder / reverseReorder of an array (reorder.cpp:368-426) on device
+ * vectors, out of place: out[i] = in[map[i]] (reverse 0) or
+ * out[map[i]] = in[i] (reverse 1). map is a device array; an index outside
+ * [0, size) is CIE_HIP_ERROR_SHAPE. */
+int cie_hip_reorder_f64(cie_hip_context context, const int32_t* map, int reverse, const double* in, double* out,
+                        int64_t size);
+int cie_hip_reorder_f32(cie_hip_context context, const int32_t* map, int reverse, const float* in, float* out,
+                        int64_t size);
 
 #ifdef __cplusplus
 }

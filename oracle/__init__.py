@@ -44,6 +44,11 @@ def lib():
         l.oracle_element_gradient.argtypes = [c_int64, c_int32, P, P, P, P, P]
         l.oracle_assemble.argtypes = [c_int64, P, P, c_int64, c_int32, P, P, c_int64, P, P]
         l.oracle_assemble_vector.argtypes = [c_int64, c_int64, c_int32, P, P, c_int64, P, P]
+        l.oracle_degree_sort.argtypes = [c_int64, P, P]
+        l.oracle_heap_sort.argtypes = [c_int64, P, P]
+        l.oracle_make_reordering.argtypes = [c_int64, P, P, c_int, P]
+        l.oracle_reorder_csr.argtypes = [c_int64, P, c_int, P, P, P, P, P, P]
+        l.oracle_reorder_vector.argtypes = [c_int64, P, c_int, P, P]
         _lib = l
     return _lib
 
@@ -164,4 +169,54 @@ def assemble_vector(rows, dofs, vectors, scale=None):
                                           None if scale is None else _p(scale), _p(out))
     if status:
         raise OracleError(status, "row index out of range")
+    return out
+
+
+REORDERINGS = {"none": 0, "cuthill-mckee": 1, "reverse-cuthill-mckee": 2}
+
+
+def degree_sort(items, key):
+    """items sorted by key[item] in the order GCC's std::sort leaves them (ties included)."""
+    items, key = _i32(items).copy(), _i32(key)
+    lib().oracle_degree_sort(items.size, _p(items), _p(key))
+    return items
+
+
+def heap_sort(items, key):
+    """The heapsort step of that order alone (std::partial_sort over the whole range)."""
+    items, key = _i32(items).copy(), _i32(key)
+    lib().oracle_heap_sort(items.size, _p(items), _p(key))
+    return items
+
+
+def make_reordering(csr, strategy="reverse-cuthill-mckee"):
+    """map[new] = old (makeReordering, reorder.cpp:23-173)."""
+    ext, cols = _i32(csr.row_extents), _i32(csr.column_indices)
+    out = np.empty(csr.rows, dtype=np.int32)
+    status = lib().oracle_make_reordering(csr.rows, _p(ext), _p(cols), REORDERINGS[strategy], _p(out))
+    if status:
+        raise OracleError(status, "make_reordering")
+    return out
+
+
+def reorder_csr(csr, mapping, reverse=False):
+    """The reordered CSR arrays (reorder.cpp:215-365), as (row_extents, column_indices, entries)."""
+    mapping = _i32(mapping)
+    ext, cols, vals = _i32(csr.row_extents), _i32(csr.column_indices), _f64(csr.entries)
+    new_ext = np.empty(csr.rows + 1, dtype=np.int32)
+    new_cols = np.empty(max(1, csr.nnz), dtype=np.int32)
+    new_vals = np.empty(max(1, csr.nnz), dtype=np.float64)
+    status = lib().oracle_reorder_csr(csr.rows, _p(mapping), 1 if reverse else 0, _p(ext), _p(cols), _p(vals),
+                                      _p(new_ext), _p(new_cols), _p(new_vals))
+    if status:
+        raise OracleError(status, "reorder_csr: the map is not a permutation")
+    return new_ext, new_cols[:csr.nnz], new_vals[:csr.nnz]
+
+
+def reorder_vector(mapping, values, reverse=False):
+    mapping, values = _i32(mapping), _f64(values)
+    out = np.empty_like(values)
+    status = lib().oracle_reorder_vector(values.size, _p(mapping), 1 if reverse else 0, _p(values), _p(out))
+    if status:
+        raise OracleError(status, "reorder_vector")
     return out

@@ -36,6 +36,27 @@
  *                           The contribution is scale_e * K_e (scale NULL: K_e),
  *                           the product of LinearIsotropicStiffnessIntegrand_impl.hpp:74.
  *   oracle_assemble_vector  Assembler_impl.hpp:197-220 (the vector overload).
+ *   oracle_make_reordering  linalg/packages/utilities/src/reorder.cpp:23-173
+ *                           (makeReordering, serial path): Cuthill-McKee from
+ *                           the first row of least degree (row length); every
+ *                           visited row queues its unvisited columns in column
+ *                           order, and that batch is then sorted by degree with
+ *                           std::sort (:130-135). std::sort is not stable, so
+ *                           ties follow the C++ library's algorithm; this is
+ *                           GCC libstdc++'s introsort (bits/stl_algo.h
+ *                           std::__sort: quicksort to runs of 16 with
+ *                           median-of-three pivots, heapsort below 2*lg(n)
+ *                           depth, then one insertion-sort pass), restated in
+ *                           degree_sort below. When the queue empties, the
+ *                           first unvisited row of least degree starts the next
+ *                           component (:141-158). Reverse Cuthill-McKee reverses
+ *                           the map (:168-170). map[new] = old.
+ *   oracle_reorder_csr      reorder.cpp:215-365 (reorder<..., Reverse>): row
+ *                           i of the result is row mapView[i] of the input with
+ *                           columns renumbered by the inverse and re-sorted;
+ *                           reverse = 1 reads the map as old -> new (:273-274).
+ *   oracle_reorder_vector   reorder.cpp:368-394: out[i] = in[map[i]], or
+ *                           out[map[i]] = in[i] when reverse.
  *   oracle_element_gradient dJ/dk_e = -lambda_e^T K_e0 u_e (no reference
  *                           counterpart; K_e = k_e K_e0 as in
  *                           fem/.../LinearIsotropicStiffnessIntegrand_impl.hpp:74).
@@ -521,6 +542,231 @@ int oracle_assemble_vector(int64_t rows, int64_t elements, int32_t nodesPerEleme
             if (d[a] >= rows) return 2;
             out[d[a]] += scale ? scale[e] * v[a] : v[a];
         }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reordering                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* The order GCC's std::sort leaves items in, for items compared by
+ * key[item] (strictly less). Each helper below is one step of that algorithm. */
+static int key_less(const int32_t* key, int32_t left, int32_t right) { return key[left] < key[right]; }
+
+static void swap_items(int32_t* a, int64_t i, int64_t j) {
+    const int32_t t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+
+/* Sift-down of the hole at `hole` to a leaf, then sift the value back up to `top`. */
+static void heap_adjust(int32_t* a, int64_t hole, int64_t len, int32_t value, const int32_t* key) {
+    const int64_t top = hole;
+    int64_t child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (key_less(key, a[child], a[child - 1])) --child;
+        a[hole] = a[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        a[hole] = a[child - 1];
+        hole = child - 1;
+    }
+    int64_t parent = (hole - 1) / 2;
+    while (hole > top && key_less(key, a[parent], value)) {
+        a[hole] = a[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    a[hole] = value;
+}
+
+static void heap_sort_range(int32_t* a, int64_t n, const int32_t* key) {
+    if (n >= 2)
+        for (int64_t parent = (n - 2) / 2;; --parent) {
+            heap_adjust(a, parent, n, a[parent], key);
+            if (parent == 0) break;
+        }
+    for (int64_t last = n; last > 1;) {
+        --last;
+        const int32_t value = a[last];
+        a[last] = a[0];
+        heap_adjust(a, 0, last, value, key);
+    }
+}
+
+/* Moves the median of a[x], a[y], a[z] to a[r]. */
+static void median_to_first(int32_t* a, int64_t r, int64_t x, int64_t y, int64_t z, const int32_t* key) {
+    if (key_less(key, a[x], a[y])) {
+        if (key_less(key, a[y], a[z])) swap_items(a, r, y);
+        else if (key_less(key, a[x], a[z])) swap_items(a, r, z);
+        else swap_items(a, r, x);
+    } else if (key_less(key, a[x], a[z])) swap_items(a, r, x);
+    else if (key_less(key, a[y], a[z])) swap_items(a, r, z);
+    else swap_items(a, r, y);
+}
+
+/* Partition of a[lo, hi) around the pivot a[p] (p < lo); returns the cut. */
+static int64_t partition_around(int32_t* a, int64_t lo, int64_t hi, int64_t p, const int32_t* key) {
+    for (;;) {
+        while (key_less(key, a[lo], a[p])) ++lo;
+        --hi;
+        while (key_less(key, a[p], a[hi])) --hi;
+        if (!(lo < hi)) return lo;
+        swap_items(a, lo, hi);
+        ++lo;
+    }
+}
+
+static void introsort_loop(int32_t* a, int64_t lo, int64_t hi, int64_t depth, const int32_t* key) {
+    while (hi - lo > 16) {
+        if (depth == 0) {
+            heap_sort_range(a + lo, hi - lo, key);
+            return;
+        }
+        --depth;
+        median_to_first(a, lo, lo + 1, lo + (hi - lo) / 2, hi - 1, key);
+        const int64_t cut = partition_around(a, lo + 1, hi, lo, key);
+        introsort_loop(a, cut, hi, depth, key);
+        hi = cut;
+    }
+}
+
+/* Insertion of a[i] into the sorted a[lo, i); `guarded` checks against lo first. */
+static void insert_item(int32_t* a, int64_t lo, int64_t i, int guarded, const int32_t* key) {
+    const int32_t value = a[i];
+    if (guarded && key_less(key, value, a[lo])) {
+        memmove(a + lo + 1, a + lo, sizeof(int32_t) * (size_t)(i - lo));
+        a[lo] = value;
+        return;
+    }
+    int64_t hole = i;
+    while (key_less(key, value, a[hole - 1])) {
+        a[hole] = a[hole - 1];
+        --hole;
+    }
+    a[hole] = value;
+}
+
+static void degree_sort(int32_t* a, int64_t n, const int32_t* key) {
+    if (n <= 0) return;
+    int64_t lg = 0;
+    while (((int64_t)1 << (lg + 1)) <= n) ++lg;
+    introsort_loop(a, 0, n, 2 * lg, key);
+    const int64_t head = n > 16 ? 16 : n;
+    for (int64_t i = 1; i < head; ++i) insert_item(a, 0, i, 1, key);
+    for (int64_t i = head; i < n; ++i) insert_item(a, 0, i, 0, key);
+}
+
+/* items[0, n) sorted in place by key[item], in the order described above. */
+int oracle_degree_sort(int64_t n, int32_t* items, const int32_t* key) {
+    degree_sort(items, n, key);
+    return 0;
+}
+
+/* The heapsort step alone (std::partial_sort over the whole range). */
+int oracle_heap_sort(int64_t n, int32_t* items, const int32_t* key) {
+    heap_sort_range(items, n, key);
+    return 0;
+}
+
+/* strategy: 0 none (identity), 1 Cuthill-McKee, 2 reverse Cuthill-McKee. */
+int oracle_make_reordering(int64_t rows, const int32_t* rowptr, const int32_t* cols, int strategy, int32_t* map) {
+    if (strategy < 0 || strategy > 2) return 1;
+    for (int64_t r = 0; r < rows; ++r) map[r] = (int32_t)r;
+    if (strategy == 0 || rows == 0 || rowptr[rows] == 0) return 0;
+    for (int64_t k = 0; k < rowptr[rows]; ++k)
+        if (cols[k] < 0 || cols[k] >= rows) return 2;
+    int32_t* degree = (int32_t*)malloc(sizeof(int32_t) * (size_t)rows);
+    int32_t* queue = (int32_t*)malloc(sizeof(int32_t) * (size_t)rows);
+    char* seen = (char*)calloc((size_t)rows, 1);
+    int64_t least = 0;
+    for (int64_t r = 0; r < rows; ++r) {
+        degree[r] = rowptr[r + 1] - rowptr[r];
+        if (degree[r] < degree[least]) least = r;
+    }
+    int64_t head = 0, tail = 0, placed = 0;
+    queue[tail++] = (int32_t)least;
+    seen[least] = 1;
+    while (head < tail) {
+        const int32_t row = queue[head++];
+        const int64_t batch = tail;
+        for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
+            if (!seen[cols[k]]) {
+                seen[cols[k]] = 1;
+                queue[tail++] = cols[k];
+            }
+        degree_sort(queue + batch, tail - batch, degree);
+        map[placed++] = row;
+        if (head == tail) {
+            int64_t next = -1;
+            for (int64_t r = 0; r < rows; ++r)
+                if (!seen[r] && (next < 0 || degree[r] < degree[next])) next = r;
+            if (next >= 0) {
+                seen[next] = 1;
+                queue[tail++] = (int32_t)next;
+            }
+        }
+    }
+    if (strategy == 2)
+        for (int64_t i = 0, j = rows - 1; i < j; ++i, --j) {
+            const int32_t t = map[i];
+            map[i] = map[j];
+            map[j] = t;
+        }
+    free(degree);
+    free(queue);
+    free(seen);
+    return placed == rows ? 0 : 2;
+}
+
+/* Row i of the result is row source[i] with columns target[c], re-sorted;
+ * source = map, target = inverse(map) (reverse = 0), or the other way round. */
+int oracle_reorder_csr(int64_t rows, const int32_t* map, int reverse, const int32_t* rowptr, const int32_t* cols,
+                       const double* vals, int32_t* newRowptr, int32_t* newCols, double* newVals) {
+    int32_t* inverse = (int32_t*)malloc(sizeof(int32_t) * (size_t)(rows > 0 ? rows : 1));
+    for (int64_t r = 0; r < rows; ++r) inverse[r] = -1;
+    for (int64_t i = 0; i < rows; ++i) {
+        if (map[i] < 0 || map[i] >= rows || inverse[map[i]] >= 0) {
+            free(inverse);
+            return 2;
+        }
+        inverse[map[i]] = (int32_t)i;
+    }
+    const int32_t* source = reverse ? inverse : map;
+    const int32_t* target = reverse ? map : inverse;
+    newRowptr[0] = 0;
+    for (int64_t i = 0; i < rows; ++i) {
+        const int32_t old = source[i];
+        const int32_t begin = rowptr[old], end = rowptr[old + 1];
+        const int32_t out = newRowptr[i];
+        newRowptr[i + 1] = out + (end - begin);
+        /* insertion sort of the renumbered row (stable) */
+        for (int32_t k = begin; k < end; ++k) {
+            const int32_t column = target[cols[k]];
+            const double value = vals[k];
+            int32_t hole = out + (k - begin);
+            while (hole > out && newCols[hole - 1] > column) {
+                newCols[hole] = newCols[hole - 1];
+                newVals[hole] = newVals[hole - 1];
+                --hole;
+            }
+            newCols[hole] = column;
+            newVals[hole] = value;
+        }
+    }
+    free(inverse);
+    return 0;
+}
+
+int oracle_reorder_vector(int64_t n, const int32_t* map, int reverse, const double* in, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (map[i] < 0 || map[i] >= n) return 2;
+        if (reverse) out[map[i]] = in[i];
+        else out[i] = in[map[i]];
     }
     return 0;
 }
