@@ -173,6 +173,15 @@ _SIGNATURES = {
     "cie_hip_reorder_f64": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
     "cie_hip_reorder_f32": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
     "cie_hip_csr_copy_out": (c_int, [_P, _P, _P, _P]),
+    # MatrixMarket files
+    "cie_hip_mm_write_csr_f64": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),
+    "cie_hip_mm_write_csr_f32": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),
+    "cie_hip_mm_write_array_f64": (c_int, [c_char_p, _P, c_int64, c_int]),
+    "cie_hip_mm_write_array_f32": (c_int, [c_char_p, _P, c_int64, c_int]),
+    "cie_hip_mm_read": (c_int, [c_char_p, POINTER(_P)]),
+    "cie_hip_mm_shape": (c_int, [_P, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64), POINTER(c_int)]),
+    "cie_hip_mm_arrays": (c_int, [_P, POINTER(_P), POINTER(_P), POINTER(_P)]),
+    "cie_hip_mm_free": (c_int, [_P]),
     # structured systems (include/cie_hip_systems.h)
     "cie_sys_last_error": (c_char_p, []),
     "cie_sys_splitmix64": (c_int, [c_uint64, c_int64, _P]),

@@ -327,6 +327,32 @@ int cie_hip_reorder_f64(cie_hip_context context, const int32_t* map, int reverse
 int cie_hip_reorder_f32(cie_hip_context context, const int32_t* map, int reverse, const float* in, float* out,
                         int64_t size);
 
+/* ---- MatrixMarket files (cieutils/packages/io/inc/MatrixMarket.hpp) ------ */
+/* MatrixMarket::Output (MatrixMarket.cpp:36-78): host arrays to a file.
+ * CSR -> "coordinate real general", one "row column value" line per stored
+ * entry in CSR order, 1-based; a vector -> "array real general", size x 1.
+ * digits 0 prints values as std::ostream does by default (6 significant
+ * digits, the reference's files); digits > 0 prints that many (17 round-trips
+ * a double). */
+int cie_hip_mm_write_csr_f64(const char* path, int64_t rows, int64_t columns, int64_t nnz, const int32_t* rowExtents,
+                             const int32_t* columnIndices, const double* entries, int digits);
+int cie_hip_mm_write_csr_f32(const char* path, int64_t rows, int64_t columns, int64_t nnz, const int32_t* rowExtents,
+                             const int32_t* columnIndices, const float* entries, int digits);
+int cie_hip_mm_write_array_f64(const char* path, const double* values, int64_t size, int digits);
+int cie_hip_mm_write_array_f32(const char* path, const float* values, int64_t size, int digits);
+/* Reading (no reference counterpart: MatrixMarket::Input is empty,
+ * MatrixMarket.hpp:25-26). Coordinate files (real, integer or pattern;
+ * general, symmetric or skew-symmetric) become CSR with rows sorted by column,
+ * duplicates summed in file order and symmetric mirrors added; general
+ * real/integer arrays are kept dense, column-major. The arrays stay valid
+ * until cie_hip_mm_free; dense data has no row extents or column indices. */
+typedef struct cie_hip_mm_s* cie_hip_mm;
+int cie_hip_mm_read(const char* path, cie_hip_mm* data);
+int cie_hip_mm_shape(cie_hip_mm data, int64_t* rows, int64_t* columns, int64_t* nnz, int* dense);
+int cie_hip_mm_arrays(cie_hip_mm data, const int32_t** rowExtents, const int32_t** columnIndices,
+                      const double** entries);
+int cie_hip_mm_free(cie_hip_mm data);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,3 +220,41 @@ def reorder_vector(mapping, values, reverse=False):
     if status:
         raise OracleError(status, "reorder_vector")
     return out
+
+
+_ref = None
+
+
+def ref_lib():
+    """oracle/_ref/libref_matrix_market.so: the reference's own MatrixMarket writer (None if not built and the
+    reference tree is absent)."""
+    global _ref
+    if _ref is None:
+        from .build import REF_OUTPUT, build_reference
+
+        if not REF_OUTPUT.exists() and build_reference() is None:
+            return None
+        l = ctypes.CDLL(str(REF_OUTPUT))
+        P = c_void_p
+        for suffix in ("f64", "f32"):
+            getattr(l, f"ref_mm_write_csr_{suffix}").argtypes = [ctypes.c_char_p, c_int, c_int, c_int, P, P, P]
+            getattr(l, f"ref_mm_write_array_{suffix}").argtypes = [ctypes.c_char_p, P, c_int64]
+        _ref = l
+    return _ref
+
+
+def ref_write_csr(path, csr, dtype=np.float64):
+    """Writes `csr` with the reference's MatrixMarket::Output (coordinate real general)."""
+    suffix = "f32" if dtype == np.float32 else "f64"
+    entries = np.ascontiguousarray(csr.entries, dtype=dtype)
+    ext, cols = _i32(csr.row_extents), _i32(csr.column_indices)
+    status = getattr(ref_lib(), f"ref_mm_write_csr_{suffix}")(str(path).encode(), csr.rows, csr.columns, csr.nnz,
+                                                               _p(ext), _p(cols), _p(entries))
+    assert status == 0
+
+
+def ref_write_array(path, values, dtype=np.float64):
+    suffix = "f32" if dtype == np.float32 else "f64"
+    values = np.ascontiguousarray(values, dtype=dtype)
+    status = getattr(ref_lib(), f"ref_mm_write_array_{suffix}")(str(path).encode(), _p(values), values.size)
+    assert status == 0
