@@ -5,7 +5,7 @@ The shared library is built in-tree by ``cie_amd._build`` and loaded from
 importing a compute entry point raises.
 """
 import ctypes
-from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint16, c_uint64, c_void_p
 from pathlib import Path
 
 import os
@@ -24,6 +24,7 @@ ERROR_MISSING_DIAGONAL = 6
 ERROR_HIP = 7
 ERROR_OUT_OF_MEMORY = 8
 ERROR_NO_DEVICE = 9
+ERROR_NOT_IMPLEMENTED = 10
 
 HOST = 0
 DEVICE = 1
@@ -63,8 +64,14 @@ class MixedSettings(ctypes.Structure):
     _fields_ = [("settings", Statistics), ("refinementFactor", c_double), ("restart", c_int32), ("pad", c_int32)]
 
 
+class PmgInfo(ctypes.Structure):
+    _fields_ = [("cycles", c_uint64), ("coarseIterations", c_uint64), ("initialResidual", c_double),
+                ("residual", c_double), ("converged", c_int32), ("status", c_int32), ("seconds", c_double)]
+
+
 class LagrangeSpec(ctypes.Structure):
-    _fields_ = [("dim", c_int32), ("order", c_int32), ("elements", c_int32), ("eliminate", c_int32)]
+    _fields_ = [("dim", c_int32), ("order", c_int32), ("elements", c_int32), ("eliminate", c_int32),
+                ("basis", c_int32)]
 
 
 class CieHipError(RuntimeError):
@@ -91,6 +98,10 @@ class MissingDiagonal(CieHipError):
     pass
 
 
+class NotImplementedForArguments(CieHipError):
+    """NotImplementedException (MaskedJacobiOperator.cpp:64-65, ApproximateSubstitution.cpp:40-41)."""
+
+
 class ShapeError(CieHipError):
     pass
 
@@ -101,6 +112,7 @@ _ERRORS = {
     ERROR_BREAKDOWN: Breakdown,
     ERROR_ZERO_DIAGONAL: ZeroDiagonal,
     ERROR_MISSING_DIAGONAL: MissingDiagonal,
+    ERROR_NOT_IMPLEMENTED: NotImplementedForArguments,
 }
 
 _P = c_void_p
@@ -173,6 +185,18 @@ _SIGNATURES = {
     "cie_hip_reorder_f64": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
     "cie_hip_reorder_f32": (c_int, [_P, _P, c_int, _P, _P, c_int64]),
     "cie_hip_csr_copy_out": (c_int, [_P, _P, _P, _P]),
+    # masked operators, Jacobi smoothers, p-multigrid
+    "cie_hip_mask_create": (c_int, [_P, _P, c_int64, POINTER(_P)]),
+    "cie_hip_mask_destroy": (c_int, [_P]),
+    "cie_hip_csr_masked": (c_int, [_P, _P, c_uint16, POINTER(_P)]),
+    "cie_hip_masked_identity_f64": (c_int, [_P, _P, c_uint16, c_double, _P, c_double, _P, c_int64]),
+    "cie_hip_jacobi_f64": (c_int, [_P, _P, _P, c_uint64, c_double, c_double, _P, c_double, _P, c_int64]),
+    "cie_hip_masked_jacobi_f64": (c_int, [_P, _P, _P, c_uint16, c_uint64, c_double, c_double, _P, c_double, _P,
+                                          c_int64]),
+    "cie_hip_approximate_substitution_f64": (c_int, [_P, _P, c_uint64, c_double, _P, c_double, _P, c_int64]),
+    "cie_hip_pmg_create": (c_int, [_P, _P, c_int, POINTER(_P)]),
+    "cie_hip_pmg_destroy": (c_int, [_P]),
+    "cie_hip_pmg_solve": (c_int, [_P, _P, _P, c_int64, c_double, c_uint64, POINTER(PmgInfo)]),
     # MatrixMarket files
     "cie_hip_mm_write_csr_f64": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),
     "cie_hip_mm_write_csr_f32": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),
@@ -195,6 +219,7 @@ _SIGNATURES = {
     "cie_sys_lagrange_element_matrix": (c_int, [POINTER(LagrangeSpec), _P]),
     "cie_sys_lagrange_element_dofs": (c_int, [POINTER(LagrangeSpec), _P]),
     "cie_sys_csr_slots": (c_int, [c_int64, _P, _P, c_int64, _P, _P, _P]),
+    "cie_sys_lagrange_ansatz_mask": (c_int, [POINTER(LagrangeSpec), _P]),
 }
 
 _library = None

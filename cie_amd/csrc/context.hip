@@ -47,6 +47,7 @@ const char* cie_hip_status_string(int status) {
     case CIE_HIP_ERROR_HIP: return "HIP runtime error";
     case CIE_HIP_ERROR_OUT_OF_MEMORY: return "out of device memory";
     case CIE_HIP_ERROR_NO_DEVICE: return "no HIP device";
+    case CIE_HIP_ERROR_NOT_IMPLEMENTED: return "not implemented for these arguments";
     default: return "unknown status";
     }
 }

@@ -13,7 +13,15 @@
 //     an n^d unit-square/cube mesh (d = 2, 3), nodes at the Chebyshev-Lobatto
 //     points cos(i*pi/p) of each element (poisson2D/mesh.cpp:73-77), with a
 //     per-element conductivity k_e multiplying the element matrix and the
-//     Dirichlet boundary rows and columns eliminated.
+//     Dirichlet boundary rows and columns eliminated;
+//   * the same with the hierarchical integrated Legendre basis
+//     (fem/packages/maths/src/LegendrePolynomial.cpp: N_0 = (1 - x)/2,
+//     N_1 = (1 + x)/2, N_k = int_1^x P_{k-1} = (P_k - P_{k-2}) / (2k - 1) on
+//     [-1, 1]), the basis poisson2D uses by default and its p-multigrid
+//     needs. Along each dimension the element's vertex functions sit at the
+//     element's end positions and N_k (k >= 2) at interior position k - 1, so
+//     the node-grid numbering, the coupling pattern and the Dirichlet
+//     elimination are those of the Lagrange case.
 //
 // Node numbering is lexicographic with x fastest, so every row's columns come
 // out sorted and unique, as CSRView users expect (DiagonalOperator.cpp:144-148).
@@ -145,8 +153,36 @@ void lagrange(const std::vector<double>& nodes, double x, std::vector<double>& v
     }
 }
 
+// Legendre polynomials P_0..P_n at x (three-term recurrence).
+void legendre(int n, double x, std::vector<double>& values) {
+    values.assign(size_t(n) + 1, 0.0);
+    values[0] = 1.0;
+    if (n >= 1) values[1] = x;
+    for (int k = 2; k <= n; ++k) values[k] = ((2.0 * k - 1.0) * x * values[k - 1] - (k - 1.0) * values[k - 2]) / k;
+}
+
+// Integrated Legendre basis of order p in position order on t in [0, 1]
+// (x = 2t - 1): position 0 holds N_0, position p holds N_1, position l
+// (0 < l < p) holds N_{l+1}. Slopes are d/dt = 2 d/dx.
+void integratedLegendre(int p, double t, std::vector<double>& value, std::vector<double>& slope) {
+    const double x = 2.0 * t - 1.0;
+    std::vector<double> P;
+    legendre(p, x, P);
+    value.assign(size_t(p) + 1, 0.0);
+    slope.assign(size_t(p) + 1, 0.0);
+    value[0] = 0.5 * (1.0 - x);
+    slope[0] = -1.0;
+    value[size_t(p)] = 0.5 * (1.0 + x);
+    slope[size_t(p)] = 1.0;
+    for (int l = 1; l < p; ++l) {
+        const int k = l + 1;
+        value[size_t(l)] = (P[size_t(k)] - P[size_t(k) - 2]) / (2.0 * k - 1.0);
+        slope[size_t(l)] = 2.0 * P[size_t(k) - 1];
+    }
+}
+
 // stiffness[a*(p+1)+b] = int_0^1 phi_a' phi_b',  mass[...] = int_0^1 phi_a phi_b.
-void referenceMatrices(int p, std::vector<double>& stiffness, std::vector<double>& mass) {
+void referenceMatrices(int p, int basis, std::vector<double>& stiffness, std::vector<double>& mass) {
     const int n = p + 1;
     const std::vector<double> nodes = elementNodes(p);
     std::vector<double> points, weights;
@@ -157,7 +193,8 @@ void referenceMatrices(int p, std::vector<double>& stiffness, std::vector<double
     for (size_t q = 0; q < points.size(); ++q) {
         const double x = 0.5 * (points[q] + 1.0);
         const double w = 0.5 * weights[q];
-        lagrange(nodes, x, value, slope);
+        if (basis == CIE_SYS_BASIS_LEGENDRE) integratedLegendre(p, x, value, slope);
+        else lagrange(nodes, x, value, slope);
         for (int a = 0; a < n; ++a)
             for (int b = 0; b < n; ++b) {
                 stiffness[a * n + b] += w * slope[a] * slope[b];
@@ -167,7 +204,7 @@ void referenceMatrices(int p, std::vector<double>& stiffness, std::vector<double
 }
 
 struct Lagrange {
-    int dim = 3, order = 2, elements = 1, eliminate = 1;
+    int dim = 3, order = 2, elements = 1, eliminate = 1, basis = CIE_SYS_BASIS_LAGRANGE;
     int nodesPerDim = 0;       // global nodes per dimension, order*elements + 1
     int first = 0, last = 0;   // first/last retained node index per dimension
     int retained = 0;          // retained nodes per dimension
@@ -180,7 +217,9 @@ struct Lagrange {
         order = spec->order;
         elements = spec->elements;
         eliminate = spec->eliminate ? 1 : 0;
+        basis = spec->basis;
         if ((dim != 2 && dim != 3) || order < 1 || order > 6 || elements < 1) return false;
+        if (basis != CIE_SYS_BASIS_LAGRANGE && basis != CIE_SYS_BASIS_LEGENDRE) return false;
         nodesPerDim = order * elements + 1;
         first = eliminate ? 1 : 0;
         last = eliminate ? nodesPerDim - 2 : nodesPerDim - 1;
@@ -194,7 +233,7 @@ struct Lagrange {
 
     void buildElementMatrix() {
         std::vector<double> k1, m1;
-        referenceMatrices(order, k1, m1);
+        referenceMatrices(order, basis, k1, m1);
         const int n = order + 1;
         const double h = 1.0 / elements;
         // Physical 1D blocks: K1 = k1 / h, M1 = h m1, so the d-dimensional
@@ -427,6 +466,26 @@ int cie_sys_lagrange_element_dofs(const cie_sys_lagrange_spec* spec, int32_t* do
             for (int d = 0; d < mesh.dim; ++d) kept = kept && mesh.first <= idx[d] && idx[d] <= mesh.last;
             dofs[e * mesh.npe + a] = kept ? int32_t(mesh.pack(idx)) : -1;
         }
+    }
+    return 0;
+}
+
+int cie_sys_lagrange_ansatz_mask(const cie_sys_lagrange_spec* spec, uint16_t* mask) {
+    Lagrange mesh;
+    if (!mesh.init(spec)) return fail("cie_sys_lagrange_ansatz_mask: invalid spec");
+    if (mesh.basis != CIE_SYS_BASIS_LEGENDRE)
+        return fail("cie_sys_lagrange_ansatz_mask: the mask is defined for the hierarchical (Legendre) basis");
+    if (!mask) return fail("cie_sys_lagrange_ansatz_mask: null output");
+    const int64_t rows = mesh.rowCount();
+    for (int64_t row = 0; row < rows; ++row) {
+        int idx[3];
+        mesh.unpack(row, idx);
+        int degree = 1;  // vertex functions N_0, N_1 count as degree 1
+        for (int d = 0; d < mesh.dim; ++d) {
+            const int local = idx[d] % mesh.order;
+            if (local != 0) degree = std::max(degree, local + 1);  // N_{local+1}
+        }
+        mask[row] = uint16_t(degree);
     }
     return 0;
 }

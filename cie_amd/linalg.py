@@ -478,6 +478,156 @@ class MixedPCG(IterativeSolver):
         self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
 
 
+class Mask:
+    """An ansatz mask on the device (one uint16 per row; makeAnsatzMask, fem/packages/graph/src/Assembler.cpp:194-213)."""
+
+    def __init__(self, values, context=None):
+        self.context = context or Context.default()
+        self.values = np.ascontiguousarray(values, dtype=np.uint16)
+        self.handle = c_void_p()
+        check(library().cie_hip_mask_create(self.context.handle, self.values.ctypes.data_as(c_void_p),
+                                            self.values.size, byref(self.handle)), "cie_hip_mask_create")
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_mask_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def maskedMatrix(matrix, mask, threshold):
+    """The matrix of MaskedCSROperator(lhs, mask, threshold) (MaskedCSROperator.cpp:37-118): rows and columns with
+    mask >= threshold compacted away, so its products are the masked products."""
+    handle = c_void_p()
+    _sync_torch()
+    check(library().cie_hip_csr_masked(matrix.handle, mask.handle, int(threshold), byref(handle)), "cie_hip_csr_masked")
+    rows, columns, nnz = c_int64(), c_int64(), c_int64()
+    check(library().cie_hip_csr_shape(handle, byref(rows), byref(columns), byref(nnz)))
+    masked = CSRMatrix._adopt(handle, matrix.context, rows.value, columns.value, nnz.value)
+    masked.source = matrix  # keeps the source alive as long as the masked copy is used with it
+    return masked
+
+
+class MaskedCSROperator(HIPCSROperator):
+    """MaskedCSROperator(lhs, mask, threshold) over a HIPSpace (MaskedCSROperator.hpp)."""
+
+    def __init__(self, matrix, mask, threshold, space):
+        super().__init__(maskedMatrix(matrix, mask, threshold), space)
+
+
+class MaskedIdentityOperator(LinearOperator):
+    """out = inScale out + outScale (mask < threshold ? in : 0) (MaskedIdentityOperator.cpp:14-70)."""
+
+    def __init__(self, space, mask, threshold):
+        self.space, self.mask, self.threshold = space, mask, int(threshold)
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        _sync_torch()
+        check(library().cie_hip_masked_identity_f64(self.space.context.handle, self.mask.handle, self.threshold,
+                                                    inScale, _dptr(in_), outScale, _dptr(out), out.numel()),
+              "MaskedIdentityOperator.product")
+
+
+class JacobiOperator(LinearOperator):
+    """JacobiOperator(space, size, lhs, inverseDiagonal, iterations, relaxation) (JacobiOperator.cpp:15-62).
+
+    lhs is a CSRMatrix (a masked one included), inverseDiagonal a DiagonalOperator."""
+
+    def __init__(self, space, size, lhs, inverseDiagonal, iterations=1, relaxation=1.0):
+        self.space, self.size, self.lhs = space, int(size), lhs
+        self.inverseDiagonal, self.iterations, self.relaxation = inverseDiagonal, int(iterations), float(relaxation)
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        _sync_torch()
+        check(library().cie_hip_jacobi_f64(self.space.context.handle, self.lhs.handle,
+                                           _dptr(self.inverseDiagonal.inverseDiagonal), self.iterations,
+                                           self.relaxation, inScale, _dptr(in_), outScale, _dptr(out), out.numel()),
+              "JacobiOperator.product")
+
+
+class MaskedJacobiOperator(LinearOperator):
+    """MaskedJacobiOperator (MaskedJacobiOperator.cpp:46-161) on the device."""
+
+    def __init__(self, matrix, iterations, relaxation, mask, threshold, space):
+        self.matrix, self.iterations, self.relaxation = matrix, int(iterations), float(relaxation)
+        self.mask, self.threshold, self.space = mask, int(threshold), space
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        _sync_torch()
+        check(library().cie_hip_masked_jacobi_f64(self.space.context.handle, self.matrix.handle, self.mask.handle,
+                                                  self.threshold, self.iterations, self.relaxation, inScale,
+                                                  _dptr(in_), outScale, _dptr(out), out.numel()),
+              "MaskedJacobiOperator.product")
+
+
+class ApproximateSubstitution(LinearOperator):
+    """ApproximateSubstitution (ApproximateSubstitution.cpp:27-111) on the device."""
+
+    def __init__(self, matrix, iterations, space):
+        self.matrix, self.iterations, self.space = matrix, int(iterations), space
+
+    def product(self, inScale, in_, outScale, out):
+        self.space._check(in_, out)
+        _sync_torch()
+        check(library().cie_hip_approximate_substitution_f64(self.space.context.handle, self.matrix.handle,
+                                                             self.iterations, inScale, _dptr(in_), outScale,
+                                                             _dptr(out), out.numel()),
+              "ApproximateSubstitution.product")
+
+
+class NestedProductOperator(LinearOperator):
+    """lhs(rhs(in)) (NestedProductOperator_impl.hpp:24-32): buffer = rhs(in), then lhs.product(inScale, buffer, ...)."""
+
+    def __init__(self, space, lhs, rhs, rowCount):
+        self.space, self.lhs, self.rhs = space, lhs, rhs
+        self.buffer = space.makeVector(rowCount)
+        space.fill(self.buffer, 0)
+
+    def product(self, inScale, in_, outScale, out):
+        self.rhs.product(0, in_, 1, self.buffer)
+        self.lhs.product(inScale, self.buffer, outScale, out)
+
+
+class PMultigrid:
+    """The p-multigrid of poisson2D/solver.cpp:111-232 (solveMultigrid) on the device (cie_hip_pmg_*)."""
+
+    def __init__(self, matrix, mask, order):
+        self.matrix, self.mask, self.order = matrix, mask, int(order)
+        self.handle = c_void_p()
+        check(library().cie_hip_pmg_create(matrix.handle, mask.handle, self.order, byref(self.handle)),
+              "cie_hip_pmg_create")
+        self.info = None
+
+    def solve(self, b, x, relative_tolerance=1e-6, max_cycles=10000):
+        """Cycles from x (expected zero, as in the driver) until ||r|| <= relative_tolerance ||b||."""
+        info = _lib.PmgInfo()
+        _sync_torch()
+        status = library().cie_hip_pmg_solve(self.handle, _dptr(b), _dptr(x), x.numel(), relative_tolerance,
+                                             max_cycles, byref(info))
+        self.info = info
+        check(status, "cie_hip_pmg_solve")
+        return info
+
+    def close(self):
+        if self.handle:
+            library().cie_hip_pmg_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 _REORDERINGS = {"none": _lib.REORDER_NONE, "cuthill-mckee": _lib.REORDER_CUTHILL_MCKEE,
                 "reverse-cuthill-mckee": _lib.REORDER_REVERSE_CUTHILL_MCKEE}
 

@@ -71,8 +71,11 @@ def poisson5(m):
     return CSR(rows.value, rows.value, extents, cols, vals)
 
 
-def _spec(dim, order, elements, eliminate=True):
-    return LagrangeSpec(dim, order, elements, 1 if eliminate else 0)
+BASES = {"lagrange": 0, "legendre": 1}
+
+
+def _spec(dim, order, elements, eliminate=True, basis="lagrange"):
+    return LagrangeSpec(dim, order, elements, 1 if eliminate else 0, BASES[basis])
 
 
 def lagrange_size(dim, order, elements, eliminate=True):
@@ -83,9 +86,10 @@ def lagrange_size(dim, order, elements, eliminate=True):
     return rows.value, nnz.value, count.value, npe.value
 
 
-def lagrange(dim, order, elements, conductivity=None, eliminate=True, threads=0):
-    """Laplace stiffness of order-`order` Lagrange elements on an elements^dim unit mesh."""
-    spec = _spec(dim, order, elements, eliminate)
+def lagrange(dim, order, elements, conductivity=None, eliminate=True, threads=0, basis="lagrange"):
+    """Laplace stiffness of order-`order` tensor-product elements on an elements^dim unit mesh
+    (basis "lagrange": nodal, "legendre": integrated Legendre, hierarchical)."""
+    spec = _spec(dim, order, elements, eliminate, basis)
     rows, nnz, count, _ = lagrange_size(dim, order, elements, eliminate)
     if conductivity is not None:
         conductivity = np.ascontiguousarray(conductivity, dtype=np.float64)
@@ -99,12 +103,21 @@ def lagrange(dim, order, elements, conductivity=None, eliminate=True, threads=0)
     return CSR(rows, rows, extents, cols, vals)
 
 
-def element_matrix(dim, order, elements):
+def element_matrix(dim, order, elements, basis="lagrange"):
     """Element stiffness for k_e = 1 (nodes per element squared, row-major)."""
     _, _, _, npe = lagrange_size(dim, order, elements)
     out = np.empty((npe, npe), dtype=np.float64)
-    check_sys(library().cie_sys_lagrange_element_matrix(byref(_spec(dim, order, elements)), _ptr(out)),
-              "element_matrix")
+    check_sys(library().cie_sys_lagrange_element_matrix(byref(_spec(dim, order, elements, basis=basis)),
+                                                        _ptr(out)), "element_matrix")
+    return out
+
+
+def ansatz_mask(dim, order, elements, eliminate=True):
+    """Per-row ansatz mask of the integrated Legendre basis (largest 1D degree, at least 1), uint16."""
+    rows = lagrange_size(dim, order, elements, eliminate)[0]
+    out = np.empty(rows, dtype=np.uint16)
+    check_sys(library().cie_sys_lagrange_ansatz_mask(byref(_spec(dim, order, elements, eliminate, "legendre")),
+                                                     _ptr(out)), "ansatz_mask")
     return out
 
 

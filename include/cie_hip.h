@@ -37,6 +37,7 @@ extern "C" {
 #define CIE_HIP_ERROR_HIP 7
 #define CIE_HIP_ERROR_OUT_OF_MEMORY 8
 #define CIE_HIP_ERROR_NO_DEVICE 9
+#define CIE_HIP_ERROR_NOT_IMPLEMENTED 10 /* NotImplementedException: MaskedJacobiOperator.cpp:64-65 */
 
 /* memory locations for cie_hip_csr_create / cie_hip_memcpy */
 #define CIE_HIP_HOST 0
@@ -352,6 +353,64 @@ int cie_hip_mm_shape(cie_hip_mm data, int64_t* rows, int64_t* columns, int64_t* 
 int cie_hip_mm_arrays(cie_hip_mm data, const int32_t** rowExtents, const int32_t** columnIndices,
                       const double** entries);
 int cie_hip_mm_free(cie_hip_mm data);
+
+/* ---- Masked operators, Jacobi smoothers, p-multigrid (solvers/src) ------- */
+/* An ansatz mask, one uint16 per row (fem/packages/graph/src/Assembler.cpp:194-213),
+ * uploaded once; the host copy is kept for set-up work. */
+typedef struct cie_hip_mask_s* cie_hip_mask;
+int cie_hip_mask_create(cie_hip_context context, const uint16_t* values, int64_t size, cie_hip_mask* mask);
+int cie_hip_mask_destroy(cie_hip_mask mask);
+/* MaskedCSROperator (MaskedCSROperator.cpp:37-118) as a matrix of its own:
+ * the entries of rows and columns with mask < threshold, compacted on the
+ * device. Its products through cie_hip_csr_product_f64/f32 are the masked
+ * products (a masked row sums to 0). Values are copied: a later
+ * cie_hip_csr_update_entries on `matrix` does not reach `masked`. */
+int cie_hip_csr_masked(cie_hip_csr matrix, cie_hip_mask mask, uint16_t threshold, cie_hip_csr* masked);
+/* MaskedIdentityOperator (MaskedIdentityOperator.cpp:14-70, CPU form):
+ * out = inScale out + outScale (mask < threshold ? in : 0). */
+int cie_hip_masked_identity_f64(cie_hip_context context, cie_hip_mask mask, uint16_t threshold, double inScale,
+                                const double* in, double outScale, double* out, int64_t size);
+/* JacobiOperator (JacobiOperator.cpp:36-62) over lhs (any matrix handle, a
+ * masked one included) with the inverse diagonal `inverseDiagonal` (device;
+ * cie_hip_inverse_diagonal_f64 of the full matrix, as the driver's): x = 0,
+ * `iterations` times x += (w (in - lhs x)) / d, then out = outScale x + inScale out. */
+int cie_hip_jacobi_f64(cie_hip_context context, cie_hip_csr lhs, const double* inverseDiagonal, uint64_t iterations,
+                       double relaxation, double inScale, const double* in, double outScale, double* out,
+                       int64_t size);
+/* MaskedJacobiOperator (MaskedJacobiOperator.cpp:46-161; previous = out before
+ * each sweep, the reference's threaded path) and ApproximateSubstitution
+ * (ApproximateSubstitution.cpp:27-111). inScale != 0 with more than one sweep
+ * is CIE_HIP_ERROR_NOT_IMPLEMENTED, as there; a kept row without a diagonal
+ * entry is CIE_HIP_ERROR_MISSING_DIAGONAL. */
+int cie_hip_masked_jacobi_f64(cie_hip_context context, cie_hip_csr matrix, cie_hip_mask mask, uint16_t threshold,
+                              uint64_t iterations, double relaxation, double inScale, const double* in,
+                              double outScale, double* out, int64_t size);
+int cie_hip_approximate_substitution_f64(cie_hip_context context, cie_hip_csr matrix, uint64_t iterations,
+                                         double inScale, const double* in, double outScale, double* out,
+                                         int64_t size);
+/* The p-multigrid of poisson2D/solver.cpp:111-232 (solveMultigrid). Grid 0
+ * keeps mask < 2 and is Jacobi-PCG to a relative residual of 0.5 (at most 5000
+ * steps); grid g = 1..order-1 keeps mask < g + 2 and is 6 Jacobi sweeps with
+ * relaxation 2/3 on the masked matrix; a cycle visits the grids from the
+ * finest down to 0 and back up from 1, each time adding the grid's update to x
+ * and subtracting the masked product of the update from the residual. The
+ * residual starts as b (x is expected to hold zeros, as in the driver); cycles
+ * run until ||r|| <= relativeTolerance ||b|| (the driver's 1e-6) or maxCycles.
+ * The grids are built once per plan. */
+typedef struct cie_hip_pmg_s* cie_hip_pmg;
+typedef struct cie_hip_pmg_info {
+    uint64_t cycles;
+    uint64_t coarseIterations;   /* summed iteration counts of the coarse CG solves */
+    double initialResidual;      /* ||b|| */
+    double residual;             /* the tracked residual norm after the last cycle */
+    int32_t converged;
+    int32_t status;
+    double seconds;              /* device time of the solve */
+} cie_hip_pmg_info;
+int cie_hip_pmg_create(cie_hip_csr matrix, cie_hip_mask mask, int order, cie_hip_pmg* plan);
+int cie_hip_pmg_destroy(cie_hip_pmg plan);
+int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size, double relativeTolerance,
+                      uint64_t maxCycles, cie_hip_pmg_info* info);
 
 #ifdef __cplusplus
 }

@@ -17,12 +17,17 @@
 extern "C" {
 #endif
 
-/* Tensor-product Lagrange Laplace stiffness on the unit square/cube. */
+/* Tensor-product Laplace stiffness on the unit square/cube. */
+#define CIE_SYS_BASIS_LAGRANGE 0
+#define CIE_SYS_BASIS_LEGENDRE 1
 typedef struct cie_sys_lagrange_spec {
     int32_t dim;       /* 2 or 3 */
-    int32_t order;     /* polynomial order p, nodes cos(i pi / p) per element */
+    int32_t order;     /* polynomial order p */
     int32_t elements;  /* elements per dimension */
     int32_t eliminate; /* 1: drop Dirichlet boundary rows and columns */
+    int32_t basis;     /* CIE_SYS_BASIS_LAGRANGE: nodes cos(i pi / p) per element (poisson2D/mesh.cpp:73-77);
+                          CIE_SYS_BASIS_LEGENDRE: integrated Legendre polynomials, the hierarchical basis of
+                          fem/packages/maths/src/LegendrePolynomial.cpp (poisson2D's default) */
 } cie_sys_lagrange_spec;
 
 const char* cie_sys_last_error(void);
@@ -46,6 +51,10 @@ int cie_sys_lagrange_fill(const cie_sys_lagrange_spec* spec, const double* condu
 int cie_sys_lagrange_element_matrix(const cie_sys_lagrange_spec* spec, double* elementMatrix);
 /* Row index of every element node (elementCount x nodesPerElement), -1 if eliminated. */
 int cie_sys_lagrange_element_dofs(const cie_sys_lagrange_spec* spec, int32_t* dofs);
+/* Ansatz mask of every row for the integrated Legendre basis: the largest
+ * polynomial degree among the DOF's 1D factors, at least 1 (makeAnsatzMask,
+ * fem/packages/graph/src/connectivity.cpp:62-90 and Assembler.cpp:194-213). */
+int cie_sys_lagrange_ansatz_mask(const cie_sys_lagrange_spec* spec, uint16_t* mask);
 /* CSR slot of each (row, column) pair, -1 if absent. */
 int cie_sys_csr_slots(int64_t rows, const int32_t* rowExtents, const int32_t* columnIndices,
                       int64_t pairCount, const int32_t* pairRows, const int32_t* pairColumns,

@@ -14,7 +14,8 @@ import numpy as np
 LIBRARY = Path(__file__).resolve().parent / "liboracle.so"
 
 OP_SCALE, OP_SCALE_UNIFORM, OP_ADD, OP_ASSIGN, OP_FILL = range(5)
-STATUS = {0: "ok", 2: "shape", 3: "not spd", 4: "breakdown", 5: "zero diagonal", 6: "missing diagonal"}
+STATUS = {0: "ok", 2: "shape", 3: "not spd", 4: "breakdown", 5: "zero diagonal", 6: "missing diagonal",
+          10: "not implemented"}
 
 
 class Statistics(ctypes.Structure):
@@ -49,6 +50,15 @@ def lib():
         l.oracle_make_reordering.argtypes = [c_int64, P, P, c_int, P]
         l.oracle_reorder_csr.argtypes = [c_int64, P, c_int, P, P, P, P, P, P]
         l.oracle_reorder_vector.argtypes = [c_int64, P, c_int, P, P]
+        l.oracle_masked_csr_product.argtypes = [c_int64, P, P, P, P, ctypes.c_uint16, c_double, P, c_double, P, c_int]
+        l.oracle_masked_identity.argtypes = [c_int64, P, ctypes.c_uint16, c_double, P, c_double, P]
+        l.oracle_jacobi.argtypes = [c_int64, P, P, P, P, ctypes.c_uint16, c_uint64, c_double, c_double, P, c_double,
+                                    P, c_int]
+        l.oracle_masked_jacobi.argtypes = [c_int64, P, P, P, P, ctypes.c_uint16, c_uint64, c_double, c_double, P,
+                                           c_double, P]
+        l.oracle_approximate_substitution.argtypes = [c_int64, P, P, P, c_uint64, c_double, P, c_double, P]
+        l.oracle_pmg.argtypes = [c_int64, P, P, P, P, c_int, P, P, c_double, c_int64, P, P, c_int64,
+                                 POINTER(c_int64), POINTER(c_uint64), c_int]
         _lib = l
     return _lib
 
@@ -258,3 +268,74 @@ def ref_write_array(path, values, dtype=np.float64):
     values = np.ascontiguousarray(values, dtype=dtype)
     status = getattr(ref_lib(), f"ref_mm_write_array_{suffix}")(str(path).encode(), _p(values), values.size)
     assert status == 0
+
+
+def _u16(a):
+    return np.ascontiguousarray(a, dtype=np.uint16)
+
+
+def _csr(csr):
+    return _i32(csr.row_extents), _i32(csr.column_indices), _f64(csr.entries)
+
+
+def _check(status, what):
+    if status:
+        raise OracleError(status, what)
+
+
+def masked_csr_product(csr, mask, threshold, in_scale, x, out_scale, y, threads=1):
+    """MaskedCSROperator::product, in place on y."""
+    ext, cols, vals = _csr(csr)
+    mask, x = _u16(mask), _f64(x)
+    assert y.dtype == np.float64 and y.flags.c_contiguous
+    lib().oracle_masked_csr_product(csr.rows, _p(ext), _p(cols), _p(vals), _p(mask), threshold, in_scale, _p(x),
+                                    out_scale, _p(y), threads)
+    return y
+
+
+def masked_identity(mask, threshold, in_scale, x, out_scale, y):
+    mask, x = _u16(mask), _f64(x)
+    lib().oracle_masked_identity(y.size, _p(mask), threshold, in_scale, _p(x), out_scale, _p(y))
+    return y
+
+
+def jacobi(csr, iterations, relaxation, in_scale, x, out_scale, y, mask=None, threshold=0, threads=1):
+    """JacobiOperator over csr (or the masked csr) with its full inverse diagonal, in place on y."""
+    ext, cols, vals = _csr(csr)
+    m = None if mask is None else _u16(mask)
+    x = _f64(x)
+    _check(lib().oracle_jacobi(csr.rows, _p(ext), _p(cols), _p(vals), None if m is None else _p(m), threshold,
+                               iterations, relaxation, in_scale, _p(x), out_scale, _p(y), threads), "jacobi")
+    return y
+
+
+def masked_jacobi(csr, mask, threshold, iterations, relaxation, in_scale, x, out_scale, y):
+    ext, cols, vals = _csr(csr)
+    mask, x = _u16(mask), _f64(x)
+    _check(lib().oracle_masked_jacobi(csr.rows, _p(ext), _p(cols), _p(vals), _p(mask), threshold, iterations,
+                                      relaxation, in_scale, _p(x), out_scale, _p(y)), "masked_jacobi")
+    return y
+
+
+def approximate_substitution(csr, iterations, in_scale, x, out_scale, y):
+    ext, cols, vals = _csr(csr)
+    x = _f64(x)
+    _check(lib().oracle_approximate_substitution(csr.rows, _p(ext), _p(cols), _p(vals), iterations, in_scale, _p(x),
+                                                 out_scale, _p(y)), "approximate_substitution")
+    return y
+
+
+def pmg(csr, mask, order, b, relative_tolerance=1e-6, max_cycles=500, threads=1):
+    """solveMultigrid (poisson2D/solver.cpp:111-232) from x = 0.
+
+    Returns (x, residual norm after each cycle, coarse CG iterations of each cycle)."""
+    ext, cols, vals = _csr(csr)
+    mask, b = _u16(mask), _f64(b)
+    x = np.zeros(csr.rows)
+    history = np.zeros(max(1, max_cycles))
+    coarse_history = np.zeros(max(1, max_cycles), dtype=np.uint64)
+    cycles, coarse = c_int64(), c_uint64()
+    _check(lib().oracle_pmg(csr.rows, _p(ext), _p(cols), _p(vals), _p(mask), order, _p(b), _p(x),
+                            relative_tolerance, max_cycles, _p(history), _p(coarse_history), max_cycles,
+                            byref(cycles), byref(coarse), threads), "pmg")
+    return x, history[:cycles.value].copy(), coarse_history[:cycles.value].astype(np.int64)

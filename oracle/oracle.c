@@ -57,12 +57,43 @@
  *                           reverse = 1 reads the map as old -> new (:273-274).
  *   oracle_reorder_vector   reorder.cpp:368-394: out[i] = in[map[i]], or
  *                           out[map[i]] = in[i] when reverse.
+ *   oracle_masked_csr_product
+ *                           solvers/src/MaskedCSROperator.cpp:37-118: rows
+ *                           and columns with mask >= threshold are left out
+ *                           of the row sums (a masked row sums to 0).
+ *   oracle_masked_identity  solvers/src/MaskedIdentityOperator.cpp:14-70 (CPU):
+ *                           op(out, mask < threshold ? in : 0).
+ *   oracle_jacobi           solvers/src/JacobiOperator.cpp:36-62 over a CSR or
+ *                           masked CSR lhs with the DiagonalOperator of the
+ *                           full matrix (DiagonalOperator.cpp:37-53).
+ *   oracle_masked_jacobi    solvers/src/MaskedJacobiOperator.cpp:46-161, the
+ *                           threaded path (previous = out before each sweep,
+ *                           :103); inScale != 0 with more than one sweep is
+ *                           not implemented there (:64-65, status 10).
+ *   oracle_approximate_substitution
+ *                           solvers/src/ApproximateSubstitution.cpp:27-111,
+ *                           threaded path as above (:65).
+ *   oracle_pmg              executables/fem_examples/src/poisson2D/solver.cpp:111-232
+ *                           (solveMultigrid): the coarsest grid (ansatz mask
+ *                           < 2) is Jacobi-PCG on the masked matrix to a
+ *                           relative residual of 0.5 (at most 5000 steps); grid
+ *                           k = 2..p is 6 Jacobi sweeps (relaxation 2/3) on
+ *                           the masked matrix of mask < k + 1 after the mask;
+ *                           a cycle visits the grids from the finest down and
+ *                           back up from the second, each time restricting
+ *                           the residual, adding the grid's update to the
+ *                           solution and subtracting the masked product of the
+ *                           update from the residual, until
+ *                           ||r|| <= 1e-6 ||b|| (the relative tolerance is a
+ *                           parameter here, and so is a cycle cap). The
+ *                           residual starts as b, as in the driver, so x is
+ *                           expected to hold zeros.
  *   oracle_element_gradient dJ/dk_e = -lambda_e^T K_e0 u_e (no reference
  *                           counterpart; K_e = k_e K_e0 as in
  *                           fem/.../LinearIsotropicStiffnessIntegrand_impl.hpp:74).
  *
  * Status codes match include/cie_hip.h: 0 ok, 2 shape, 3 p^T A p <= 0,
- * 4 r^T z <= 0, 5 zero diagonal, 6 missing diagonal.
+ * 4 r^T z <= 0, 5 zero diagonal, 6 missing diagonal, 10 not implemented.
  */
 #include <float.h>
 #include <math.h>
@@ -165,41 +196,70 @@ typedef struct {
     double inScale, outScale;
     const double* in;
     double* out;
+    const uint16_t* mask; /* NULL: every row and column; else only those with mask < threshold */
+    uint16_t threshold;
 } spmv_args_t;
+
+/* out = op(out, right) for one (inScale, outScale) class: CSROperator.cpp:78-110
+ * and the same table in MaskedCSROperator.cpp:85-118, MaskedIdentityOperator.cpp:40-72,
+ * MaskedJacobiOperator.cpp:119-152, ApproximateSubstitution.cpp:78-111. For
+ * inScale = outScale = -1 those compute right - out; this follows the
+ * documented contract (LinearOperator.hpp:15-33), -right - out. */
+static void apply_op(double inScale, double outScale, double* o, double right) {
+    if (inScale == 1.0) {
+        if (outScale == 1.0) *o += right;
+        else if (outScale == -1.0) *o -= right;
+        else *o += outScale * right;
+    } else if (inScale == 0.0) {
+        if (outScale == 1.0) *o = right;
+        else if (outScale == -1.0) *o = -right;
+        else *o = outScale * right;
+    } else if (inScale == -1.0) {
+        if (outScale == 1.0) *o = right - *o;
+        else if (outScale == -1.0) *o = -right - *o;
+        else *o = outScale * right - *o;
+    } else {
+        if (outScale == 1.0) *o = inScale * *o + right;
+        else if (outScale == -1.0) *o = inScale * *o - right;
+        else *o = inScale * *o + outScale * right;
+    }
+}
 
 static void spmv_task(void* raw, int64_t begin, int64_t end, int worker) {
     (void)worker;
     const spmv_args_t* a = (const spmv_args_t*)raw;
     for (int64_t row = begin; row < end; ++row) {
         double contribution = 0.0;
-        for (int32_t k = a->rowptr[row]; k < a->rowptr[row + 1]; ++k)
-            contribution += a->vals[k] * a->in[a->cols[k]];
-        double* o = a->out + row;
-        /* CSROperator.cpp:78-110, one case per (inScale, outScale) class. */
-        if (a->inScale == 1.0) {
-            if (a->outScale == 1.0) *o += contribution;
-            else if (a->outScale == -1.0) *o -= contribution;
-            else *o += a->outScale * contribution;
-        } else if (a->inScale == 0.0) {
-            if (a->outScale == 1.0) *o = contribution;
-            else if (a->outScale == -1.0) *o = -contribution;
-            else *o = a->outScale * contribution;
-        } else if (a->inScale == -1.0) {
-            if (a->outScale == 1.0) *o = contribution - *o;
-            else if (a->outScale == -1.0) *o = -contribution - *o;
-            else *o = a->outScale * contribution - *o;
-        } else {
-            if (a->outScale == 1.0) *o = a->inScale * *o + contribution;
-            else if (a->outScale == -1.0) *o = a->inScale * *o - contribution;
-            else *o = a->inScale * *o + a->outScale * contribution;
+        if (!a->mask) {
+            for (int32_t k = a->rowptr[row]; k < a->rowptr[row + 1]; ++k)
+                contribution += a->vals[k] * a->in[a->cols[k]];
+        } else if (a->mask[row] < a->threshold) {  /* MaskedCSROperator.cpp:53-66: masked entries are skipped */
+            for (int32_t k = a->rowptr[row]; k < a->rowptr[row + 1]; ++k)
+                if (a->mask[a->cols[k]] < a->threshold) contribution += a->vals[k] * a->in[a->cols[k]];
         }
+        apply_op(a->inScale, a->outScale, a->out + row, contribution);
     }
 }
 
 static void spmv(pool_t* pool, int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
                  double inScale, const double* in, double outScale, double* out) {
-    spmv_args_t a = {rowptr, cols, vals, inScale, outScale, in, out};
+    spmv_args_t a = {rowptr, cols, vals, inScale, outScale, in, out, NULL, 0};
     pool_run(pool, spmv_task, &a, rows);
+}
+
+/* A CSR operator, optionally masked (MaskedCSROperator). */
+typedef struct {
+    int64_t rows;
+    const int32_t *rowptr, *cols;
+    const double* vals;
+    const uint16_t* mask;
+    uint16_t threshold;
+} csr_operator_t;
+
+static void apply_csr(pool_t* pool, const csr_operator_t* A, double inScale, const double* in, double outScale,
+                      double* out) {
+    spmv_args_t a = {A->rowptr, A->cols, A->vals, inScale, outScale, in, out, A->mask, A->threshold};
+    pool_run(pool, spmv_task, &a, A->rows);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -371,37 +431,29 @@ typedef struct {
  * a Jacobi (DiagonalOperator) preconditioner when `precondition` is nonzero.
  * history (optional, capacity historyCapacity) receives the absolute residual
  * after every loop trip. */
-int oracle_pcg(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
-               double inScale, const double* b, double outScale, double* x,
-               uint64_t maxIterations, double absoluteTolerance, double relativeTolerance,
-               int precondition, int threads, oracle_statistics_t* stats,
-               double* history, int64_t historyCapacity, double* seconds) {
-    struct timespec t0, t1;
-    clock_gettime(CLOCK_MONOTONIC, &t0);
-    pool_t* pool = pool_create(resolve_threads(threads));
-    const int64_t n = rows;
+/* ConjugateGradients<TSpace>::product(inScale, b, outScale, x) over the
+ * operator A with the Jacobi DiagonalOperator `inverse` (NULL: no
+ * preconditioner); line numbers are ConjugateGradients_impl.hpp's. */
+static int pcg_core(pool_t* pool, const csr_operator_t* A, const double* inverse, double inScale, const double* b,
+                    double outScale, double* x, uint64_t maxIterations, double absoluteTolerance,
+                    double relativeTolerance, oracle_statistics_t* stats, double* history, int64_t historyCapacity) {
+    const int64_t n = A->rows;
+    const int precondition = inverse != NULL;
     int status = 0;
     double* solution = (double*)malloc((size_t)n * sizeof(double));
     double* search = (double*)malloc((size_t)n * sizeof(double));
     double* product = (double*)malloc((size_t)n * sizeof(double));
     double* residual = (double*)malloc((size_t)n * sizeof(double));
     double* preconditioned = precondition ? (double*)malloc((size_t)n * sizeof(double)) : NULL;
-    double* inverse = precondition ? (double*)malloc((size_t)n * sizeof(double)) : NULL;
     double* buffer = precondition ? (double*)malloc((size_t)n * sizeof(double)) : NULL;
 
     oracle_statistics_t s = {0, DBL_MAX, DBL_MAX};
-
-    if (precondition) {
-        int64_t failed = -1;
-        status = oracle_inverse_diagonal(rows, rowptr, cols, vals, inverse, &failed);
-        if (status) goto done;
-    }
 
     vec(pool, OP_ASSIGN, solution, x, 0.0, n);                                   /* :74-75 */
     if (maxIterations) {
         vec(pool, OP_FILL, product, NULL, 0.0, n);                               /* :82 */
         vec(pool, OP_ASSIGN, residual, b, 0.0, n);                               /* :86-88 */
-        spmv(pool, rows, rowptr, cols, vals, 1.0, solution, -1.0, residual);
+        apply_csr(pool, A, 1.0, solution, -1.0, residual);
 
         double residualNorm = dot(pool, residual, residual, n);                 /* :93 */
         double preconditionedNorm = 0.0;
@@ -422,7 +474,7 @@ int oracle_pcg(int64_t rows, const int32_t* rowptr, const int32_t* cols, const d
         if (!(0.0 < preconditionedNorm)) { status = 4; goto done; }
 
         for (; s.iterationCount < maxIterations; ++s.iterationCount) {           /* :134-186 */
-            spmv(pool, rows, rowptr, cols, vals, 0.0, search, 1.0, product);
+            apply_csr(pool, A, 0.0, search, 1.0, product);
             double searchScale = preconditionedNorm;
             const double denominator = dot(pool, search, product, n);
             if (!(0.0 < denominator)) { status = 3; goto done; }
@@ -466,7 +518,32 @@ finish:
 done:
     if (stats) *stats = s;
     free(solution); free(search); free(product); free(residual);
-    free(preconditioned); free(inverse); free(buffer);
+    free(preconditioned); free(buffer);
+    return status;
+}
+
+int oracle_pcg(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+               double inScale, const double* b, double outScale, double* x,
+               uint64_t maxIterations, double absoluteTolerance, double relativeTolerance,
+               int precondition, int threads, oracle_statistics_t* stats,
+               double* history, int64_t historyCapacity, double* seconds) {
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    pool_t* pool = pool_create(resolve_threads(threads));
+    int status = 0;
+    double* inverse = precondition ? (double*)malloc((size_t)rows * sizeof(double)) : NULL;
+    if (precondition) {
+        int64_t failed = -1;
+        status = oracle_inverse_diagonal(rows, rowptr, cols, vals, inverse, &failed);
+    }
+    if (status) {
+        if (stats) { stats->iterationCount = 0; stats->absoluteResidual = DBL_MAX; stats->relativeResidual = DBL_MAX; }
+    } else {
+        const csr_operator_t A = {rows, rowptr, cols, vals, NULL, 0};
+        status = pcg_core(pool, &A, inverse, inScale, b, outScale, x, maxIterations, absoluteTolerance,
+                          relativeTolerance, stats, history, historyCapacity);
+    }
+    free(inverse);
     pool_destroy(pool);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     if (seconds) *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
@@ -769,4 +846,181 @@ int oracle_reorder_vector(int64_t n, const int32_t* map, int reverse, const doub
         else out[i] = in[map[i]];
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Masked operators, Jacobi smoothers and the p-multigrid cycle             */
+/* ------------------------------------------------------------------------ */
+
+int oracle_masked_csr_product(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                              const uint16_t* mask, uint16_t threshold, double inScale, const double* in,
+                              double outScale, double* out, int threads) {
+    pool_t* pool = pool_create(resolve_threads(threads));
+    const csr_operator_t A = {rows, rowptr, cols, vals, mask, threshold};
+    apply_csr(pool, &A, inScale, in, outScale, out);
+    pool_destroy(pool);
+    return 0;
+}
+
+int oracle_masked_identity(int64_t n, const uint16_t* mask, uint16_t threshold, double inScale, const double* in,
+                           double outScale, double* out) {
+    for (int64_t i = 0; i < n; ++i) apply_op(inScale, outScale, out + i, mask[i] < threshold ? in[i] : 0.0);
+    return 0;
+}
+
+/* JacobiOperator::product over A with the inverse diagonal `inverse`; work holds 4 n doubles. */
+static void jacobi_core(pool_t* pool, const csr_operator_t* A, const double* inverse, uint64_t iterations,
+                        double relaxation, double inScale, const double* in, double outScale, double* out,
+                        double* work) {
+    const int64_t n = A->rows;
+    double *memory = work, *residual = work + n, *buffer = work + 2 * n;
+    vec(pool, OP_ASSIGN, memory, out, 0.0, n);                      /* JacobiOperator.cpp:48-49 */
+    vec(pool, OP_FILL, out, NULL, 0.0, n);
+    for (uint64_t k = 0; k < iterations; ++k) {                      /* :52-57 */
+        vec(pool, OP_ASSIGN, residual, in, 0.0, n);
+        apply_csr(pool, A, 1.0, out, -1.0, residual);
+        diagonal_product(pool, inverse, 1.0, residual, relaxation, out, buffer, n);
+    }
+    vec(pool, OP_SCALE_UNIFORM, out, NULL, outScale, n);             /* :59-60 */
+    vec(pool, OP_ADD, out, memory, inScale, n);
+}
+
+int oracle_jacobi(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                  const uint16_t* mask, uint16_t threshold, uint64_t iterations, double relaxation, double inScale,
+                  const double* in, double outScale, double* out, int threads) {
+    double* inverse = (double*)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+    int64_t failed = -1;
+    int status = oracle_inverse_diagonal(rows, rowptr, cols, vals, inverse, &failed);
+    if (!status) {
+        pool_t* pool = pool_create(resolve_threads(threads));
+        double* work = (double*)malloc(sizeof(double) * 4 * (size_t)(rows > 0 ? rows : 1));
+        const csr_operator_t A = {rows, rowptr, cols, vals, mask, threshold};
+        jacobi_core(pool, &A, inverse, iterations, relaxation, inScale, in, outScale, out, work);
+        free(work);
+        pool_destroy(pool);
+    }
+    free(inverse);
+    return status;
+}
+
+/* One sweep of MaskedJacobiOperator (masked = 1) or ApproximateSubstitution
+ * (masked = 0) from `previous` into out; 6 if a kept row has no diagonal. */
+static int substitution_sweep(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                              const uint16_t* mask, uint16_t threshold, int masked, double relaxation,
+                              double inScale, const double* in, double outScale, double* out,
+                              const double* previous) {
+    for (int64_t row = 0; row < rows; ++row) {
+        if (masked && !(mask[row] < threshold)) {
+            apply_op(inScale, outScale, out + row, 0.0);             /* MaskedJacobiOperator.cpp:89 */
+            continue;
+        }
+        double contribution = 0.0, diagonal = 0.0;
+        int found = 0;
+        for (int32_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+            const int32_t column = cols[k];
+            if (column != row) {
+                if (!masked || mask[column] < threshold) contribution += vals[k] * previous[column];
+            } else {
+                diagonal = vals[k];
+                found = 1;
+            }
+        }
+        if (!found) return 6;  /* std::optional::value() of an empty diagonal throws */
+        const double value = masked
+            ? relaxation * (in[row] - contribution) / diagonal + (1.0 - relaxation) * previous[row]  /* :87 */
+            : previous[row] + (in[row] - contribution) / diagonal;               /* ApproximateSubstitution.cpp:57 */
+        apply_op(inScale, outScale, out + row, value);
+    }
+    return 0;
+}
+
+static int substitution(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                        const uint16_t* mask, uint16_t threshold, int masked, uint64_t iterations,
+                        double relaxation, double inScale, const double* in, double outScale, double* out) {
+    if (inScale != 0.0 && iterations != 1) return 10;                 /* NotImplementedException, :64-65 / :40-41 */
+    double* previous = (double*)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+    int status = 0;
+    for (uint64_t k = 0; k < iterations && !status; ++k) {
+        memcpy(previous, out, sizeof(double) * (size_t)rows);         /* previous = out before each sweep */
+        status = substitution_sweep(rows, rowptr, cols, vals, mask, threshold, masked, relaxation, inScale, in,
+                                    outScale, out, previous);
+    }
+    free(previous);
+    return status;
+}
+
+int oracle_masked_jacobi(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                         const uint16_t* mask, uint16_t threshold, uint64_t iterations, double relaxation,
+                         double inScale, const double* in, double outScale, double* out) {
+    return substitution(rows, rowptr, cols, vals, mask, threshold, 1, iterations, relaxation, inScale, in, outScale,
+                        out);
+}
+
+int oracle_approximate_substitution(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals,
+                                    uint64_t iterations, double inScale, const double* in, double outScale,
+                                    double* out) {
+    return substitution(rows, rowptr, cols, vals, NULL, 0, 0, iterations, 1.0, inScale, in, outScale, out);
+}
+
+/* solveMultigrid (poisson2D/solver.cpp:111-232). x accumulates the updates;
+ * history[c] is ||r|| after cycle c and coarseHistory[c] the iteration count
+ * of its coarse CG; *cycles the cycle count. */
+int oracle_pmg(int64_t rows, const int32_t* rowptr, const int32_t* cols, const double* vals, const uint16_t* mask,
+               int order, const double* b, double* x, double relativeTolerance, int64_t maxCycles, double* history,
+               uint64_t* coarseHistory, int64_t historyCapacity, int64_t* cycles, uint64_t* coarseIterations,
+               int threads) {
+    if (order < 1) return 1;
+    pool_t* pool = pool_create(resolve_threads(threads));
+    const int64_t n = rows;
+    double* inverse = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* residual = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* gridResidual = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* update = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* buffer = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* work = (double*)malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+    int64_t failed = -1;
+    int status = oracle_inverse_diagonal(rows, rowptr, cols, vals, inverse, &failed);
+    int64_t count = 0;
+    uint64_t coarse = 0;
+    if (!status) {
+        vec(pool, OP_ASSIGN, residual, b, 0.0, n);                                     /* :135-136 */
+        const double initial = sqrt(dot(pool, residual, residual, n));
+        double norm = initial;
+        const int grids = order;  /* grid 0: mask < 2 (CG); grid g >= 1: mask < g + 2 (Jacobi) */
+        while (relativeTolerance * initial < norm && count < maxCycles && !status) {    /* :212 */
+            for (int pass = 0; pass < 2 && !status; ++pass) {
+                const int first = pass == 0 ? grids - 1 : 1, last = pass == 0 ? 0 : grids - 1;
+                const int step = pass == 0 ? -1 : 1;
+                for (int g = first; pass == 0 ? g >= last : g <= last; g += step) {
+                    const uint16_t threshold = (uint16_t)(g + 2);
+                    const csr_operator_t A = {rows, rowptr, cols, vals, mask, threshold};
+                    oracle_masked_identity(n, mask, threshold, 0.0, residual, 1.0, gridResidual);  /* restriction */
+                    vec(pool, OP_FILL, update, NULL, 0.0, n);
+                    if (g == 0) {
+                        oracle_statistics_t stats;
+                        status = pcg_core(pool, &A, inverse, 0.0, gridResidual, 1.0, update, 5000, 0.0, 0.5, &stats,
+                                          NULL, 0);                                  /* :148-171 */
+                        coarse += stats.iterationCount;
+                        if (coarseHistory && count < historyCapacity) coarseHistory[count] = stats.iterationCount;
+                        if (status) break;
+                    } else {
+                        /* NestedProductOperator(smoother, mask): buffer = mask(gridResidual), then Jacobi */
+                        vec(pool, OP_FILL, buffer, NULL, 0.0, n);
+                        oracle_masked_identity(n, mask, threshold, 0.0, gridResidual, 1.0, buffer);
+                        jacobi_core(pool, &A, inverse, 6, 2.0 / 3.0, 0.0, buffer, 1.0, update, work);
+                    }
+                    vec(pool, OP_ADD, x, update, 1.0, n);                                  /* :217, :225 */
+                    apply_csr(pool, &A, 1.0, update, -1.0, residual);                      /* :218, :226 */
+                }
+            }
+            norm = sqrt(dot(pool, residual, residual, n));                                 /* :229 */
+            if (history && count < historyCapacity) history[count] = norm;
+            ++count;
+        }
+    }
+    if (cycles) *cycles = count;
+    if (coarseIterations) *coarseIterations = coarse;
+    free(inverse); free(residual); free(gridResidual); free(update); free(buffer); free(work);
+    pool_destroy(pool);
+    return status;
 }

@@ -139,3 +139,83 @@ def test_batch_workload_layout():
     for lo, hi in zip(offsets[:-1], offsets[1:]):  # block diagonal: nothing outside the diagonal blocks
         assert a[lo:hi, :lo].nnz == 0 and a[lo:hi, hi:].nnz == 0
     assert w.matrix.nnz == sum((8 * n - 9) ** 2 for n in sizes)
+
+
+def _legendre_1d(order, t):
+    """Integrated Legendre basis in position order at t in [0, 1] (x = 2t - 1), evaluated with scipy."""
+    from scipy.special import eval_legendre
+
+    x = 2.0 * np.asarray(t) - 1.0
+    out = np.zeros((order + 1,) + x.shape)
+    out[0] = 0.5 * (1.0 - x)
+    out[order] = 0.5 * (1.0 + x)
+    for pos in range(1, order):
+        k = pos + 1
+        out[pos] = (eval_legendre(k, x) - eval_legendre(k - 2, x)) / (2 * k - 1)
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_legendre_1d_stiffness_closed_form(order):
+    # 1D factor of the 2D element matrix on one element (h = 1): K = K1 (x) M1 + M1 (x) K1. With the
+    # vertex block [[1,-1],[-1,1]] and bubbles 4/(2k-1) on the diagonal K1 is known exactly; M1 from scipy.
+    from scipy.special import roots_legendre
+
+    points, weights = roots_legendre(order + 2)
+    t, w = 0.5 * (points + 1.0), 0.5 * weights
+    values = _legendre_1d(order, t)
+    m1 = (values * w) @ values.T
+    k1 = np.zeros((order + 1, order + 1))
+    k1[np.ix_([0, order], [0, order])] = [[1.0, -1.0], [-1.0, 1.0]]
+    for pos in range(1, order):
+        k1[pos, pos] = 4.0 / (2 * (pos + 1) - 1)
+    expected = np.kron(m1, k1) + np.kron(k1, m1)  # local index x fastest
+    got = systems.element_matrix(2, order, 2, basis="legendre")  # in 2D the element matrix does not depend on h
+    assert np.allclose(got, expected, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 2, 3), (2, 3, 2), (2, 4, 2), (3, 2, 2), (3, 3, 2)])
+def test_legendre_system_is_the_lagrange_system_in_another_basis(dim, order, n):
+    # Both bases span Q_p with u = 0 on the boundary, so K_leg = T^T K_lag T with T[i, j] the value of
+    # Legendre function j at Lagrange node i (interior positions only).
+    positions = order * n + 1
+    lagrange_points = np.concatenate([[0.0]] + [
+        (e + 0.5 * (1.0 - np.cos(np.arange(1, order + 1) * np.pi / order))) / n for e in range(n)])
+    t1 = np.zeros((positions, positions))
+    for e in range(n):
+        local = np.arange(order + 1)
+        for i in range(order * e, order * e + order + 1):
+            t = lagrange_points[i] * n - e
+            t1[i, order * e + local] = _legendre_1d(order, np.array([t]))[:, 0]
+    interior = t1[1:-1, 1:-1]
+    t_full = interior
+    for _ in range(dim - 1):
+        t_full = np.kron(interior, t_full)  # x fastest
+    k_lag = systems.lagrange(dim, order, n).to_scipy().toarray()
+    k_leg = systems.lagrange(dim, order, n, basis="legendre").to_scipy().toarray()
+    assert np.allclose(k_leg, t_full.T @ k_lag @ t_full, rtol=1e-11, atol=1e-11 * np.abs(k_leg).max())
+    # same pattern as the Lagrange system, symmetric, positive definite
+    assert np.array_equal(systems.lagrange(dim, order, n, basis="legendre").row_extents,
+                          systems.lagrange(dim, order, n).row_extents)
+    assert np.allclose(k_leg, k_leg.T, rtol=0, atol=1e-14 * np.abs(k_leg).max())
+    assert np.linalg.eigvalsh(k_leg).min() > 0
+
+
+def test_ansatz_mask_degrees():
+    order, n = 4, 3
+    mask = systems.ansatz_mask(2, order, n)
+    positions = np.arange(1, order * n)  # interior positions along one dimension
+    degree = np.where(positions % order == 0, 1, positions % order + 1)
+    expected = np.maximum(degree[None, :], degree[:, None]).ravel()  # row = x + (n_pos) * y, x fastest
+    assert np.array_equal(mask, expected.astype(np.uint16))
+    assert np.count_nonzero(mask == 1) == (n - 1) ** 2  # interior vertex DOFs
+    assert mask.max() == order
+    # the mask is only defined for the hierarchical basis
+    from ctypes import byref
+
+    from cie_amd._lib import LagrangeSpec, check_sys, library
+
+    out = np.empty(mask.size, dtype=np.uint16)
+    with pytest.raises(RuntimeError):
+        check_sys(library().cie_sys_lagrange_ansatz_mask(byref(LagrangeSpec(2, order, n, 1, 0)),
+                                                         out.ctypes.data_as(__import__("ctypes").c_void_p)))
