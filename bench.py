@@ -14,7 +14,9 @@ of the dominant kernel (the FP32 SpMV q = A p inside the mixed solve, timed
 with HIP events on the solver stream over the timed steps) and the threaded
 CPU restatement of the reference CG on a bounded sample.
 
-Prints one JSON line.
+Prints one JSON line. --workload assembly | reorder | pmg measures the widened
+paths instead (device assembly of the benchmark stiffness, its RCM reordering,
+poisson2D's p-multigrid), each with its own roofline and CPU baseline.
 """
 import argparse
 import json
@@ -95,12 +97,18 @@ def main():
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
+    parser.add_argument("--workload", choices=("pcg", "assembly", "reorder", "pmg"), default="pcg",
+                        help="pcg: the headline solve; assembly / reorder / pmg: the widened paths, one JSON line each")
     args = parser.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the solve path is single-device; see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "pcg":
+        {"assembly": bench_assembly, "reorder": bench_reorder, "pmg": bench_pmg}[args.workload](args)
+        return
 
     import torch
 
@@ -237,6 +245,230 @@ def main():
         source = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
         torch.empty_like(source).copy_(source)
         torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# The widened paths: device assembly, RCM reordering, p-multigrid.
+# ---------------------------------------------------------------------------
+
+def _library_stream_events(torch, linalg, context):
+    """A timer of work on the library context's stream (HIP events recorded on that stream)."""
+    from ctypes import byref, c_void_p
+
+    pointer = c_void_p()
+    linalg.library().cie_hip_context_stream(context.handle, byref(pointer))
+    stream = torch.cuda.ExternalStream(pointer.value)
+
+    def timed(call, repetitions):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(repetitions):
+            call()
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop) * 1e-3 / repetitions
+
+    return timed
+
+
+def _kernel_seconds(linalg, context, call, repetitions):
+    """Mean main-kernel duration of `call` from the context's profiling events (HIP events on its stream)."""
+    from ctypes import byref, c_double
+
+    library = linalg.library()
+    library.cie_hip_context_set_profiling(context.handle, 1)
+    seconds = []
+    try:
+        for _ in range(repetitions):
+            call()
+            value = c_double()
+            library.cie_hip_context_kernel_seconds(context.handle, byref(value))
+            seconds.append(value.value)
+    finally:
+        library.cie_hip_context_set_profiling(context.handle, 0)
+    return statistics.mean(seconds)
+
+
+def _line(args, metric, value, unit, dtype, config, roofline, cpu, extra=None):
+    result = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (structured meshes, SplitMix64 inputs)", "config": config,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    result.update(extra or {})
+    print(json.dumps(result), flush=True)
+
+
+def _roofline(kernel, bytes_per_launch, seconds, note):
+    achieved = bytes_per_launch / seconds / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
+            "bytes_per_launch": bytes_per_launch, "mean_launch_seconds": seconds, "bytes": note}
+
+
+def bench_assembly(args):
+    """K = sum_e k_e P_e^T K_e0 P_e on the benchmark pattern (3D Q2 64^3, k_e = exp(N(0,1))), device assembly."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from cie_amd import linalg, systems
+
+    n = args.elements
+    count = systems.lagrange_size(3, 2, n)[2]
+    k = np.exp(systems.normal(2, 1.0, count))
+    pattern = systems.lagrange(3, 2, n, threads=0)
+    dofs = systems.element_dofs(3, 2, n)
+    element = systems.element_matrix(3, 2, n)
+    context = linalg.Context.default()
+    matrix = linalg.CSRMatrix(pattern, context)
+    assembly = linalg.Assembly(matrix, dofs)
+    element_dev, k_dev = torch.from_numpy(element).cuda(), torch.from_numpy(k).cuda()
+    timed = _library_stream_events(torch, linalg, context)
+    for _ in range(args.warmup):
+        assembly.assemble(element_dev, k_dev)
+    seconds = timed(lambda: assembly.assemble(element_dev, k_dev), args.steps)
+    kernel = _kernel_seconds(linalg, context, lambda: assembly.assemble(element_dev, k_dev), args.steps)
+    rows, nnz = pattern.rows, pattern.nnz
+    kept = np.count_nonzero(dofs >= 0, axis=1).astype(np.int64)
+    contributions = int(np.sum(kept * kept))  # one per (element, kept local row, kept local column)
+    algorithmic = 12 * nnz + 4 * (nnz + 1) + 6 * contributions + 8 * count
+    t0 = time.perf_counter()
+    oracle.assemble(pattern, dofs, element, k)
+    cpu = time.perf_counter() - t0
+    _line(args, "device stiffness assembly time (s), K = sum_e k_e P_e^T K_e0 P_e, FP64 + FP32 planes", seconds,
+          "s", "f64", {"workload": f"3D Q2 Laplace {n}^3 hexahedra ({count} elements), k_e = exp(N(0,1)) seed 2, "
+                       f"benchmark pattern", "rows": rows, "nnz": nnz, "contributions": contributions},
+          _roofline("assembleSlotsKernel<uint16_t> (one launch per assembly)", algorithmic, kernel,
+                    "12 nnz (FP64 + FP32 values written) + 4 (nnz + 1) (slot offsets) + 6 per contribution "
+                    "(element, local pair) + 8 E (k_e); K_e0 stays in cache"),
+          {"value": cpu, "unit": "s", "cores": 1, "kind": "port",
+           "sample": "oracle_assemble (the Serial addContribution scatter, Assembler_impl.hpp:156-194) over all "
+                     "elements, one thread"})
+
+
+def bench_reorder(args):
+    """RCM of the benchmark matrix numbered at random: host map, device CSR permutation, SpMV before/after."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from cie_amd import linalg, systems
+
+    n = args.elements
+    natural = systems.lagrange(3, 2, n, threads=0)
+    rows, nnz = natural.rows, natural.nnz
+    shuffle = np.random.default_rng(11).permutation(rows).astype(np.int32)
+    context = linalg.Context.default()
+    base = linalg.CSRMatrix(natural, context)
+    shuffled = base.reorder(shuffle)
+    host_shuffled = shuffled.copy_out()
+    t0 = time.perf_counter()
+    mapping = linalg.makeReordering(host_shuffled, "reverse-cuthill-mckee")
+    map_seconds = time.perf_counter() - t0
+    timed = _library_stream_events(torch, linalg, context)
+    results = []
+
+    def permute():
+        results.append(shuffled.reorder(mapping))
+
+    for _ in range(args.warmup):
+        permute()
+    seconds = timed(permute, args.steps)
+    kernel = _kernel_seconds(linalg, context, permute, args.steps)
+    reordered = results.pop()
+    for extra in results:
+        extra.close()
+    spmv = {name: m.autotune(64, repetitions=10) for name, m in
+            (("natural", base), ("random", shuffled), ("rcm", reordered))}
+    algorithmic = 2 * (4 + 8 + 4) * nnz + 2 * 4 * (rows + 1) + 3 * 4 * rows
+    t0 = time.perf_counter()
+    oracle.make_reordering(host_shuffled, "reverse-cuthill-mckee")
+    cpu_map = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.reorder_csr(host_shuffled, mapping)
+    cpu_permute = time.perf_counter() - t0
+    _line(args, "device CSR permutation time (s) by the reverse Cuthill-McKee map", seconds, "s", "int32 + f64",
+          {"workload": f"3D Q2 Laplace {n}^3 hexahedra, rows numbered at random (seed 11), reverse Cuthill-McKee",
+           "rows": rows, "nnz": nnz},
+          _roofline("permuteRowsKernel (one launch per permutation; the value above is the whole call, with its "
+                    "host preparation, allocations and map uploads)", algorithmic, kernel,
+                    "(4 + 8 + 4) nnz read + written (columns, FP64, FP32) + 2 x 4 (N + 1) extents + 3 x 4 N maps"),
+          {"value": cpu_permute, "unit": "s", "cores": 1, "kind": "port",
+           "sample": f"oracle_reorder_csr (reorder.cpp:215-365) of the whole matrix, one thread; the oracle's RCM "
+                     f"map took {cpu_map:.3f} s"},
+          {"rcm_map_host_seconds": map_seconds,
+           "spmv_fp64_seconds": spmv, "spmv_rcm_vs_random": spmv["random"] / spmv["rcm"]})
+
+
+def bench_pmg(args):
+    """solveMultigrid on 2D Q4 integrated Legendre, 512^2 elements, to 1e-6 of ||b||."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from cie_amd import linalg, systems
+    from cie_amd._lib import Statistics
+
+    order, n = 4, 512
+    csr = systems.lagrange(2, order, n, basis="legendre", threads=0)
+    mask_values = systems.ansatz_mask(2, order, n)
+    b = systems.uniform(1, csr.rows)
+    rows, nnz = csr.rows, csr.nnz
+    context = linalg.Context.default()
+    matrix = linalg.CSRMatrix(csr, context)
+    mask = linalg.Mask(mask_values, context)
+    t0 = time.perf_counter()
+    pmg = linalg.PMultigrid(matrix, mask, order)
+    setup = time.perf_counter() - t0
+    b_dev = torch.from_numpy(b).cuda()
+    x = torch.zeros_like(b_dev)
+    for _ in range(args.warmup):
+        x.zero_()
+        pmg.solve(b_dev, x)
+    infos = []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        x.zero_()
+        infos.append(pmg.solve(b_dev, x))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - start) / args.steps
+    info = infos[-1]
+    # Dominant kernel: a Jacobi sweep on the finest grid (the whole matrix), from 7-sweep minus 1-sweep runs.
+    space = linalg.HIPSpace()
+    jacobi = {k: linalg.JacobiOperator(space, rows, matrix, linalg.makeDiagonalOperator(matrix, space), k, 2.0 / 3.0)
+              for k in (1, 7)}
+    y = torch.zeros_like(b_dev)
+    timed = _library_stream_events(torch, linalg, context)
+    one = timed(lambda: jacobi[1].product(0, b_dev, 1, y), 5)
+    seven = timed(lambda: jacobi[7].product(0, b_dev, 1, y), 5)
+    sweep = (seven - one) / 6
+    index = 2 if matrix.column_bytes() == 2 else 4
+    algorithmic = spmv_bytes(rows, nnz, 8, index) + 2 * 8 * rows
+    solver = linalg.FusedPCG(matrix, Statistics(100000, 0.0, 1e-6))
+    xp = torch.zeros_like(b_dev)
+    solver.product(0, b_dev, 1, xp)
+    sample_cycles = 3
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    oracle.pmg(csr, mask_values, order, b, max_cycles=sample_cycles, threads=threads)
+    cpu_sample = time.perf_counter() - t0
+    _line(args, "p-multigrid time to ||r|| <= 1e-6 ||b|| (s), poisson2D's solveMultigrid", wall, "s", "f64",
+          {"workload": f"2D Q{order} integrated Legendre Laplace, {n}^2 elements, Dirichlet eliminated, "
+                       f"b ~ U(-1,1) seed 1, x0 = 0", "rows": rows, "nnz": nnz},
+          _roofline("jacobiSweepKernel<BlockTiles<double, double, 8, 256>> on the finest grid", algorithmic, sweep,
+                    "B_SpMV(FP64, format index bytes) + 2 x 8 N (right-hand side and inverse diagonal)"),
+          {"value": cpu_sample * info.cycles / sample_cycles, "unit": "s", "cores": threads, "kind": "port",
+           "sample": f"oracle_pmg (solver.cpp:111-232 restated) for {sample_cycles} cycles in {cpu_sample:.2f} s, "
+                     f"scaled to the {info.cycles} cycles of the solve"},
+          {"cycles": int(info.cycles), "coarse_cg_steps": int(info.coarseIterations),
+           "device_seconds": info.seconds, "setup_seconds": setup, "tracked_relative_residual":
+               info.residual / info.initialResidual,
+           "jacobi_pcg_to_1e-6": {"iterations": int(solver.info.stats.iterationCount),
+                                  "seconds": solver.info.seconds}})
 
 
 def measured_traffic(launch):

@@ -126,6 +126,7 @@ _SIGNATURES = {
     "cie_hip_context_synchronize": (c_int, [_P]),
     "cie_hip_context_stream": (c_int, [_P, POINTER(_P)]),
     "cie_hip_context_set_profiling": (c_int, [_P, c_int]),
+    "cie_hip_context_kernel_seconds": (c_int, [_P, POINTER(c_double)]),
     "cie_hip_malloc": (c_int, [_P, ctypes.c_size_t, POINTER(_P)]),
     "cie_hip_free": (c_int, [_P, _P]),
     "cie_hip_memcpy": (c_int, [_P, _P, _P, ctypes.c_size_t, c_int]),
@@ -179,6 +180,7 @@ _SIGNATURES = {
     "cie_hip_assembly_destroy": (c_int, [_P]),
     "cie_hip_assemble": (c_int, [_P, _P, c_int, _P]),
     "cie_hip_assemble_vector": (c_int, [_P, _P, c_int, _P, _P]),
+    "cie_hip_element_stiffness": (c_int, [_P, c_int64, c_int32, c_int32, _P, _P, _P, _P]),
     # reordering
     "cie_hip_make_reordering": (c_int, [c_int64, _P, _P, c_int, _P]),
     "cie_hip_csr_reorder": (c_int, [_P, _P, c_int, POINTER(_P)]),
@@ -220,6 +222,7 @@ _SIGNATURES = {
     "cie_sys_lagrange_element_dofs": (c_int, [POINTER(LagrangeSpec), _P]),
     "cie_sys_csr_slots": (c_int, [c_int64, _P, _P, c_int64, _P, _P, _P]),
     "cie_sys_lagrange_ansatz_mask": (c_int, [POINTER(LagrangeSpec), _P]),
+    "cie_sys_lagrange_gradient_moments": (c_int, [POINTER(LagrangeSpec), _P]),
 }
 
 _library = None

@@ -5,18 +5,20 @@
 // (Assembler::addContribution) for an element integrand linear in the cell
 // modulus (fem/packages/integrands/impl/LinearIsotropicStiffnessIntegrand_impl.hpp:74:
 // K_e = modulus * D^T D), restated as a gather so that it needs no atomics
-// and is deterministic: each row is owned by a group of lanes of one
-// wavefront, which visits the elements of the row node in increasing element
-// order and adds scale_e * K_e0[a][b] into the slot of every column node b
-// (one lane per column node). Every slot thus
-// receives its contributions in element order, starting from zero -- the
-// order of a sequential element loop -- so the values are bitwise those of
-// the structured generator and of a CPU scatter in element order.
+// and is deterministic: every CSR slot (r, c) is owned by one thread, which
+// sums the contributions of the elements holding both nodes in increasing
+// element order, starting from zero -- the order of a sequential element
+// loop -- so the values are bitwise those of the structured generator and of
+// a CPU scatter in element order.
 //
 // The plan (built once per pattern): the elements of each row node in
-// increasing element order with the node's local index, and the element
-// -> row map on the device; every (row, column) pair of an element is checked
-// to be in the CSR pattern when the plan is made.
+// increasing element order with the node's local index; every (row, column)
+// pair of an element is checked to be in the CSR pattern; and, for every CSR
+// slot, its contributions {element, local pair} in element order, found once
+// by walking the two nodes' element lists together (int32 slot offsets, int32
+// elements, uint16 local pairs up to 256 nodes per element). Each assembly is
+// then one streaming pass over tiles of consecutive slots (6 B per
+// contribution and 4 B per slot read, 12 B per slot written).
 #include "dispatch.hpp"
 #include "kernels.hpp"
 
@@ -55,53 +57,87 @@ __global__ __launch_bounds__(kAssemblyBlock) void checkPatternKernel(
     }
 }
 
-// A group of G lanes (G divides the wavefront) per row. The group zeroes the
-// row, then visits the row node's elements in increasing element order; for
-// one element its lanes take the element's column nodes b = lane, lane + G, ...
-// and add scale_e * K_e0[a][b] into the slot of d_e[b]. The nodes of one
-// element are distinct rows, so the lanes of one element write distinct slots;
-// a slot hit by two elements is hit in element order, the later add reading
-// the earlier one's store. Both live in one wavefront, whose vector memory
-// operations are performed in program order, and a wavefront-scope fence after
-// every element keeps the compiler from moving the loads and stores across it.
-template <int G>
-__global__ __launch_bounds__(kAssemblyBlock) void assembleRowsKernel(
-    int rows, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columns,
-    const int64_t* __restrict__ nodeBegin, const int2* __restrict__ nodeElements, const int32_t* __restrict__ dofs,
-    int nodes, const double* __restrict__ elementMatrices, int64_t elementStride, const double* __restrict__ scale,
-    double* entries64, float* __restrict__ entries32) {
-    static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "a row group lies inside one wavefront");
-    const int lane = int(threadIdx.x) % G;
-    const int row = int(blockIdx.x) * (kAssemblyBlock / G) + int(threadIdx.x) / G;
-    if (row >= rows) return;
-    const int begin = rowExtents[row], end = rowExtents[row + 1];
-    for (int k = begin + lane; k < end; k += G) entries64[k] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int64_t c = nodeBegin[row]; c < nodeBegin[row + 1]; ++c) {
-        const int2 entry = nodeElements[c];  // {element, local index of the row node}
-        const double factor = scale ? scale[entry.x] : 1.0;
-        const int32_t* d = dofs + int64_t(entry.x) * nodes;
-        const double* matrixRow = elementMatrices + int64_t(entry.x) * elementStride + int64_t(entry.y) * nodes;
-        for (int b = lane; b < nodes; b += G) {
-            const int column = d[b];
-            if (column < 0) continue;
-            const int slot = findSlot(columns, begin, end, column);  // present: checked with the plan
-            entries64[slot] += factor * matrixRow[b];
+// The contributions to CSR slot (r, c): the elements holding both nodes, in
+// increasing element order, and for each the local indices a of r and b of c
+// (a-major) -- the order in which a sequential scatter in element order
+// (local row, then local column) reaches the slot. Found by walking the
+// element lists of r and c together (both sorted by element, then by local
+// index). With `elements` null the walk only counts; else it writes the
+// element and the local pair a * nodes + b of each contribution.
+template <class TLocal>
+__device__ __forceinline__ int walkSlot(int row, int column, const int64_t* __restrict__ nodeBegin,
+                                        const int2* __restrict__ nodeElements, int nodes,
+                                        int32_t* __restrict__ elements, TLocal* __restrict__ locals) {
+    int64_t i = nodeBegin[row], j = nodeBegin[column];
+    const int64_t rowLast = nodeBegin[row + 1], columnLast = nodeBegin[column + 1];
+    int count = 0;
+    while (i < rowLast && j < columnLast) {
+        const int2 r = nodeElements[i], c = nodeElements[j];
+        if (r.x < c.x) {
+            ++i;
+        } else if (c.x < r.x) {
+            ++j;
+        } else {
+            int64_t j2 = j;
+            while (j2 < columnLast && nodeElements[j2].x == r.x) ++j2;
+            for (; i < rowLast && nodeElements[i].x == r.x; ++i)
+                for (int64_t jj = j; jj < j2; ++jj) {
+                    if (elements) {
+                        elements[count] = r.x;
+                        locals[count] = TLocal(nodeElements[i].y * nodes + nodeElements[jj].y);
+                    }
+                    ++count;
+                }
+            j = j2;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    if (entries32)
-        for (int k = begin + lane; k < end; k += G) entries32[k] = float(entries64[k]);
+    return count;
 }
 
-template <int G>
-void launchAssembly(cie_hip_csr m, const int64_t* nodeBegin, const int2* nodeElements, const int32_t* dofs, int nodes,
-                    const double* elementMatrices, int64_t elementStride, const double* scale, hipStream_t stream) {
-    constexpr int kRowsPerBlock = kAssemblyBlock / G;
-    const unsigned grid = unsigned((m->rows + kRowsPerBlock - 1) / kRowsPerBlock);
-    hipLaunchKernelGGL((assembleRowsKernel<G>), dim3(grid), dim3(kAssemblyBlock), 0, stream, int(m->rows),
-                       m->rowExtents, m->columnIndices, nodeBegin, nodeElements, dofs, nodes, elementMatrices,
-                       elementStride, scale, m->entries64, m->entries32);
+// counts[k] (elements null) or the contributions of every slot, one thread per row.
+template <class TLocal>
+__global__ __launch_bounds__(kAssemblyBlock) void slotPairsKernel(
+    int rows, const int32_t* __restrict__ rowExtents, const int32_t* __restrict__ columns,
+    const int64_t* __restrict__ nodeBegin, const int2* __restrict__ nodeElements, int nodes,
+    int32_t* __restrict__ counts, const int32_t* __restrict__ slotBegin, int32_t* __restrict__ elements,
+    TLocal* __restrict__ locals) {
+    const int row = int(blockIdx.x) * kAssemblyBlock + int(threadIdx.x);
+    if (row >= rows) return;
+    for (int k = rowExtents[row]; k < rowExtents[row + 1]; ++k) {
+        if (elements)
+            walkSlot(row, columns[k], nodeBegin, nodeElements, nodes, elements + slotBegin[k], locals + slotBegin[k]);
+        else
+            counts[k] = walkSlot<TLocal>(row, columns[k], nodeBegin, nodeElements, nodes, nullptr, nullptr);
+    }
+}
+
+constexpr int kSlotTile = 2048;  // contributions per tile (at most kAssemblyBlock slots)
+
+// One block per tile of consecutive slots: the lanes stream the tile's
+// contributions (coalesced), stage each product scale_e * K_e[a][b] in LDS,
+// and after one barrier the lane of each slot adds its products from zero in
+// their order; no two lanes share a slot.
+template <class TLocal>
+__global__ __launch_bounds__(kAssemblyBlock) void assembleSlotsKernel(
+    const int2* __restrict__ tiles, const int32_t* __restrict__ slotBegin, const int32_t* __restrict__ elements,
+    const TLocal* __restrict__ locals, const double* __restrict__ elementMatrices, int64_t elementStride,
+    const double* __restrict__ scale, double* __restrict__ entries64, float* __restrict__ entries32) {
+    __shared__ double stage[kSlotTile];
+    const int2 tile = tiles[blockIdx.x], next = tiles[blockIdx.x + 1];  // {first slot, first contribution}
+    for (int p = tile.y + int(threadIdx.x); p < next.y; p += kAssemblyBlock) {
+        const int element = elements[p];
+        const double factor = scale ? scale[element] : 1.0;
+        stage[p - tile.y] = factor * elementMatrices[int64_t(element) * elementStride + int64_t(locals[p])];
+    }
+    __syncthreads();
+    const int slot = tile.x + int(threadIdx.x);
+    if (slot < next.x) {
+        double sum = 0.0;
+        const int last = slotBegin[slot + 1];
+        for (int p = slotBegin[slot]; p < last; ++p) sum += stage[p - tile.y];
+        entries64[slot] = sum;
+        if (entries32) entries32[slot] = float(sum);
+    }
 }
 
 // One thread per row: out[row] = sum over the row node's elements, in
@@ -121,6 +157,71 @@ __global__ __launch_bounds__(kAssemblyBlock) void assembleVectorKernel(
     out[row] = sum;
 }
 
+// ---- element stiffness of affine cells ---------------------------------------
+
+constexpr int kStiffnessBlock = 256;
+constexpr int kStiffnessEntries = 4096;  // entries of the block's elements per pass
+constexpr int kStiffnessElements = 256;  // at most this many elements per block
+
+// |det J| k_e and G = J^-1 J^-T of one affine cell x = x0 + J xi (J row-major,
+// dim 2 or 3), by cofactors: inverse[i][j] = cofactor[j][i] / det.
+__device__ __forceinline__ void cellGeometry(int dim, const double* J, double modulus, double* G, double* factor) {
+    double inverse[9];
+    double det;
+    if (dim == 2) {
+        det = J[0] * J[3] - J[1] * J[2];
+        inverse[0] = J[3] / det;
+        inverse[1] = -J[1] / det;
+        inverse[2] = -J[2] / det;
+        inverse[3] = J[0] / det;
+    } else {
+        const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+        const double c10 = J[2] * J[7] - J[1] * J[8], c11 = J[0] * J[8] - J[2] * J[6], c12 = J[1] * J[6] - J[0] * J[7];
+        const double c20 = J[1] * J[5] - J[2] * J[4], c21 = J[2] * J[3] - J[0] * J[5], c22 = J[0] * J[4] - J[1] * J[3];
+        det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+        inverse[0] = c00 / det; inverse[1] = c10 / det; inverse[2] = c20 / det;
+        inverse[3] = c01 / det; inverse[4] = c11 / det; inverse[5] = c21 / det;
+        inverse[6] = c02 / det; inverse[7] = c12 / det; inverse[8] = c22 / det;
+    }
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) {
+            double g = 0.0;
+            for (int k = 0; k < dim; ++k) g += inverse[i * dim + k] * inverse[j * dim + k];
+            G[i * dim + j] = g;
+        }
+    *factor = fabs(det) * modulus;
+}
+
+// out[e][a][b] = |det J_e| k_e sum_ij G_ij moments[i][j][a][b] for a run of
+// consecutive elements per block: the block's elements get their geometry in
+// LDS first, then every lane writes entries of those elements in order, so
+// the stores are contiguous and the moments (the same for every element) are
+// read in the same order by every element.
+__global__ __launch_bounds__(kStiffnessBlock) void elementStiffnessKernel(
+    int64_t elements, int dim, int entries, int perBlock, const double* __restrict__ moments,
+    const double* __restrict__ jacobians, const double* __restrict__ modulus, double* __restrict__ out) {
+    __shared__ double geometry[kStiffnessElements][10];  // G (up to 9) and the factor
+    for (int64_t first = int64_t(blockIdx.x) * perBlock; first < elements; first += int64_t(gridDim.x) * perBlock) {
+        const int count = int(min<int64_t>(perBlock, elements - first));
+        __syncthreads();  // the previous run's geometry is no longer read
+        if (int(threadIdx.x) < count) {
+            const int64_t e = first + threadIdx.x;
+            cellGeometry(dim, jacobians + e * dim * dim, modulus ? modulus[e] : 1.0, geometry[threadIdx.x],
+                         &geometry[threadIdx.x][9]);
+        }
+        __syncthreads();
+        const int total = count * entries;
+        const int planes = dim * dim;
+        for (int t = int(threadIdx.x); t < total; t += kStiffnessBlock) {
+            const int local = t / entries, entry = t - local * entries;
+            const double* G = geometry[local];
+            double v = 0.0;
+            for (int ij = 0; ij < planes; ++ij) v += G[ij] * moments[int64_t(ij) * entries + entry];
+            out[(first + local) * entries + entry] = geometry[local][9] * v;
+        }
+    }
+}
+
 }  // namespace
 
 struct cie_hip_assembly_s {
@@ -130,6 +231,12 @@ struct cie_hip_assembly_s {
     int64_t* nodeBegin = nullptr;   // rows + 1
     int2* nodeElements = nullptr;   // {element, local index}, per row in element order
     int32_t* dofs = nullptr;        // elements x nodes
+    int32_t* slotBegin = nullptr;   // nnz + 1: each slot's contributions, in element order
+    int32_t* pairElements = nullptr;  // element of each contribution
+    void* pairLocals = nullptr;     // a * nodes + b: uint16 if nodes^2 <= 65536, else int32
+    bool wideLocals = false;
+    int tileCount = 0;
+    int2* tiles = nullptr;          // tileCount + 1: {first slot, first contribution}
 };
 
 namespace {
@@ -139,6 +246,10 @@ void releasePlan(cie_hip_assembly plan) {
     if (plan->nodeBegin) (void)hipFree(plan->nodeBegin);
     if (plan->nodeElements) (void)hipFree(plan->nodeElements);
     if (plan->dofs) (void)hipFree(plan->dofs);
+    if (plan->slotBegin) (void)hipFree(plan->slotBegin);
+    if (plan->pairElements) (void)hipFree(plan->pairElements);
+    if (plan->pairLocals) (void)hipFree(plan->pairLocals);
+    if (plan->tiles) (void)hipFree(plan->tiles);
     delete plan;
 }
 
@@ -150,7 +261,7 @@ int cie_hip_assembly_create(cie_hip_csr matrix, int64_t elements, int32_t nodesP
                             cie_hip_assembly* plan) {
     if (!plan) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_assembly_create: null output");
     *plan = nullptr;
-    if (!matrix || elements < 0 || nodesPerElement < 1 || (elements && !dofs))
+    if (!matrix || elements < 0 || nodesPerElement < 1 || nodesPerElement > 46340 || (elements && !dofs))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_assembly_create: invalid arguments");
     if (matrix->rows != matrix->columns)
         return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_assembly_create: the stiffness matrix must be square");
@@ -205,6 +316,70 @@ int cie_hip_assembly_create(cie_hip_csr matrix, int64_t elements, int32_t nodesP
          hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (!ok) return bail(CIE_HIP_ERROR_HIP, "cie_hip_assembly_create: upload or pattern check failed");
     if (*pinned) return bail(CIE_HIP_ERROR_SHAPE, "cie_hip_assembly_create: an element couples rows the CSR pattern lacks");
+    // The contribution lists of every slot: count, prefix on the host, fill.
+    const int64_t nnz = matrix->nnz;
+    p->wideLocals = int64_t(nodesPerElement) * nodesPerElement > 65536;
+    int32_t* counts = nullptr;
+    if (hipMalloc(&counts, sizeof(int32_t) * size_t(std::max<int64_t>(1, nnz))) != hipSuccess ||
+        hipMalloc(&p->slotBegin, sizeof(int32_t) * size_t(nnz + 1)) != hipSuccess) {
+        if (counts) (void)hipFree(counts);
+        return bail(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_assembly_create: slot list allocation failed");
+    }
+    const unsigned rowBlocks = unsigned((rows + kAssemblyBlock - 1) / kAssemblyBlock);
+    auto walk = [&](auto local, int32_t* countOut, int32_t* elementsOut, void* localsOut) {
+        using TLocal = decltype(local);
+        hipLaunchKernelGGL((slotPairsKernel<TLocal>), dim3(rowBlocks), dim3(kAssemblyBlock), 0, ctx->stream, int(rows),
+                           matrix->rowExtents, matrix->columnIndices, p->nodeBegin, p->nodeElements, nodesPerElement,
+                           countOut, static_cast<const int32_t*>(p->slotBegin), elementsOut,
+                           static_cast<TLocal*>(localsOut));
+    };
+    std::vector<int32_t> hostBegin(static_cast<size_t>(nnz) + 1, 0);
+    if (rows) {
+        if (p->wideLocals) walk(int32_t(0), counts, nullptr, nullptr);
+        else walk(uint16_t(0), counts, nullptr, nullptr);
+        ok = hipGetLastError() == hipSuccess &&
+             (!nnz || hipMemcpyAsync(hostBegin.data() + 1, counts, sizeof(int32_t) * size_t(nnz),
+                                     hipMemcpyDeviceToHost, ctx->stream) == hipSuccess) &&
+             hipStreamSynchronize(ctx->stream) == hipSuccess;
+    }
+    (void)hipFree(counts);
+    if (!ok) return bail(CIE_HIP_ERROR_HIP, "cie_hip_assembly_create: counting slot contributions failed");
+    int64_t pairCount = 0;
+    for (int64_t k = 0; k < nnz; ++k) {
+        pairCount += hostBegin[size_t(k) + 1];
+        if (pairCount > INT32_MAX)
+            return bail(CIE_HIP_ERROR_SHAPE, "cie_hip_assembly_create: more than 2^31 element contributions");
+        hostBegin[size_t(k) + 1] = int32_t(pairCount);
+    }
+    const size_t localBytes = p->wideLocals ? sizeof(int32_t) : sizeof(uint16_t);
+    ok = hipMalloc(&p->pairElements, sizeof(int32_t) * size_t(std::max<int64_t>(1, pairCount))) == hipSuccess &&
+         hipMalloc(&p->pairLocals, localBytes * size_t(std::max<int64_t>(1, pairCount))) == hipSuccess &&
+         hipMemcpyAsync(p->slotBegin, hostBegin.data(), sizeof(int32_t) * size_t(nnz + 1), hipMemcpyHostToDevice,
+                        ctx->stream) == hipSuccess;
+    if (ok && rows) {
+        if (p->wideLocals) walk(int32_t(0), nullptr, p->pairElements, p->pairLocals);
+        else walk(uint16_t(0), nullptr, p->pairElements, p->pairLocals);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) return bail(CIE_HIP_ERROR_HIP, "cie_hip_assembly_create: writing slot contributions failed");
+    // Tiles of whole consecutive slots: at most kSlotTile contributions and kAssemblyBlock slots.
+    std::vector<int2> tiles;
+    for (int64_t slot = 0; slot < nnz;) {
+        tiles.push_back(make_int2(int(slot), hostBegin[size_t(slot)]));
+        int64_t last = slot;
+        while (last < nnz && last - slot < kAssemblyBlock &&
+               int64_t(hostBegin[size_t(last) + 1]) - hostBegin[size_t(slot)] <= kSlotTile)
+            ++last;
+        if (last == slot)
+            return bail(CIE_HIP_ERROR_SHAPE, "cie_hip_assembly_create: a slot has more than 2048 contributions");
+        slot = last;
+    }
+    tiles.push_back(make_int2(int(nnz), hostBegin[size_t(nnz)]));
+    p->tileCount = int(tiles.size()) - 1;
+    if (hipMalloc(&p->tiles, sizeof(int2) * tiles.size()) != hipSuccess ||
+        hipMemcpy(p->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_assembly_create: tile upload failed");
     *plan = p;
     return CIE_HIP_OK;
 }
@@ -221,25 +396,54 @@ int cie_hip_assemble(cie_hip_assembly plan, const double* elementMatrices, int p
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_assemble: the matrix must own its FP64 entries");
     cie_hip_context ctx = m->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
-    if (m->rows) {
-        // The smallest group that covers one element's nodes, at most a wavefront.
-        const int n = plan->nodes;
-        const int64_t stride = perElement ? int64_t(n) * n : 0;
-        auto launch = [&](auto group) {
-            launchAssembly<decltype(group)::value>(m, plan->nodeBegin, plan->nodeElements, plan->dofs, n,
-                                                   elementMatrices, stride, scale, ctx->stream);
-        };
-        if (n <= 4) launch(std::integral_constant<int, 4>{});
-        else if (n <= 8) launch(std::integral_constant<int, 8>{});
-        else if (n <= 16) launch(std::integral_constant<int, 16>{});
-        else if (n <= 32) launch(std::integral_constant<int, 32>{});
-        else launch(std::integral_constant<int, 64>{});
+    if (m->nnz && plan->tileCount) {
+        const int64_t stride = perElement ? int64_t(plan->nodes) * plan->nodes : 0;
+        const dim3 grid(unsigned(plan->tileCount)), block(kAssemblyBlock);
+        const KernelTimer timer(ctx);
+        if (plan->wideLocals)
+            hipLaunchKernelGGL((assembleSlotsKernel<int32_t>), grid, block, 0, ctx->stream,
+                               static_cast<const int2*>(plan->tiles), static_cast<const int32_t*>(plan->slotBegin),
+                               static_cast<const int32_t*>(plan->pairElements),
+                               static_cast<const int32_t*>(plan->pairLocals), elementMatrices, stride, scale,
+                               m->entries64, m->entries32);
+        else
+            hipLaunchKernelGGL((assembleSlotsKernel<uint16_t>), grid, block, 0, ctx->stream,
+                               static_cast<const int2*>(plan->tiles), static_cast<const int32_t*>(plan->slotBegin),
+                               static_cast<const int32_t*>(plan->pairElements),
+                               static_cast<const uint16_t*>(plan->pairLocals), elementMatrices, stride, scale,
+                               m->entries64, m->entries32);
+        timer.stop();
+        CIE_HIP_CHECK(hipGetLastError());
+        timer.finish();
         CIE_HIP_CHECK(hipGetLastError());
     }
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     // The Jacobi diagonal belongs to the old values.
     if (m->inverseDiagonal64) { CIE_HIP_CHECK(hipFree(m->inverseDiagonal64)); m->inverseDiagonal64 = nullptr; }
     if (m->inverseDiagonal32) { CIE_HIP_CHECK(hipFree(m->inverseDiagonal32)); m->inverseDiagonal32 = nullptr; }
+    return CIE_HIP_OK;
+}
+
+int cie_hip_element_stiffness(cie_hip_context context, int64_t elements, int32_t dim, int32_t nodesPerElement,
+                              const double* moments, const double* jacobians, const double* modulus, double* out) {
+    if (!context || elements < 0 || (dim != 2 && dim != 3) || nodesPerElement < 1 ||
+        (elements && (!moments || !jacobians || !out)))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_element_stiffness: invalid arguments");
+    const int64_t entries = int64_t(nodesPerElement) * nodesPerElement;
+    if (entries > INT32_MAX / kStiffnessBlock)
+        return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_element_stiffness: element matrices too large");
+    if (!elements) return CIE_HIP_OK;
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    const int perBlock = int(std::max<int64_t>(1, std::min<int64_t>(kStiffnessElements, kStiffnessEntries / entries)));
+    const int64_t blocks = (elements + perBlock - 1) / perBlock;
+    const unsigned grid = unsigned(std::min<int64_t>(blocks, int64_t(context->computeUnits) * 64));
+    const KernelTimer timer(context);
+    hipLaunchKernelGGL(elementStiffnessKernel, dim3(std::max(1u, grid)), dim3(kStiffnessBlock), 0, context->stream,
+                       elements, int(dim), int(entries), perBlock, moments, jacobians, modulus, out);
+    timer.stop();
+    CIE_HIP_CHECK(hipGetLastError());
+    timer.finish();
+    CIE_HIP_CHECK(hipStreamSynchronize(context->stream));
     return CIE_HIP_OK;
 }
 

@@ -84,7 +84,29 @@ struct cie_hip_context_s {
     bool profiling = false;
     static constexpr int kProfileEvents = 64;
     hipEvent_t profileEvents[2 * kProfileEvents] = {};
+    double kernelSeconds = 0.0;  // profiling on: the last call's main kernel (cie_hip_context_kernel_seconds)
 };
+
+namespace ciehip {
+// With the context's profiling on, brackets one kernel launch on its stream
+// with HIP events; finish() waits for the end event and keeps the duration.
+struct KernelTimer {
+    cie_hip_context context;
+    explicit KernelTimer(cie_hip_context c) : context(c) {
+        if (context->profiling) (void)hipEventRecord(context->profileEvents[0], context->stream);
+    }
+    void stop() const {
+        if (context->profiling) (void)hipEventRecord(context->profileEvents[1], context->stream);
+    }
+    void finish() const {
+        if (!context->profiling) return;
+        float ms = 0.0f;
+        if (hipEventSynchronize(context->profileEvents[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, context->profileEvents[0], context->profileEvents[1]) == hipSuccess)
+            context->kernelSeconds = double(ms) * 1e-3;
+    }
+};
+}  // namespace ciehip
 
 // Launch configuration of a CSR SpMV for one value precision. The grid is
 // sized per launch from the kernel's occupancy (one resident round).

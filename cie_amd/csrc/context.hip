@@ -145,6 +145,12 @@ int cie_hip_context_set_profiling(cie_hip_context context, int enable) {
     return CIE_HIP_OK;
 }
 
+int cie_hip_context_kernel_seconds(cie_hip_context context, double* seconds) {
+    if (!context || !seconds) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_context_kernel_seconds: null argument");
+    *seconds = context->kernelSeconds;
+    return CIE_HIP_OK;
+}
+
 int cie_hip_malloc(cie_hip_context context, size_t bytes, void** pointer) {
     if (!context || !pointer) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_malloc: null argument");
     *pointer = nullptr;

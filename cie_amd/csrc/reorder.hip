@@ -12,11 +12,12 @@
 // map[new] = old.
 //
 // The matrix permutation (reorder.cpp:215-365) runs on the device: one
-// wavefront per row of the result gathers the old row, renumbers its columns
-// and places every entry at its rank among the row's new columns (rows of up
-// to kRankEntries entries are ranked out of LDS, longer ones out of L2). Both
-// value planes follow their entries, so the result is bitwise the reordered
-// input.
+// wavefront per row of the result gathers the old row and renumbers its
+// columns; a row of up to 128 entries is sorted by a bitonic network in
+// registers (two keys per lane), a longer one places every entry at its rank
+// among the row's new columns (counted out of LDS up to kRankEntries entries,
+// out of L2 beyond). Both value planes follow their entries, so the result is
+// bitwise the reordered input.
 #include "dispatch.hpp"
 
 #include <algorithm>
@@ -30,6 +31,30 @@ namespace {
 constexpr int kPermuteBlock = 256;
 constexpr int kPermuteWaves = kPermuteBlock / 64;
 constexpr int kRankEntries = 1024;
+
+// Sorts the 128 keys held two per lane (positions lane and lane + 64) of one
+// wavefront ascending: a bitonic network, partners across lanes exchanged by
+// __shfl_xor, the stride-64 step inside each lane.
+__device__ __forceinline__ void bitonicSort128(uint64_t& low, uint64_t& high, int lane) {
+    for (int k = 2; k <= 128; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == 64) {
+                // positions lane and lane + 64: ascending iff ((lane & k) == 0), here k == 128
+                const uint64_t a = low, b = high;
+                low = a < b ? a : b;
+                high = a < b ? b : a;
+            } else {
+                const uint64_t otherLow = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(low), j));
+                const uint64_t otherHigh = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(high), j));
+                const bool lower = (lane & j) == 0;  // this lane holds the smaller position of the pair
+                const bool ascendingLow = (lane & k) == 0, ascendingHigh = ((lane + 64) & k) == 0;
+                const bool keepMinLow = lower == ascendingLow, keepMinHigh = lower == ascendingHigh;
+                low = keepMinLow ? (low < otherLow ? low : otherLow) : (low < otherLow ? otherLow : low);
+                high = keepMinHigh ? (high < otherHigh ? high : otherHigh) : (high < otherHigh ? otherHigh : high);
+            }
+        }
+    }
+}
 
 __global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
     int rows, const int32_t* __restrict__ source, const int32_t* __restrict__ target,
@@ -45,6 +70,27 @@ __global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
     const int begin = oldExtents[old];
     const int length = oldExtents[old + 1] - begin;
     const int out = newExtents[row];
+    if (length <= 128) {  // uniform over the wavefront
+        // keys (new column, position in the old row): distinct, and equal columns keep their order
+        const uint64_t pad = ~uint64_t(0);
+        const int k0 = lane, k1 = lane + 64;
+        uint64_t low = k0 < length ? (uint64_t(uint32_t(target[oldColumns[begin + k0]])) << 32) | uint32_t(k0) : pad;
+        uint64_t high = k1 < length ? (uint64_t(uint32_t(target[oldColumns[begin + k1]])) << 32) | uint32_t(k1) : pad;
+        bitonicSort128(low, high, lane);
+        if (k0 < length) {
+            const int from = begin + int(uint32_t(low));
+            newColumns[out + k0] = int32_t(low >> 32);
+            newEntries64[out + k0] = oldEntries64[from];
+            if (newEntries32) newEntries32[out + k0] = oldEntries32[from];
+        }
+        if (k1 < length) {
+            const int from = begin + int(uint32_t(high));
+            newColumns[out + k1] = int32_t(high >> 32);
+            newEntries64[out + k1] = oldEntries64[from];
+            if (newEntries32) newEntries32[out + k1] = oldEntries32[from];
+        }
+        return;
+    }
     int32_t* lds = staged[wave];
     const bool inLds = length <= kRankEntries;  // uniform over the wavefront
     if (inLds) {
@@ -222,11 +268,14 @@ int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie
                     hipMemcpyAsync(dTarget, target.data(), sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice,
                                    ctx->stream) == hipSuccess));
     if (ok && rows) {
+        const KernelTimer timer(ctx);
         hipLaunchKernelGGL(permuteRowsKernel, dim3(unsigned((rows + kPermuteWaves - 1) / kPermuteWaves)),
                            dim3(kPermuteBlock), 0, ctx->stream, int(rows), dSource, dTarget, matrix->rowExtents,
                            matrix->columnIndices, matrix->entries64, matrix->entries32, dExtents, dColumns,
                            dEntries64, dEntries32);
+        timer.stop();
         ok = hipGetLastError() == hipSuccess;
+        if (ok) timer.finish();
     }
     ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
     releaseMaps();

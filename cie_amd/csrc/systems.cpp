@@ -470,6 +470,53 @@ int cie_sys_lagrange_element_dofs(const cie_sys_lagrange_spec* spec, int32_t* do
     return 0;
 }
 
+int cie_sys_lagrange_gradient_moments(const cie_sys_lagrange_spec* spec, double* moments) {
+    Lagrange mesh;
+    if (!mesh.init(spec)) return fail("cie_sys_lagrange_gradient_moments: invalid spec");
+    if (!moments) return fail("cie_sys_lagrange_gradient_moments: null output");
+    // 1D values and slopes on [-1, 1] at p + 1 Gauss-Legendre points (d/dxi = (d/dt) / 2 for t in [0, 1]).
+    const int n = mesh.order + 1, dim = mesh.dim, npe = mesh.npe;
+    std::vector<double> points, weights, value, slope;
+    gaussLegendre(n, points, weights);
+    const std::vector<double> nodes = elementNodes(mesh.order);
+    std::vector<double> v1(size_t(n) * n), d1(size_t(n) * n);  // [point][local position]
+    for (int q = 0; q < n; ++q) {
+        const double t = 0.5 * (points[size_t(q)] + 1.0);
+        if (mesh.basis == CIE_SYS_BASIS_LEGENDRE) integratedLegendre(mesh.order, t, value, slope);
+        else lagrange(nodes, t, value, slope);
+        for (int a = 0; a < n; ++a) {
+            v1[size_t(q) * n + a] = value[size_t(a)];
+            d1[size_t(q) * n + a] = 0.5 * slope[size_t(a)];
+        }
+    }
+    int pointCount = 1;
+    for (int d = 0; d < dim; ++d) pointCount *= n;
+    std::vector<double> gradient(size_t(dim) * npe), w(size_t(pointCount));
+    std::fill(moments, moments + size_t(dim) * dim * npe * npe, 0.0);
+    for (int q = 0; q < pointCount; ++q) {
+        const int qi[3] = {q % n, (q / n) % n, dim == 3 ? q / (n * n) : 0};
+        double weight = 1.0;
+        for (int d = 0; d < dim; ++d) weight *= weights[size_t(qi[d])];
+        for (int a = 0; a < npe; ++a) {
+            const int ai[3] = {a % n, (a / n) % n, dim == 3 ? a / (n * n) : 0};
+            for (int i = 0; i < dim; ++i) {
+                double g = 1.0;
+                for (int d = 0; d < dim; ++d)
+                    g *= (d == i ? d1 : v1)[size_t(qi[d]) * n + ai[d]];
+                gradient[size_t(i) * npe + a] = g;
+            }
+        }
+        for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j) {
+                double* block = moments + (size_t(i) * dim + j) * npe * npe;
+                for (int a = 0; a < npe; ++a)
+                    for (int b = 0; b < npe; ++b)
+                        block[size_t(a) * npe + b] += weight * gradient[size_t(i) * npe + a] * gradient[size_t(j) * npe + b];
+            }
+    }
+    return 0;
+}
+
 int cie_sys_lagrange_ansatz_mask(const cie_sys_lagrange_spec* spec, uint16_t* mask) {
     Lagrange mesh;
     if (!mesh.init(spec)) return fail("cie_sys_lagrange_ansatz_mask: invalid spec");

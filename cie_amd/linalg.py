@@ -673,6 +673,31 @@ def reorder(mapping, vector, out=None, reverse=False, context=None):
     return out
 
 
+def elementStiffness(moments, jacobians, modulus=None, out=None, context=None):
+    """K_e = |det J_e| k_e sum_ij (J_e^-1 J_e^-T)_ij moments[i, j] for affine cells (cie_hip_element_stiffness).
+
+    moments: (dim, dim, npe, npe) device tensor; jacobians: (elements, dim, dim); modulus: (elements,) or None.
+    Returns (elements, npe, npe)."""
+    torch = _torch()
+    context = context or Context.default()
+    dim, npe = moments.shape[0], moments.shape[2]
+    elements = jacobians.shape[0]
+    for t in (moments, jacobians) + ((modulus,) if modulus is not None else ()):
+        if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
+            raise ValueError("inputs must be contiguous float64 device tensors")
+    if tuple(moments.shape) != (dim, dim, npe, npe) or tuple(jacobians.shape) != (elements, dim, dim):
+        raise ValueError("moments must be (dim, dim, npe, npe) and jacobians (elements, dim, dim)")
+    if modulus is not None and modulus.numel() != elements:
+        raise ValueError("modulus needs one value per element")
+    if out is None:
+        out = torch.empty((elements, npe, npe), dtype=torch.float64, device=jacobians.device)
+    _sync_torch()
+    check(library().cie_hip_element_stiffness(context.handle, elements, dim, npe, _dptr(moments), _dptr(jacobians),
+                                              None if modulus is None else _dptr(modulus), _dptr(out)),
+          "cie_hip_element_stiffness")
+    return out
+
+
 class Assembly:
     """K = sum_e scale[e] P_e^T K_e0 P_e on the pattern of `matrix`, assembled on the device
     (cie_hip_assembly; the element scatter of fem/packages/graph/impl/Assembler_impl.hpp:156-194)."""

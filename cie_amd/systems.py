@@ -112,6 +112,15 @@ def element_matrix(dim, order, elements, basis="lagrange"):
     return out
 
 
+def gradient_moments(dim, order, basis="lagrange"):
+    """moments[i, j] = sum_q w_q grad_i N grad_j N^T on the reference element [-1, 1]^dim (dim x dim x npe x npe)."""
+    npe = (order + 1) ** dim
+    out = np.empty((dim, dim, npe, npe), dtype=np.float64)
+    check_sys(library().cie_sys_lagrange_gradient_moments(byref(_spec(dim, order, 2, basis=basis)), _ptr(out)),
+              "gradient_moments")
+    return out
+
+
 def ansatz_mask(dim, order, elements, eliminate=True):
     """Per-row ansatz mask of the integrated Legendre basis (largest 1D degree, at least 1), uint16."""
     rows = lagrange_size(dim, order, elements, eliminate)[0]

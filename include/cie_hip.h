@@ -97,6 +97,10 @@ int cie_hip_context_stream(cie_hip_context context, void** stream);
  * each solve with HIP events and report their mean duration in
  * cie_hip_solve_info.productSeconds. */
 int cie_hip_context_set_profiling(cie_hip_context context, int enable);
+/* With profiling on: the duration (HIP events on the context's stream) of the
+ * main kernel of the last cie_hip_assemble, cie_hip_element_stiffness or
+ * cie_hip_csr_reorder call on this context. */
+int cie_hip_context_kernel_seconds(cie_hip_context context, double* seconds);
 
 /* Device memory for callers without an allocator of their own
  * (cieutils/packages/concurrency/inc/sycl.hpp:35-70 DeviceMemory). */
@@ -292,6 +296,20 @@ int cie_hip_assembly_destroy(cie_hip_assembly plan);
  * contributions of Assembler::addContribution). Device arrays. Drops the
  * cached Jacobi diagonal. The matrix must own its FP64 entries. */
 int cie_hip_assemble(cie_hip_assembly plan, const double* elementMatrices, int perElement, const double* scale);
+/* Element stiffness of affine cells (the LinearIsotropicStiffnessIntegrand,
+ * K_e = modulus D^T D, fem/packages/integrands/impl/LinearIsotropicStiffnessIntegrand_impl.hpp:74,
+ * integrated over the cell as IntegrandProcessor does,
+ * fem/packages/utilities/impl/IntegrandProcessor_impl.hpp:519-680): with
+ * x = x0 + J_e xi on the reference element and the reference's gradient
+ * moments M[i][j] = sum_q w_q grad_i N grad_j N^T (dim^2 x npe^2,
+ * cie_sys_lagrange_gradient_moments for tensor-product elements),
+ *     out[e] = |det J_e| modulus[e] sum_ij (J_e^-1 J_e^-T)_ij M[i][j],
+ * the sums over i then j in order. jacobians: elements x dim x dim,
+ * row-major (J[r][c] = dx_r / dxi_c); modulus NULL means 1; dim 2 or 3.
+ * poisson2D's cells (ScaleTranslateTransform) are affine. Device arrays;
+ * out holds elements x npe^2 and feeds cie_hip_assemble(plan, out, 1, NULL). */
+int cie_hip_element_stiffness(cie_hip_context context, int64_t elements, int32_t dim, int32_t nodesPerElement,
+                              const double* moments, const double* jacobians, const double* modulus, double* out);
 /* out[rows] = sum_e scale[e] P_e^T v_e, elements in increasing order: the
  * vector overload of Assembler::addContribution (Assembler_impl.hpp:197-220).
  * perElement 0: one nodesPerElement vector; 1: elements x nodesPerElement.

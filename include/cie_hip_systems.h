@@ -51,6 +51,12 @@ int cie_sys_lagrange_fill(const cie_sys_lagrange_spec* spec, const double* condu
 int cie_sys_lagrange_element_matrix(const cie_sys_lagrange_spec* spec, double* elementMatrix);
 /* Row index of every element node (elementCount x nodesPerElement), -1 if eliminated. */
 int cie_sys_lagrange_element_dofs(const cie_sys_lagrange_spec* spec, int32_t* dofs);
+/* Gradient moments of the reference element [-1, 1]^dim: moments[i][j][a][b] =
+ * sum_q w_q dN_a/dxi_i dN_b/dxi_j over (order + 1)^dim Gauss-Legendre points,
+ * dim^2 x npe^2 values, element nodes in the element matrix's order. With an
+ * affine map x = x0 + J xi the element stiffness is
+ * |det J| k sum_ij (J^-1 J^-T)_ij moments[i][j] (cie_hip_element_stiffness). */
+int cie_sys_lagrange_gradient_moments(const cie_sys_lagrange_spec* spec, double* moments);
 /* Ansatz mask of every row for the integrated Legendre basis: the largest
  * polynomial degree among the DOF's 1D factors, at least 1 (makeAnsatzMask,
  * fem/packages/graph/src/connectivity.cpp:62-90 and Assembler.cpp:194-213). */

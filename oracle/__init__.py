@@ -46,6 +46,7 @@ def lib():
         l.oracle_assemble.argtypes = [c_int64, P, P, c_int64, c_int32, P, P, c_int64, P, P]
         l.oracle_assemble_vector.argtypes = [c_int64, c_int64, c_int32, P, P, c_int64, P, P]
         l.oracle_degree_sort.argtypes = [c_int64, P, P]
+        l.oracle_element_stiffness.argtypes = [c_int64, c_int, c_int, P, P, P, P]
         l.oracle_heap_sort.argtypes = [c_int64, P, P]
         l.oracle_make_reordering.argtypes = [c_int64, P, P, c_int, P]
         l.oracle_reorder_csr.argtypes = [c_int64, P, c_int, P, P, P, P, P, P]
@@ -339,3 +340,14 @@ def pmg(csr, mask, order, b, relative_tolerance=1e-6, max_cycles=500, threads=1)
                             relative_tolerance, max_cycles, _p(history), _p(coarse_history), max_cycles,
                             byref(cycles), byref(coarse), threads), "pmg")
     return x, history[:cycles.value].copy(), coarse_history[:cycles.value].astype(np.int64)
+
+
+def element_stiffness(moments, jacobians, modulus=None):
+    """K_e for affine cells from the reference gradient moments (dim, dim, npe, npe); jacobians (E, dim, dim)."""
+    moments, jacobians = _f64(moments), _f64(jacobians)
+    dim, npe, elements = moments.shape[0], moments.shape[2], jacobians.shape[0]
+    modulus = None if modulus is None else _f64(modulus)
+    out = np.empty((elements, npe, npe))
+    _check(lib().oracle_element_stiffness(elements, dim, npe, _p(moments), _p(jacobians),
+                                          None if modulus is None else _p(modulus), _p(out)), "element_stiffness")
+    return out

@@ -36,6 +36,12 @@
  *                           The contribution is scale_e * K_e (scale NULL: K_e),
  *                           the product of LinearIsotropicStiffnessIntegrand_impl.hpp:74.
  *   oracle_assemble_vector  Assembler_impl.hpp:197-220 (the vector overload).
+ *   oracle_element_stiffness
+ *                           LinearIsotropicStiffnessIntegrand_impl.hpp:74
+ *                           (K = modulus D^T D) integrated over an affine cell
+ *                           x = x0 + J xi (IntegrandProcessor_impl.hpp:519-680):
+ *                           |det J| k sum_ij (J^-1 J^-T)_ij M_ij with the
+ *                           reference gradient moments M_ij.
  *   oracle_make_reordering  linalg/packages/utilities/src/reorder.cpp:23-173
  *                           (makeReordering, serial path): Cuthill-McKee from
  *                           the first row of least degree (row length); every
@@ -1023,4 +1029,43 @@ int oracle_pmg(int64_t rows, const int32_t* rowptr, const int32_t* cols, const d
     free(inverse); free(residual); free(gridResidual); free(update); free(buffer); free(work);
     pool_destroy(pool);
     return status;
+}
+
+/* out[e] = |det J_e| k_e sum_ij G_ij moments[i][j], G = J^-1 J^-T by cofactors. */
+int oracle_element_stiffness(int64_t elements, int dim, int npe, const double* moments, const double* jacobians,
+                             const double* modulus, double* out) {
+    if (dim != 2 && dim != 3) return 2;
+    const int64_t entries = (int64_t)npe * npe;
+    for (int64_t e = 0; e < elements; ++e) {
+        const double* J = jacobians + e * dim * dim;
+        double inverse[9], G[9], det;
+        if (dim == 2) {
+            det = J[0] * J[3] - J[1] * J[2];
+            inverse[0] = J[3] / det;
+            inverse[1] = -J[1] / det;
+            inverse[2] = -J[2] / det;
+            inverse[3] = J[0] / det;
+        } else {
+            const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+            const double c10 = J[2] * J[7] - J[1] * J[8], c11 = J[0] * J[8] - J[2] * J[6], c12 = J[1] * J[6] - J[0] * J[7];
+            const double c20 = J[1] * J[5] - J[2] * J[4], c21 = J[2] * J[3] - J[0] * J[5], c22 = J[0] * J[4] - J[1] * J[3];
+            det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+            inverse[0] = c00 / det; inverse[1] = c10 / det; inverse[2] = c20 / det;
+            inverse[3] = c01 / det; inverse[4] = c11 / det; inverse[5] = c21 / det;
+            inverse[6] = c02 / det; inverse[7] = c12 / det; inverse[8] = c22 / det;
+        }
+        for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j) {
+                double g = 0.0;
+                for (int k = 0; k < dim; ++k) g += inverse[i * dim + k] * inverse[j * dim + k];
+                G[i * dim + j] = g;
+            }
+        const double factor = fabs(det) * (modulus ? modulus[e] : 1.0);
+        for (int64_t entry = 0; entry < entries; ++entry) {
+            double v = 0.0;
+            for (int ij = 0; ij < dim * dim; ++ij) v += G[ij] * moments[ij * entries + entry];
+            out[e * entries + entry] = factor * v;
+        }
+    }
+    return 0;
 }

@@ -22,6 +22,14 @@ def _dev(torch, array, dtype=None):
     return tensor if dtype is None else tensor.to(dtype)
 
 
+def _entries_match(matrix, expected):
+    """The stored FP64 values, read back, equal the expected CSR's bit for bit."""
+    got = matrix.copy_out()
+    assert np.array_equal(got.row_extents, expected.row_extents)
+    assert np.array_equal(got.column_indices, expected.column_indices)
+    assert np.array_equal(got.entries, expected.entries)
+
+
 def _products_match(hip, torch, matrix, expected, seeds=(3, 4), threads=1):
     matrix.set_launch(64, "staged", 1024, 8)
     matrix.set_launch(32, "staged", 1024, 8)
@@ -47,6 +55,7 @@ def test_assembly_matches_generator_bitwise(hip, torch_gpu, dim, order, n):
     assembly = hip.Assembly(matrix, systems.element_dofs(dim, order, n))
     element = _dev(torch_gpu, systems.element_matrix(dim, order, n))
     assembly.assemble(element, _dev(torch_gpu, k))
+    _entries_match(matrix, expected)
     _products_match(hip, torch_gpu, matrix, expected)
     # scale NULL means k = 1; assembling twice gives the same values (every slot starts from zero)
     assembly.assemble(element)
@@ -68,11 +77,29 @@ def test_assembly_of_per_element_matrices_matches_oracle(hip, torch_gpu, dim, or
     matrix = hip.CSRMatrix(pattern)
     assembly = hip.Assembly(matrix, dofs)
     assembly.assemble(_dev(torch_gpu, matrices), _dev(torch_gpu, scale))
+    _entries_match(matrix, expected)
     _products_match(hip, torch_gpu, matrix, expected)
     unscaled = systems.CSR(pattern.rows, pattern.rows, pattern.row_extents, pattern.column_indices,
                            oracle.assemble(pattern, dofs, matrices))
     assembly.assemble(_dev(torch_gpu, matrices))
     _products_match(hip, torch_gpu, matrix, unscaled, seeds=(7,))
+
+
+def test_assembly_with_a_row_listed_twice_in_one_element(hip, torch_gpu):
+    """A dof map that names one row at two local nodes of an element (the scatter adds both)."""
+    dim, order, n = 2, 2, 5
+    pattern = systems.lagrange(dim, order, n)
+    dofs = systems.element_dofs(dim, order, n)
+    e = 7
+    kept = np.flatnonzero(dofs[e] >= 0)
+    dofs[e, kept[1]] = dofs[e, kept[0]]
+    rng = np.random.default_rng(6)
+    matrices = rng.standard_normal((dofs.shape[0], dofs.shape[1], dofs.shape[1]))
+    expected = systems.CSR(pattern.rows, pattern.rows, pattern.row_extents, pattern.column_indices,
+                           oracle.assemble(pattern, dofs, matrices))
+    matrix = hip.CSRMatrix(pattern)
+    hip.Assembly(matrix, dofs).assemble(_dev(torch_gpu, matrices))
+    _entries_match(matrix, expected)
 
 
 @pytest.mark.parametrize("dim,order,n", [(2, 1, 9), (2, 4, 5), (3, 2, 5)])
@@ -87,6 +114,38 @@ def test_vector_assembly_matches_oracle_bitwise(hip, torch_gpu, dim, order, n):
     assert np.array_equal(got, oracle.assemble_vector(pattern.rows, dofs, vectors, scale))
     got = assembly.assemble_vector(_dev(torch_gpu, shared)).cpu().numpy()
     assert np.array_equal(got, oracle.assemble_vector(pattern.rows, dofs, shared))
+
+
+@pytest.mark.parametrize("dim,order,basis", [(2, 2, "lagrange"), (2, 4, "legendre"), (3, 1, "lagrange"),
+                                             (3, 2, "lagrange"), (3, 4, "lagrange")])
+def test_element_stiffness_matches_oracle_bitwise(hip, torch_gpu, dim, order, basis):
+    rng = np.random.default_rng(dim * 10 + order)
+    moments = systems.gradient_moments(dim, order, basis)
+    count = 700
+    jac = rng.standard_normal((count, dim, dim)) * 0.3 + np.eye(dim) * rng.uniform(0.5, 2.0, (count, 1, 1))
+    k = np.exp(rng.standard_normal(count))
+    got = hip.elementStiffness(_dev(torch_gpu, moments), _dev(torch_gpu, jac), _dev(torch_gpu, k))
+    assert np.array_equal(got.cpu().numpy(), oracle.element_stiffness(moments, jac, k))
+    got = hip.elementStiffness(_dev(torch_gpu, moments), _dev(torch_gpu, jac))
+    assert np.array_equal(got.cpu().numpy(), oracle.element_stiffness(moments, jac))
+
+
+@pytest.mark.parametrize("dim,order", [(2, 3), (3, 2)])
+def test_stiffness_from_cell_geometry_assembles_the_generator_system(hip, torch_gpu, dim, order):
+    """Uniform cells x = x0 + (h/2) xi with k_e: element stiffness on the device, then the per-element
+    assembly, equals the generator's system (which sums tensor-product factors in another order)."""
+    n = 6
+    count = systems.lagrange_size(dim, order, n)[2]
+    k = np.exp(systems.normal(3, 1.0, count))
+    expected = systems.lagrange(dim, order, n, conductivity=k)
+    moments = _dev(torch_gpu, systems.gradient_moments(dim, order))
+    jac = _dev(torch_gpu, np.broadcast_to(np.eye(dim) * (0.5 / n), (count, dim, dim)).copy())
+    stiffness = hip.elementStiffness(moments, jac, _dev(torch_gpu, k))
+    matrix = hip.CSRMatrix(systems.lagrange(dim, order, n))
+    assembly = hip.Assembly(matrix, systems.element_dofs(dim, order, n))
+    assembly.assemble(stiffness)
+    got = matrix.copy_out()
+    assert np.allclose(got.entries, expected.entries, rtol=1e-12, atol=1e-12 * np.abs(expected.entries).max())
 
 
 def test_assembly_then_solve_uses_new_values(hip, torch_gpu):
@@ -156,6 +215,22 @@ def test_full_size_assembly_q2_64(hip, torch_gpu):
     for _ in range(3):
         assembly.assemble(element, k_dev)
     device_seconds = (time.perf_counter() - t0) / 3
-    _products_match(hip, torch_gpu, matrix, expected, seeds=(9,), threads=16)
-    print(f"\nassembly of {expected.nnz} nonzeros: device {device_seconds * 1e3:.2f} ms "
-          f"(FP64 + FP32 planes), host generator {host_seconds * 1e3:.0f} ms (16 threads)")
+    _entries_match(matrix, expected)
+    # element stiffness of the 262,144 cells from their geometry, then the per-element assembly
+    moments = _dev(torch_gpu, systems.gradient_moments(dim, order))
+    jac = _dev(torch_gpu, np.broadcast_to(np.eye(dim) * (0.5 / n), (count, dim, dim)).copy())
+    stiffness = hip.elementStiffness(moments, jac, k_dev)
+    torch_gpu.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        hip.elementStiffness(moments, jac, k_dev, out=stiffness)
+    stiffness_seconds = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        assembly.assemble(stiffness)
+    per_element_seconds = (time.perf_counter() - t0) / 3
+    got = matrix.copy_out()
+    assert np.allclose(got.entries, expected.entries, rtol=1e-12, atol=1e-12 * np.abs(expected.entries).max())
+    print(f"\nassembly of {expected.nnz} nonzeros: device {device_seconds * 1e3:.2f} ms (shared K_e0, FP64 + FP32 "
+          f"planes), host generator {host_seconds * 1e3:.0f} ms (16 threads); element stiffness of {count} cells "
+          f"{stiffness_seconds * 1e3:.2f} ms, per-element assembly {per_element_seconds * 1e3:.2f} ms")
