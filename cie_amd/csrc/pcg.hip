@@ -44,10 +44,13 @@ constexpr int kVectorThreads = 1024;  // update and direction blocks
 constexpr int kVectorBlocksMax = kVectorPartials;  // at most this many update partials
 constexpr int kFinishBlock = 1024;
 
-// Blocks of the update and direction kernels for n rows (about 4 rows per thread).
+// Blocks of the update and direction kernels for n rows: about 4 rows per
+// thread for large systems; a smaller system still spreads over up to 256
+// blocks (one per CU) with fewer rows per thread.
 int vectorBlocks(int64_t n) {
-    const int64_t blocks = (n + int64_t(kVectorThreads) * 4 - 1) / (int64_t(kVectorThreads) * 4);
-    return int(std::max<int64_t>(1, std::min<int64_t>(kVectorBlocksMax, blocks)));
+    const int64_t four = (n + int64_t(kVectorThreads) * 4 - 1) / (int64_t(kVectorThreads) * 4);
+    const int64_t one = std::min<int64_t>(256, (n + kVectorThreads - 1) / kVectorThreads);
+    return int(std::max<int64_t>(1, std::min<int64_t>(kVectorBlocksMax, std::max(four, one))));
 }
 
 // Sum over the block of `count` partials (value `which` of each slot of two),
