@@ -97,8 +97,10 @@ def main():
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
-    parser.add_argument("--workload", choices=("pcg", "assembly", "reorder", "pmg"), default="pcg",
-                        help="pcg: the headline solve; assembly / reorder / pmg: the widened paths, one JSON line each")
+    parser.add_argument("--workload", choices=("pcg", "batch", "adjoint", "assembly", "reorder", "pmg"),
+                        default="pcg",
+                        help="pcg: the headline solve; batch / adjoint: the added forms; assembly / reorder / pmg: "
+                             "the widened paths; one JSON line each")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -107,7 +109,8 @@ def main():
         sys.exit(2)
 
     if args.workload != "pcg":
-        {"assembly": bench_assembly, "reorder": bench_reorder, "pmg": bench_pmg}[args.workload](args)
+        {"batch": bench_batch, "adjoint": bench_adjoint, "assembly": bench_assembly, "reorder": bench_reorder,
+         "pmg": bench_pmg}[args.workload](args)
         return
 
     import torch
@@ -306,6 +309,139 @@ def _roofline(kernel, bytes_per_launch, seconds, note):
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
             "bytes_per_launch": bytes_per_launch, "mean_launch_seconds": seconds, "bytes": note}
+
+
+def bench_batch(args):
+    """256 uneven 2D Q2 systems (n_i in [16, 256], seed 3) solved together, mixed-refined to 1e-10."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from cie_amd import linalg, systems
+    from cie_amd._lib import Statistics
+
+    w = systems.q2_quad_batch(256)
+    offsets, rows, nnz = w.extra["offsets"], w.matrix.rows, w.matrix.nnz
+    context = linalg.Context.default()
+    matrix = linalg.CSRMatrix(w.matrix, context)
+    batch = linalg.Batch(matrix, offsets)
+    batch.autotune(32, 2)
+    b = torch.from_numpy(w.rhs).cuda()
+    x = torch.zeros_like(b)
+    settings = Statistics(100000, 0.0, w.relative_tolerance)
+
+    def solve(mixed=True, graph=True):
+        x.zero_()
+        return batch.solve(b, x, settings, mixed=mixed, graph=graph)
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        stats, status, refinements, summary = solve()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - start) / args.steps
+    assert np.all(status == 1)
+    iterations = np.array([st.iterationCount for st in stats], dtype=np.int64)
+    fp64_stats, _, _, fp64_summary = solve(mixed=False)
+    fp64_iterations = np.array([st.iterationCount for st in fp64_stats], dtype=np.int64)
+    # Dominant kernel: the batched FP32 product; the first 8 launches (every system still runs) timed with events.
+    linalg.library().cie_hip_context_set_profiling(context.handle, 1)
+    _, _, _, profiled = solve(graph=False)
+    linalg.library().cie_hip_context_set_profiling(context.handle, 0)
+    assert iterations.min() >= 8 and profiled.productSamples > 0
+    algorithmic = spmv_bytes(rows, nnz, 4, index_bytes(matrix, 32))
+    # CPU baseline: the reference solves each system with its own ConjugateGradients; the oracle's FP64 PCG on
+    # systems in order until the sample time is spent, scaled by nonzeros x FP64 iterations.
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    extents = w.matrix.row_extents
+    sampled_work, sampled_seconds, sampled = 0, 0.0, 0
+    for i in range(256):
+        lo, hi = int(offsets[i]), int(offsets[i + 1])
+        sub = systems.CSR(hi - lo, hi - lo, (extents[lo:hi + 1] - extents[lo]).astype(np.int32),
+                          (w.matrix.column_indices[extents[lo]:extents[hi]] - lo).astype(np.int32),
+                          w.matrix.entries[extents[lo]:extents[hi]])
+        _, st, _, secs = oracle.pcg(sub, w.rhs[lo:hi], relative_tolerance=w.relative_tolerance, max_iterations=100000,
+                                    threads=threads)
+        sampled_work += sub.nnz * (int(st.iterationCount) + 1)
+        sampled_seconds += secs
+        sampled += 1
+        if sampled_seconds > 15.0:
+            break
+    system_nnz = np.diff(extents[offsets])
+    total_work = int(np.sum(system_nnz * (fp64_iterations + 1)))
+    _line(args, "batched solve time (s), 256 uneven 2D Q2 systems to 1e-10, mixed-refined", wall, "s",
+          "f32 inner / f64 refinement",
+          {"workload": "256 2D Q2 Laplace systems, n_i ~ U{16..256} (seed 3), k_e lognormal(0, 0.5), b ~ U(-1,1), "
+                       "relative residual 1e-10 each", "rows": rows, "nnz": nnz, "systems": 256},
+          _roofline("batchProductKernel<float> (all systems active)", algorithmic, profiled.productSeconds,
+                    "B_SpMV(FP32 plane, format index bytes) over the whole block-diagonal matrix"),
+          {"value": sampled_seconds * total_work / sampled_work, "unit": "s", "cores": threads, "kind": "port",
+           "sample": f"oracle FP64 Jacobi-PCG (ConjugateGradients_impl.hpp:49-201) on the first {sampled} systems "
+                     f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
+          {"device_seconds": summary.seconds, "max_iterations": int(iterations.max()),
+           "refinements": int(refinements.sum()), "fp64": {"device_seconds": fp64_summary.seconds,
+                                                           "max_iterations": int(fp64_iterations.max())}})
+
+
+def bench_adjoint(args):
+    """dJ/dg for J = 0.5 ||u(g) - u_t||^2 on 3D Q2 48^3 (k = exp(g)): assembly, forward and adjoint PCG, gather."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from cie_amd import adjoint, linalg, systems
+
+    n = 48
+    w = systems.q2_adjoint_3d(n)
+    g, gt = w.extra["log_conductivity"], w.extra["target_log_conductivity"]
+    problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
+    problem.assemble(gt)
+    ut, _ = problem.solve(problem.rhs)
+    problem.target = ut
+    context = problem.context
+
+    def gradient():
+        g_t = torch.tensor(g, dtype=torch.float64, device="cuda:0", requires_grad=True)
+        j = adjoint.Objective.apply(g_t, problem)
+        j.backward()
+        return g_t.grad
+
+    for _ in range(args.warmup):
+        gradient()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        gradient()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - start) / args.steps
+    linalg.library().cie_hip_context_set_profiling(context.handle, 1)
+    problem.assemble(g)
+    u, info = problem.solve(problem.rhs)
+    linalg.library().cie_hip_context_set_profiling(context.handle, 0)
+    lam, info_adjoint = problem.solve(u - problem.target)
+    rows, nnz = w.matrix.rows, w.matrix.nnz
+    algorithmic = spmv_bytes(rows, nnz, 8, index_bytes(problem.matrix, 64))
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sample = 20
+    _, _, _, secs = oracle.pcg(w.matrix, w.rhs, relative_tolerance=0.0, max_iterations=sample, threads=threads)
+    total_iterations = int(info.stats.iterationCount) + int(info_adjoint.stats.iterationCount) + 2
+    t0 = time.perf_counter()
+    oracle.element_gradient(problem.dofs_host, problem.element_matrix_host, lam.cpu().numpy(), u.cpu().numpy())
+    gather = time.perf_counter() - t0
+    _line(args, "adjoint gradient time (s), dJ/dg over every element, forward + adjoint FP64 PCG to 1e-11", wall,
+          "s", "f64",
+          {"workload": f"3D Q2 Laplace {n}^3 hexahedra, k_e = exp(g_e), g ~ N(0, 0.5) seed 4, target from seed 5, "
+                       f"b ~ U(-1,1) seed 1", "rows": rows, "nnz": nnz, "elements": int(problem.count)},
+          _roofline("productKernel<BlockTiles<double, double, ...>> (q = A p) inside the forward solve", algorithmic,
+                    info.productSeconds, "B_SpMV(FP64, format index bytes)"),
+          {"value": secs * total_iterations / sample + gather, "unit": "s", "cores": threads, "kind": "port",
+           "sample": f"oracle FP64 Jacobi-PCG, {sample} iterations in {secs:.2f} s, scaled to the "
+                     f"{total_iterations} forward + adjoint iterations, plus the oracle's element gradient "
+                     f"({gather:.2f} s)"},
+          {"forward_iterations": int(info.stats.iterationCount),
+           "adjoint_iterations": int(info_adjoint.stats.iterationCount)})
 
 
 def bench_assembly(args):

@@ -593,9 +593,11 @@ void launchSelect(cie_hip_batch batch, const ActiveWork& work) {
 }
 
 // Enqueues the kernels of one iteration on the context stream.
+constexpr int kProfiledTrips = 8;  // product launches averaged by the profiling summary
+
 template <class TR>
 void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
-                      const TR* inverse, const BatchSettings& settings, bool mixed) {
+                      const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
     cie_hip_context ctx = batch->context;
     cie_hip_csr m = batch->matrix;
     const hipStream_t stream = ctx->stream;
@@ -604,8 +606,10 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     const TR* entries;
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
+    const int mark = profile ? profile->begin(ctx) : -1;
     hipLaunchKernelGGL((batchProductKernel<TR>), workGrid(batchProductKernel<TR>, ctx, kTileBlock, batch->tileCount),
                        tileBlock, 0, stream, layout, work, entries, p, q, batch->tilePartials);
+    if (profile) profile->end(ctx, mark);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
                        batch->tilePartials, batch->state, settings, batch->active);
     hipLaunchKernelGGL((batchUpdateKernel<TR, double>),
@@ -632,7 +636,7 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
 
 template <class TR>
 int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSettings& settings, bool mixed,
-             bool useGraph, double* seconds, int* launches) {
+             bool useGraph, double* seconds, int* launches, ProductProfile* profile) {
     cie_hip_context ctx = batch->context;
     cie_hip_csr m = batch->matrix;
     const int64_t n = m->rows;
@@ -693,7 +697,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             hipGraph_t graph = nullptr;
             CIE_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed);
+                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, nullptr);
             const hipError_t captured = hipStreamEndCapture(ctx->stream, &graph);
             if (captured != hipSuccess) {
                 cleanup();
@@ -721,7 +725,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             CIE_HIP_CHECK(hipGraphLaunch(batch->graph, ctx->stream));
         } else {
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed);
+                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
             CIE_HIP_CHECK(hipGetLastError());
         }
         ++count;
@@ -915,8 +919,9 @@ int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const doub
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pcg_batched: iterationCount must be positive");
     double seconds = 0.0;
     int launches = 0;
-    const int run = mixed ? runBatch<float>(batch, b, x, s, true, useGraph != 0, &seconds, &launches)
-                          : runBatch<double>(batch, b, x, s, false, useGraph != 0, &seconds, &launches);
+    ProductProfile profile;  // product kernels bracketed with events (profiling on, no graph)
+    const int run = mixed ? runBatch<float>(batch, b, x, s, true, useGraph != 0, &seconds, &launches, &profile)
+                          : runBatch<double>(batch, b, x, s, false, useGraph != 0, &seconds, &launches, &profile);
     if (run) return run;
     CIE_HIP_CHECK(hipMemcpy(batch->pinnedStates, batch->state, sizeof(SystemState) * batch->systems,
                             hipMemcpyDeviceToHost));
@@ -944,6 +949,12 @@ int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const doub
         summary->seconds = seconds;
         summary->refinements = totalRefinements;
         summary->spmvCount = uint64_t(launches) * kGraphIterations;
+        if (context->profiling && profile.used) {
+            // The first trips, where every system still runs (the bench checks the fewest iterations).
+            const int samples = std::min(profile.used, kProfiledTrips);
+            summary->productSamples = uint64_t(samples);
+            summary->productSeconds = profile.mean(context, samples);
+        }
     }
     if (worst) return setError(worst, "cie_hip_pcg_batched: a system broke down (see the per-system status)");
     return CIE_HIP_OK;

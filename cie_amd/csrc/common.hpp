@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -88,6 +89,33 @@ struct cie_hip_context_s {
 };
 
 namespace ciehip {
+// With the context's profiling on, brackets up to kProfileEvents launches of
+// one kernel (the product of a solver loop) with HIP events; mean() is their
+// average duration over the first `samples` of them.
+struct ProductProfile {
+    int used = 0;
+    int begin(cie_hip_context ctx) {
+        if (!ctx->profiling || used >= cie_hip_context_s::kProfileEvents) return -1;
+        (void)hipEventRecord(ctx->profileEvents[2 * used], ctx->stream);
+        return used++;
+    }
+    void end(cie_hip_context ctx, int mark) {
+        if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
+    }
+    double mean(cie_hip_context ctx, int samples) const {
+        double total = 0.0;
+        int count = 0;
+        for (int i = 0; i < std::min(used, samples); ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->profileEvents[2 * i], ctx->profileEvents[2 * i + 1]) == hipSuccess) {
+                total += ms * 1e-3;
+                ++count;
+            }
+        }
+        return count ? total / count : 0.0;
+    }
+};
+
 // With the context's profiling on, brackets one kernel launch on its stream
 // with HIP events; finish() waits for the end event and keeps the duration.
 struct KernelTimer {

@@ -408,31 +408,6 @@ int launchRefine(cie_hip_csr m, int slot, int updateCount, const double* x, cons
 
 // Brackets the first kProfileEvents loop SpMV kernels with events on the
 // solver stream (profiling on).
-struct ProductProfile {
-    int used = 0;
-    int begin(cie_hip_context ctx) {
-        if (!ctx->profiling || used >= cie_hip_context_s::kProfileEvents) return -1;
-        (void)hipEventRecord(ctx->profileEvents[2 * used], ctx->stream);
-        return used++;
-    }
-    void end(cie_hip_context ctx, int mark) {
-        if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
-    }
-    // Mean over the first `samples` bracketed launches (those that did work).
-    double mean(cie_hip_context ctx, int samples) const {
-        double total = 0.0;
-        int count = 0;
-        for (int i = 0; i < std::min(used, samples); ++i) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ctx->profileEvents[2 * i], ctx->profileEvents[2 * i + 1]) == hipSuccess) {
-                total += ms * 1e-3;
-                ++count;
-            }
-        }
-        return count ? total / count : 0.0;
-    }
-};
-
 // q = A p with its p.q partials (first-level sum included); returns where the
 // update kernel reads them.
 template <class T>

@@ -248,7 +248,10 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions);
  * (each optional): stats[s] (Statistics as for the single solve; for mixed,
  * the FP64 true residual), status[s] (1 converged, 2 iteration cap, 3 p^T A p
  * <= 0, 4 r^T z <= 0), refinements[s]; summary holds the batch time and the
- * largest iteration count. Returns the first breakdown status, if any. */
+ * largest iteration count. With the context's profiling on and useGraph 0, the
+ * first 8 product kernels are bracketed with HIP events and their mean
+ * duration is summary.productSeconds. Returns the first breakdown status, if
+ * any. */
 int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const double* b, double* x,
                         const cie_hip_mixed_settings* settings, int mixed, int useGraph, cie_hip_statistics* stats,
                         int32_t* status, uint64_t* refinements, cie_hip_solve_info* summary);
