@@ -379,11 +379,10 @@ struct BlockTiles {
                 entry[u] = live ? entry[u] : TM(0);  // the re-read column is a real one: harmless to gather
             }
         } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                column[u] = 0;
-                entry[u] = TM(0);
-            }
+            // Only empty rows: each sums to +0, as below, and nothing is
+            // gathered (a matrix without columns has no in[0]).
+            if (tid < rows) epilogue(first + tid, TV(0));
+            return;
         }
         if (tid < rows) extents[tid] = source.rowExtents[first + tid] - begin;
         if (tid == 0) extents[rows] = end - begin;

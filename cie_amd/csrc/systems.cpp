@@ -491,7 +491,7 @@ int cie_sys_lagrange_gradient_moments(const cie_sys_lagrange_spec* spec, double*
     }
     int pointCount = 1;
     for (int d = 0; d < dim; ++d) pointCount *= n;
-    std::vector<double> gradient(size_t(dim) * npe), w(size_t(pointCount));
+    std::vector<double> gradient(size_t(dim) * npe);
     std::fill(moments, moments + size_t(dim) * dim * npe * npe, 0.0);
     for (int q = 0; q < pointCount; ++q) {
         const int qi[3] = {q % n, (q / n) % n, dim == 3 ? q / (n * n) : 0};

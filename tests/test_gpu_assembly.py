@@ -130,6 +130,31 @@ def test_element_stiffness_matches_oracle_bitwise(hip, torch_gpu, dim, order, ba
     assert np.array_equal(got.cpu().numpy(), oracle.element_stiffness(moments, jac))
 
 
+def test_elements_of_more_than_256_nodes(hip, torch_gpu):
+    """3D Q6 (343 nodes): local pairs a * 343 + b exceed 16 bits, so the slot lists store int32 locals, and one
+    element matrix (117,649 entries) is more than one pass of the stiffness kernel's block."""
+    dim, order, n = 3, 6, 2
+    rng = np.random.default_rng(36)
+    moments = systems.gradient_moments(dim, order)
+    jac = rng.standard_normal((20, dim, dim)) * 0.2 + np.eye(dim) * rng.uniform(0.5, 2.0, (20, 1, 1))
+    k = np.exp(rng.standard_normal(20))
+    got = hip.elementStiffness(_dev(torch_gpu, moments), _dev(torch_gpu, jac), _dev(torch_gpu, k))
+    assert np.array_equal(got.cpu().numpy(), oracle.element_stiffness(moments, jac, k))
+    pattern = systems.lagrange(dim, order, n)
+    dofs = systems.element_dofs(dim, order, n)
+    matrices = rng.standard_normal((dofs.shape[0], dofs.shape[1], dofs.shape[1]))
+    scale = rng.uniform(0.5, 2.0, dofs.shape[0])
+    expected = systems.CSR(pattern.rows, pattern.rows, pattern.row_extents, pattern.column_indices,
+                           oracle.assemble(pattern, dofs, matrices, scale))
+    matrix = hip.CSRMatrix(pattern)
+    assembly = hip.Assembly(matrix, dofs)
+    assembly.assemble(_dev(torch_gpu, matrices), _dev(torch_gpu, scale))
+    _entries_match(matrix, expected)
+    vectors = rng.standard_normal((dofs.shape[0], dofs.shape[1]))
+    got = assembly.assemble_vector(_dev(torch_gpu, vectors), _dev(torch_gpu, scale))
+    assert np.array_equal(got.cpu().numpy(), oracle.assemble_vector(pattern.rows, dofs, vectors, scale))
+
+
 @pytest.mark.parametrize("dim,order", [(2, 3), (3, 2)])
 def test_stiffness_from_cell_geometry_assembles_the_generator_system(hip, torch_gpu, dim, order):
     """Uniform cells x = x0 + (h/2) xi with k_e: element stiffness on the device, then the per-element
