@@ -668,15 +668,10 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     CIE_HIP_CHECK(hipMemcpyAsync(batch->active, batch->pinnedActive, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
-    hipEvent_t start, marks[2];
-    CIE_HIP_CHECK(hipEventCreate(&start));
-    CIE_HIP_CHECK(hipEventCreate(&marks[0]));
-    CIE_HIP_CHECK(hipEventCreate(&marks[1]));
-    auto cleanup = [&]() {
-        (void)hipEventDestroy(start);
-        (void)hipEventDestroy(marks[0]);
-        (void)hipEventDestroy(marks[1]);
-    };
+    ScopedEvents<3> events;  // the start, and one mark per polling slot
+    CIE_HIP_CHECK(events.create());
+    const hipEvent_t start = events.event[0];
+    hipEvent_t* const marks = events.event + 1;
     CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
     const ActiveWork& work = batch->running;
     launchSelect<kRunning>(batch, work);  // every system runs
@@ -700,14 +695,12 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
                 enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, nullptr);
             const hipError_t captured = hipStreamEndCapture(ctx->stream, &graph);
             if (captured != hipSuccess) {
-                cleanup();
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph capture: ") + hipGetErrorString(captured));
             }
             const hipError_t made = hipGraphInstantiate(&batch->graph, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (made != hipSuccess) {
                 batch->graph = nullptr;
-                cleanup();
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(made));
             }
             std::copy(key, key + 4, batch->graphKey);
@@ -746,7 +739,6 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
         slot ^= 1;
     }
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    cleanup();
     *launches = count;
     return CIE_HIP_OK;
 }
@@ -845,10 +837,13 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     const size_t valueSize = precision == 32 ? sizeof(float) : sizeof(double);
     void* buffer = nullptr;
     CIE_HIP_CHECK(hipMalloc(&buffer, 2 * valueSize * size_t(std::max<int64_t>(1, m->rows))));
-    CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, 2 * valueSize * size_t(std::max<int64_t>(1, m->rows)), ctx->stream));
-    hipEvent_t start, stop;
-    CIE_HIP_CHECK(hipEventCreate(&start));
-    CIE_HIP_CHECK(hipEventCreate(&stop));
+    ScopedEvents<2> events;
+    if (const hipError_t error = events.create(); error != hipSuccess ||
+        hipMemsetAsync(buffer, 0, 2 * valueSize * size_t(std::max<int64_t>(1, m->rows)), ctx->stream) != hipSuccess) {
+        (void)hipFree(buffer);
+        return setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing set-up failed");
+    }
+    const hipEvent_t start = events.event[0], stop = events.event[1];
     std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
     std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
     int status = CIE_HIP_OK;
@@ -860,8 +855,11 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
         status = batchLayout(batch, &layout);
         if (status) break;
         std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
-        CIE_HIP_CHECK(hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
-                                hipMemcpyDeviceToHost));
+        if (hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: reading the tile layout failed");
+            break;
+        }
         for (int s = 0; s < batch->systems && status == CIE_HIP_OK; ++s) {
             const int t0 = systemTiles[s], tiles = systemTiles[s + 1] - t0;
             for (int pass = 0; pass < 2; ++pass) {
@@ -890,8 +888,6 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
             }
         }
     }
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
     (void)hipFree(buffer);
     if (status) return status;
     batch->tileRows = choice;

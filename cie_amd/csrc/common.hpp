@@ -116,6 +116,24 @@ struct ProductProfile {
     }
 };
 
+// N events, destroyed on every return path of the scope that holds them.
+template <int N>
+struct ScopedEvents {
+    hipEvent_t event[N] = {};
+    ScopedEvents() = default;
+    ScopedEvents(const ScopedEvents&) = delete;
+    ScopedEvents& operator=(const ScopedEvents&) = delete;
+    ~ScopedEvents() {
+        for (hipEvent_t e : event)
+            if (e) (void)hipEventDestroy(e);
+    }
+    hipError_t create() {
+        for (hipEvent_t& e : event)
+            if (const hipError_t error = hipEventCreate(&e); error != hipSuccess) return error;
+        return hipSuccess;
+    }
+};
+
 // With the context's profiling on, brackets one kernel launch on its stream
 // with HIP events; finish() waits for the end event and keeps the duration.
 struct KernelTimer {

@@ -533,10 +533,12 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
     const int64_t inCount = std::max<int64_t>(1, matrix->columns);
     const size_t bytes = valueSize * size_t(inCount + std::max<int64_t>(1, matrix->rows));
     CIE_HIP_CHECK(hipMalloc(&buffer, bytes));
-    CIE_HIP_CHECK(hipMemsetAsync(buffer, 0, bytes, ctx->stream));
-    hipEvent_t start, stop;
-    CIE_HIP_CHECK(hipEventCreate(&start));
-    CIE_HIP_CHECK(hipEventCreate(&stop));
+    ScopedEvents<2> events;
+    if (const hipError_t error = events.create(); error != hipSuccess || hipMemsetAsync(buffer, 0, bytes, ctx->stream) != hipSuccess) {
+        (void)hipFree(buffer);
+        return setError(CIE_HIP_ERROR_HIP, "cie_hip_csr_autotune: timing set-up failed");
+    }
+    const hipEvent_t start = events.event[0], stop = events.event[1];
 
     double best = std::numeric_limits<double>::infinity();
     CsrLaunch bestLaunch = candidates.front();
@@ -566,8 +568,6 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
         }
         if (status != CIE_HIP_OK) break;
     }
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
     (void)hipFree(buffer);
     if (status != CIE_HIP_OK) return setError(status, std::string("cie_hip_csr_autotune: ") + cie_hip_last_error());
     launchOf(matrix, precision) = bestLaunch;

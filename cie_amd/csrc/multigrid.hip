@@ -529,10 +529,9 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
     const uint16_t* maskValues = plan->mask->values;
     constexpr double kRelaxation = 2.0 / 3.0;  // solver.cpp:196
     constexpr int kSweeps = 6;                 // solver.cpp:195
-    hipEvent_t start, stop;
-    CIE_HIP_CHECK(hipEventCreate(&start));
-    CIE_HIP_CHECK(hipEventCreate(&stop));
-    CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
+    ScopedEvents<2> events;  // start, stop
+    CIE_HIP_CHECK(events.create());
+    CIE_HIP_CHECK(hipEventRecord(events.event[0], ctx->stream));
 
     CIE_HIP_CHECK(hipMemcpyAsync(residual, b, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, ctx->stream));
     double initial = 0.0;
@@ -592,12 +591,10 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
         norm = std::sqrt(squared);
         ++cycles;
     }
-    CIE_HIP_CHECK(hipEventRecord(stop, ctx->stream));
-    CIE_HIP_CHECK(hipEventSynchronize(stop));
+    CIE_HIP_CHECK(hipEventRecord(events.event[1], ctx->stream));
+    CIE_HIP_CHECK(hipEventSynchronize(events.event[1]));
     float ms = 0.0f;
-    CIE_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
+    CIE_HIP_CHECK(hipEventElapsedTime(&ms, events.event[0], events.event[1]));
     if (info) {
         info->cycles = cycles;
         info->coarseIterations = coarseIterations;
