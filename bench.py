@@ -33,14 +33,30 @@ HBM_PEAK_GBPS = 8000.0
 
 
 def spmv_bytes(rows, nnz, value_bytes, index_bytes=4):
-    """Algorithmic bytes of one SpMV y = A x: entries + columns, row extents, x read once, y written once.
-    index_bytes is 4 for int32 columns, 2 when the tiled kernel reads the narrow 16-bit column plane."""
+    """Algorithmic bytes of one SpMV y = A x over int32 CSR: entries + columns, row extents, x read once,
+    y written once (index_bytes 4). With index_bytes 2: the same over the narrow 16-bit column plane."""
     return (value_bytes + index_bytes) * nnz + 4 * (rows + 1) + 2 * value_bytes * rows
 
 
 def index_bytes(matrix, precision):
     """Bytes per column index the selected SpMV launch reads."""
-    return matrix.column_bytes() if matrix.launch(precision)[0] == "tiled" else 4
+    kernel, width, _ = matrix.launch(precision)
+    if kernel == "tiled":
+        return matrix.column_bytes()
+    if kernel == "blocks":
+        return matrix.column_blocks(width)[1]
+    return 4
+
+
+def format_bytes(matrix, precision):
+    """Bytes of the format the selected SpMV launch reads: per entry (CSR) or per column block
+    (w values + one column each), plus row extents, x once and y once."""
+    value = 8 if precision == 64 else 4
+    kernel, width, _ = matrix.launch(precision)
+    if kernel == "blocks":
+        blocks, column = matrix.column_blocks(width)
+        return (width * value + column) * blocks + 4 * (matrix.rows + 1) + 2 * value * matrix.rows
+    return spmv_bytes(matrix.rows, matrix.nnz, value, index_bytes(matrix, precision))
 
 
 def time_spmv(torch, linalg, matrix, precision, repetitions=50):
@@ -90,16 +106,18 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=5)
     parser.add_argument("--warmup", type=int, default=1)
-    parser.add_argument("--elements", type=int, default=64, help="hexahedra per dimension (64 = the benchmark)")
+    parser.add_argument("--elements", type=int, default=None,
+                        help="hexahedra per dimension (default 64 for pcg, 32 for mixed-q4: the configurations)")
     parser.add_argument("--tolerance", type=float, default=1e-10)
     parser.add_argument("--cpu-iterations", type=int, default=20, help="CPU baseline sample (0 disables)")
     parser.add_argument("--cpu-threads", type=int, default=0)
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
-    parser.add_argument("--workload", choices=("pcg", "batch", "adjoint", "assembly", "reorder", "pmg"),
+    parser.add_argument("--workload", choices=("pcg", "mixed-q4", "batch", "adjoint", "assembly", "reorder", "pmg"),
                         default="pcg",
-                        help="pcg: the headline solve; batch / adjoint: the added forms; assembly / reorder / pmg: "
+                        help="pcg: the headline solve (3D Q2 Laplace); mixed-q4: the same solves on 3D Q4 with "
+                             "lognormal conductivity; batch / adjoint: the added forms; assembly / reorder / pmg: "
                              "the widened paths; one JSON line each")
     args = parser.parse_args()
 
@@ -108,7 +126,7 @@ def main():
               "(the solve path is single-device; see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload != "pcg":
+    if args.workload not in ("pcg", "mixed-q4"):
         {"batch": bench_batch, "adjoint": bench_adjoint, "assembly": bench_assembly, "reorder": bench_reorder,
          "pmg": bench_pmg}[args.workload](args)
         return
@@ -118,8 +136,18 @@ def main():
     from cie_amd import linalg, systems
     from cie_amd._lib import Statistics
 
+    q4 = args.workload == "mixed-q4"
+    if args.elements is None:
+        args.elements = 32 if q4 else 64
     t0 = time.perf_counter()
-    workload = systems.q2_laplace_3d(args.elements, tolerance=args.tolerance)
+    if q4:
+        workload = systems.q4_lognormal_3d(args.elements, tolerance=args.tolerance)
+        description = (f"3D Q4 Laplace, {args.elements}^3 hexahedra, k_e = exp(g_e), g_e ~ N(0,1) seed 2, "
+                       f"Dirichlet eliminated, Jacobi-PCG to relative residual {args.tolerance:g}")
+    else:
+        workload = systems.q2_laplace_3d(args.elements, tolerance=args.tolerance)
+        description = (f"3D Q2 Laplace, {args.elements}^3 hexahedra, Dirichlet eliminated, Jacobi-PCG to "
+                       f"relative residual {args.tolerance:g}")
     generate_seconds = time.perf_counter() - t0
     csr = workload.matrix
     rows, nnz = csr.rows, csr.nnz
@@ -130,8 +158,8 @@ def main():
     t0 = time.perf_counter()
     matrix = linalg.CSRMatrix(csr, context)
     upload_seconds = time.perf_counter() - t0
-    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled"))
-    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled"))
+    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks"))
+    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks"))
     b = torch.from_numpy(workload.rhs).to("cuda:0")
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, args.tolerance)
@@ -181,18 +209,18 @@ def main():
 
     spmv64 = time_spmv(torch, linalg, matrix, 64)
     spmv32 = time_spmv(torch, linalg, matrix, 32)
-    # Bytes of the format each launch reads (narrow column plane: 2 B per index), and of plain
-    # int32 CSR (the effective rate an int32-index kernel would need for the same time).
-    bytes64 = spmv_bytes(rows, nnz, 8, index_bytes(matrix, 64))
-    bytes32 = spmv_bytes(rows, nnz, 4, index_bytes(matrix, 32))
+    # Algorithmic bytes (int32 CSR: the roofline's measure) and the bytes of the format each launch
+    # actually reads (narrow 16-bit columns, or column blocks with their padding).
     csr64 = spmv_bytes(rows, nnz, 8)
     csr32 = spmv_bytes(rows, nnz, 4)
+    bytes64 = format_bytes(matrix, 64)
+    bytes32 = format_bytes(matrix, 32)
 
     m = mixed_infos[-1]
     f = fp64_infos[-1]
     product_seconds = statistics.mean(i.productSeconds for i in mixed_infos if i.productSamples) \
         if any(i.productSamples for i in mixed_infos) else spmv32
-    achieved = bytes32 / product_seconds / 1e9
+    achieved = csr32 / product_seconds / 1e9
 
     result = {
         "metric": "time-to-1e-10 residual (s) + SpMV HBM GB/s, FP64 vs mixed-refined PCG, 1 GPU",
@@ -206,10 +234,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 inner / f64 refinement",
-        "data": "synthetic (structured Q2 stiffness, SplitMix64 rhs)",
+        "data": f"synthetic (structured {'Q4' if q4 else 'Q2'} stiffness, SplitMix64 rhs)",
         "config": {
-            "workload": f"3D Q2 Laplace, {args.elements}^3 hexahedra, Dirichlet eliminated, Jacobi-PCG to "
-                        f"relative residual {args.tolerance:g}",
+            "workload": description,
             "rows": rows, "nnz": nnz, "rhs_seed": 1, "x0": "zero",
             "parallelism": "single GPU",
         },
@@ -225,19 +252,21 @@ def main():
         },
         "speedup_mixed_vs_fp64": fp64_wall / mixed_wall,
         "spmv": {
-            "fp64_seconds": spmv64, "fp64_GBps": bytes64 / spmv64 / 1e9, "fp64_bytes": bytes64,
-            "fp64_int32_csr_GBps": csr64 / spmv64 / 1e9,
-            "fp32_seconds": spmv32, "fp32_GBps": bytes32 / spmv32 / 1e9, "fp32_bytes": bytes32,
-            "fp32_int32_csr_GBps": csr32 / spmv32 / 1e9,
+            "fp64_seconds": spmv64, "fp64_GBps": csr64 / spmv64 / 1e9, "fp64_bytes": csr64,
+            "fp64_format_bytes": bytes64, "fp64_format_GBps": bytes64 / spmv64 / 1e9,
+            "fp32_seconds": spmv32, "fp32_GBps": csr32 / spmv32 / 1e9, "fp32_bytes": csr32,
+            "fp32_format_bytes": bytes32, "fp32_format_GBps": bytes32 / spmv32 / 1e9,
             "launch_fp64": list(matrix.launch(64)), "launch_fp32": list(matrix.launch(32)),
             "index_bytes_fp64": index_bytes(matrix, 64), "index_bytes_fp32": index_bytes(matrix, 32),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(matrix.launch(32)),
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": measured_traffic(matrix.launch(32), f"spmv_{'q4' if q4 else 'q2'}_{args.elements}_traffic.json"),
             "kernel": f"productKernel {list(matrix.launch(32))} (q = A p, FP32 plane, fused p.q) inside the mixed solve",
-            "bytes_per_launch": bytes32, "mean_launch_seconds": product_seconds,
-            "int32_csr_bytes_per_launch": csr32, "int32_csr_equivalent_GBps": csr32 / product_seconds / 1e9,
+            "bytes_per_launch": csr32, "bytes_rule": "int32 CSR: (4 + 4) nnz + 4 (N + 1) + 2 * 4 N",
+            "mean_launch_seconds": product_seconds,
+            "format_bytes_per_launch": bytes32, "format_GBps": bytes32 / product_seconds / 1e9,
         },
     }
     if args.cpu_iterations > 0:
@@ -351,7 +380,7 @@ def bench_batch(args):
     _, _, _, profiled = solve(graph=False)
     linalg.library().cie_hip_context_set_profiling(context.handle, 0)
     assert iterations.min() >= 8 and profiled.productSamples > 0
-    algorithmic = spmv_bytes(rows, nnz, 4, index_bytes(matrix, 32))
+    algorithmic = spmv_bytes(rows, nnz, 4)
     # CPU baseline: the reference solves each system with its own ConjugateGradients; the oracle's FP64 PCG on
     # systems in order until the sample time is spent, scaled by nonzeros x FP64 iterations.
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -376,7 +405,7 @@ def bench_batch(args):
           {"workload": "256 2D Q2 Laplace systems, n_i ~ U{16..256} (seed 3), k_e lognormal(0, 0.5), b ~ U(-1,1), "
                        "relative residual 1e-10 each", "rows": rows, "nnz": nnz, "systems": 256},
           _roofline("batchProductKernel<float> (all systems active)", algorithmic, profiled.productSeconds,
-                    "B_SpMV(FP32 plane, format index bytes) over the whole block-diagonal matrix"),
+                    "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix"),
           {"value": sampled_seconds * total_work / sampled_work, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG (ConjugateGradients_impl.hpp:49-201) on the first {sampled} systems "
                      f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
@@ -398,6 +427,7 @@ def bench_adjoint(args):
     g, gt = w.extra["log_conductivity"], w.extra["target_log_conductivity"]
     problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
     problem.assemble(gt)
+    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks"))
     ut, _ = problem.solve(problem.rhs)
     problem.target = ut
     context = problem.context
@@ -422,7 +452,7 @@ def bench_adjoint(args):
     linalg.library().cie_hip_context_set_profiling(context.handle, 0)
     lam, info_adjoint = problem.solve(u - problem.target)
     rows, nnz = w.matrix.rows, w.matrix.nnz
-    algorithmic = spmv_bytes(rows, nnz, 8, index_bytes(problem.matrix, 64))
+    algorithmic = spmv_bytes(rows, nnz, 8)
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     sample = 20
     _, _, _, secs = oracle.pcg(w.matrix, w.rhs, relative_tolerance=0.0, max_iterations=sample, threads=threads)
@@ -434,8 +464,8 @@ def bench_adjoint(args):
           "s", "f64",
           {"workload": f"3D Q2 Laplace {n}^3 hexahedra, k_e = exp(g_e), g ~ N(0, 0.5) seed 4, target from seed 5, "
                        f"b ~ U(-1,1) seed 1", "rows": rows, "nnz": nnz, "elements": int(problem.count)},
-          _roofline("productKernel<BlockTiles<double, double, ...>> (q = A p) inside the forward solve", algorithmic,
-                    info.productSeconds, "B_SpMV(FP64, format index bytes)"),
+          _roofline(f"productKernel {list(problem.matrix.launch(64))} (q = A p) inside the forward solve", algorithmic,
+                    info.productSeconds, "B_SpMV(FP64, int32 CSR)"),
           {"value": secs * total_iterations / sample + gather, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG, {sample} iterations in {secs:.2f} s, scaled to the "
                      f"{total_iterations} forward + adjoint iterations, plus the oracle's element gradient "
@@ -452,7 +482,7 @@ def bench_assembly(args):
     import oracle
     from cie_amd import linalg, systems
 
-    n = args.elements
+    n = args.elements or 64
     count = systems.lagrange_size(3, 2, n)[2]
     k = np.exp(systems.normal(2, 1.0, count))
     pattern = systems.lagrange(3, 2, n, threads=0)
@@ -493,7 +523,7 @@ def bench_reorder(args):
     import oracle
     from cie_amd import linalg, systems
 
-    n = args.elements
+    n = args.elements or 64
     natural = systems.lagrange(3, 2, n, threads=0)
     rows, nnz = natural.rows, natural.nnz
     shuffle = np.random.default_rng(11).permutation(rows).astype(np.int32)
@@ -582,8 +612,7 @@ def bench_pmg(args):
     one = timed(lambda: jacobi[1].product(0, b_dev, 1, y), 5)
     seven = timed(lambda: jacobi[7].product(0, b_dev, 1, y), 5)
     sweep = (seven - one) / 6
-    index = 2 if matrix.column_bytes() == 2 else 4
-    algorithmic = spmv_bytes(rows, nnz, 8, index) + 2 * 8 * rows
+    algorithmic = spmv_bytes(rows, nnz, 8) + 2 * 8 * rows
     solver = linalg.FusedPCG(matrix, Statistics(100000, 0.0, 1e-6))
     xp = torch.zeros_like(b_dev)
     solver.product(0, b_dev, 1, xp)
@@ -596,7 +625,7 @@ def bench_pmg(args):
           {"workload": f"2D Q{order} integrated Legendre Laplace, {n}^2 elements, Dirichlet eliminated, "
                        f"b ~ U(-1,1) seed 1, x0 = 0", "rows": rows, "nnz": nnz},
           _roofline("jacobiSweepKernel<BlockTiles<double, double, 8, 256>> on the finest grid", algorithmic, sweep,
-                    "B_SpMV(FP64, format index bytes) + 2 x 8 N (right-hand side and inverse diagonal)"),
+                    "B_SpMV(FP64, int32 CSR) + 2 x 8 N (right-hand side and inverse diagonal)"),
           {"value": cpu_sample * info.cycles / sample_cycles, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle_pmg (solver.cpp:111-232 restated) for {sample_cycles} cycles in {cpu_sample:.2f} s, "
                      f"scaled to the {info.cycles} cycles of the solve"},
@@ -607,15 +636,18 @@ def bench_pmg(args):
                                   "seconds": solver.info.seconds}})
 
 
-def measured_traffic(launch):
+def measured_traffic(launch, profile):
     """HBM bytes per launch of the in-solve FP32 product kernel for this launch shape, from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy), else None."""
-    path = ROOT / "profiles" / "spmv_q2_64_traffic.json"
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (`profile`, calibrated on a
+    1 GiB copy), else None."""
+    path = ROOT / "profiles" / profile
     if not path.exists():
         return None
     kernel, width, unroll = launch
     if kernel == "tiled":
         walker = f"BlockTiles<float, float, {unroll}, {width}> >"
+    elif kernel == "blocks":
+        walker = f"ColumnBlockTiles<float, float, {unroll}, 256, {width}> >"
     else:
         walker = f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<float, float, {width}, {unroll}, 256> >"
     kernels = json.loads(path.read_text()).get("kernels", {})

@@ -31,6 +31,7 @@ DEVICE = 1
 SPMV_VECTOR = 0
 SPMV_STAGED = 1
 SPMV_TILED = 2
+SPMV_BLOCKS = 3
 REORDER_NONE, REORDER_CUTHILL_MCKEE, REORDER_REVERSE_CUTHILL_MCKEE = 0, 1, 2
 PLANE_F64 = 1
 PLANE_F32 = 2
@@ -106,7 +107,12 @@ class ShapeError(CieHipError):
     pass
 
 
+class InvalidArgument(CieHipError):
+    pass
+
+
 _ERRORS = {
+    ERROR_INVALID_ARGUMENT: InvalidArgument,
     ERROR_SHAPE: ShapeError,
     ERROR_NOT_SPD: NotPositiveDefinite,
     ERROR_BREAKDOWN: Breakdown,
@@ -136,6 +142,7 @@ _SIGNATURES = {
     "cie_hip_csr_shape": (c_int, [_P, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
     "cie_hip_csr_update_entries": (c_int, [_P, _P]),
     "cie_hip_csr_column_bytes": (c_int, [_P, POINTER(c_int)]),
+    "cie_hip_csr_column_blocks": (c_int, [_P, c_int, POINTER(c_int64), POINTER(c_int)]),
     "cie_hip_csr_set_launch": (c_int, [_P, c_int, c_int, c_int, c_int]),
     "cie_hip_csr_get_launch": (c_int, [_P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "cie_hip_csr_autotune": (c_int, [_P, c_int, c_int, c_int, POINTER(c_double)]),

@@ -106,10 +106,7 @@ int cie_hip_csr_add_element_matrices(cie_hip_csr matrix, int64_t count, int32_t 
                        slots, elementMatrix, factors, matrix->entries64, matrix->entries32);
     CIE_HIP_CHECK(hipGetLastError());
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    // The Jacobi diagonal depends on the entries: rebuild it on next use.
-    if (matrix->inverseDiagonal64) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal64)); matrix->inverseDiagonal64 = nullptr; }
-    if (matrix->inverseDiagonal32) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal32)); matrix->inverseDiagonal32 = nullptr; }
-    return CIE_HIP_OK;
+    return entriesChanged(matrix);  // the Jacobi diagonal and the column-block values are rebuilt on next use
 }
 
 }  // extern "C"

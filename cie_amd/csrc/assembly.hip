@@ -418,10 +418,7 @@ int cie_hip_assemble(cie_hip_assembly plan, const double* elementMatrices, int p
         CIE_HIP_CHECK(hipGetLastError());
     }
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    // The Jacobi diagonal belongs to the old values.
-    if (m->inverseDiagonal64) { CIE_HIP_CHECK(hipFree(m->inverseDiagonal64)); m->inverseDiagonal64 = nullptr; }
-    if (m->inverseDiagonal32) { CIE_HIP_CHECK(hipFree(m->inverseDiagonal32)); m->inverseDiagonal32 = nullptr; }
-    return CIE_HIP_OK;
+    return entriesChanged(m);  // the Jacobi diagonal and the column-block values belong to the old values
 }
 
 int cie_hip_element_stiffness(cie_hip_context context, int64_t elements, int32_t dim, int32_t nodesPerElement,

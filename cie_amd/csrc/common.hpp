@@ -157,8 +157,9 @@ struct KernelTimer {
 // Launch configuration of a CSR SpMV for one value precision. The grid is
 // sized per launch from the kernel's occupancy (one resident round).
 struct CsrLaunch {
-    int kernel = 0;   // CIE_HIP_SPMV_VECTOR, CIE_HIP_SPMV_STAGED or CIE_HIP_SPMV_TILED
-    int width = 0;    // vector: lanes per row; staged: LDS entries per segment; tiled: threads per block
+    int kernel = 0;   // CIE_HIP_SPMV_VECTOR, CIE_HIP_SPMV_STAGED, CIE_HIP_SPMV_TILED or CIE_HIP_SPMV_BLOCKS
+    int width = 0;    // vector: lanes per row; staged: LDS entries per segment; tiled: threads per block;
+                      // blocks: columns per block
     int unroll = 0;   // entries per lane loaded before they are used
     int block = 0;    // threads per block
 };
@@ -191,6 +192,15 @@ struct cie_hip_csr_s {
     int narrowRuns = 0;
     uint16_t* narrowColumns = nullptr;
     int32_t* narrowBases = nullptr;
+    uint64_t valuesVersion = 0;        // bumped whenever the entries change
+    // The matrix as 1 x w blocks of adjacent columns, per block width w (built
+    // on first use): itself a matrix whose "entries" are the blocks (extents,
+    // first columns, tiles and narrow plane as above) and whose value planes
+    // hold w values per block, zero where the row has no entry.
+    static constexpr int kMaxBlockWidth = 4;
+    cie_hip_csr_s* columnBlocks[kMaxBlockWidth + 1] = {};
+    int blockWidth = 0;                // of a block matrix; 0 for an ordinary one
+    uint64_t filledVersion = 0;        // of a block matrix: the parent's valuesVersion its values hold
 };
 
 namespace ciehip {
@@ -203,6 +213,12 @@ int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.
 int ensureNarrowColumns(cie_hip_csr matrix);
+// The 1 x width column-block form of `matrix` with values current (built on
+// first use, refilled after the entries changed).
+int ensureColumnBlocks(cie_hip_csr matrix, int width, cie_hip_csr* blocks);
+// After the entries changed: derived value planes (the Jacobi inverse
+// diagonals, the column-block values) are rebuilt before their next use.
+int entriesChanged(cie_hip_csr matrix);
 // A library-owned matrix over device arrays allocated by the caller of this
 // function (ownership passes to the matrix; entries32 may be null).
 int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t* rowExtents, int32_t* columnIndices,

@@ -105,19 +105,19 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
 
 int cie_hip_context_destroy(cie_hip_context context) {
     if (!context) return CIE_HIP_OK;
-    hipSetDevice(context->device);
-    if (context->stream) hipStreamSynchronize(context->stream);
+    (void)hipSetDevice(context->device);
+    if (context->stream) (void)hipStreamSynchronize(context->stream);
     auto& s = context->scratch;
-    if (s.partials) hipFree(s.partials);
-    if (s.level) hipFree(s.level);
-    if (s.vectorPartials) hipFree(s.vectorPartials);
-    if (s.scalars) hipFree(s.scalars);
-    if (s.state) hipFree(s.state);
-    if (s.pinned) hipHostFree(s.pinned);
-    if (s.pinnedState) hipHostFree(s.pinnedState);
+    if (s.partials) (void)hipFree(s.partials);
+    if (s.level) (void)hipFree(s.level);
+    if (s.vectorPartials) (void)hipFree(s.vectorPartials);
+    if (s.scalars) (void)hipFree(s.scalars);
+    if (s.state) (void)hipFree(s.state);
+    if (s.pinned) (void)hipHostFree(s.pinned);
+    if (s.pinnedState) (void)hipHostFree(s.pinnedState);
     for (hipEvent_t& event : context->profileEvents)
         if (event) (void)hipEventDestroy(event);
-    if (context->ownsStream && context->stream) hipStreamDestroy(context->stream);
+    if (context->ownsStream && context->stream) (void)hipStreamDestroy(context->stream);
     delete context;
     return CIE_HIP_OK;
 }

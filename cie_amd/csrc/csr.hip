@@ -55,6 +55,71 @@ __global__ __launch_bounds__(kReduceBlock) void narrowColumnsKernel(int64_t nnz,
         narrow[k] = uint16_t(columns[k] - bases[k >> kNarrowShift]);
 }
 
+// ---- column-block form ------------------------------------------------------
+// A row's entries are covered by 1 x width blocks: each block starts at the
+// column of the row's first entry not yet covered (or at columns - width, when
+// that would pass the last column) and covers [start, start + width). One
+// thread per row walks its entries; the count, structure and value passes
+// take the same walk, so they agree block for block.
+
+__device__ __forceinline__ int blockStart(int column, int width, int columns) {
+    return column + width > columns ? columns - width : column;
+}
+
+__global__ __launch_bounds__(kReduceBlock) void columnBlockCountKernel(int rows, const int32_t* __restrict__ extents,
+                                                                       const int32_t* __restrict__ columnIndices,
+                                                                       int width, int columns,
+                                                                       int32_t* __restrict__ counts) {
+    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
+        const int end = extents[row + 1];
+        int count = 0;
+        for (int k = extents[row]; k < end;) {
+            const int start = blockStart(columnIndices[k], width, columns);
+            ++count;
+            while (k < end && columnIndices[k] < start + width) ++k;
+        }
+        counts[row] = count;
+    }
+}
+
+__global__ __launch_bounds__(kReduceBlock) void columnBlockStartsKernel(int rows, const int32_t* __restrict__ extents,
+                                                                        const int32_t* __restrict__ columnIndices,
+                                                                        int width, int columns,
+                                                                        const int32_t* __restrict__ blockExtents,
+                                                                        int32_t* __restrict__ blockColumns) {
+    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
+        const int end = extents[row + 1];
+        int b = blockExtents[row];
+        for (int k = extents[row]; k < end;) {
+            const int start = blockStart(columnIndices[k], width, columns);
+            blockColumns[b++] = start;
+            while (k < end && columnIndices[k] < start + width) ++k;
+        }
+    }
+}
+
+// The width values of every block: the row's entries at their columns, zero
+// elsewhere (a column listed twice in a row adds up).
+template <class T>
+__global__ __launch_bounds__(kReduceBlock) void columnBlockValuesKernel(
+    int rows, const int32_t* __restrict__ extents, const int32_t* __restrict__ columnIndices,
+    const T* __restrict__ entries, int width, const int32_t* __restrict__ blockExtents,
+    const int32_t* __restrict__ blockColumns, T* __restrict__ values) {
+    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
+        const int end = extents[row + 1];
+        int k = extents[row];
+        for (int b = blockExtents[row]; b < blockExtents[row + 1]; ++b) {
+            const int start = blockColumns[b];
+            T* v = values + int64_t(b) * width;
+            for (int i = 0; i < width; ++i) v[i] = T(0);
+            while (k < end && columnIndices[k] < start + width) {
+                v[columnIndices[k] - start] += entries[k];
+                ++k;
+            }
+        }
+    }
+}
+
 template <class T>
 int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
                   const T* entries) {
@@ -79,22 +144,23 @@ CsrLaunch& launchOf(cie_hip_csr m, int precision) { return precision == 32 ? m->
 void releaseMatrix(cie_hip_csr m) {
     if (!m) return;
     if (m->context) {
-        hipSetDevice(m->context->device);
-        hipStreamSynchronize(m->context->stream);
+        (void)hipSetDevice(m->context->device);
+        (void)hipStreamSynchronize(m->context->stream);
     }
     if (m->ownsIndices) {
-        hipFree(m->rowExtents);
-        hipFree(m->columnIndices);
+        (void)hipFree(m->rowExtents);
+        (void)hipFree(m->columnIndices);
     }
-    if (m->ownsEntries64) hipFree(m->entries64);
-    if (m->entries32) hipFree(m->entries32);
-    if (m->inverseDiagonal64) hipFree(m->inverseDiagonal64);
-    if (m->inverseDiagonal32) hipFree(m->inverseDiagonal32);
-    if (m->workspace) hipFree(m->workspace);
+    if (m->ownsEntries64) (void)hipFree(m->entries64);
+    if (m->entries32) (void)hipFree(m->entries32);
+    if (m->inverseDiagonal64) (void)hipFree(m->inverseDiagonal64);
+    if (m->inverseDiagonal32) (void)hipFree(m->inverseDiagonal32);
+    if (m->workspace) (void)hipFree(m->workspace);
     for (TileSet& set : m->tileSets)
-        if (set.tiles) hipFree(set.tiles);
-    if (m->narrowColumns) hipFree(m->narrowColumns);
-    if (m->narrowBases) hipFree(m->narrowBases);
+        if (set.tiles) (void)hipFree(set.tiles);
+    if (m->narrowColumns) (void)hipFree(m->narrowColumns);
+    if (m->narrowBases) (void)hipFree(m->narrowBases);
+    for (cie_hip_csr blocks : m->columnBlocks) releaseMatrix(blocks);
     delete m;
 }
 
@@ -273,7 +339,7 @@ int ensureInverseDiagonal(cie_hip_csr m, int precision) {
         CIE_HIP_CHECK(hipMalloc(&inverse, sizeof(float) * std::max<int64_t>(1, m->rows)));
         const int status = inverseDiagonal<float>(m, inverse, nullptr);
         if (status != CIE_HIP_OK) {
-            hipFree(inverse);
+            (void)hipFree(inverse);
             return status;
         }
         m->inverseDiagonal32 = inverse;
@@ -284,7 +350,7 @@ int ensureInverseDiagonal(cie_hip_csr m, int precision) {
     CIE_HIP_CHECK(hipMalloc(&inverse, sizeof(double) * std::max<int64_t>(1, m->rows)));
     const int status = inverseDiagonal<double>(m, inverse, nullptr);
     if (status != CIE_HIP_OK) {
-        hipFree(inverse);
+        (void)hipFree(inverse);
         return status;
     }
     m->inverseDiagonal64 = inverse;
@@ -308,6 +374,119 @@ int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t*
     m->launch64 = m->launch32 = defaultLaunch();
     *matrix = m;
     return CIE_HIP_OK;
+}
+
+int entriesChanged(cie_hip_csr m) {
+    ++m->valuesVersion;
+    if (m->inverseDiagonal64) {
+        CIE_HIP_CHECK(hipFree(m->inverseDiagonal64));
+        m->inverseDiagonal64 = nullptr;
+    }
+    if (m->inverseDiagonal32) {
+        CIE_HIP_CHECK(hipFree(m->inverseDiagonal32));
+        m->inverseDiagonal32 = nullptr;
+    }
+    return CIE_HIP_OK;
+}
+
+int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
+    if (m->blockWidth) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "column blocks of a column-block form");
+    if (width < 2 || width > cie_hip_csr_s::kMaxBlockWidth)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "column blocks: width must be 2.." +
+                                                            std::to_string(cie_hip_csr_s::kMaxBlockWidth));
+    if (m->columns < width)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "column blocks: the matrix has fewer columns than the block width");
+    cie_hip_context ctx = m->context;
+    const int rows = int(m->rows), columns = int(m->columns);
+    cie_hip_csr blocks = m->columnBlocks[width];
+    if (!blocks) {
+        // Structure: blocks per row, their prefix on the host, then the first column of each block.
+        int32_t *counts = nullptr, *extents = nullptr, *starts = nullptr;
+        double* values64 = nullptr;
+        float* values32 = nullptr;
+        auto release = [&]() {
+            for (void* p : {static_cast<void*>(counts), static_cast<void*>(extents), static_cast<void*>(starts),
+                            static_cast<void*>(values64), static_cast<void*>(values32)})
+                if (p) (void)hipFree(p);
+        };
+        std::vector<int32_t> hostCounts(static_cast<size_t>(rows)), hostExtents(static_cast<size_t>(rows) + 1, 0);
+        bool ok = hipMalloc(&counts, sizeof(int32_t) * size_t(std::max(1, rows))) == hipSuccess &&
+                  hipMalloc(&extents, sizeof(int32_t) * (size_t(rows) + 1)) == hipSuccess;
+        if (ok && rows) {
+            hipLaunchKernelGGL(columnBlockCountKernel, dim3(vectorGrid(rows)), dim3(kReduceBlock), 0, ctx->stream, rows,
+                               m->rowExtents, m->columnIndices, width, columns, counts);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipMemcpyAsync(hostCounts.data(), counts, sizeof(int32_t) * size_t(rows), hipMemcpyDeviceToHost,
+                                ctx->stream) == hipSuccess &&
+                 hipStreamSynchronize(ctx->stream) == hipSuccess;
+        }
+        if (!ok) {
+            release();
+            return setError(CIE_HIP_ERROR_HIP, "column blocks: counting the blocks failed");
+        }
+        int64_t total = 0;
+        for (int row = 0; row < rows; ++row) {
+            total += hostCounts[size_t(row)];
+            hostExtents[size_t(row) + 1] = int32_t(total);  // total <= nnz <= INT32_MAX
+        }
+        const size_t count = size_t(std::max<int64_t>(1, total));
+        ok = hipMalloc(&starts, sizeof(int32_t) * count) == hipSuccess &&
+             hipMalloc(&values64, sizeof(double) * count * width) == hipSuccess &&
+             (!m->entries32 || hipMalloc(&values32, sizeof(float) * count * width) == hipSuccess) &&
+             hipMemcpyAsync(extents, hostExtents.data(), sizeof(int32_t) * (size_t(rows) + 1), hipMemcpyHostToDevice,
+                            ctx->stream) == hipSuccess;
+        if (ok && rows) {
+            hipLaunchKernelGGL(columnBlockStartsKernel, dim3(vectorGrid(rows)), dim3(kReduceBlock), 0, ctx->stream,
+                               rows, m->rowExtents, m->columnIndices, width, columns, extents, starts);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
+        (void)hipFree(counts);
+        counts = nullptr;
+        if (!ok) {
+            release();
+            return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "column blocks: building the block structure failed");
+        }
+        adoptMatrix(ctx, rows, columns, extents, starts, values64, values32, std::move(hostExtents), &blocks);
+        blocks->blockWidth = width;
+        blocks->filledVersion = std::numeric_limits<uint64_t>::max();  // values not written yet
+        m->columnBlocks[width] = blocks;
+    }
+    if (m->entries32 && !blocks->entries32) {  // an FP32 plane the block form does not mirror yet
+        CIE_HIP_CHECK(hipMalloc(&blocks->entries32, sizeof(float) * size_t(std::max<int64_t>(1, blocks->nnz)) * width));
+        blocks->filledVersion = std::numeric_limits<uint64_t>::max();
+    }
+    if (blocks->filledVersion != m->valuesVersion) {
+        if (rows) {
+            hipLaunchKernelGGL((columnBlockValuesKernel<double>), dim3(vectorGrid(rows)), dim3(kReduceBlock), 0,
+                               ctx->stream, rows, m->rowExtents, m->columnIndices,
+                               static_cast<const double*>(m->entries64), width, blocks->rowExtents,
+                               blocks->columnIndices, blocks->entries64);
+            CIE_HIP_CHECK(hipGetLastError());
+            if (blocks->entries32 && m->entries32) {
+                hipLaunchKernelGGL((columnBlockValuesKernel<float>), dim3(vectorGrid(rows)), dim3(kReduceBlock), 0,
+                                   ctx->stream, rows, m->rowExtents, m->columnIndices,
+                                   static_cast<const float*>(m->entries32), width, blocks->rowExtents,
+                                   blocks->columnIndices, blocks->entries32);
+                CIE_HIP_CHECK(hipGetLastError());
+            }
+        }
+        blocks->filledVersion = m->valuesVersion;
+    }
+    *out = blocks;
+    return CIE_HIP_OK;
+}
+
+// Frees the column-block forms no launch of the matrix uses.
+void releaseUnusedColumnBlocks(cie_hip_csr m) {
+    for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
+        const bool used = (m->launch64.kernel == CIE_HIP_SPMV_BLOCKS && m->launch64.width == width) ||
+                          (m->launch32.kernel == CIE_HIP_SPMV_BLOCKS && m->launch32.width == width);
+        if (!used && m->columnBlocks[width]) {
+            releaseMatrix(m->columnBlocks[width]);
+            m->columnBlocks[width] = nullptr;
+        }
+    }
 }
 
 }  // namespace ciehip
@@ -432,9 +611,7 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries) {
         CIE_HIP_CHECK(hipGetLastError());
     }
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (matrix->inverseDiagonal64) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal64)); matrix->inverseDiagonal64 = nullptr; }
-    if (matrix->inverseDiagonal32) { CIE_HIP_CHECK(hipFree(matrix->inverseDiagonal32)); matrix->inverseDiagonal32 = nullptr; }
-    return CIE_HIP_OK;
+    return entriesChanged(matrix);
 }
 
 int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll) {
@@ -449,6 +626,11 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     } else if (kernel == CIE_HIP_SPMV_TILED) {
         if (!validTiles(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: tiled width must be 256, unroll 4, 8 or 16");
+    } else if (kernel == CIE_HIP_SPMV_BLOCKS) {
+        if (!validBlocks(width, unroll))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: block width must be 2, 3 or 4, unroll 4, 8 or 16");
+        if (matrix->columns < width)
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: the matrix has fewer columns than the block width");
     } else {
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: unknown kernel");
     }
@@ -469,6 +651,19 @@ int cie_hip_csr_column_bytes(cie_hip_csr matrix, int* bytes) {
     return CIE_HIP_OK;
 }
 
+int cie_hip_csr_column_blocks(cie_hip_csr matrix, int width, int64_t* blocks, int* bytes) {
+    if (!matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_column_blocks: null matrix");
+    CIE_HIP_CHECK(hipSetDevice(matrix->context->device));
+    cie_hip_csr form = nullptr;
+    int status = ensureColumnBlocks(matrix, width, &form);
+    if (status != CIE_HIP_OK) return status;
+    status = ensureNarrowColumns(form);
+    if (status != CIE_HIP_OK) return status;
+    if (blocks) *blocks = form->nnz;
+    if (bytes) *bytes = form->narrowState > 0 ? int(sizeof(uint16_t)) : int(sizeof(int32_t));
+    return CIE_HIP_OK;
+}
+
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll) {
     if (!matrix || (precision != 32 && precision != 64))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_get_launch: invalid matrix or precision");
@@ -484,7 +679,8 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: invalid matrix or precision");
     if (precision == 32 && !matrix->entries32)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
-    if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED) | (1 << CIE_HIP_SPMV_TILED))))
+    if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED) | (1 << CIE_HIP_SPMV_TILED) |
+                     (1 << CIE_HIP_SPMV_BLOCKS))))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: empty kernel mask");
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
@@ -512,6 +708,23 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
             candidates.push_back(c);
         }
     }
+    if (kernels & (1 << CIE_HIP_SPMV_BLOCKS)) {
+        // Widths whose blocks hold at most 2 entries' worth of padding per 3 entries (fill <= 5/3).
+        for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth && width <= matrix->columns; ++width) {
+            cie_hip_csr blocks = nullptr;
+            const int status = ensureColumnBlocks(matrix, width, &blocks);
+            if (status != CIE_HIP_OK) return status;
+            if (3 * blocks->nnz * width > 5 * std::max<int64_t>(1, matrix->nnz)) continue;
+            for (int unroll : {4, 8, 16}) {
+                CsrLaunch c;
+                c.kernel = CIE_HIP_SPMV_BLOCKS;
+                c.width = width;
+                c.unroll = unroll;
+                c.block = kSpmvBlock;
+                candidates.push_back(c);
+            }
+        }
+    }
     if (kernels & (1 << CIE_HIP_SPMV_VECTOR)) {
         // Lanes within a factor 4 of about one pass of 4 entries per lane per row.
         const int seed = std::min(64, nextPow2(int(std::ceil(std::max(1.0, matrix->meanRowLength) / 4))));
@@ -525,8 +738,10 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
                 candidates.push_back(c);
             }
     }
-    if (candidates.empty())
+    if (candidates.empty()) {
+        releaseUnusedColumnBlocks(matrix);
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: no candidate for this matrix");
+    }
 
     const size_t valueSize = precision == 32 ? sizeof(float) : sizeof(double);
     void* buffer = nullptr;
@@ -569,8 +784,12 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
         if (status != CIE_HIP_OK) break;
     }
     (void)hipFree(buffer);
-    if (status != CIE_HIP_OK) return setError(status, std::string("cie_hip_csr_autotune: ") + cie_hip_last_error());
+    if (status != CIE_HIP_OK) {
+        releaseUnusedColumnBlocks(matrix);
+        return setError(status, std::string("cie_hip_csr_autotune: ") + cie_hip_last_error());
+    }
     launchOf(matrix, precision) = bestLaunch;
+    releaseUnusedColumnBlocks(matrix);
     if (bestSeconds) *bestSeconds = best;
     return CIE_HIP_OK;
 }

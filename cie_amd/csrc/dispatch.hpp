@@ -65,9 +65,32 @@ inline bool validTiles(int block, int unroll) {
     }
 }
 
+// Columns per block x blocks per lane in flight of the column-block kernel.
+#define CIEHIP_BLOCKS(X) X(2, 4) X(2, 8) X(2, 16) X(3, 4) X(3, 8) X(3, 16) X(4, 4) X(4, 8) X(4, 16)
+
+inline bool validBlocks(int width, int unroll) {
+    switch (width * 32 + unroll) {
+#define CIEHIP_BLOCKS_VALID(W, U) case W * 32 + U: return true;
+        CIEHIP_BLOCKS(CIEHIP_BLOCKS_VALID)
+#undef CIEHIP_BLOCKS_VALID
+    default: return false;
+    }
+}
+
 // Calls f(Walker{}) with the row walker the launch selects; false if unsupported.
 template <class TV, class TM, class F>
 bool withWalker(const CsrLaunch& launch, F&& f) {
+    if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {
+        switch (launch.width * 32 + launch.unroll) {
+#define CIEHIP_BLOCKS_CASE(W, U)                                  \
+    case W * 32 + U:                                              \
+        f(ColumnBlockTiles<TV, TM, U, kSpmvBlock, W>{});          \
+        return true;
+            CIEHIP_BLOCKS(CIEHIP_BLOCKS_CASE)
+#undef CIEHIP_BLOCKS_CASE
+        default: return false;
+        }
+    }
     if (launch.kernel == CIE_HIP_SPMV_TILED) {
         if (launch.width != kSpmvBlock) return false;
         switch (launch.unroll) {
@@ -131,8 +154,32 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     source.narrowRuns = 0;
     source.narrowColumns = nullptr;
     source.narrowBases = nullptr;
+    source.blockValues64 = nullptr;
+    source.blockValues32 = nullptr;
     const int64_t rows = m->rows;
-    if (launch.kernel == CIE_HIP_SPMV_TILED) {
+    if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {  // the tiled walker over the column-block form
+        cie_hip_csr blocks = nullptr;
+        int status = ensureColumnBlocks(m, launch.width, &blocks);
+        if (status != CIE_HIP_OK) return status;
+        const TileSet* set = nullptr;
+        status = ensureTiles(blocks, launch.block * launch.unroll, &set);
+        if (status != CIE_HIP_OK) return status;
+        status = ensureNarrowColumns(blocks);
+        if (status != CIE_HIP_OK) return status;
+        source.rowExtents = blocks->rowExtents;
+        source.columnIndices = blocks->columnIndices;
+        source.tiles = set->tiles;
+        source.tileCount = set->count;
+        if (blocks->narrowState > 0) {
+            source.narrowRuns = blocks->narrowRuns;
+            source.narrowColumns = blocks->narrowColumns;
+            source.narrowBases = blocks->narrowBases;
+        }
+        source.blockValues64 = blocks->entries64;
+        source.blockValues32 = blocks->entries32;
+        source.param = 1;
+        grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
+    } else if (launch.kernel == CIE_HIP_SPMV_TILED) {
         const TileSet* set = nullptr;
         const int status = ensureTiles(m, launch.block * launch.unroll, &set);
         if (status != CIE_HIP_OK) return status;

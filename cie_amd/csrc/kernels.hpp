@@ -145,6 +145,8 @@ struct RowSource {
     int narrowRuns;                            // runs of the narrow column plane (0: none)
     const uint16_t* __restrict__ narrowColumns;  // column - narrowBases[entry >> kNarrowShift]
     const int32_t* __restrict__ narrowBases;     // smallest column of each run of kNarrowRun entries
+    const void* blockValues64;                 // column-block walker: w values per block (FP64, FP32)
+    const void* blockValues32;
 };
 
 // L lanes per row, BS / L rows per block step; rows split into one contiguous
@@ -428,6 +430,172 @@ struct BlockTiles {
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (base + tid + BS * u < end) part += TV(entry[u]) * gathered[u];
+            part = groupSum<kWave>(part);
+            if ((tid & (kWave - 1)) == 0) stage[tid / kWave] = part;
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int w = 0; w < BS / kWave; ++w) total += stage[w];
+            }
+            __syncthreads();
+        }
+        return total;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Column-block tiles: the block-tile walker over the matrix's column-block
+// form, 1 x W blocks of adjacent columns (W values each, zero where the row
+// has no entry). A lane's unit is a block: its W values arrive in one load
+// and its W x values in one gather, so a row whose entries come in runs of
+// adjacent columns (FEM rows in node order) costs about a third of the
+// gathers at W = 3; the gather instructions, not the bytes, bound the tiled
+// product in FP32. A block's products are added in column order, a zero value
+// adding nothing (so padding never turns an infinite x into NaN); then each
+// row sums its blocks as BlockTiles sums entries. The `entries` argument (the
+// CSR plane) is not read: the block values come with the row source.
+// ---------------------------------------------------------------------------
+
+template <class T, int W>
+struct ValueRun {  // W consecutive values; loads of it need only T's alignment
+    T v[W];
+};
+
+template <class TV, class TM, int U, int BS, int W>
+struct ColumnBlockTiles {
+    using Value = TV;
+    using Matrix = TM;
+    static constexpr int kBlock = BS;
+    static constexpr int kCapacity = BS * U;  // blocks per tile
+    static_assert(kCapacity <= kNarrowRun, "a tile must touch at most two narrow runs");
+
+    __device__ __forceinline__ static int slot() { return blockIdx.x; }
+
+    __device__ __forceinline__ static const TM* values(const RowSource& source) {
+        if constexpr (sizeof(TM) == sizeof(double)) return static_cast<const TM*>(source.blockValues64);
+        else return static_cast<const TM*>(source.blockValues32);
+    }
+
+    __device__ __forceinline__ static ValueRun<TM, W> loadBlock(const TM* __restrict__ values, int k) {
+        ValueRun<TM, W> run;
+        __builtin_memcpy(&run, values + int64_t(k) * W, sizeof(run));
+        return run;
+    }
+
+    __device__ __forceinline__ static ValueRun<TV, W> gather(const TV* __restrict__ in, int column) {
+        ValueRun<TV, W> run;
+        __builtin_memcpy(&run, in + column, sizeof(run));
+        return run;
+    }
+
+    __device__ __forceinline__ static TV blockProduct(const ValueRun<TM, W>& a, const ValueRun<TV, W>& x) {
+        TV sum = TV(0);
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (a.v[i] != TM(0)) sum += TV(a.v[i]) * x.v[i];
+        return sum;
+    }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void run(const RowSource& source, const TM* __restrict__,
+                                               const TV* __restrict__ in, Epilogue&& epilogue) {
+        __shared__ TV stage[kCapacity];
+        __shared__ int32_t extents[BS + 1];
+        for (int t = blockIdx.x; t < source.tileCount; t += gridDim.x) {
+            tile(t, source, in, epilogue, stage, extents);
+            if (t + int(gridDim.x) < source.tileCount) __syncthreads();  // stage and extents are reused
+        }
+    }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void tile(int t, const RowSource& source, const TV* __restrict__ in,
+                                                Epilogue& epilogue, TV* stage, int32_t* extents) {
+        const TM* __restrict__ blockValues = values(source);
+        const int tid = threadIdx.x;
+        const int2 head = source.tiles[t];
+        const int2 next = source.tiles[t + 1];
+        const int first = head.x, rows = next.x - head.x, begin = head.y, end = next.y;
+        if (end - begin > kCapacity) {
+            const TV sum = longRow(begin, end, source.columnIndices, blockValues, in, stage);
+            if (tid == 0) epilogue(first, sum);
+            return;
+        }
+        if (end == begin) {  // only empty rows: nothing to gather
+            if (tid < rows) epilogue(first + tid, TV(0));
+            return;
+        }
+        // Unconditional loads, indices clamped into the tile, masked afterwards (see BlockTiles).
+        int column[U];
+        ValueRun<TM, W> block[U];
+        const int last = end - 1;
+        if (source.narrowRuns) {
+            const int run = begin >> kNarrowShift;
+            const int base0 = source.narrowBases[run];
+            const int base1 = source.narrowBases[min(run + 1, source.narrowRuns - 1)];
+            uint32_t offset[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(begin + tid + BS * u, last);
+                offset[u] = CIEHIP_STREAM_LOAD(source.narrowColumns + k);
+                block[u] = loadBlock(blockValues, k);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = begin + tid + BS * u;
+                column[u] = ((min(k, last) >> kNarrowShift) == run ? base0 : base1) + int(offset[u]);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(begin + tid + BS * u, last);
+                column[u] = CIEHIP_STREAM_LOAD(source.columnIndices + k);
+                block[u] = loadBlock(blockValues, k);
+            }
+        }
+        if (tid < rows) extents[tid] = source.rowExtents[first + tid] - begin;
+        if (tid == 0) extents[rows] = end - begin;
+        ValueRun<TV, W> gathered[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) gathered[u] = gather(in, column[u]);  // clamped lanes re-read a real block
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (begin + tid + BS * u < end) stage[tid + BS * u] = blockProduct(block[u], gathered[u]);
+        __syncthreads();
+        int group = kWave;
+        while (group > 1 && rows * group > BS) group >>= 1;
+        const int row = tid / group;
+        const int g = tid % group;
+        TV sum = TV(0);
+        if (row < rows) {
+            const int rowEnd = extents[row + 1];
+            for (int k = extents[row] + g; k < rowEnd; k += group) sum += stage[k];
+        }
+        for (int offset = group / 2; offset > 0; offset /= 2) sum += __shfl_xor(sum, offset, kWave);
+        if (row < rows && g == 0) epilogue(first + row, sum);
+    }
+
+    // Sum of one row of more than kCapacity blocks, valid in thread 0 (as BlockTiles::longRow).
+    __device__ __forceinline__ static TV longRow(int begin, int end, const int32_t* __restrict__ columnIndices,
+                                                 const TM* __restrict__ blockValues, const TV* __restrict__ in,
+                                                 TV* stage) {
+        const int tid = threadIdx.x;
+        TV total = TV(0);
+        for (int base = begin; base < end; base += kCapacity) {
+            int column[U];
+            ValueRun<TM, W> block[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(base + tid + BS * u, end - 1);
+                column[u] = CIEHIP_STREAM_LOAD(columnIndices + k);
+                block[u] = loadBlock(blockValues, k);
+            }
+            ValueRun<TV, W> gathered[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) gathered[u] = gather(in, column[u]);
+            TV part = TV(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (base + tid + BS * u < end) part += blockProduct(block[u], gathered[u]);
             part = groupSum<kWave>(part);
             if ((tid & (kWave - 1)) == 0) stage[tid / kWave] = part;
             __syncthreads();

@@ -36,11 +36,12 @@ def _torch():
     return torch
 
 
-_KERNEL_NAMES = {_lib.SPMV_VECTOR: "vector", _lib.SPMV_STAGED: "staged", _lib.SPMV_TILED: "tiled"}
+_KERNEL_NAMES = {_lib.SPMV_VECTOR: "vector", _lib.SPMV_STAGED: "staged", _lib.SPMV_TILED: "tiled",
+                 _lib.SPMV_BLOCKS: "blocks"}
 
 
 def _kernel_code(name):
-    """C-ABI code of an SpMV kernel name ("vector", "staged", "tiled")."""
+    """C-ABI code of an SpMV kernel name ("vector", "staged", "tiled", "blocks")."""
     for code, known in _KERNEL_NAMES.items():
         if known == name:
             return code
@@ -125,12 +126,19 @@ class CSRMatrix:
         check(library().cie_hip_csr_column_bytes(self.handle, byref(value)), "cie_hip_csr_column_bytes")
         return value.value
 
+    def column_blocks(self, width):
+        """(blocks, bytes per block column) of the 1 x width column-block form the "blocks" kernel reads."""
+        blocks, value = c_int64(), c_int()
+        check(library().cie_hip_csr_column_blocks(self.handle, width, byref(blocks), byref(value)),
+              "cie_hip_csr_column_blocks")
+        return blocks.value, value.value
+
     def set_launch(self, precision, kernel, width, unroll):
         code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
-    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "staged", "vector")):
-        """Time the candidate launches of `kernels` ("staged", "vector", "tiled") and keep the fastest."""
+    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "blocks", "staged", "vector")):
+        """Time the candidate launches of `kernels` ("staged", "vector", "tiled", "blocks") and keep the fastest."""
         mask = sum(1 << _kernel_code(k) for k in kernels)
         best = c_double()
         check(library().cie_hip_csr_autotune(self.handle, precision, mask, repetitions, byref(best)),

@@ -127,7 +127,7 @@ int cie_hip_csr_shape(cie_hip_csr matrix, int64_t* rows, int64_t* columns, int64
  * the sparsity pattern is unchanged. */
 int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 
-/* SpMV launch shape, per value precision (64 or 32). Three kernels:
+/* SpMV launch shape, per value precision (64 or 32). Four kernels:
  *   CIE_HIP_SPMV_STAGED: one wavefront per 64-row tile; coalesced entry
  *     streams, products staged in LDS, each row summed in CSR entry order by
  *     one lane -- rows bit-identical to the sequential CPU loop of
@@ -141,6 +141,14 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
  *     lane streams entries, products are staged in LDS and each row is
  *     summed by a power-of-two group of lanes. Tiles are cut once per matrix
  *     and capacity, on the host, on first use.
+ *   CIE_HIP_SPMV_BLOCKS: the tiled kernel over 1 x width blocks of adjacent
+ *     columns (width 2, 3 or 4; unroll 8 or 16 blocks per lane): each row's
+ *     entries are covered by blocks that start at its next uncovered column,
+ *     and a block stores width values, zero where the row has no entry. One
+ *     gather loads the width x values of a block. The block form is built on
+ *     the device on first use (values refreshed after the entries change);
+ *     stored zeros, padding included, contribute nothing to a row sum, also
+ *     where x is not finite. Needs at least `width` columns.
  * All are bitwise reproducible run to run for a fixed launch. The default is
  * TILED/256/8; STAGED gives rows bit-identical to the CPU loop. cie_hip_csr_autotune
  * times the candidates of the kernels in `kernels` (a mask of 1 << kernel)
@@ -148,13 +156,20 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 #define CIE_HIP_SPMV_VECTOR 0
 #define CIE_HIP_SPMV_STAGED 1
 #define CIE_HIP_SPMV_TILED 2
+#define CIE_HIP_SPMV_BLOCKS 3
 /* The tiled kernel reads a narrow copy of the column indices when every run of
  * 4096 consecutive entries spans at most 65535 columns (banded FEM matrices in
  * their natural node order do): 16-bit offsets from each run's smallest
  * column, built on the device on first use. The int32 columns stay the
  * matrix's indices for every other kernel. *bytes receives the bytes per
- * column index the tiled kernel reads: 2 (narrow copy) or 4. */
+ * column index the tiled kernel reads: 2 (narrow copy) or 4. The block
+ * kernel keeps a narrow copy of its block columns by the same rule. */
 int cie_hip_csr_column_bytes(cie_hip_csr matrix, int* bytes);
+/* The 1 x width column-block form the CIE_HIP_SPMV_BLOCKS kernel reads (built
+ * on first use): *blocks receives its block count (nnz <= blocks * width) and
+ * *bytes the bytes per block column it reads (2 or 4, by the rule above). An
+ * introspection call with no counterpart in the reference. */
+int cie_hip_csr_column_blocks(cie_hip_csr matrix, int width, int64_t* blocks, int* bytes);
 int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll);
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll);
 int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds);

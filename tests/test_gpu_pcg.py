@@ -83,7 +83,7 @@ def test_fused_pcg_output_scaling_and_warm_start(hip, torch_gpu, in_scale, out_s
     assert rel(x.cpu().numpy(), x_ref) < 1e-9
 
 
-@pytest.mark.parametrize("kernel", ["autotuned", "tiled"])
+@pytest.mark.parametrize("kernel", ["autotuned", "tiled", "blocks"])
 @pytest.mark.parametrize("dim,order,n", [(3, 2, 10), (2, 4, 12), (3, 4, 4)])
 def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n, kernel):
     count = n ** dim
@@ -94,8 +94,10 @@ def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n, kernel):
     matrix = hip.CSRMatrix(csr)
     if kernel == "tiled":
         matrix.set_launch(64, "tiled", 256, 8)
+    elif kernel == "blocks":
+        matrix.set_launch(64, "blocks", 3, 8)
     else:
-        matrix.autotune(64, 3, kernels=("staged", "vector", "tiled"))
+        matrix.autotune(64, 3, kernels=("staged", "vector", "tiled", "blocks"))
     solver = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
     solver.product(0, to_dev(torch_gpu, b), 1, x)
@@ -104,7 +106,7 @@ def test_fused_pcg_lagrange_lognormal(hip, torch_gpu, dim, order, n, kernel):
     assert solver.info.trueRelativeResidual < 2e-10
 
 
-@pytest.mark.parametrize("restart,tiled", [(False, False), (True, False), (False, True)])
+@pytest.mark.parametrize("restart,tiled", [(False, False), (True, False), (False, True), (False, "blocks")])
 def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart, tiled):
     count = 8 ** 3
     k = np.exp(systems.normal(2, 1.0, count))
@@ -112,7 +114,10 @@ def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart, tiled):
     b = systems.uniform(1, csr.rows)
     x_ref, _, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-14, max_iterations=20000)
     matrix = hip.CSRMatrix(csr)
-    if tiled:
+    if tiled == "blocks":
+        matrix.set_launch(64, "blocks", 3, 16)
+        matrix.set_launch(32, "blocks", 3, 8)
+    elif tiled:
         matrix.set_launch(64, "tiled", 256, 16)
         matrix.set_launch(32, "tiled", 256, 8)
     solver = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), restart=restart)
@@ -167,8 +172,8 @@ def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu):
     solution agrees with the FP64 one to the conditioning-limited level."""
     w = systems.q2_laplace_3d(64)
     matrix = hip.CSRMatrix(w.matrix)
-    matrix.autotune(64, 3, kernels=("staged", "vector", "tiled"))
-    matrix.autotune(32, 3, kernels=("staged", "vector", "tiled"))
+    matrix.autotune(64, 3, kernels=("staged", "vector", "tiled", "blocks"))
+    matrix.autotune(32, 3, kernels=("staged", "vector", "tiled", "blocks"))
     b = to_dev(torch_gpu, w.rhs)
     fp64 = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
     x64 = torch_gpu.zeros_like(b)
@@ -183,3 +188,32 @@ def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu):
     assert rel(host, x64.cpu().numpy()) < 1e-6
     print(f"fp64 iterations {fp64.getStats().iterationCount} {fp64.info.seconds:.4f}s; mixed iterations "
           f"{mixed.getStats().iterationCount} refinements {mixed.info.refinements} {mixed.info.seconds:.4f}s")
+
+
+def test_full_size_q4_32_lognormal_mixed(hip, torch_gpu):
+    """3D Q4 32^3 with k_e = exp(N(0,1)) (2,048,383 DOFs, 423,564,751 nonzeros, up to 729 per row): the
+    mixed solve reaches FP64 true relative residual 1e-10 and is as close to a 1e-14 FP64 solve as the
+    FP64 solve to 1e-10 is."""
+    w = systems.q4_lognormal_3d(32)
+    assert (w.matrix.rows, w.matrix.nnz) == (2048383, 423564751)
+    matrix = hip.CSRMatrix(w.matrix)
+    b = to_dev(torch_gpu, w.rhs)
+    fp64 = hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))
+    x64 = torch_gpu.zeros_like(b)
+    fp64.product(0, b, 1, x64)
+    assert fp64.info.converged == 1 and fp64.info.trueRelativeResidual < 2e-10
+    mixed = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10))
+    xm = torch_gpu.zeros_like(b)
+    mixed.product(0, b, 1, xm)
+    assert mixed.info.converged == 1 and mixed.info.trueRelativeResidual < 1e-10
+    tight = hip.FusedPCG(matrix, Statistics(40000, 0.0, 1e-14))
+    xt = torch_gpu.zeros_like(b)
+    tight.product(0, b, 1, xt)
+    assert tight.info.converged == 1
+    host, reference = xm.cpu().numpy(), xt.cpu().numpy()
+    assert true_relative_residual(w.matrix, host, w.rhs) < 1e-10
+    error_mixed, error_fp64 = rel(host, reference), rel(x64.cpu().numpy(), reference)
+    assert error_mixed < 10 * error_fp64 + 1e-12
+    print(f"fp64 {fp64.getStats().iterationCount} iterations {fp64.info.seconds:.3f}s error {error_fp64:.2e}; mixed "
+          f"{mixed.getStats().iterationCount} iterations, {mixed.info.refinements} refinements "
+          f"{mixed.info.seconds:.3f}s error {error_mixed:.2e}; speedup {fp64.info.seconds / mixed.info.seconds:.2f}")

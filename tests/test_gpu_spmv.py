@@ -55,7 +55,8 @@ SCALES = [(0, 1), (0, -1), (0, 2.5), (1, 1), (1, -1), (1, 0.5), (-1, 1), (-1, 3.
 
 LAUNCHES = [("staged", 512, 4), ("staged", 512, 8), ("staged", 1024, 4), ("staged", 1024, 8), ("vector", 1, 1),
             ("vector", 2, 4), ("vector", 4, 2), ("vector", 8, 8), ("vector", 16, 4), ("vector", 32, 1),
-            ("vector", 64, 2), ("tiled", 256, 4), ("tiled", 256, 8), ("tiled", 256, 16)]
+            ("vector", 64, 2), ("tiled", 256, 4), ("tiled", 256, 8), ("tiled", 256, 16), ("blocks", 2, 8),
+            ("blocks", 3, 4), ("blocks", 3, 16), ("blocks", 4, 8)]
 
 
 @pytest.mark.parametrize("kernel,width,unroll", LAUNCHES)
@@ -102,6 +103,52 @@ def test_spmv_tiled_long_short_and_empty_rows(hip, torch_gpu, unroll, cols):
     matrix.set_launch(32, "tiled", 256, unroll)
     assert matrix.launch(64) == ("tiled", 256, unroll)
     assert matrix.column_bytes() == (2 if cols < 65536 else 4)
+    x = rng.standard_normal(cols)
+    y0 = rng.standard_normal(rows)
+    for in_scale, out_scale in [(0, 1), (1, -1), (0.5, 2.0)]:
+        expected = oracle.csr_product(csr, in_scale, x, out_scale, y0.copy())
+        space = hip.HIPSpace(torch_gpu.float64)
+        results = []
+        for _ in range(2):
+            y = to_dev(torch_gpu, y0)
+            hip.HIPCSROperator(matrix, space).product(in_scale, to_dev(torch_gpu, x), out_scale, y)
+            results.append(y.cpu().numpy())
+        assert np.array_equal(results[0], results[1])  # bitwise run to run
+        assert np.all(np.abs(results[0] - expected) <= spmv_bound(csr, x, y0, in_scale, out_scale, EPS64))
+        space32 = hip.HIPSpace(torch_gpu.float32)
+        y32 = to_dev(torch_gpu, y0, torch_gpu.float32)
+        hip.HIPCSROperator(matrix, space32).product(in_scale, to_dev(torch_gpu, x, torch_gpu.float32), out_scale, y32)
+        assert np.all(np.abs(y32.cpu().numpy() - expected) <= 4 * spmv_bound(csr, x, y0, in_scale, out_scale, EPS32))
+
+
+@pytest.mark.parametrize("width,unroll", [(2, 4), (3, 8), (4, 16)])
+@pytest.mark.parametrize("cols", [16000, 200000])
+def test_spmv_blocks_long_short_and_empty_rows(hip, torch_gpu, width, unroll, cols):
+    """Column blocks: rows with more blocks than a tile (summed in passes), runs of adjacent columns cut at
+    every offset, a lone entry in the last column (its block starts at columns - width), empty rows."""
+    rng = np.random.default_rng(width * 100 + unroll)
+    rows = 2000
+    lengths = rng.integers(1, 9, rows)
+    lengths[rng.choice(rows, 40, replace=False)] = 0
+    lengths[[0, 700, 701, 1999]] = [5000, 256 * unroll + 1, 3 * 256 * unroll, 4000]
+    rows_columns = []
+    for n in lengths:
+        if n >= 256:  # scattered: one block per entry
+            rows_columns.append(np.sort(rng.choice(cols, n, replace=False)))
+        else:  # runs of adjacent columns of random length
+            starts = np.sort(rng.choice(cols // 16, max(1, n // 3), replace=False)) * 16
+            run = np.unique(np.concatenate([s + np.arange(rng.integers(1, 6)) for s in starts]))
+            rows_columns.append(run[:n])
+    rows_columns[5] = np.array([cols - 1])
+    rows_columns[6] = np.array([0, cols - 2, cols - 1])
+    lengths = np.array([len(c) for c in rows_columns])
+    ext = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    csr = systems.CSR(rows, cols, ext, np.concatenate(rows_columns).astype(np.int32),
+                      rng.standard_normal(int(ext[-1])))
+    matrix = hip.CSRMatrix(csr)
+    matrix.set_launch(64, "blocks", width, unroll)
+    matrix.set_launch(32, "blocks", width, unroll)
+    assert matrix.launch(32) == ("blocks", width, unroll)
     x = rng.standard_normal(cols)
     y0 = rng.standard_normal(rows)
     for in_scale, out_scale in [(0, 1), (1, -1), (0.5, 2.0)]:
@@ -185,6 +232,16 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
         y = space.makeVector(w.matrix.rows)
         hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
         assert np.all(np.abs(y.cpu().numpy() - expected) <= bound)
+    # column blocks (runs of 3 and 5 adjacent columns: 1 x 3 blocks are 13% padding)
+    for width, unroll in ((3, 16), (2, 8), (4, 4)):
+        matrix.set_launch(64, "blocks", width, unroll)
+        matrix.set_launch(32, "blocks", width, unroll)
+        y = space.makeVector(w.matrix.rows)
+        hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
+        assert np.all(np.abs(y.cpu().numpy() - expected) <= bound), (width, unroll)
+        y32 = space32.makeVector(w.matrix.rows)
+        hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
+        assert np.linalg.norm(y32.cpu().numpy() - expected) / np.linalg.norm(expected) < 1e-5
 
 
 def test_spmv_shape_error(hip, torch_gpu):
