@@ -59,8 +59,10 @@ __global__ __launch_bounds__(kReduceBlock) void narrowColumnsKernel(int64_t nnz,
 // A row's entries are covered by 1 x width blocks: each block starts at the
 // column of the row's first entry not yet covered (or at columns - width, when
 // that would pass the last column) and covers [start, start + width). One
-// thread per row walks its entries; the count, structure and value passes
-// take the same walk, so they agree block for block.
+// thread per row walks its entries in the count and structure passes (the
+// same walk, so they agree block for block); the structure pass also records
+// each block's first entry, so refilling the values after the entries change
+// is one coalesced pass with a thread per block.
 
 __device__ __forceinline__ int blockStart(int column, int width, int columns) {
     return column + width > columns ? columns - width : column;
@@ -82,41 +84,39 @@ __global__ __launch_bounds__(kReduceBlock) void columnBlockCountKernel(int rows,
     }
 }
 
+// The first column of every block, and its first CSR entry (blocks hold
+// consecutive runs of entries, so block b's entries are
+// [entryBegin[b], entryBegin[b + 1]); the last slot, nnz, is set by the host).
 __global__ __launch_bounds__(kReduceBlock) void columnBlockStartsKernel(int rows, const int32_t* __restrict__ extents,
                                                                         const int32_t* __restrict__ columnIndices,
                                                                         int width, int columns,
                                                                         const int32_t* __restrict__ blockExtents,
-                                                                        int32_t* __restrict__ blockColumns) {
+                                                                        int32_t* __restrict__ blockColumns,
+                                                                        int32_t* __restrict__ entryBegin) {
     for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
         const int end = extents[row + 1];
         int b = blockExtents[row];
         for (int k = extents[row]; k < end;) {
             const int start = blockStart(columnIndices[k], width, columns);
+            entryBegin[b] = k;
             blockColumns[b++] = start;
             while (k < end && columnIndices[k] < start + width) ++k;
         }
     }
 }
 
-// The width values of every block: the row's entries at their columns, zero
-// elsewhere (a column listed twice in a row adds up).
+// The width values of every block, one thread per block: its entries at their
+// columns, zero elsewhere (a column listed twice in a row adds up).
 template <class T>
 __global__ __launch_bounds__(kReduceBlock) void columnBlockValuesKernel(
-    int rows, const int32_t* __restrict__ extents, const int32_t* __restrict__ columnIndices,
-    const T* __restrict__ entries, int width, const int32_t* __restrict__ blockExtents,
-    const int32_t* __restrict__ blockColumns, T* __restrict__ values) {
-    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
-        const int end = extents[row + 1];
-        int k = extents[row];
-        for (int b = blockExtents[row]; b < blockExtents[row + 1]; ++b) {
-            const int start = blockColumns[b];
-            T* v = values + int64_t(b) * width;
-            for (int i = 0; i < width; ++i) v[i] = T(0);
-            while (k < end && columnIndices[k] < start + width) {
-                v[columnIndices[k] - start] += entries[k];
-                ++k;
-            }
-        }
+    int64_t blocks, const int32_t* __restrict__ columnIndices, const T* __restrict__ entries, int width,
+    const int32_t* __restrict__ blockColumns, const int32_t* __restrict__ entryBegin, T* __restrict__ values) {
+    for (int64_t b = int64_t(blockIdx.x) * kReduceBlock + threadIdx.x; b < blocks;
+         b += int64_t(gridDim.x) * kReduceBlock) {
+        const int start = blockColumns[b];
+        T v[cie_hip_csr_s::kMaxBlockWidth] = {};
+        for (int k = entryBegin[b]; k < entryBegin[b + 1]; ++k) v[columnIndices[k] - start] += entries[k];
+        for (int i = 0; i < width; ++i) values[b * width + i] = v[i];
     }
 }
 
@@ -160,6 +160,7 @@ void releaseMatrix(cie_hip_csr m) {
         if (set.tiles) (void)hipFree(set.tiles);
     if (m->narrowColumns) (void)hipFree(m->narrowColumns);
     if (m->narrowBases) (void)hipFree(m->narrowBases);
+    if (m->blockEntryBegin) (void)hipFree(m->blockEntryBegin);
     for (cie_hip_csr blocks : m->columnBlocks) releaseMatrix(blocks);
     delete m;
 }
@@ -401,12 +402,12 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
     cie_hip_csr blocks = m->columnBlocks[width];
     if (!blocks) {
         // Structure: blocks per row, their prefix on the host, then the first column of each block.
-        int32_t *counts = nullptr, *extents = nullptr, *starts = nullptr;
+        int32_t *counts = nullptr, *extents = nullptr, *starts = nullptr, *entryBegin = nullptr;
         double* values64 = nullptr;
         float* values32 = nullptr;
         auto release = [&]() {
             for (void* p : {static_cast<void*>(counts), static_cast<void*>(extents), static_cast<void*>(starts),
-                            static_cast<void*>(values64), static_cast<void*>(values32)})
+                            static_cast<void*>(entryBegin), static_cast<void*>(values64), static_cast<void*>(values32)})
                 if (p) (void)hipFree(p);
         };
         std::vector<int32_t> hostCounts(static_cast<size_t>(rows)), hostExtents(static_cast<size_t>(rows) + 1, 0);
@@ -430,14 +431,18 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
             hostExtents[size_t(row) + 1] = int32_t(total);  // total <= nnz <= INT32_MAX
         }
         const size_t count = size_t(std::max<int64_t>(1, total));
+        const int32_t entryCount = int32_t(m->nnz);
         ok = hipMalloc(&starts, sizeof(int32_t) * count) == hipSuccess &&
+             hipMalloc(&entryBegin, sizeof(int32_t) * (count + 1)) == hipSuccess &&
+             hipMemcpyAsync(entryBegin + total, &entryCount, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) ==
+                 hipSuccess &&
              hipMalloc(&values64, sizeof(double) * count * width) == hipSuccess &&
              (!m->entries32 || hipMalloc(&values32, sizeof(float) * count * width) == hipSuccess) &&
              hipMemcpyAsync(extents, hostExtents.data(), sizeof(int32_t) * (size_t(rows) + 1), hipMemcpyHostToDevice,
                             ctx->stream) == hipSuccess;
         if (ok && rows) {
             hipLaunchKernelGGL(columnBlockStartsKernel, dim3(vectorGrid(rows)), dim3(kReduceBlock), 0, ctx->stream,
-                               rows, m->rowExtents, m->columnIndices, width, columns, extents, starts);
+                               rows, m->rowExtents, m->columnIndices, width, columns, extents, starts, entryBegin);
             ok = hipGetLastError() == hipSuccess;
         }
         ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
@@ -449,6 +454,7 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
         }
         adoptMatrix(ctx, rows, columns, extents, starts, values64, values32, std::move(hostExtents), &blocks);
         blocks->blockWidth = width;
+        blocks->blockEntryBegin = entryBegin;
         blocks->filledVersion = std::numeric_limits<uint64_t>::max();  // values not written yet
         m->columnBlocks[width] = blocks;
     }
@@ -457,17 +463,16 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
         blocks->filledVersion = std::numeric_limits<uint64_t>::max();
     }
     if (blocks->filledVersion != m->valuesVersion) {
-        if (rows) {
-            hipLaunchKernelGGL((columnBlockValuesKernel<double>), dim3(vectorGrid(rows)), dim3(kReduceBlock), 0,
-                               ctx->stream, rows, m->rowExtents, m->columnIndices,
-                               static_cast<const double*>(m->entries64), width, blocks->rowExtents,
-                               blocks->columnIndices, blocks->entries64);
+        if (blocks->nnz) {
+            const dim3 grid(unsigned(vectorGrid(blocks->nnz)));
+            hipLaunchKernelGGL((columnBlockValuesKernel<double>), grid, dim3(kReduceBlock), 0, ctx->stream,
+                               blocks->nnz, m->columnIndices, static_cast<const double*>(m->entries64), width,
+                               blocks->columnIndices, blocks->blockEntryBegin, blocks->entries64);
             CIE_HIP_CHECK(hipGetLastError());
             if (blocks->entries32 && m->entries32) {
-                hipLaunchKernelGGL((columnBlockValuesKernel<float>), dim3(vectorGrid(rows)), dim3(kReduceBlock), 0,
-                                   ctx->stream, rows, m->rowExtents, m->columnIndices,
-                                   static_cast<const float*>(m->entries32), width, blocks->rowExtents,
-                                   blocks->columnIndices, blocks->entries32);
+                hipLaunchKernelGGL((columnBlockValuesKernel<float>), grid, dim3(kReduceBlock), 0, ctx->stream,
+                                   blocks->nnz, m->columnIndices, static_cast<const float*>(m->entries32), width,
+                                   blocks->columnIndices, blocks->blockEntryBegin, blocks->entries32);
                 CIE_HIP_CHECK(hipGetLastError());
             }
         }
