@@ -354,7 +354,7 @@ def bench_batch(args):
     context = linalg.Context.default()
     matrix = linalg.CSRMatrix(w.matrix, context)
     batch = linalg.Batch(matrix, offsets)
-    batch.autotune(32, 2)
+    batch.autotune(32, 5)
     b = torch.from_numpy(w.rhs).cuda()
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, w.relative_tolerance)
@@ -404,12 +404,14 @@ def bench_batch(args):
           "f32 inner / f64 refinement",
           {"workload": "256 2D Q2 Laplace systems, n_i ~ U{16..256} (seed 3), k_e lognormal(0, 0.5), b ~ U(-1,1), "
                        "relative residual 1e-10 each", "rows": rows, "nnz": nnz, "systems": 256},
-          _roofline("batchProductKernel<float> (all systems active)", algorithmic, profiled.productSeconds,
-                    "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix"),
+          _roofline(f"batchProductKernel<float, {batch.column_blocks()}> (all systems active; tiles of "
+                    f"{'1 x 3 column blocks' if batch.column_blocks() else 'entries'})", algorithmic,
+                    profiled.productSeconds, "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix"),
           {"value": sampled_seconds * total_work / sampled_work, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG (ConjugateGradients_impl.hpp:49-201) on the first {sampled} systems "
                      f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
           {"device_seconds": summary.seconds, "max_iterations": int(iterations.max()),
+           "column_blocks": batch.column_blocks(),
            "refinements": int(refinements.sum()), "fp64": {"device_seconds": fp64_summary.seconds,
                                                            "max_iterations": int(fp64_iterations.max())}})
 

@@ -45,8 +45,12 @@ constexpr int kCompactThreads = 1024;
 constexpr int kGraphIterations = 8;
 constexpr int kDefaultTileRows = 256;
 
-template <class T>
-using BatchTiles = BlockTiles<T, T, kTileUnroll, kTileBlock>;
+constexpr int kBatchBlockWidth = 3;  // column-block width of a batch that walks blocks
+
+// The tile walker of a batch: matrix entries (W = 0) or 1 x W column blocks.
+template <class T, int W>
+using BatchTiles = std::conditional_t<W == 0, BlockTiles<T, T, kTileUnroll, kTileBlock>,
+                                      ColumnBlockTiles<T, T, kTileUnroll, kTileBlock, (W > 0 ? W : 2)>>;
 
 struct SystemState {
     double rho, alpha, beta, rr, initialNorm, absolute, relative, updateNorm;
@@ -102,7 +106,7 @@ __device__ __forceinline__ int locate(int item, const int32_t* __restrict__ pref
 // ---------------------------------------------------------------------------
 
 // r = b - A x (FP64), z = D^-1 r; start: p = z. Tile partials r.r (FP64), r.z (TR).
-template <class TR, bool kStart>
+template <class TR, bool kStart, int W>
 __global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
     BatchLayout layout, ActiveWork work, const double* __restrict__ entries, const double* __restrict__ x,
     const double* __restrict__ b, TR* __restrict__ r, const TR* __restrict__ inverse, TR* __restrict__ p,
@@ -125,7 +129,7 @@ __global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
             rr += residual64 * residual64;
             rz += residual * z;
         };
-        BlockTiles<double, double, kTileUnroll, kTileBlock>::tile(t, layout.source, entries, x, epilogue, stage, extents);
+        BatchTiles<double, W>::tile(t, layout.source, entries, x, epilogue, stage, extents);
         rr = blockSum<kTileBlock>(rr, ldsRR);
         rz = blockSum<kTileBlock>(rz, ldsRZ);
         if (threadIdx.x == 0) {
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
 }
 
 // q = A p; tile partial p.q.
-template <class T>
+template <class T, int W>
 __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout layout, ActiveWork work,
                                                                  const T* __restrict__ entries, const T* __restrict__ p,
                                                                  T* __restrict__ q, double* __restrict__ partials) {
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout lay
             q[row] = sum;
             pq += p[row] * sum;
         };
-        BatchTiles<T>::tile(t, layout.source, entries, p, epilogue, stage, extents);
+        BatchTiles<T, W>::tile(t, layout.source, entries, p, epilogue, stage, extents);
         pq = blockSum<kTileBlock>(pq, lds);
         if (threadIdx.x == 0) partials[2 * t] = double(pq);
         __syncthreads();
@@ -160,14 +164,14 @@ __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout lay
 }
 
 // The same product over an explicit tile range (the autotuner's timing runs).
-template <class T>
+template <class T, int W>
 __global__ __launch_bounds__(kTileBlock) void batchTileRangeKernel(RowSource source, int firstTile,
                                                                    const T* __restrict__ entries,
                                                                    const T* __restrict__ p, T* __restrict__ q) {
     __shared__ T stage[kTileEntries];
     __shared__ int32_t extents[kTileBlock + 1];
     auto epilogue = [&](int row, T sum) { q[row] = sum; };
-    BatchTiles<T>::tile(firstTile + int(blockIdx.x), source, entries, p, epilogue, stage, extents);
+    BatchTiles<T, W>::tile(firstTile + int(blockIdx.x), source, entries, p, epilogue, stage, extents);
 }
 
 // x += alpha p; r -= alpha q; z = D^-1 r; chunk partials r.r and r.z.
@@ -445,7 +449,8 @@ struct cie_hip_batch_s {
     void* workspace = nullptr;
     size_t workspaceBytes = 0;
     hipGraphExec_t graph = nullptr;
-    int graphKey[4] = {0, 0, 0, 0};     // what the captured graph was built for
+    int blockWidth = 0;                 // 0: tiles of entries; kBatchBlockWidth: tiles of column blocks
+    int graphKey[5] = {0, 0, 0, 0, 0};  // what the captured graph was built for
     const void* graphPointers[3] = {nullptr, nullptr, nullptr};  // b, x, inverse diagonal
 };
 
@@ -465,7 +470,13 @@ void freeDevice(T*& pointer) {
 // Cuts every system into block tiles (<= kTileEntries entries, <= tileRows[s]
 // rows; a longer row is a tile of its own) and into chunks of kChunkRows rows.
 int buildWork(cie_hip_batch batch) {
-    const int32_t* extents = batch->matrix->hostExtents.data();
+    // Tiles are cut over the rows' entries, or over their column blocks (as many per row as the block form holds).
+    cie_hip_csr form = batch->matrix;
+    if (batch->blockWidth) {
+        const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &form);
+        if (status != CIE_HIP_OK) return status;
+    }
+    const int32_t* extents = form->hostExtents.data();
     std::vector<int2> tiles;
     std::vector<int32_t> systemTiles(size_t(batch->systems) + 1, 0);
     std::vector<int32_t> chunkRows;
@@ -521,9 +532,15 @@ int buildWork(cie_hip_batch batch) {
 
 int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
     cie_hip_csr m = batch->matrix;
+    if (batch->blockWidth) {  // the block form, its values current
+        const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &m);
+        if (status != CIE_HIP_OK) return status;
+    }
     const int status = ensureNarrowColumns(m);
     if (status != CIE_HIP_OK) return status;
     RowSource& source = layout->source;
+    source.blockValues64 = batch->blockWidth ? m->entries64 : nullptr;
+    source.blockValues32 = batch->blockWidth ? m->entries32 : nullptr;
     source.rows = int(m->rows);
     source.param = 1;
     source.rowExtents = m->rowExtents;
@@ -554,6 +571,8 @@ dim3 workGrid(Kernel kernel, cie_hip_context ctx, int block, int items, bool res
 
 void releaseBatch(cie_hip_batch batch) {
     if (!batch) return;
+    if (batch->blockWidth && batch->matrix->columnBlocks[batch->blockWidth])
+        --batch->matrix->columnBlocks[batch->blockWidth]->blockUsers;
     if (batch->context) {
         (void)hipSetDevice(batch->context->device);
         (void)hipStreamSynchronize(batch->context->stream);
@@ -581,6 +600,44 @@ void releaseBatch(cie_hip_batch batch) {
     delete batch;
 }
 
+// Switches the batch between tiles of entries (0) and of column blocks; the
+// block form in use is pinned so the matrix's own autotuner keeps it.
+int setBlockWidth(cie_hip_batch batch, int width) {
+    if (width == batch->blockWidth) return CIE_HIP_OK;
+    cie_hip_csr m = batch->matrix;
+    if (width) {
+        cie_hip_csr form = nullptr;
+        const int status = ensureColumnBlocks(m, width, &form);
+        if (status != CIE_HIP_OK) return status;
+        ++form->blockUsers;
+    }
+    if (batch->blockWidth && m->columnBlocks[batch->blockWidth]) --m->columnBlocks[batch->blockWidth]->blockUsers;
+    batch->blockWidth = width;
+    return buildWork(batch);  // the tiles always match the walker
+}
+
+// The batched product over tiles [t0, t0 + tiles) (the autotuner's timing runs).
+void launchTileRange(int width, int precision, int tiles, hipStream_t stream, const RowSource& source, int t0,
+                     cie_hip_csr m, void* buffer) {
+    if (precision == 32) {
+        float* in = static_cast<float*>(buffer);
+        if (width)
+            hipLaunchKernelGGL((batchTileRangeKernel<float, kBatchBlockWidth>), dim3(tiles), dim3(kTileBlock), 0, stream,
+                               source, t0, m->entries32, in, in + m->rows);
+        else
+            hipLaunchKernelGGL((batchTileRangeKernel<float, 0>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
+                               m->entries32, in, in + m->rows);
+    } else {
+        double* in = static_cast<double*>(buffer);
+        if (width)
+            hipLaunchKernelGGL((batchTileRangeKernel<double, kBatchBlockWidth>), dim3(tiles), dim3(kTileBlock), 0,
+                               stream, source, t0, m->entries64, in, in + m->rows);
+        else
+            hipLaunchKernelGGL((batchTileRangeKernel<double, 0>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
+                               m->entries64, in, in + m->rows);
+    }
+}
+
 // Rebuilds a work list from the system states.
 template <int kSelect>
 void launchSelect(cie_hip_batch batch, const ActiveWork& work) {
@@ -595,7 +652,7 @@ void launchSelect(cie_hip_batch batch, const ActiveWork& work) {
 // Enqueues the kernels of one iteration on the context stream.
 constexpr int kProfiledTrips = 8;  // product launches averaged by the profiling summary
 
-template <class TR>
+template <class TR, int W>
 void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
                       const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
     cie_hip_context ctx = batch->context;
@@ -607,7 +664,7 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
     const int mark = profile ? profile->begin(ctx) : -1;
-    hipLaunchKernelGGL((batchProductKernel<TR>), workGrid(batchProductKernel<TR>, ctx, kTileBlock, batch->tileCount),
+    hipLaunchKernelGGL((batchProductKernel<TR, W>), workGrid(batchProductKernel<TR, W>, ctx, kTileBlock, batch->tileCount),
                        tileBlock, 0, stream, layout, work, entries, p, q, batch->tilePartials);
     if (profile) profile->end(ctx, mark);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
@@ -619,8 +676,8 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
         hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdateMixed>), systems, tileBlock, 0, stream,
                            batch->systemChunks, batch->chunkPartials, batch->state, settings, batch->active);
         launchSelect<kRefining>(batch, batch->refining);
-        hipLaunchKernelGGL((batchResidualKernel<TR, false>),
-                           workGrid(batchResidualKernel<TR, false>, ctx, kTileBlock, batch->tileCount, true), tileBlock, 0,
+        hipLaunchKernelGGL((batchResidualKernel<TR, false, W>),
+                           workGrid(batchResidualKernel<TR, false, W>, ctx, kTileBlock, batch->tileCount, true), tileBlock, 0,
                            stream, layout, batch->refining, m->entries64, x, b, r, inverse, p, batch->tilePartials,
                            batch->state);
         hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseRefine>), systems, tileBlock, 0, stream, batch->systemTiles,
@@ -632,6 +689,14 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     hipLaunchKernelGGL((batchDirectionKernel<TR>), workGrid(batchDirectionKernel<TR>, ctx, kChunkRows, batch->chunkCount),
                        chunkBlock, 0, stream, layout, work, p, r, inverse, batch->state);
     launchSelect<kRunning>(batch, work);
+}
+
+// One trip with the batch's walker.
+template <class TR>
+void enqueueTrip(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
+                 const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
+    if (batch->blockWidth) enqueueIteration<TR, kBatchBlockWidth>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+    else enqueueIteration<TR, 0>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
 }
 
 template <class TR>
@@ -675,24 +740,31 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
     const ActiveWork& work = batch->running;
     launchSelect<kRunning>(batch, work);  // every system runs
-    hipLaunchKernelGGL((batchResidualKernel<TR, true>),
-                       workGrid(batchResidualKernel<TR, true>, ctx, kTileBlock, batch->tileCount), dim3(kTileBlock), 0,
-                       ctx->stream, layout, work, m->entries64, x, b, r, inverse, p, batch->tilePartials, batch->state);
+    if (batch->blockWidth)
+        hipLaunchKernelGGL((batchResidualKernel<TR, true, kBatchBlockWidth>),
+                           workGrid(batchResidualKernel<TR, true, kBatchBlockWidth>, ctx, kTileBlock, batch->tileCount),
+                           dim3(kTileBlock), 0, ctx->stream, layout, work, m->entries64, x, b, r, inverse, p,
+                           batch->tilePartials, batch->state);
+    else
+        hipLaunchKernelGGL((batchResidualKernel<TR, true, 0>),
+                           workGrid(batchResidualKernel<TR, true, 0>, ctx, kTileBlock, batch->tileCount),
+                           dim3(kTileBlock), 0, ctx->stream, layout, work, m->entries64, x, b, r, inverse, p,
+                           batch->tilePartials, batch->state);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseStart>), dim3(batch->systems), dim3(kTileBlock), 0, ctx->stream,
                        batch->systemTiles, batch->tilePartials, batch->state, settings, batch->active);
     launchSelect<kRunning>(batch, work);
     CIE_HIP_CHECK(hipGetLastError());
 
     if (useGraph) {
-        const int key[4] = {mixed ? 1 : 0, settings.restart, int(sizeof(TR)), batch->tileCount};
-        const bool same = batch->graph && std::equal(key, key + 4, batch->graphKey) && batch->graphPointers[0] == b &&
+        const int key[5] = {mixed ? 1 : 0, settings.restart, int(sizeof(TR)), batch->tileCount, batch->blockWidth};
+        const bool same = batch->graph && std::equal(key, key + 5, batch->graphKey) && batch->graphPointers[0] == b &&
                           batch->graphPointers[1] == x && batch->graphPointers[2] == inverse;
         if (!same) {
             releaseGraph(batch);
             hipGraph_t graph = nullptr;
             CIE_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, nullptr);
+                enqueueTrip<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, nullptr);
             const hipError_t captured = hipStreamEndCapture(ctx->stream, &graph);
             if (captured != hipSuccess) {
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph capture: ") + hipGetErrorString(captured));
@@ -703,7 +775,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
                 batch->graph = nullptr;
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(made));
             }
-            std::copy(key, key + 4, batch->graphKey);
+            std::copy(key, key + 5, batch->graphKey);
             batch->graphPointers[0] = b;
             batch->graphPointers[1] = x;
             batch->graphPointers[2] = inverse;
@@ -718,7 +790,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             CIE_HIP_CHECK(hipGraphLaunch(batch->graph, ctx->stream));
         } else {
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueIteration<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+                enqueueTrip<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
             CIE_HIP_CHECK(hipGetLastError());
         }
         ++count;
@@ -844,54 +916,88 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
         return setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing set-up failed");
     }
     const hipEvent_t start = events.event[0], stop = events.event[1];
-    std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
-    std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
-    int status = CIE_HIP_OK;
-    for (int rows : {32, 64, 128, 256}) {
-        batch->tileRows.assign(size_t(batch->systems), rows);
-        status = buildWork(batch);
-        if (status) break;
-        BatchLayout layout{};
-        status = batchLayout(batch, &layout);
-        if (status) break;
-        std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
-        if (hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
-                      hipMemcpyDeviceToHost) != hipSuccess) {
-            status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: reading the tile layout failed");
-            break;
+    // Formats: tiles of entries, and tiles of 1 x 3 column blocks where their padding stays small (fill <= 5/3).
+    std::vector<int> widths{0};
+    if (m->columns >= kBatchBlockWidth) {
+        cie_hip_csr form = nullptr;
+        const int status = ensureColumnBlocks(m, kBatchBlockWidth, &form);
+        if (status != CIE_HIP_OK) {
+            (void)hipFree(buffer);
+            return status;
         }
-        for (int s = 0; s < batch->systems && status == CIE_HIP_OK; ++s) {
-            const int t0 = systemTiles[s], tiles = systemTiles[s + 1] - t0;
-            for (int pass = 0; pass < 2; ++pass) {
-                (void)hipEventRecord(start, ctx->stream);
-                for (int r = 0; r < (pass ? repetitions : 1); ++r) {
-                    if (precision == 32) {
-                        float* in = static_cast<float*>(buffer);
-                        hipLaunchKernelGGL((batchTileRangeKernel<float>), dim3(tiles), dim3(kTileBlock), 0, ctx->stream,
-                                           layout.source, t0, m->entries32, in, in + m->rows);
-                    } else {
-                        double* in = static_cast<double*>(buffer);
-                        hipLaunchKernelGGL((batchTileRangeKernel<double>), dim3(tiles), dim3(kTileBlock), 0,
-                                           ctx->stream, layout.source, t0, m->entries64, in, in + m->rows);
+        if (3 * form->nnz * kBatchBlockWidth <= 5 * std::max<int64_t>(1, m->nnz)) widths.push_back(kBatchBlockWidth);
+    }
+    double bestTotal = std::numeric_limits<double>::infinity();
+    int bestWidth = 0;
+    std::vector<int32_t> bestChoice(size_t(batch->systems), kDefaultTileRows);
+    int status = CIE_HIP_OK;
+    for (int width : widths) {
+        status = setBlockWidth(batch, width);
+        if (status) break;
+        std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
+        std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
+        for (int rows : {32, 64, 128, 256}) {
+            batch->tileRows.assign(size_t(batch->systems), rows);
+            status = buildWork(batch);
+            if (status) break;
+            BatchLayout layout{};
+            status = batchLayout(batch, &layout);
+            if (status) break;
+            std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
+            if (hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
+                          hipMemcpyDeviceToHost) != hipSuccess) {
+                status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: reading the tile layout failed");
+                break;
+            }
+            for (int s = 0; s < batch->systems && status == CIE_HIP_OK; ++s) {
+                const int t0 = systemTiles[s], tiles = systemTiles[s + 1] - t0;
+                for (int pass = 0; pass < 2; ++pass) {
+                    (void)hipEventRecord(start, ctx->stream);
+                    for (int r = 0; r < (pass ? repetitions : 1); ++r)
+                        launchTileRange(width, precision, tiles, ctx->stream, layout.source, t0, m, buffer);
+                    if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
+                        status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing failed");
+                        break;
+                    }
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, start, stop);
+                    if (pass && ms < best[s]) {
+                        best[s] = ms;
+                        choice[s] = rows;
                     }
                 }
-                if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
-                    status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing failed");
-                    break;
-                }
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, start, stop);
-                if (pass && ms < best[s]) {
-                    best[s] = ms;
-                    choice[s] = rows;
-                }
             }
+            if (status) break;
+        }
+        if (status) break;
+        double total = 0.0;
+        for (double t : best) total += t;
+        if (total < bestTotal) {
+            bestTotal = total;
+            bestWidth = width;
+            bestChoice = choice;
         }
     }
     (void)hipFree(buffer);
     if (status) return status;
-    batch->tileRows = choice;
+    status = setBlockWidth(batch, bestWidth);
+    if (status) return status;
+    batch->tileRows = bestChoice;
     return buildWork(batch);
+}
+
+int cie_hip_batch_get_column_blocks(cie_hip_batch batch, int* width) {
+    if (!batch || !width) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_get_column_blocks: null argument");
+    *width = batch->blockWidth;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_batch_set_column_blocks(cie_hip_batch batch, int width) {
+    if (!batch || (width != 0 && width != kBatchBlockWidth))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_column_blocks: width must be 0 or " +
+                                                            std::to_string(kBatchBlockWidth));
+    CIE_HIP_CHECK(hipSetDevice(batch->context->device));
+    return setBlockWidth(batch, width);
 }
 
 int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const double* b, double* x,

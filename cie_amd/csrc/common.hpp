@@ -201,6 +201,7 @@ struct cie_hip_csr_s {
     cie_hip_csr_s* columnBlocks[kMaxBlockWidth + 1] = {};
     int blockWidth = 0;                // of a block matrix; 0 for an ordinary one
     int32_t* blockEntryBegin = nullptr;// of a block matrix: each block's first entry of the parent (blocks + 1)
+    int blockUsers = 0;                // of a block matrix: holders (batches) that keep it from being freed
     uint64_t filledVersion = 0;        // of a block matrix: the parent's valuesVersion its values hold
 };
 

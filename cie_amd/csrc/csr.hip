@@ -487,7 +487,7 @@ void releaseUnusedColumnBlocks(cie_hip_csr m) {
     for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
         const bool used = (m->launch64.kernel == CIE_HIP_SPMV_BLOCKS && m->launch64.width == width) ||
                           (m->launch32.kernel == CIE_HIP_SPMV_BLOCKS && m->launch32.width == width);
-        if (!used && m->columnBlocks[width]) {
+        if (!used && m->columnBlocks[width] && !m->columnBlocks[width]->blockUsers) {
             releaseMatrix(m->columnBlocks[width]);
             m->columnBlocks[width] = nullptr;
         }

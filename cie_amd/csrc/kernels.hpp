@@ -507,6 +507,14 @@ struct ColumnBlockTiles {
         }
     }
 
+    // BlockTiles::tile's signature (the CSR plane argument is not read).
+    template <class Epilogue>
+    __device__ __forceinline__ static void tile(int t, const RowSource& source, const TM* __restrict__,
+                                                const TV* __restrict__ in, Epilogue& epilogue, TV* stage,
+                                                int32_t* extents) {
+        tile(t, source, in, epilogue, stage, extents);
+    }
+
     template <class Epilogue>
     __device__ __forceinline__ static void tile(int t, const RowSource& source, const TV* __restrict__ in,
                                                 Epilogue& epilogue, TV* stage, int32_t* extents) {

@@ -788,6 +788,15 @@ class Batch:
     def autotune(self, precision=32, repetitions=3):
         check(library().cie_hip_batch_autotune(self.handle, precision, repetitions), "cie_hip_batch_autotune")
 
+    def column_blocks(self):
+        """3 when the batch's tiles hold 1 x 3 column blocks, 0 when they hold matrix entries."""
+        width = c_int()
+        check(library().cie_hip_batch_get_column_blocks(self.handle, byref(width)), "cie_hip_batch_get_column_blocks")
+        return width.value
+
+    def set_column_blocks(self, width):
+        check(library().cie_hip_batch_set_column_blocks(self.handle, int(width)), "cie_hip_batch_set_column_blocks")
+
     def tile_rows(self):
         out = np.empty(self.systems, dtype=np.int32)
         check(library().cie_hip_batch_get_tile_rows(self.handle, out.ctypes.data_as(c_void_p)))

@@ -249,7 +249,11 @@ int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScal
  * rowsPerTile[s] rows: 32, 64, 128 or 256, chosen per system by
  * cie_hip_batch_autotune from timed runs; default 256); the vector steps walk
  * chunks of 1024 rows of one system. After every step the finished systems are
- * dropped from the work lists on the device. */
+ * dropped from the work lists on the device. The tiles hold matrix entries,
+ * or 1 x 3 column blocks (CIE_HIP_SPMV_BLOCKS, at most 2048 per tile):
+ * cie_hip_batch_autotune times both where the blocks' padding is small and
+ * keeps the faster, cie_hip_batch_set_column_blocks (width 3, or 0 for
+ * entries) sets it. */
 typedef struct cie_hip_batch_s* cie_hip_batch;
 int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t systems, const int64_t* offsets,
                          cie_hip_batch* batch);
@@ -257,6 +261,8 @@ int cie_hip_batch_destroy(cie_hip_batch batch);
 int cie_hip_batch_set_tile_rows(cie_hip_batch batch, const int32_t* rowsPerTile);
 int cie_hip_batch_get_tile_rows(cie_hip_batch batch, int32_t* rowsPerTile);
 int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions);
+int cie_hip_batch_set_column_blocks(cie_hip_batch batch, int width);
+int cie_hip_batch_get_column_blocks(cie_hip_batch batch, int* width);
 /* Solves every system in place: x (device, all rows) is the initial guess and
  * the result. mixed != 0 runs the mixed-precision refined form (settings as
  * for cie_hip_pcg_mixed), else FP64 (settings.settings only). Host outputs

@@ -19,10 +19,13 @@ def per_system_residuals(w, x):
     return np.array([np.linalg.norm(r[a:b]) / np.linalg.norm(w.rhs[a:b]) for a, b in zip(offsets[:-1], offsets[1:])])
 
 
-def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu):
+@pytest.mark.parametrize("blocks", [0, 3])
+def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu, blocks):
     w = small_batch()
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.set_column_blocks(blocks)
+    assert batch.column_blocks() == blocks
     b = torch_gpu.from_numpy(w.rhs).cuda()
     x = torch_gpu.zeros_like(b)
     stats, status, _, summary = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=False)
@@ -38,11 +41,12 @@ def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu):
         assert np.linalg.norm(got[a:c] - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
 
 
-@pytest.mark.parametrize("restart", [False, True])
-def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart):
+@pytest.mark.parametrize("restart,blocks", [(False, 0), (True, 0), (False, 3)])
+def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks):
     w = small_batch(seed=8)
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.set_column_blocks(blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     x = torch_gpu.zeros_like(b)
     stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, restart=restart)
@@ -51,12 +55,16 @@ def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart):
     assert all(s.relativeResidual < 1e-10 for s in stats)
 
 
-def test_batched_graph_replays_match_eager(hip, torch_gpu):
+@pytest.mark.parametrize("blocks", [None, 0, 3])
+def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
     w = small_batch(seed=9)
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
     batch.autotune(32, 2)
     assert set(batch.tile_rows().tolist()) <= {32, 64, 128, 256}
+    assert batch.column_blocks() in (0, 3)
+    if blocks is not None:
+        batch.set_column_blocks(blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     results = []
     for graph in (False, True, True, True):   # the graph is captured once and replayed on later calls
@@ -90,7 +98,8 @@ def test_full_batch_256_uneven_systems(hip, torch_gpu):
     residuals = per_system_residuals(w, x.cpu().numpy())
     assert np.all(residuals < 1e-10)
     print(f"batch: {w.matrix.rows} rows, {w.matrix.nnz} nnz, {summary.seconds:.3f} s, max iterations "
-          f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, tile rows "
+          f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, column blocks "
+          f"{batch.column_blocks()}, tile rows "
           f"{np.bincount(batch.tile_rows(), minlength=257)[[32, 64, 128, 256]].tolist()}")
 
 
@@ -112,3 +121,24 @@ def test_batched_tile_rows(hip, torch_gpu):
     assert np.linalg.norm(results[0][0] - results[1][0]) <= 1e-9 * np.linalg.norm(results[1][0])
     with pytest.raises(hip._lib.CieHipError):
         batch.set_tile_rows(np.full(batch.systems, 48, dtype=np.int32))
+    with pytest.raises(hip._lib.InvalidArgument):
+        batch.set_column_blocks(2)
+
+
+def test_batch_keeps_its_column_blocks_through_the_matrix_autotuner(hip, torch_gpu):
+    """A batch walking column blocks pins the block form: the matrix's own autotuner, which frees the block forms
+    its launches do not use, keeps it, and later value updates reach it."""
+    w = small_batch(seed=11)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.set_column_blocks(3)
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x0 = torch_gpu.zeros_like(b)
+    batch.solve(b, x0, Statistics(20000, 0.0, 1e-10), mixed=False)
+    matrix.autotune(64, 2, kernels=("tiled",))
+    matrix.update_entries(torch_gpu.from_numpy(2.0 * w.matrix.entries).cuda())
+    x1 = torch_gpu.zeros_like(b)
+    stats, status, _, _ = batch.solve(b, x1, Statistics(20000, 0.0, 1e-10), mixed=False)
+    assert np.all(status == 1)
+    # (2A) x1 = b: x1 = x0 / 2 to the solve's tolerance
+    assert np.linalg.norm(2.0 * x1.cpu().numpy() - x0.cpu().numpy()) <= 1e-8 * np.linalg.norm(x0.cpu().numpy())
