@@ -77,3 +77,38 @@ def test_padding_ignores_non_finite_x(hip, torch_gpu):
         assert np.isnan(expected[1]) and got[1] == 12.0
         got32 = _product(hip, torch_gpu, matrix, x, torch_gpu.float32)
         assert np.array_equal(got32[[0, 2, 3]], expected[[0, 2, 3]]) and got32[1] == 12.0
+
+
+def _host_block_count(csr, width):
+    """Blocks of the covering rule, restated on the host: each block starts at the row's first uncovered
+    column (at columns - width when it would pass the last column) and covers width adjacent columns."""
+    count = 0
+    ext, cols = csr.row_extents, csr.column_indices
+    for row in range(csr.rows):
+        k, end = int(ext[row]), int(ext[row + 1])
+        while k < end:
+            start = min(int(cols[k]), csr.columns - width)
+            count += 1
+            while k < end and cols[k] < start + width:
+                k += 1
+    return count
+
+
+@pytest.mark.parametrize("dim,order,n", [(3, 2, 5), (2, 4, 7), (3, 1, 6)])
+def test_block_counts_match_the_covering_rule(hip, torch_gpu, dim, order, n):
+    csr = systems.lagrange(dim, order, n)
+    matrix = hip.CSRMatrix(csr)
+    for width in (2, 3, 4):
+        blocks, column_bytes = matrix.column_blocks(width)
+        assert blocks == _host_block_count(csr, width), width
+        assert width * blocks >= csr.nnz and column_bytes == 2
+    # scattered rows of 0-6 columns, some at the last columns (blocks shifted to columns - width)
+    rng = np.random.default_rng(4)
+    rows = [np.unique(rng.integers(0, 40, rng.integers(0, 7))) for _ in range(50)]
+    rows[7] = np.array([38, 39])
+    ext = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    scattered = systems.CSR(50, 40, ext, np.concatenate(rows).astype(np.int32), np.ones(int(ext[-1])))
+    matrix = hip.CSRMatrix(scattered)
+    assert matrix.column_blocks(3)[0] == _host_block_count(scattered, 3)
+    with pytest.raises(InvalidArgument):
+        matrix.column_blocks(5)
