@@ -587,6 +587,7 @@ def bench_pmg(args):
     rows, nnz = csr.rows, csr.nnz
     context = linalg.Context.default()
     matrix = linalg.CSRMatrix(csr, context)
+    matrix.autotune(64, 5, kernels=("tiled", "blocks"))  # the finest grid's sweeps and updates use this launch
     mask = linalg.Mask(mask_values, context)
     t0 = time.perf_counter()
     pmg = linalg.PMultigrid(matrix, mask, order)
@@ -626,7 +627,7 @@ def bench_pmg(args):
     _line(args, "p-multigrid time to ||r|| <= 1e-6 ||b|| (s), poisson2D's solveMultigrid", wall, "s", "f64",
           {"workload": f"2D Q{order} integrated Legendre Laplace, {n}^2 elements, Dirichlet eliminated, "
                        f"b ~ U(-1,1) seed 1, x0 = 0", "rows": rows, "nnz": nnz},
-          _roofline("jacobiSweepKernel<BlockTiles<double, double, 8, 256>> on the finest grid", algorithmic, sweep,
+          _roofline(f"jacobiSweepKernel {list(matrix.launch(64))} on the finest grid", algorithmic, sweep,
                     "B_SpMV(FP64, int32 CSR) + 2 x 8 N (right-hand side and inverse diagonal)"),
           {"value": cpu_sample * info.cycles / sample_cycles, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle_pmg (solver.cpp:111-232 restated) for {sample_cycles} cycles in {cpu_sample:.2f} s, "
