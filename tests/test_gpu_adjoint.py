@@ -40,7 +40,9 @@ def cpu_reference_gradient(dim, order, n, g, g_target, rhs, tolerance=1e-13):
     return grad, 0.5 * np.dot(u - ut, u - ut), ut
 
 
-def test_autograd_matches_cpu_reference(torch_gpu, hip):
+@pytest.mark.parametrize("blocks", [False, True])
+def test_autograd_matches_cpu_reference(torch_gpu, hip, blocks):
+    """With blocks the solves walk 1 x 3 column blocks, whose values follow every re-assembly."""
     dim, order, n = 3, 2, 10
     _, _, count, _ = systems.lagrange_size(dim, order, n)
     g = systems.normal(4, np.sqrt(0.5), count)
@@ -48,6 +50,9 @@ def test_autograd_matches_cpu_reference(torch_gpu, hip):
     rhs = systems.uniform(1, systems.lagrange_size(dim, order, n)[0])
     grad_ref, j_ref, ut = cpu_reference_gradient(dim, order, n, g, gt, rhs)
     problem = adjoint.LagrangeProblem(dim, order, n, rhs, ut, tolerance=1e-13)
+    if blocks:
+        problem.assemble(gt)  # creates the matrix; the objective re-assembles it for g
+        problem.matrix.set_launch(64, "blocks", 3, 8)
     g_t = torch_gpu.tensor(g, dtype=torch_gpu.float64, requires_grad=True)
     j = adjoint.Objective.apply(g_t, problem)
     j.backward()

@@ -126,12 +126,16 @@ def test_nested_product_operator(hip, torch_gpu):
     assert np.allclose(y.cpu().numpy(), expected, rtol=1e-13, atol=1e-13 * np.abs(expected).max())
 
 
-@pytest.mark.parametrize("order,n", [(1, 20), (2, 16), (3, 12), (4, 10), (4, 32)])
-def test_pmg_matches_oracle(hip, torch_gpu, order, n):
+@pytest.mark.parametrize("order,n,blocks", [(1, 20, False), (2, 16, False), (3, 12, False), (4, 10, False),
+                                            (4, 32, False), (3, 12, True), (4, 32, True)])
+def test_pmg_matches_oracle(hip, torch_gpu, order, n, blocks):
+    """blocks: the finest grid's sweeps and updates walk 1 x 3 column blocks."""
     csr, mask_values = _legendre(order, n)
     b = systems.uniform(1, csr.rows)
     x_ref, history, coarse = oracle.pmg(csr, mask_values, order, b)
     matrix = hip.CSRMatrix(csr)
+    if blocks:
+        matrix.set_launch(64, "blocks", 3, 8)
     pmg = hip.PMultigrid(matrix, hip.Mask(mask_values), order)
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
     info = pmg.solve(_dev(torch_gpu, b), x)
