@@ -133,8 +133,10 @@ __device__ __forceinline__ TV rowProduct(int row, int lane, const int32_t* __res
 
 // The sparsity pattern as the walkers read it. param is the block's share:
 // rows per block (VectorRows), 64-row tiles per block (StagedRows) or block
-// tiles per block (BlockTiles). tiles/tileCount describe the block tiles of
-// BlockTiles (tileCount + 1 entries {first row, first entry}); unused otherwise.
+// tiles per block (BlockTiles, ColumnBlockTiles). tiles/tileCount describe the
+// block tiles (tileCount + 1 entries {first row, first entry}, or {first row,
+// first column block}); unused otherwise. For ColumnBlockTiles the extents,
+// columns and narrow plane are those of the matrix's column-block form.
 struct RowSource {
     int rows;
     int param;
