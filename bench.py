@@ -45,16 +45,18 @@ def index_bytes(matrix, precision):
         return matrix.column_bytes()
     if kernel == "blocks":
         return matrix.column_blocks(width)[1]
-    return 4
+    return 4  # int32 columns, or the 32-bit key per sorted block
 
 
 def format_bytes(matrix, precision):
     """Bytes of the format the selected SpMV launch reads: per entry (CSR) or per column block
-    (w values + one column each), plus row extents, x once and y once."""
+    (w values + one column each; a sorted block's column is its 32-bit key), plus row extents, x once and
+    y once (tile descriptors, 8-12 B per tile of thousands of blocks, are not counted for any kernel)."""
     value = 8 if precision == 64 else 4
     kernel, width, _ = matrix.launch(precision)
-    if kernel == "blocks":
+    if kernel in ("blocks", "sorted"):
         blocks, column = matrix.column_blocks(width)
+        column = 4 if kernel == "sorted" else column
         return (width * value + column) * blocks + 4 * (matrix.rows + 1) + 2 * value * matrix.rows
     return spmv_bytes(matrix.rows, matrix.nnz, value, index_bytes(matrix, precision))
 
@@ -158,8 +160,8 @@ def main():
     t0 = time.perf_counter()
     matrix = linalg.CSRMatrix(csr, context)
     upload_seconds = time.perf_counter() - t0
-    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks"))
-    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks"))
+    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
+    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
     b = torch.from_numpy(workload.rhs).to("cuda:0")
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, args.tolerance)
@@ -429,7 +431,7 @@ def bench_adjoint(args):
     g, gt = w.extra["log_conductivity"], w.extra["target_log_conductivity"]
     problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
     problem.assemble(gt)
-    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks"))
+    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted"))
     ut, _ = problem.solve(problem.rhs)
     problem.target = ut
     context = problem.context
@@ -587,7 +589,7 @@ def bench_pmg(args):
     rows, nnz = csr.rows, csr.nnz
     context = linalg.Context.default()
     matrix = linalg.CSRMatrix(csr, context)
-    matrix.autotune(64, 5, kernels=("tiled", "blocks"))  # the finest grid's sweeps and updates use this launch
+    matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted"))  # the finest grid's sweeps and updates use this launch
     mask = linalg.Mask(mask_values, context)
     t0 = time.perf_counter()
     pmg = linalg.PMultigrid(matrix, mask, order)
@@ -651,6 +653,8 @@ def measured_traffic(launch, profile):
         walker = f"BlockTiles<float, float, {unroll}, {width}> >"
     elif kernel == "blocks":
         walker = f"ColumnBlockTiles<float, float, {unroll}, 256, {width}> >"
+    elif kernel == "sorted":
+        walker = f"SortedBlockTiles<float, float, {unroll}, 256, {width}> >"
     else:
         walker = f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<float, float, {width}, {unroll}, 256> >"
     kernels = json.loads(path.read_text()).get("kernels", {})

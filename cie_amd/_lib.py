@@ -32,6 +32,7 @@ SPMV_VECTOR = 0
 SPMV_STAGED = 1
 SPMV_TILED = 2
 SPMV_BLOCKS = 3
+SPMV_SORTED = 4
 REORDER_NONE, REORDER_CUTHILL_MCKEE, REORDER_REVERSE_CUTHILL_MCKEE = 0, 1, 2
 PLANE_F64 = 1
 PLANE_F32 = 2

@@ -169,6 +169,14 @@ struct TileSet {
     int capacity = 0;
     int count = 0;          // tiles; the device array holds count + 1 entries
     int2* tiles = nullptr;  // {first row, first entry}
+    // Of a column-block matrix, built on first use by a sorted launch: each
+    // tile's blocks in column order (see SortedBlockTiles).
+    int sortedState = 0;    // 0 not built, 1 built, -1 a tile spans too many columns
+    int32_t* sortedBases = nullptr;    // per tile: smallest block column
+    uint32_t* sortedKeys = nullptr;    // per block: column offset and tile position
+    double* sortedValues64 = nullptr;  // per block: w values, sorted order
+    float* sortedValues32 = nullptr;
+    uint64_t sortedVersion = 0;        // the block matrix's filledVersion its values hold
 };
 
 struct cie_hip_csr_s {
@@ -212,6 +220,11 @@ int matrixWorkspace(cie_hip_csr matrix, size_t bytes, void** pointer);
 int ensureInverseDiagonal(cie_hip_csr matrix, int precision);
 // The block tiles of `capacity` entries (built and uploaded on first use).
 int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
+// The tiles of `capacity` blocks of a column-block matrix with each tile's
+// blocks sorted by column (built on first use; values refreshed with the block
+// values). Fails with CIE_HIP_ERROR_INVALID_ARGUMENT when a tile's block
+// columns span more than the keys hold.
+int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.
 int ensureNarrowColumns(cie_hip_csr matrix);

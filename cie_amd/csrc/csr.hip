@@ -120,6 +120,76 @@ __global__ __launch_bounds__(kReduceBlock) void columnBlockValuesKernel(
     }
 }
 
+// Column-sorted tiles of a column-block matrix, one block per tile of at most
+// `capacity` (<= kSortMax) blocks: the tile's blocks sorted by (first column,
+// position) with a bitonic network in LDS; writes the tile's smallest column
+// and each block's key ((column - smallest) << kSortedPositionBits | its
+// position in the tile). A tile of one row longer than the capacity is
+// skipped (it is walked unsorted). A tile whose columns span more than the
+// keys hold sets *failed.
+constexpr int kSortThreads = 1024;
+constexpr int kSortMax = 1 << kSortedPositionBits;
+
+__global__ __launch_bounds__(kSortThreads) void sortedKeysKernel(const int2* __restrict__ tiles, int capacity,
+                                                                 const int32_t* __restrict__ blockColumns,
+                                                                 int32_t* __restrict__ bases,
+                                                                 uint32_t* __restrict__ keys, int* __restrict__ failed) {
+    __shared__ uint64_t items[kSortMax];
+    const int t = blockIdx.x;
+    const int begin = tiles[t].y, count = tiles[t + 1].y - begin;
+    if (count > capacity || count == 0) {
+        if (threadIdx.x == 0) bases[t] = 0;
+        return;
+    }
+    int n = 1;
+    while (n < count) n <<= 1;
+    for (int i = threadIdx.x; i < n; i += kSortThreads)
+        items[i] = i < count ? (uint64_t(uint32_t(blockColumns[begin + i])) << kSortedPositionBits) | uint64_t(i)
+                             : ~uint64_t(0);
+    __syncthreads();
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += kSortThreads) {
+                const int partner = i ^ j;
+                if (partner > i) {
+                    const uint64_t a = items[i], b = items[partner];
+                    if ((a > b) == ((i & k) == 0)) {
+                        items[i] = b;
+                        items[partner] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t base = int64_t(items[0] >> kSortedPositionBits);
+    if (threadIdx.x == 0) {
+        bases[t] = int32_t(base);
+        if (int64_t(items[count - 1] >> kSortedPositionBits) - base > kSortedMaxSpan) atomicOr(failed, 1);
+    }
+    for (int i = threadIdx.x; i < count; i += kSortThreads) {
+        const uint64_t item = items[i];
+        const uint32_t offset = uint32_t(int64_t(item >> kSortedPositionBits) - base);
+        keys[begin + i] = (offset << kSortedPositionBits) | uint32_t(item & kSortedPositionMask);
+    }
+}
+
+// The width values of each sorted block, copied from its block (one block of
+// threads per tile; tiles walked unsorted are skipped).
+template <class T>
+__global__ __launch_bounds__(kReduceBlock) void sortedValuesKernel(const int2* __restrict__ tiles, int capacity,
+                                                                   int width, const uint32_t* __restrict__ keys,
+                                                                   const T* __restrict__ values,
+                                                                   T* __restrict__ sorted) {
+    const int t = blockIdx.x;
+    const int begin = tiles[t].y, end = tiles[t + 1].y;
+    if (end - begin > capacity) return;
+    for (int i = begin + int(threadIdx.x); i < end; i += kReduceBlock) {
+        const int64_t source = int64_t(begin) + int64_t(keys[i] & kSortedPositionMask);
+        for (int w = 0; w < width; ++w) sorted[int64_t(i) * width + w] = values[source * width + w];
+    }
+}
+
 template <class T>
 int launchProduct(cie_hip_csr m, const CsrLaunch& launch, T inScale, const T* in, T outScale, T* out,
                   const T* entries) {
@@ -157,7 +227,10 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->inverseDiagonal32) (void)hipFree(m->inverseDiagonal32);
     if (m->workspace) (void)hipFree(m->workspace);
     for (TileSet& set : m->tileSets)
-        if (set.tiles) (void)hipFree(set.tiles);
+        for (void* p : {static_cast<void*>(set.tiles), static_cast<void*>(set.sortedBases),
+                        static_cast<void*>(set.sortedKeys), static_cast<void*>(set.sortedValues64),
+                        static_cast<void*>(set.sortedValues32)})
+            if (p) (void)hipFree(p);
     if (m->narrowColumns) (void)hipFree(m->narrowColumns);
     if (m->narrowBases) (void)hipFree(m->narrowBases);
     if (m->blockEntryBegin) (void)hipFree(m->blockEntryBegin);
@@ -275,6 +348,72 @@ int ensureTiles(cie_hip_csr m, int capacity, const TileSet** tiles) {
     }
     m->tileSets.push_back(set);
     *tiles = &m->tileSets.back();
+    return CIE_HIP_OK;
+}
+
+int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles) {
+    if (!blocks->blockWidth) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles of a matrix that is not a column-block form");
+    if (capacity > kSortMax)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles hold at most " + std::to_string(kSortMax) + " blocks");
+    const TileSet* found = nullptr;
+    int status = ensureTiles(blocks, capacity, &found);
+    if (status != CIE_HIP_OK) return status;
+    TileSet& set = *const_cast<TileSet*>(found);
+    cie_hip_context ctx = blocks->context;
+    const int width = blocks->blockWidth;
+    const size_t count = size_t(std::max<int64_t>(1, blocks->nnz));
+    const auto spanError = [&]() {
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles: a tile's block columns span more than " +
+                                                            std::to_string(kSortedMaxSpan) + " columns");
+    };
+    if (set.sortedState < 0) return spanError();
+    if (set.sortedState == 0) {
+        int* failed = reinterpret_cast<int*>(ctx->scratch.scalars);
+        int* pinned = reinterpret_cast<int*>(ctx->scratch.pinned);
+        bool ok = hipMalloc(&set.sortedBases, sizeof(int32_t) * size_t(std::max(1, set.count))) == hipSuccess &&
+                  hipMalloc(&set.sortedKeys, sizeof(uint32_t) * count) == hipSuccess &&
+                  hipMalloc(&set.sortedValues64, sizeof(double) * count * width) == hipSuccess &&
+                  hipMemsetAsync(failed, 0, sizeof(int), ctx->stream) == hipSuccess;
+        if (ok && set.count) {
+            hipLaunchKernelGGL(sortedKeysKernel, dim3(set.count), dim3(kSortThreads), 0, ctx->stream, set.tiles,
+                               capacity, blocks->columnIndices, set.sortedBases, set.sortedKeys, failed);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        ok = ok && hipMemcpyAsync(pinned, failed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+             hipStreamSynchronize(ctx->stream) == hipSuccess;
+        if (!ok || pinned[0]) {
+            for (void* p : {static_cast<void*>(set.sortedBases), static_cast<void*>(set.sortedKeys),
+                            static_cast<void*>(set.sortedValues64)})
+                if (p) (void)hipFree(p);
+            set.sortedBases = nullptr;
+            set.sortedKeys = nullptr;
+            set.sortedValues64 = nullptr;
+            if (!ok) return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "sorted tiles: building the keys failed");
+            set.sortedState = -1;
+            return spanError();
+        }
+        set.sortedState = 1;
+        set.sortedVersion = std::numeric_limits<uint64_t>::max();  // values not written yet
+    }
+    if (blocks->entries32 && !set.sortedValues32) {
+        CIE_HIP_CHECK(hipMalloc(&set.sortedValues32, sizeof(float) * count * width));
+        set.sortedVersion = std::numeric_limits<uint64_t>::max();
+    }
+    if (set.sortedVersion != blocks->filledVersion) {
+        if (set.count && blocks->nnz) {
+            hipLaunchKernelGGL((sortedValuesKernel<double>), dim3(set.count), dim3(kReduceBlock), 0, ctx->stream,
+                               set.tiles, capacity, width, set.sortedKeys, blocks->entries64, set.sortedValues64);
+            CIE_HIP_CHECK(hipGetLastError());
+            if (set.sortedValues32) {
+                hipLaunchKernelGGL((sortedValuesKernel<float>), dim3(set.count), dim3(kReduceBlock), 0, ctx->stream,
+                                   set.tiles, capacity, width, set.sortedKeys,
+                                   static_cast<const float*>(blocks->entries32), set.sortedValues32);
+                CIE_HIP_CHECK(hipGetLastError());
+            }
+        }
+        set.sortedVersion = blocks->filledVersion;
+    }
+    *tiles = &set;
     return CIE_HIP_OK;
 }
 
@@ -485,8 +624,11 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
 // Frees the column-block forms no launch of the matrix uses.
 void releaseUnusedColumnBlocks(cie_hip_csr m) {
     for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
-        const bool used = (m->launch64.kernel == CIE_HIP_SPMV_BLOCKS && m->launch64.width == width) ||
-                          (m->launch32.kernel == CIE_HIP_SPMV_BLOCKS && m->launch32.width == width);
+        const auto uses = [width](const CsrLaunch& launch) {
+            return (launch.kernel == CIE_HIP_SPMV_BLOCKS || launch.kernel == CIE_HIP_SPMV_SORTED) &&
+                   launch.width == width;
+        };
+        const bool used = uses(m->launch64) || uses(m->launch32);
         if (!used && m->columnBlocks[width] && !m->columnBlocks[width]->blockUsers) {
             releaseMatrix(m->columnBlocks[width]);
             m->columnBlocks[width] = nullptr;
@@ -631,7 +773,7 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     } else if (kernel == CIE_HIP_SPMV_TILED) {
         if (!validTiles(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: tiled width must be 256, unroll 4, 8 or 16");
-    } else if (kernel == CIE_HIP_SPMV_BLOCKS) {
+    } else if (kernel == CIE_HIP_SPMV_BLOCKS || kernel == CIE_HIP_SPMV_SORTED) {
         if (!validBlocks(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: block width must be 2, 3 or 4, unroll 4, 8 or 16");
         if (matrix->columns < width)
@@ -685,7 +827,7 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
     if (precision == 32 && !matrix->entries32)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
     if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED) | (1 << CIE_HIP_SPMV_TILED) |
-                     (1 << CIE_HIP_SPMV_BLOCKS))))
+                     (1 << CIE_HIP_SPMV_BLOCKS) | (1 << CIE_HIP_SPMV_SORTED))))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: empty kernel mask");
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
@@ -713,20 +855,33 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
             candidates.push_back(c);
         }
     }
-    if (kernels & (1 << CIE_HIP_SPMV_BLOCKS)) {
+    if (kernels & ((1 << CIE_HIP_SPMV_BLOCKS) | (1 << CIE_HIP_SPMV_SORTED))) {
         // Widths whose blocks hold at most 2 entries' worth of padding per 3 entries (fill <= 5/3).
         for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth && width <= matrix->columns; ++width) {
             cie_hip_csr blocks = nullptr;
             const int status = ensureColumnBlocks(matrix, width, &blocks);
             if (status != CIE_HIP_OK) return status;
             if (3 * blocks->nnz * width > 5 * std::max<int64_t>(1, matrix->nnz)) continue;
-            for (int unroll : {4, 8, 16}) {
-                CsrLaunch c;
-                c.kernel = CIE_HIP_SPMV_BLOCKS;
-                c.width = width;
-                c.unroll = unroll;
-                c.block = kSpmvBlock;
-                candidates.push_back(c);
+            for (int kernel : {CIE_HIP_SPMV_BLOCKS, CIE_HIP_SPMV_SORTED}) {
+                if (!(kernels & (1 << kernel))) continue;
+                for (int unroll : {4, 8, 16}) {
+                    if (kernel == CIE_HIP_SPMV_SORTED) {  // sorted tiles only where every tile's columns fit the keys
+                        const TileSet* set = nullptr;
+                        const int sorted = ensureSortedTiles(blocks, kSpmvBlock * unroll, &set);
+                        if (sorted != CIE_HIP_OK) {
+                            const TileSet* plain = nullptr;
+                            if (ensureTiles(blocks, kSpmvBlock * unroll, &plain) == CIE_HIP_OK && plain->sortedState < 0)
+                                continue;  // too wide for the keys: not a candidate
+                            return sorted;
+                        }
+                    }
+                    CsrLaunch c;
+                    c.kernel = kernel;
+                    c.width = width;
+                    c.unroll = unroll;
+                    c.block = kSpmvBlock;
+                    candidates.push_back(c);
+                }
             }
         }
     }

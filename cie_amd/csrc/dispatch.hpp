@@ -80,6 +80,17 @@ inline bool validBlocks(int width, int unroll) {
 // Calls f(Walker{}) with the row walker the launch selects; false if unsupported.
 template <class TV, class TM, class F>
 bool withWalker(const CsrLaunch& launch, F&& f) {
+    if (launch.kernel == CIE_HIP_SPMV_SORTED) {
+        switch (launch.width * 32 + launch.unroll) {
+#define CIEHIP_SORTED_CASE(W, U)                                  \
+    case W * 32 + U:                                              \
+        f(SortedBlockTiles<TV, TM, U, kSpmvBlock, W>{});          \
+        return true;
+            CIEHIP_BLOCKS(CIEHIP_SORTED_CASE)
+#undef CIEHIP_SORTED_CASE
+        default: return false;
+        }
+    }
     if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {
         switch (launch.width * 32 + launch.unroll) {
 #define CIEHIP_BLOCKS_CASE(W, U)                                  \
@@ -156,8 +167,31 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     source.narrowBases = nullptr;
     source.blockValues64 = nullptr;
     source.blockValues32 = nullptr;
+    source.tileBases = nullptr;
+    source.sortedKeys = nullptr;
+    source.sortedValues64 = nullptr;
+    source.sortedValues32 = nullptr;
     const int64_t rows = m->rows;
-    if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {  // the tiled walker over the column-block form
+    if (launch.kernel == CIE_HIP_SPMV_SORTED) {  // the column-block form with column-sorted tiles
+        cie_hip_csr blocks = nullptr;
+        int status = ensureColumnBlocks(m, launch.width, &blocks);
+        if (status != CIE_HIP_OK) return status;
+        const TileSet* set = nullptr;
+        status = ensureSortedTiles(blocks, launch.block * launch.unroll, &set);
+        if (status != CIE_HIP_OK) return status;
+        source.rowExtents = blocks->rowExtents;
+        source.columnIndices = blocks->columnIndices;
+        source.tiles = set->tiles;
+        source.tileCount = set->count;
+        source.blockValues64 = blocks->entries64;
+        source.blockValues32 = blocks->entries32;
+        source.tileBases = set->sortedBases;
+        source.sortedKeys = set->sortedKeys;
+        source.sortedValues64 = set->sortedValues64;
+        source.sortedValues32 = set->sortedValues32;
+        source.param = 1;
+        grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
+    } else if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {  // the tiled walker over the column-block form
         cie_hip_csr blocks = nullptr;
         int status = ensureColumnBlocks(m, launch.width, &blocks);
         if (status != CIE_HIP_OK) return status;

@@ -149,7 +149,21 @@ struct RowSource {
     const int32_t* __restrict__ narrowBases;     // smallest column of each run of kNarrowRun entries
     const void* blockValues64;                 // column-block walker: w values per block (FP64, FP32)
     const void* blockValues32;
+    // Column-sorted block tiles: per tile the smallest block column, per block
+    // (in the tile's column order) its column offset and tile position, and
+    // its w values in that order.
+    const int32_t* __restrict__ tileBases;
+    const uint32_t* __restrict__ sortedKeys;
+    const void* sortedValues64;
+    const void* sortedValues32;
 };
+
+// Sorted-tile keys: (column - tile base) << kSortedPositionBits | position of
+// the block in the tile's column-block order. Tiles hold at most 4096 blocks,
+// and a tile's block columns may span at most 2^20 - 1 columns.
+constexpr int kSortedPositionBits = 12;
+constexpr uint32_t kSortedPositionMask = (1u << kSortedPositionBits) - 1;
+constexpr int64_t kSortedMaxSpan = (int64_t(1) << (32 - kSortedPositionBits)) - 1;
 
 // L lanes per row, BS / L rows per block step; rows split into one contiguous
 // chunk per block. Sum order: per-lane strided FMAs, then an xor butterfly.
@@ -616,6 +630,108 @@ struct ColumnBlockTiles {
             __syncthreads();
         }
         return total;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Column-sorted block tiles: the column-block walker with each tile's blocks
+// stored in the order of their first column (ties in block order) and a key
+// per block holding its column offset from the tile's smallest column and its
+// position among the tile's blocks. The lanes of a wavefront then gather
+// adjacent x values (the blocks of a tile's rows that reach one node plane lie
+// together) instead of one run per row across all the planes a row couples
+// to, so each gather instruction touches a few cache lines instead of tens.
+// Each product is staged in LDS at the block's position, so the row sums, and
+// every result bit, are those of ColumnBlockTiles over the same tiles. A row
+// longer than a tile is walked as ColumnBlockTiles walks it (its blocks are
+// already in column order).
+// ---------------------------------------------------------------------------
+
+template <class TV, class TM, int U, int BS, int W>
+struct SortedBlockTiles {
+    using Value = TV;
+    using Matrix = TM;
+    using Plain = ColumnBlockTiles<TV, TM, U, BS, W>;
+    static constexpr int kBlock = BS;
+    static constexpr int kWidth = W;
+    static constexpr int kCapacity = BS * U;  // blocks per tile
+    static_assert(kCapacity <= (1 << kSortedPositionBits), "a tile position must fit the key");
+
+    __device__ __forceinline__ static int slot() { return blockIdx.x; }
+
+    __device__ __forceinline__ static const TM* sortedValues(const RowSource& source) {
+        if constexpr (sizeof(TM) == sizeof(double)) return static_cast<const TM*>(source.sortedValues64);
+        else return static_cast<const TM*>(source.sortedValues32);
+    }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void run(const RowSource& source, const TM* __restrict__,
+                                               const TV* __restrict__ in, Epilogue&& epilogue) {
+        __shared__ TV stage[kCapacity];
+        __shared__ int32_t extents[BS + 1];
+        for (int t = blockIdx.x; t < source.tileCount; t += gridDim.x) {
+            tile(t, source, in, epilogue, stage, extents);
+            if (t + int(gridDim.x) < source.tileCount) __syncthreads();  // stage and extents are reused
+        }
+    }
+
+    // BlockTiles::tile's signature (the CSR plane argument is not read).
+    template <class Epilogue>
+    __device__ __forceinline__ static void tile(int t, const RowSource& source, const TM* __restrict__,
+                                                const TV* __restrict__ in, Epilogue& epilogue, TV* stage,
+                                                int32_t* extents) {
+        tile(t, source, in, epilogue, stage, extents);
+    }
+
+    template <class Epilogue>
+    __device__ __forceinline__ static void tile(int t, const RowSource& source, const TV* __restrict__ in,
+                                                Epilogue& epilogue, TV* stage, int32_t* extents) {
+        const int tid = threadIdx.x;
+        const int2 head = source.tiles[t];
+        const int2 next = source.tiles[t + 1];
+        const int first = head.x, rows = next.x - head.x, begin = head.y, end = next.y;
+        if (end - begin > kCapacity) {
+            const TV sum = Plain::longRow(begin, end, source.columnIndices, Plain::values(source), in, stage);
+            if (tid == 0) epilogue(first, sum);
+            return;
+        }
+        if (end == begin) {  // only empty rows: nothing to gather
+            if (tid < rows) epilogue(first + tid, TV(0));
+            return;
+        }
+        const TM* __restrict__ values = sortedValues(source);
+        const int base = source.tileBases[t];
+        const int last = end - 1;
+        // Unconditional loads, indices clamped into the tile, masked afterwards (see BlockTiles).
+        uint32_t key[U];
+        ValueRun<TM, W> block[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = min(begin + tid + BS * u, last);
+            key[u] = CIEHIP_STREAM_LOAD(source.sortedKeys + k);
+            block[u] = Plain::loadBlock(values, k);
+        }
+        if (tid < rows) extents[tid] = source.rowExtents[first + tid] - begin;
+        if (tid == 0) extents[rows] = end - begin;
+        ValueRun<TV, W> gathered[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) gathered[u] = Plain::gather(in, base + int(key[u] >> kSortedPositionBits));
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (begin + tid + BS * u < end)
+                stage[key[u] & kSortedPositionMask] = Plain::blockProduct(block[u], gathered[u]);
+        __syncthreads();
+        int group = kWave;
+        while (group > 1 && rows * group > BS) group >>= 1;
+        const int row = tid / group;
+        const int g = tid % group;
+        TV sum = TV(0);
+        if (row < rows) {
+            const int rowEnd = extents[row + 1];
+            for (int k = extents[row] + g; k < rowEnd; k += group) sum += stage[k];
+        }
+        for (int offset = group / 2; offset > 0; offset /= 2) sum += __shfl_xor(sum, offset, kWave);
+        if (row < rows && g == 0) epilogue(first + row, sum);
     }
 };
 

@@ -37,11 +37,11 @@ def _torch():
 
 
 _KERNEL_NAMES = {_lib.SPMV_VECTOR: "vector", _lib.SPMV_STAGED: "staged", _lib.SPMV_TILED: "tiled",
-                 _lib.SPMV_BLOCKS: "blocks"}
+                 _lib.SPMV_BLOCKS: "blocks", _lib.SPMV_SORTED: "sorted"}
 
 
 def _kernel_code(name):
-    """C-ABI code of an SpMV kernel name ("vector", "staged", "tiled", "blocks")."""
+    """C-ABI code of an SpMV kernel name ("vector", "staged", "tiled", "blocks", "sorted")."""
     for code, known in _KERNEL_NAMES.items():
         if known == name:
             return code
@@ -137,8 +137,9 @@ class CSRMatrix:
         code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
-    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "blocks", "staged", "vector")):
-        """Time the candidate launches of `kernels` ("staged", "vector", "tiled", "blocks") and keep the fastest."""
+    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "blocks", "sorted", "staged", "vector")):
+        """Time the candidate launches of `kernels` ("staged", "vector", "tiled", "blocks", "sorted") and keep the
+        fastest."""
         mask = sum(1 << _kernel_code(k) for k in kernels)
         best = c_double()
         check(library().cie_hip_csr_autotune(self.handle, precision, mask, repetitions, byref(best)),

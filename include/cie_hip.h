@@ -157,6 +157,15 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
 #define CIE_HIP_SPMV_STAGED 1
 #define CIE_HIP_SPMV_TILED 2
 #define CIE_HIP_SPMV_BLOCKS 3
+/*   CIE_HIP_SPMV_SORTED: the BLOCKS kernel (same width and unroll) with each
+ *     tile's blocks stored in the order of their first column and a 32-bit key
+ *     per block (column offset from the tile's smallest column, position in
+ *     the tile): adjacent lanes gather adjacent x values. Products are staged
+ *     at the block's position, so results are bitwise those of BLOCKS with the
+ *     same width and unroll. Built on the device on first use; needs every
+ *     tile's block columns to span fewer than 2^20 columns (else the launch
+ *     fails and the autotuner skips it). */
+#define CIE_HIP_SPMV_SORTED 4
 /* The tiled kernel reads a narrow copy of the column indices when every run of
  * 4096 consecutive entries spans at most 65535 columns (banded FEM matrices in
  * their natural node order do): 16-bit offsets from each run's smallest
