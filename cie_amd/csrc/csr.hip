@@ -174,8 +174,9 @@ __global__ __launch_bounds__(kSortThreads) void sortedKeysKernel(const int2* __r
     }
 }
 
-// The width values of each sorted block, copied from its block (one block of
-// threads per tile; tiles walked unsorted are skipped).
+// The width values of each sorted block, copied from its block: one block of
+// threads per tile, one thread per value in sorted order (coalesced writes,
+// reads within the tile's own span); tiles walked unsorted are skipped.
 template <class T>
 __global__ __launch_bounds__(kReduceBlock) void sortedValuesKernel(const int2* __restrict__ tiles, int capacity,
                                                                    int width, const uint32_t* __restrict__ keys,
@@ -184,9 +185,11 @@ __global__ __launch_bounds__(kReduceBlock) void sortedValuesKernel(const int2* _
     const int t = blockIdx.x;
     const int begin = tiles[t].y, end = tiles[t + 1].y;
     if (end - begin > capacity) return;
-    for (int i = begin + int(threadIdx.x); i < end; i += kReduceBlock) {
+    const int64_t first = int64_t(begin) * width, last = int64_t(end) * width;
+    for (int64_t v = first + threadIdx.x; v < last; v += kReduceBlock) {
+        const int64_t i = v / width, w = v - i * width;
         const int64_t source = int64_t(begin) + int64_t(keys[i] & kSortedPositionMask);
-        for (int w = 0; w < width; ++w) sorted[int64_t(i) * width + w] = values[source * width + w];
+        sorted[v] = values[source * width + w];
     }
 }
 
@@ -621,8 +624,30 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
     return CIE_HIP_OK;
 }
 
-// Frees the column-block forms no launch of the matrix uses.
+// Frees the column-block forms no launch of the matrix uses, and the sorted
+// copies of tile capacities no sorted launch uses (rebuilt if asked for again).
 void releaseUnusedColumnBlocks(cie_hip_csr m) {
+    for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
+        cie_hip_csr blocks = m->columnBlocks[width];
+        if (!blocks) continue;
+        for (TileSet& set : blocks->tileSets) {
+            if (set.sortedState <= 0) continue;
+            const auto sortedUse = [&](const CsrLaunch& launch) {
+                return launch.kernel == CIE_HIP_SPMV_SORTED && launch.width == width &&
+                       launch.block * launch.unroll == set.capacity;
+            };
+            if (sortedUse(m->launch64) || sortedUse(m->launch32)) continue;
+            (void)hipStreamSynchronize(m->context->stream);
+            for (void* p : {static_cast<void*>(set.sortedBases), static_cast<void*>(set.sortedKeys),
+                            static_cast<void*>(set.sortedValues64), static_cast<void*>(set.sortedValues32)})
+                if (p) (void)hipFree(p);
+            set.sortedBases = nullptr;
+            set.sortedKeys = nullptr;
+            set.sortedValues64 = nullptr;
+            set.sortedValues32 = nullptr;
+            set.sortedState = 0;
+        }
+    }
     for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
         const auto uses = [width](const CsrLaunch& launch) {
             return (launch.kernel == CIE_HIP_SPMV_BLOCKS || launch.kernel == CIE_HIP_SPMV_SORTED) &&
