@@ -22,14 +22,15 @@
 // the other), so no cross-block synchronisation is needed. The host only polls
 // the device state once per chunk of iterations.
 //
-// The mixed form keeps A, r, p, q, z and their dot products in FP32 and the
-// solution x in FP64. Whenever the FP32 residual has fallen by the refinement
-// factor since the last refinement (or looks converged, or the iteration cap
-// is near), a refinement step recomputes r = b - A x with the FP64 matrix,
-// tests convergence on that FP64 residual, and restarts the FP32 recurrence
-// from it; the search direction is kept (restart = 0) or reset (restart = 1).
-// The refinement kernel runs every trip: its blocks take the decision from
-// the update's partials and return at once when there is nothing to do.
+// The mixed form keeps A, r, p, q, z, the correction d and their dot products
+// in FP32 and the solution x in FP64. Whenever the FP32 residual of the
+// previous trip has fallen by the refinement factor since the last refinement
+// (or looks converged, or the iteration cap is near), the update folds the
+// correction into x (x += d in FP64, d = 0) and a refinement step recomputes
+// r = b - A x with the FP64 matrix, tests convergence on that FP64 residual,
+// and restarts the FP32 recurrence from it; the search direction is kept
+// (restart = 0) or reset (restart = 1). The refinement kernel runs every
+// trip and returns at once when the update did not decide to refine.
 #include "dispatch.hpp"
 #include "kernels.hpp"
 
@@ -187,31 +188,80 @@ __global__ __launch_bounds__(kVectorThreads) void updateKernel(int slot, int64_t
     }
 }
 
-// Mixed form: does this trip refine? (FP32 r.r fell by the refinement factor
-// since the last refinement, looks converged, or the iteration cap is near.)
-__device__ __forceinline__ bool refineDue(float rr, const SolverState& s) {
-    const double absolute = sqrt(double(rr));
+// Mixed form: does this trip refine? The FP32 r.r of the previous trip (as
+// the direction kernel recorded it) fell by the refinement factor since the
+// last refinement, looks converged, or the iteration cap is near.
+__device__ __forceinline__ bool refineDue(const SolverState& s) {
+    const double absolute = sqrt(s.rr);
     const double relative = absolute / s.initialNorm;
-    return double(rr) <= s.updateThreshold * s.updateNorm || floatLess(absolute, s.absoluteTolerance) ||
+    return s.rr <= s.updateThreshold * s.updateNorm || floatLess(absolute, s.absoluteTolerance) ||
            floatLess(relative, s.relativeTolerance) || s.iteration + 1 >= s.maxIterations;
 }
 
-// Mixed form, every trip: the refinement decision from the update's partials
-// (recorded for the direction kernel); if due, r = b - A x with the FP64
-// matrix and x, and partials of the FP64 r.r and the FP32 r.z.
+// Mixed form: alpha = rho / p.q; the FP32 correction d += alpha p, folded
+// into the FP64 x (and cleared) on a trip that refines; r -= alpha q;
+// partials r.r and r.z. Every block takes the refinement decision from the
+// state the previous trip left (none of which this kernel writes) and block 0
+// records it for the refinement and direction kernels.
+__global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, int64_t n, double* __restrict__ x,
+                                                                    float* __restrict__ d, float* __restrict__ r,
+                                                                    const float* __restrict__ p,
+                                                                    const float* __restrict__ q,
+                                                                    const float* __restrict__ inverse,
+                                                                    int productCount,
+                                                                    const double* __restrict__ productPartials,
+                                                                    double* __restrict__ partials,
+                                                                    SolverState* __restrict__ state) {
+    __shared__ float lds[kVectorThreads / kWave];
+    __shared__ float lds2[kVectorThreads / kWave];
+    if (state->done) return;
+    const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
+    if (!(0.0f < pq)) {
+        if (firstThread()) state->done = kNotSpd;
+        return;
+    }
+    const float alpha = float(state->rhos[slot]) / pq;
+    const bool fold = refineDue(*state);
+    if (firstThread()) {
+        state->alpha = double(alpha);
+        state->refined[slot] = fold ? 1 : 0;
+    }
+    float rr = 0.0f, rz = 0.0f;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float correction = d[i] + alpha * p[i];
+        if (fold) {
+            x[i] = x[i] + double(correction);
+            d[i] = 0.0f;
+        } else {
+            d[i] = correction;
+        }
+        const float residual = r[i] + (-alpha) * q[i];
+        r[i] = residual;
+        const float z = residual * inverse[i];
+        rr += residual * residual;
+        rz += residual * z;
+    }
+    rr = blockSum<kVectorThreads>(rr, lds);
+    rz = blockSum<kVectorThreads>(rz, lds2);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = double(rr);
+        partials[2 * blockIdx.x + 1] = double(rz);
+    }
+}
+
+// Mixed form, every trip: if the update decided to refine (and folded the
+// correction into x), r = b - A x with the FP64 matrix and x, and partials of
+// the FP64 r.r and the FP32 r.z; else nothing.
 template <class Walker>
 __global__ __launch_bounds__(Walker::kBlock) void refineKernel(
-    int slot, int updateCount, const double* __restrict__ updatePartials, RowSource source,
-    const double* __restrict__ entries, const double* __restrict__ x, const double* __restrict__ b,
-    float* __restrict__ r, const float* __restrict__ inverse, double* __restrict__ partials,
-    SolverState* __restrict__ state) {
+    int slot, RowSource source, const double* __restrict__ entries, const double* __restrict__ x,
+    const double* __restrict__ b, float* __restrict__ r, const float* __restrict__ inverse,
+    double* __restrict__ partials, SolverState* __restrict__ state) {
     constexpr int BS = Walker::kBlock;
     __shared__ double ldsRR[BS / kWave];
     __shared__ float ldsRZ[BS / kWave];
-    if (state->done) return;
-    const bool due = refineDue(sumPartialsAll<BS, float>(updatePartials, updateCount, 0, ldsRZ), *state);
-    if (firstThread()) state->refined[slot] = due ? 1 : 0;
-    if (!due) return;
+    if (state->done || !state->refined[slot]) return;
     double rr = 0.0;
     float rz = 0.0f;
     Walker::run(source, entries, x, [&](int row, double sum) {
@@ -385,8 +435,7 @@ int launchStart(cie_hip_csr m, const double* x, const double* b, TR* r, const TR
 
 // Mixed form: the refinement kernel (resident-round grid: it usually exits at
 // once). Returns its partial count in `count`.
-int launchRefine(cie_hip_csr m, int slot, int updateCount, const double* x, const double* b, float* r,
-                 int* count) {
+int launchRefine(cie_hip_csr m, int slot, const double* x, const double* b, float* r, int* count) {
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
     int grid = 0;
@@ -397,9 +446,8 @@ int launchRefine(cie_hip_csr m, int slot, int updateCount, const double* x, cons
         status = prepareLaunch(m, launch, refineKernel<Walker>, source, grid, true);
         if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
         if (status != CIE_HIP_OK) return;
-        hipLaunchKernelGGL((refineKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, slot, updateCount,
-                           ctx->scratch.vectorPartials, source, m->entries64, x, b, r, m->inverseDiagonal32,
-                           ctx->scratch.partials, ctx->scratch.state);
+        hipLaunchKernelGGL((refineKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, slot, source,
+                           m->entries64, x, b, r, m->inverseDiagonal32, ctx->scratch.partials, ctx->scratch.state);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     *count = grid;
@@ -559,7 +607,7 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     const size_t valueSize = q.mixed ? sizeof(float) : sizeof(double);
     const size_t vec64 = alignUp(sizeof(double) * n), vecR = alignUp(valueSize * n);
     void* workspace = nullptr;
-    status = matrixWorkspace(m, 2 * vec64 + 3 * vecR, &workspace);
+    status = matrixWorkspace(m, 2 * vec64 + (q.mixed ? 4 : 3) * vecR, &workspace);
     if (status) return status;
     char* base = static_cast<char*>(workspace);
     double* solution = reinterpret_cast<double*>(base);
@@ -567,6 +615,7 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     void* r = base + 2 * vec64;
     void* p = base + 2 * vec64 + vecR;
     void* qv = base + 2 * vec64 + 2 * vecR;
+    float* correction = q.mixed ? reinterpret_cast<float*>(base + 2 * vec64 + 3 * vecR) : nullptr;
 
     SolverState init{};
     init.absoluteTolerance = q.settings.absoluteResidual;
@@ -616,16 +665,18 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         float* q32 = static_cast<float*>(qv);
         status = launchStart<float>(m, solution, q.b, r32, m->inverseDiagonal32, p32);
         if (status) return status;
+        hipLaunchKernelGGL((vectorKernel<float>), dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, int(kFill),
+                           n, correction, static_cast<const float*>(nullptr), 0.0f);
         status = runLoop(ctx, timer, [&]() -> int {
             const int slot = trip++ & 1;
             PartialSource sums{};
             int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, &sums);
             if (s) return s;
-            hipLaunchKernelGGL((updateKernel<float, double>), vgrid, vblock, 0, ctx->stream, slot, n, solution, r32,
+            hipLaunchKernelGGL(mixedUpdateKernel, vgrid, vblock, 0, ctx->stream, slot, n, solution, correction, r32,
                                p32, q32, m->inverseDiagonal32, sums.count, sums.partials, ctx->scratch.vectorPartials,
                                ctx->scratch.state);
             int refineCount = 0;
-            s = launchRefine(m, slot, updateCount, solution, q.b, r32, &refineCount);
+            s = launchRefine(m, slot, solution, q.b, r32, &refineCount);
             if (s) return s;
             hipLaunchKernelGGL((directionKernel<float, true>), vgrid, vblock, 0, ctx->stream, slot, n, p32, r32,
                                m->inverseDiagonal32, updateCount, ctx->scratch.vectorPartials, refineCount,
