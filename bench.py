@@ -342,6 +342,10 @@ def _roofline(kernel, bytes_per_launch, seconds, note):
             "bytes_per_launch": bytes_per_launch, "mean_launch_seconds": seconds, "bytes": note}
 
 
+_BATCH_WALKER = {"tiled": 0, "blocks": 3, "sorted": -3}  # the walker code of the batch's tile format
+_BATCH_TILES = {"tiled": "entries", "blocks": "1 x 3 column blocks", "sorted": "1 x 3 column blocks, column-sorted"}
+
+
 def bench_batch(args):
     """256 uneven 2D Q2 systems (n_i in [16, 256], seed 3) solved together, mixed-refined to 1e-10."""
     import numpy as np
@@ -406,14 +410,14 @@ def bench_batch(args):
           "f32 inner / f64 refinement",
           {"workload": "256 2D Q2 Laplace systems, n_i ~ U{16..256} (seed 3), k_e lognormal(0, 0.5), b ~ U(-1,1), "
                        "relative residual 1e-10 each", "rows": rows, "nnz": nnz, "systems": 256},
-          _roofline(f"batchProductKernel<float, {batch.column_blocks()}> (all systems active; tiles of "
-                    f"{'1 x 3 column blocks' if batch.column_blocks() else 'entries'})", algorithmic,
+          _roofline(f"batchProductKernel<float, {_BATCH_WALKER[batch.tile_kernel()]}> (all systems active; tiles of "
+                    f"{_BATCH_TILES[batch.tile_kernel()]})", algorithmic,
                     profiled.productSeconds, "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix"),
           {"value": sampled_seconds * total_work / sampled_work, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG (ConjugateGradients_impl.hpp:49-201) on the first {sampled} systems "
                      f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
           {"device_seconds": summary.seconds, "max_iterations": int(iterations.max()),
-           "column_blocks": batch.column_blocks(),
+           "tile_format": batch.tile_kernel(),
            "refinements": int(refinements.sum()), "fp64": {"device_seconds": fp64_summary.seconds,
                                                            "max_iterations": int(fp64_iterations.max())}})
 

@@ -180,6 +180,8 @@ _SIGNATURES = {
     "cie_hip_batch_autotune": (c_int, [_P, c_int, c_int]),
     "cie_hip_batch_set_column_blocks": (c_int, [_P, c_int]),
     "cie_hip_batch_get_column_blocks": (c_int, [_P, POINTER(c_int)]),
+    "cie_hip_batch_set_tile_kernel": (c_int, [_P, c_int]),
+    "cie_hip_batch_get_tile_kernel": (c_int, [_P, POINTER(c_int)]),
     "cie_hip_pcg_batched": (c_int, [_P, _P, _P, _P, POINTER(MixedSettings), c_int, c_int, _P, _P, _P,
                                     POINTER(SolveInfo)]),
     # adjoint

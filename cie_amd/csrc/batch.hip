@@ -47,10 +47,13 @@ constexpr int kDefaultTileRows = 256;
 
 constexpr int kBatchBlockWidth = 3;  // column-block width of a batch that walks blocks
 
-// The tile walker of a batch: matrix entries (W = 0) or 1 x W column blocks.
+// The tile walker of a batch: matrix entries (W = 0), 1 x W column blocks
+// (W > 0) or 1 x -W column blocks in column-sorted tiles (W < 0).
 template <class T, int W>
-using BatchTiles = std::conditional_t<W == 0, BlockTiles<T, T, kTileUnroll, kTileBlock>,
-                                      ColumnBlockTiles<T, T, kTileUnroll, kTileBlock, (W > 0 ? W : 2)>>;
+using BatchTiles = std::conditional_t<
+    W == 0, BlockTiles<T, T, kTileUnroll, kTileBlock>,
+    std::conditional_t<(W > 0), ColumnBlockTiles<T, T, kTileUnroll, kTileBlock, (W > 0 ? W : 2)>,
+                       SortedBlockTiles<T, T, kTileUnroll, kTileBlock, (W < 0 ? -W : 2)>>>;
 
 struct SystemState {
     double rho, alpha, beta, rr, initialNorm, absolute, relative, updateNorm;
@@ -450,6 +453,8 @@ struct cie_hip_batch_s {
     size_t workspaceBytes = 0;
     hipGraphExec_t graph = nullptr;
     int blockWidth = 0;                 // 0: tiles of entries; kBatchBlockWidth: tiles of column blocks
+    bool sortedTiles = false;           // tiles of column blocks stored in column order (blockWidth > 0)
+    SortedCopy sorted;                  // the sorted copy of the block form over the batch's tiles
     int graphKey[5] = {0, 0, 0, 0, 0};  // what the captured graph was built for
     const void* graphPointers[3] = {nullptr, nullptr, nullptr};  // b, x, inverse diagonal
 };
@@ -459,6 +464,11 @@ namespace {
 void releaseGraph(cie_hip_batch batch) {
     if (batch->graph) (void)hipGraphExecDestroy(batch->graph);
     batch->graph = nullptr;
+}
+
+// The walker code of the batch's tile format (the W of BatchTiles).
+int walkerCode(const cie_hip_batch_s* batch) {
+    return batch->blockWidth ? (batch->sortedTiles ? -batch->blockWidth : batch->blockWidth) : 0;
 }
 
 template <class T>
@@ -505,6 +515,7 @@ int buildWork(cie_hip_batch batch) {
     batch->chunkCount = int(chunkRows.size());
     tiles.push_back(make_int2(rows, extents[rows]));
     chunkRows.push_back(rows);
+    releaseSortedCopy(batch->context, batch->sorted);  // it follows the tiles
     freeDevice(batch->tiles);
     freeDevice(batch->chunkRows);
     freeDevice(batch->tilePartials);
@@ -536,9 +547,21 @@ int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
         const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &m);
         if (status != CIE_HIP_OK) return status;
     }
-    const int status = ensureNarrowColumns(m);
+    int status = ensureNarrowColumns(m);
     if (status != CIE_HIP_OK) return status;
     RowSource& source = layout->source;
+    source.tileBases = nullptr;
+    source.sortedKeys = nullptr;
+    source.sortedValues64 = nullptr;
+    source.sortedValues32 = nullptr;
+    if (batch->blockWidth && batch->sortedTiles) {
+        status = ensureSortedCopy(m, batch->tiles, batch->tileCount, kTileEntries, batch->sorted);
+        if (status != CIE_HIP_OK) return status;
+        source.tileBases = batch->sorted.bases;
+        source.sortedKeys = batch->sorted.keys;
+        source.sortedValues64 = batch->sorted.values64;
+        source.sortedValues32 = batch->sorted.values32;
+    }
     source.blockValues64 = batch->blockWidth ? m->entries64 : nullptr;
     source.blockValues32 = batch->blockWidth ? m->entries32 : nullptr;
     source.rows = int(m->rows);
@@ -578,6 +601,7 @@ void releaseBatch(cie_hip_batch batch) {
         (void)hipStreamSynchronize(batch->context->stream);
     }
     releaseGraph(batch);
+    releaseSortedCopy(batch->context, batch->sorted);
     freeDevice(batch->tiles);
     freeDevice(batch->systemTiles);
     freeDevice(batch->chunkRows);
@@ -600,9 +624,16 @@ void releaseBatch(cie_hip_batch batch) {
     delete batch;
 }
 
-// Switches the batch between tiles of entries (0) and of column blocks; the
-// block form in use is pinned so the matrix's own autotuner keeps it.
-int setBlockWidth(cie_hip_batch batch, int width) {
+// Switches the batch between tiles of entries (0) and of column blocks,
+// sorted or not; the block form in use is pinned so the matrix's own
+// autotuner keeps it.
+int setBlockWidth(cie_hip_batch batch, int width, bool sorted = false) {
+    sorted = sorted && width;
+    if (sorted != batch->sortedTiles) {
+        releaseSortedCopy(batch->context, batch->sorted);
+        batch->sortedTiles = sorted;
+        releaseGraph(batch);
+    }
     if (width == batch->blockWidth) return CIE_HIP_OK;
     cie_hip_csr m = batch->matrix;
     if (width) {
@@ -616,26 +647,29 @@ int setBlockWidth(cie_hip_batch batch, int width) {
     return buildWork(batch);  // the tiles always match the walker
 }
 
+// Calls f(integral_constant<W>) for the batch's walker code.
+template <class F>
+void withWalkerCode(int code, F&& f) {
+    if (code == kBatchBlockWidth) f(std::integral_constant<int, kBatchBlockWidth>{});
+    else if (code == -kBatchBlockWidth) f(std::integral_constant<int, -kBatchBlockWidth>{});
+    else f(std::integral_constant<int, 0>{});
+}
+
 // The batched product over tiles [t0, t0 + tiles) (the autotuner's timing runs).
-void launchTileRange(int width, int precision, int tiles, hipStream_t stream, const RowSource& source, int t0,
+void launchTileRange(int code, int precision, int tiles, hipStream_t stream, const RowSource& source, int t0,
                      cie_hip_csr m, void* buffer) {
-    if (precision == 32) {
-        float* in = static_cast<float*>(buffer);
-        if (width)
-            hipLaunchKernelGGL((batchTileRangeKernel<float, kBatchBlockWidth>), dim3(tiles), dim3(kTileBlock), 0, stream,
-                               source, t0, m->entries32, in, in + m->rows);
-        else
-            hipLaunchKernelGGL((batchTileRangeKernel<float, 0>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
+    withWalkerCode(code, [&](auto w) {
+        constexpr int W = decltype(w)::value;
+        if (precision == 32) {
+            float* in = static_cast<float*>(buffer);
+            hipLaunchKernelGGL((batchTileRangeKernel<float, W>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
                                m->entries32, in, in + m->rows);
-    } else {
-        double* in = static_cast<double*>(buffer);
-        if (width)
-            hipLaunchKernelGGL((batchTileRangeKernel<double, kBatchBlockWidth>), dim3(tiles), dim3(kTileBlock), 0,
-                               stream, source, t0, m->entries64, in, in + m->rows);
-        else
-            hipLaunchKernelGGL((batchTileRangeKernel<double, 0>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
+        } else {
+            double* in = static_cast<double*>(buffer);
+            hipLaunchKernelGGL((batchTileRangeKernel<double, W>), dim3(tiles), dim3(kTileBlock), 0, stream, source, t0,
                                m->entries64, in, in + m->rows);
-    }
+        }
+    });
 }
 
 // Rebuilds a work list from the system states.
@@ -695,8 +729,9 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
 template <class TR>
 void enqueueTrip(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
                  const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
-    if (batch->blockWidth) enqueueIteration<TR, kBatchBlockWidth>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
-    else enqueueIteration<TR, 0>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+    withWalkerCode(walkerCode(batch), [&](auto w) {
+        enqueueIteration<TR, decltype(w)::value>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+    });
 }
 
 template <class TR>
@@ -740,23 +775,20 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
     const ActiveWork& work = batch->running;
     launchSelect<kRunning>(batch, work);  // every system runs
-    if (batch->blockWidth)
-        hipLaunchKernelGGL((batchResidualKernel<TR, true, kBatchBlockWidth>),
-                           workGrid(batchResidualKernel<TR, true, kBatchBlockWidth>, ctx, kTileBlock, batch->tileCount),
+    withWalkerCode(walkerCode(batch), [&](auto w) {
+        constexpr int W = decltype(w)::value;
+        hipLaunchKernelGGL((batchResidualKernel<TR, true, W>),
+                           workGrid(batchResidualKernel<TR, true, W>, ctx, kTileBlock, batch->tileCount),
                            dim3(kTileBlock), 0, ctx->stream, layout, work, m->entries64, x, b, r, inverse, p,
                            batch->tilePartials, batch->state);
-    else
-        hipLaunchKernelGGL((batchResidualKernel<TR, true, 0>),
-                           workGrid(batchResidualKernel<TR, true, 0>, ctx, kTileBlock, batch->tileCount),
-                           dim3(kTileBlock), 0, ctx->stream, layout, work, m->entries64, x, b, r, inverse, p,
-                           batch->tilePartials, batch->state);
+    });
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseStart>), dim3(batch->systems), dim3(kTileBlock), 0, ctx->stream,
                        batch->systemTiles, batch->tilePartials, batch->state, settings, batch->active);
     launchSelect<kRunning>(batch, work);
     CIE_HIP_CHECK(hipGetLastError());
 
     if (useGraph) {
-        const int key[5] = {mixed ? 1 : 0, settings.restart, int(sizeof(TR)), batch->tileCount, batch->blockWidth};
+        const int key[5] = {mixed ? 1 : 0, settings.restart, int(sizeof(TR)), batch->tileCount, walkerCode(batch)};
         const bool same = batch->graph && std::equal(key, key + 5, batch->graphKey) && batch->graphPointers[0] == b &&
                           batch->graphPointers[1] == x && batch->graphPointers[2] == inverse;
         if (!same) {
@@ -916,8 +948,9 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
         return setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing set-up failed");
     }
     const hipEvent_t start = events.event[0], stop = events.event[1];
-    // Formats: tiles of entries, and tiles of 1 x 3 column blocks where their padding stays small (fill <= 5/3).
-    std::vector<int> widths{0};
+    // Formats (walker codes): tiles of entries, and tiles of 1 x 3 column blocks, plain or column-sorted, where
+    // their padding stays small (fill <= 5/3).
+    std::vector<int> codes{0};
     if (m->columns >= kBatchBlockWidth) {
         cie_hip_csr form = nullptr;
         const int status = ensureColumnBlocks(m, kBatchBlockWidth, &form);
@@ -925,23 +958,32 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
             (void)hipFree(buffer);
             return status;
         }
-        if (3 * form->nnz * kBatchBlockWidth <= 5 * std::max<int64_t>(1, m->nnz)) widths.push_back(kBatchBlockWidth);
+        if (3 * form->nnz * kBatchBlockWidth <= 5 * std::max<int64_t>(1, m->nnz)) {
+            codes.push_back(kBatchBlockWidth);
+            codes.push_back(-kBatchBlockWidth);
+        }
     }
     double bestTotal = std::numeric_limits<double>::infinity();
-    int bestWidth = 0;
+    int bestCode = 0;
     std::vector<int32_t> bestChoice(size_t(batch->systems), kDefaultTileRows);
     int status = CIE_HIP_OK;
-    for (int width : widths) {
-        status = setBlockWidth(batch, width);
+    for (int code : codes) {
+        status = setBlockWidth(batch, std::abs(code), code < 0);
         if (status) break;
         std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
         std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
+        bool skipped = false;
         for (int rows : {32, 64, 128, 256}) {
             batch->tileRows.assign(size_t(batch->systems), rows);
             status = buildWork(batch);
             if (status) break;
             BatchLayout layout{};
             status = batchLayout(batch, &layout);
+            if (status && code < 0 && batch->sorted.state < 0) {  // tiles too wide for the sorted keys
+                status = CIE_HIP_OK;
+                skipped = true;
+                break;
+            }
             if (status) break;
             std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
             if (hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
@@ -954,7 +996,7 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
                 for (int pass = 0; pass < 2; ++pass) {
                     (void)hipEventRecord(start, ctx->stream);
                     for (int r = 0; r < (pass ? repetitions : 1); ++r)
-                        launchTileRange(width, precision, tiles, ctx->stream, layout.source, t0, m, buffer);
+                        launchTileRange(code, precision, tiles, ctx->stream, layout.source, t0, m, buffer);
                     if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
                         status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing failed");
                         break;
@@ -970,20 +1012,37 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
             if (status) break;
         }
         if (status) break;
+        if (skipped) continue;
         double total = 0.0;
         for (double t : best) total += t;
         if (total < bestTotal) {
             bestTotal = total;
-            bestWidth = width;
+            bestCode = code;
             bestChoice = choice;
         }
     }
     (void)hipFree(buffer);
     if (status) return status;
-    status = setBlockWidth(batch, bestWidth);
+    status = setBlockWidth(batch, std::abs(bestCode), bestCode < 0);
     if (status) return status;
     batch->tileRows = bestChoice;
     return buildWork(batch);
+}
+
+int cie_hip_batch_get_tile_kernel(cie_hip_batch batch, int* kernel) {
+    if (!batch || !kernel) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_get_tile_kernel: null argument");
+    *kernel = !batch->blockWidth ? CIE_HIP_SPMV_TILED : batch->sortedTiles ? CIE_HIP_SPMV_SORTED : CIE_HIP_SPMV_BLOCKS;
+    return CIE_HIP_OK;
+}
+
+int cie_hip_batch_set_tile_kernel(cie_hip_batch batch, int kernel) {
+    if (!batch || (kernel != CIE_HIP_SPMV_TILED && kernel != CIE_HIP_SPMV_BLOCKS && kernel != CIE_HIP_SPMV_SORTED))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT,
+                        "cie_hip_batch_set_tile_kernel: kernel must be CIE_HIP_SPMV_TILED, _BLOCKS or _SORTED");
+    if (kernel != CIE_HIP_SPMV_TILED && batch->matrix->columns < kBatchBlockWidth)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_tile_kernel: fewer columns than a block");
+    CIE_HIP_CHECK(hipSetDevice(batch->context->device));
+    return setBlockWidth(batch, kernel == CIE_HIP_SPMV_TILED ? 0 : kBatchBlockWidth, kernel == CIE_HIP_SPMV_SORTED);
 }
 
 int cie_hip_batch_get_column_blocks(cie_hip_batch batch, int* width) {
@@ -997,7 +1056,7 @@ int cie_hip_batch_set_column_blocks(cie_hip_batch batch, int width) {
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_column_blocks: width must be 0 or " +
                                                             std::to_string(kBatchBlockWidth));
     CIE_HIP_CHECK(hipSetDevice(batch->context->device));
-    return setBlockWidth(batch, width);
+    return setBlockWidth(batch, width, false);
 }
 
 int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const double* b, double* x,

@@ -164,19 +164,23 @@ struct CsrLaunch {
     int block = 0;    // threads per block
 };
 
+// A column-block matrix's blocks in column order within each tile of a tile
+// list (see SortedBlockTiles), built on the device on first use.
+struct SortedCopy {
+    int state = 0;               // 0 not built, 1 built, -1 a tile spans too many columns
+    int32_t* bases = nullptr;    // per tile: smallest block column
+    uint32_t* keys = nullptr;    // per block: column offset and tile position
+    double* values64 = nullptr;  // per block: w values, sorted order
+    float* values32 = nullptr;
+    uint64_t version = 0;        // the block matrix's filledVersion its values hold
+};
+
 // Block tiles of a matrix for one tile capacity (entries per tile).
 struct TileSet {
     int capacity = 0;
     int count = 0;          // tiles; the device array holds count + 1 entries
     int2* tiles = nullptr;  // {first row, first entry}
-    // Of a column-block matrix, built on first use by a sorted launch: each
-    // tile's blocks in column order (see SortedBlockTiles).
-    int sortedState = 0;    // 0 not built, 1 built, -1 a tile spans too many columns
-    int32_t* sortedBases = nullptr;    // per tile: smallest block column
-    uint32_t* sortedKeys = nullptr;    // per block: column offset and tile position
-    double* sortedValues64 = nullptr;  // per block: w values, sorted order
-    float* sortedValues32 = nullptr;
-    uint64_t sortedVersion = 0;        // the block matrix's filledVersion its values hold
+    SortedCopy sorted;      // of a column-block matrix, built on first use by a sorted launch
 };
 
 struct cie_hip_csr_s {
@@ -220,10 +224,16 @@ int matrixWorkspace(cie_hip_csr matrix, size_t bytes, void** pointer);
 int ensureInverseDiagonal(cie_hip_csr matrix, int precision);
 // The block tiles of `capacity` entries (built and uploaded on first use).
 int ensureTiles(cie_hip_csr matrix, int capacity, const TileSet** tiles);
-// The tiles of `capacity` blocks of a column-block matrix with each tile's
-// blocks sorted by column (built on first use; values refreshed with the block
-// values). Fails with CIE_HIP_ERROR_INVALID_ARGUMENT when a tile's block
-// columns span more than the keys hold.
+// The sorted copy of a column-block matrix over `tiles` (tileCount tiles of at
+// most `capacity` blocks; a longer tile holds one row and is walked unsorted):
+// keys built on first use, values refreshed whenever the block values changed.
+// Fails with CIE_HIP_ERROR_INVALID_ARGUMENT when a tile's block columns span
+// more than the keys hold. The caller keeps `copy` with its tile list.
+int ensureSortedCopy(cie_hip_csr blocks, const int2* tiles, int tileCount, int capacity, SortedCopy& copy);
+// Frees a sorted copy (it is rebuilt on its next use); waits for the stream.
+void releaseSortedCopy(cie_hip_context context, SortedCopy& copy);
+// The tiles of `capacity` blocks of a column-block matrix with their sorted
+// copy (see ensureSortedCopy).
 int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.

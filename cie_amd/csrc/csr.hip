@@ -229,11 +229,10 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->inverseDiagonal64) (void)hipFree(m->inverseDiagonal64);
     if (m->inverseDiagonal32) (void)hipFree(m->inverseDiagonal32);
     if (m->workspace) (void)hipFree(m->workspace);
-    for (TileSet& set : m->tileSets)
-        for (void* p : {static_cast<void*>(set.tiles), static_cast<void*>(set.sortedBases),
-                        static_cast<void*>(set.sortedKeys), static_cast<void*>(set.sortedValues64),
-                        static_cast<void*>(set.sortedValues32)})
-            if (p) (void)hipFree(p);
+    for (TileSet& set : m->tileSets) {
+        if (set.tiles) (void)hipFree(set.tiles);
+        releaseSortedCopy(m->context, set.sorted);
+    }
     if (m->narrowColumns) (void)hipFree(m->narrowColumns);
     if (m->narrowBases) (void)hipFree(m->narrowBases);
     if (m->blockEntryBegin) (void)hipFree(m->blockEntryBegin);
@@ -354,14 +353,22 @@ int ensureTiles(cie_hip_csr m, int capacity, const TileSet** tiles) {
     return CIE_HIP_OK;
 }
 
-int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles) {
+void releaseSortedCopy(cie_hip_context context, SortedCopy& copy) {
+    if (!copy.bases && !copy.keys && !copy.values64 && !copy.values32) {
+        copy.state = copy.state < 0 ? copy.state : 0;
+        return;
+    }
+    if (context) (void)hipStreamSynchronize(context->stream);
+    for (void* p : {static_cast<void*>(copy.bases), static_cast<void*>(copy.keys), static_cast<void*>(copy.values64),
+                    static_cast<void*>(copy.values32)})
+        if (p) (void)hipFree(p);
+    copy = SortedCopy{};
+}
+
+int ensureSortedCopy(cie_hip_csr blocks, const int2* tiles, int tileCount, int capacity, SortedCopy& copy) {
     if (!blocks->blockWidth) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles of a matrix that is not a column-block form");
     if (capacity > kSortMax)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles hold at most " + std::to_string(kSortMax) + " blocks");
-    const TileSet* found = nullptr;
-    int status = ensureTiles(blocks, capacity, &found);
-    if (status != CIE_HIP_OK) return status;
-    TileSet& set = *const_cast<TileSet*>(found);
     cie_hip_context ctx = blocks->context;
     const int width = blocks->blockWidth;
     const size_t count = size_t(std::max<int64_t>(1, blocks->nnz));
@@ -369,53 +376,58 @@ int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles) {
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "sorted tiles: a tile's block columns span more than " +
                                                             std::to_string(kSortedMaxSpan) + " columns");
     };
-    if (set.sortedState < 0) return spanError();
-    if (set.sortedState == 0) {
+    if (copy.state < 0) return spanError();
+    if (copy.state == 0) {
         int* failed = reinterpret_cast<int*>(ctx->scratch.scalars);
         int* pinned = reinterpret_cast<int*>(ctx->scratch.pinned);
-        bool ok = hipMalloc(&set.sortedBases, sizeof(int32_t) * size_t(std::max(1, set.count))) == hipSuccess &&
-                  hipMalloc(&set.sortedKeys, sizeof(uint32_t) * count) == hipSuccess &&
-                  hipMalloc(&set.sortedValues64, sizeof(double) * count * width) == hipSuccess &&
+        bool ok = hipMalloc(&copy.bases, sizeof(int32_t) * size_t(std::max(1, tileCount))) == hipSuccess &&
+                  hipMalloc(&copy.keys, sizeof(uint32_t) * count) == hipSuccess &&
+                  hipMalloc(&copy.values64, sizeof(double) * count * width) == hipSuccess &&
                   hipMemsetAsync(failed, 0, sizeof(int), ctx->stream) == hipSuccess;
-        if (ok && set.count) {
-            hipLaunchKernelGGL(sortedKeysKernel, dim3(set.count), dim3(kSortThreads), 0, ctx->stream, set.tiles,
-                               capacity, blocks->columnIndices, set.sortedBases, set.sortedKeys, failed);
+        if (ok && tileCount) {
+            hipLaunchKernelGGL(sortedKeysKernel, dim3(tileCount), dim3(kSortThreads), 0, ctx->stream, tiles, capacity,
+                               blocks->columnIndices, copy.bases, copy.keys, failed);
             ok = hipGetLastError() == hipSuccess;
         }
         ok = ok && hipMemcpyAsync(pinned, failed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
              hipStreamSynchronize(ctx->stream) == hipSuccess;
         if (!ok || pinned[0]) {
-            for (void* p : {static_cast<void*>(set.sortedBases), static_cast<void*>(set.sortedKeys),
-                            static_cast<void*>(set.sortedValues64)})
-                if (p) (void)hipFree(p);
-            set.sortedBases = nullptr;
-            set.sortedKeys = nullptr;
-            set.sortedValues64 = nullptr;
+            releaseSortedCopy(ctx, copy);
             if (!ok) return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "sorted tiles: building the keys failed");
-            set.sortedState = -1;
+            copy.state = -1;
             return spanError();
         }
-        set.sortedState = 1;
-        set.sortedVersion = std::numeric_limits<uint64_t>::max();  // values not written yet
+        copy.state = 1;
+        copy.version = std::numeric_limits<uint64_t>::max();  // values not written yet
     }
-    if (blocks->entries32 && !set.sortedValues32) {
-        CIE_HIP_CHECK(hipMalloc(&set.sortedValues32, sizeof(float) * count * width));
-        set.sortedVersion = std::numeric_limits<uint64_t>::max();
+    if (blocks->entries32 && !copy.values32) {
+        CIE_HIP_CHECK(hipMalloc(&copy.values32, sizeof(float) * count * width));
+        copy.version = std::numeric_limits<uint64_t>::max();
     }
-    if (set.sortedVersion != blocks->filledVersion) {
-        if (set.count && blocks->nnz) {
-            hipLaunchKernelGGL((sortedValuesKernel<double>), dim3(set.count), dim3(kReduceBlock), 0, ctx->stream,
-                               set.tiles, capacity, width, set.sortedKeys, blocks->entries64, set.sortedValues64);
+    if (copy.version != blocks->filledVersion) {
+        if (tileCount && blocks->nnz) {
+            hipLaunchKernelGGL((sortedValuesKernel<double>), dim3(tileCount), dim3(kReduceBlock), 0, ctx->stream, tiles,
+                               capacity, width, copy.keys, blocks->entries64, copy.values64);
             CIE_HIP_CHECK(hipGetLastError());
-            if (set.sortedValues32) {
-                hipLaunchKernelGGL((sortedValuesKernel<float>), dim3(set.count), dim3(kReduceBlock), 0, ctx->stream,
-                                   set.tiles, capacity, width, set.sortedKeys,
-                                   static_cast<const float*>(blocks->entries32), set.sortedValues32);
+            if (copy.values32) {
+                hipLaunchKernelGGL((sortedValuesKernel<float>), dim3(tileCount), dim3(kReduceBlock), 0, ctx->stream,
+                                   tiles, capacity, width, copy.keys, static_cast<const float*>(blocks->entries32),
+                                   copy.values32);
                 CIE_HIP_CHECK(hipGetLastError());
             }
         }
-        set.sortedVersion = blocks->filledVersion;
+        copy.version = blocks->filledVersion;
     }
+    return CIE_HIP_OK;
+}
+
+int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles) {
+    const TileSet* found = nullptr;
+    int status = ensureTiles(blocks, capacity, &found);
+    if (status != CIE_HIP_OK) return status;
+    TileSet& set = *const_cast<TileSet*>(found);
+    status = ensureSortedCopy(blocks, set.tiles, set.count, capacity, set.sorted);
+    if (status != CIE_HIP_OK) return status;
     *tiles = &set;
     return CIE_HIP_OK;
 }
@@ -631,21 +643,12 @@ void releaseUnusedColumnBlocks(cie_hip_csr m) {
         cie_hip_csr blocks = m->columnBlocks[width];
         if (!blocks) continue;
         for (TileSet& set : blocks->tileSets) {
-            if (set.sortedState <= 0) continue;
+            if (set.sorted.state <= 0) continue;
             const auto sortedUse = [&](const CsrLaunch& launch) {
                 return launch.kernel == CIE_HIP_SPMV_SORTED && launch.width == width &&
                        launch.block * launch.unroll == set.capacity;
             };
-            if (sortedUse(m->launch64) || sortedUse(m->launch32)) continue;
-            (void)hipStreamSynchronize(m->context->stream);
-            for (void* p : {static_cast<void*>(set.sortedBases), static_cast<void*>(set.sortedKeys),
-                            static_cast<void*>(set.sortedValues64), static_cast<void*>(set.sortedValues32)})
-                if (p) (void)hipFree(p);
-            set.sortedBases = nullptr;
-            set.sortedKeys = nullptr;
-            set.sortedValues64 = nullptr;
-            set.sortedValues32 = nullptr;
-            set.sortedState = 0;
+            if (!sortedUse(m->launch64) && !sortedUse(m->launch32)) releaseSortedCopy(m->context, set.sorted);
         }
     }
     for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
@@ -895,7 +898,7 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
                         const int sorted = ensureSortedTiles(blocks, kSpmvBlock * unroll, &set);
                         if (sorted != CIE_HIP_OK) {
                             const TileSet* plain = nullptr;
-                            if (ensureTiles(blocks, kSpmvBlock * unroll, &plain) == CIE_HIP_OK && plain->sortedState < 0)
+                            if (ensureTiles(blocks, kSpmvBlock * unroll, &plain) == CIE_HIP_OK && plain->sorted.state < 0)
                                 continue;  // too wide for the keys: not a candidate
                             return sorted;
                         }

@@ -185,10 +185,10 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
         source.tileCount = set->count;
         source.blockValues64 = blocks->entries64;
         source.blockValues32 = blocks->entries32;
-        source.tileBases = set->sortedBases;
-        source.sortedKeys = set->sortedKeys;
-        source.sortedValues64 = set->sortedValues64;
-        source.sortedValues32 = set->sortedValues32;
+        source.tileBases = set->sorted.bases;
+        source.sortedKeys = set->sorted.keys;
+        source.sortedValues64 = set->sorted.values64;
+        source.sortedValues32 = set->sorted.values32;
         source.param = 1;
         grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
     } else if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {  // the tiled walker over the column-block form

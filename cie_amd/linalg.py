@@ -798,6 +798,15 @@ class Batch:
     def set_column_blocks(self, width):
         check(library().cie_hip_batch_set_column_blocks(self.handle, int(width)), "cie_hip_batch_set_column_blocks")
 
+    def tile_kernel(self):
+        """Tile format of the batch's products: "tiled" (entries), "blocks" or "sorted" (1 x 3 column blocks)."""
+        kernel = c_int()
+        check(library().cie_hip_batch_get_tile_kernel(self.handle, byref(kernel)), "cie_hip_batch_get_tile_kernel")
+        return _KERNEL_NAMES[kernel.value]
+
+    def set_tile_kernel(self, kernel):
+        check(library().cie_hip_batch_set_tile_kernel(self.handle, _kernel_code(kernel)), "cie_hip_batch_set_tile_kernel")
+
     def tile_rows(self):
         out = np.empty(self.systems, dtype=np.int32)
         check(library().cie_hip_batch_get_tile_rows(self.handle, out.ctypes.data_as(c_void_p)))

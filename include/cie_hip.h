@@ -258,11 +258,13 @@ int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScal
  * rowsPerTile[s] rows: 32, 64, 128 or 256, chosen per system by
  * cie_hip_batch_autotune from timed runs; default 256); the vector steps walk
  * chunks of 1024 rows of one system. After every step the finished systems are
- * dropped from the work lists on the device. The tiles hold matrix entries,
- * or 1 x 3 column blocks (CIE_HIP_SPMV_BLOCKS, at most 2048 per tile):
- * cie_hip_batch_autotune times both where the blocks' padding is small and
- * keeps the faster, cie_hip_batch_set_column_blocks (width 3, or 0 for
- * entries) sets it. */
+ * dropped from the work lists on the device. The tiles hold matrix entries
+ * (CIE_HIP_SPMV_TILED), 1 x 3 column blocks (CIE_HIP_SPMV_BLOCKS, at most 2048
+ * per tile) or the same blocks in column-sorted tiles (CIE_HIP_SPMV_SORTED,
+ * results bitwise those of BLOCKS): cie_hip_batch_autotune times them where
+ * the blocks' padding is small and keeps the fastest;
+ * cie_hip_batch_set_tile_kernel sets the format, cie_hip_batch_set_column_blocks
+ * (width 3, or 0 for entries) the unsorted ones. */
 typedef struct cie_hip_batch_s* cie_hip_batch;
 int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t systems, const int64_t* offsets,
                          cie_hip_batch* batch);
@@ -272,6 +274,8 @@ int cie_hip_batch_get_tile_rows(cie_hip_batch batch, int32_t* rowsPerTile);
 int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions);
 int cie_hip_batch_set_column_blocks(cie_hip_batch batch, int width);
 int cie_hip_batch_get_column_blocks(cie_hip_batch batch, int* width);
+int cie_hip_batch_set_tile_kernel(cie_hip_batch batch, int kernel);
+int cie_hip_batch_get_tile_kernel(cie_hip_batch batch, int* kernel);
 /* Solves every system in place: x (device, all rows) is the initial guess and
  * the result. mixed != 0 runs the mixed-precision refined form (settings as
  * for cie_hip_pcg_mixed), else FP64 (settings.settings only). Host outputs

@@ -55,7 +55,7 @@ def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks):
     assert all(s.relativeResidual < 1e-10 for s in stats)
 
 
-@pytest.mark.parametrize("blocks", [None, 0, 3])
+@pytest.mark.parametrize("blocks", [None, 0, 3, "sorted"])
 def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
     w = small_batch(seed=9)
     matrix = hip.CSRMatrix(w.matrix)
@@ -63,7 +63,11 @@ def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
     batch.autotune(32, 2)
     assert set(batch.tile_rows().tolist()) <= {32, 64, 128, 256}
     assert batch.column_blocks() in (0, 3)
-    if blocks is not None:
+    assert batch.tile_kernel() in ("tiled", "blocks", "sorted")
+    if blocks == "sorted":
+        batch.set_tile_kernel("sorted")
+        assert batch.tile_kernel() == "sorted" and batch.column_blocks() == 3
+    elif blocks is not None:
         batch.set_column_blocks(blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     results = []
@@ -75,6 +79,30 @@ def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
         results.append((x.cpu().numpy(), [s.iterationCount for s in stats]))
     for x, iterations in results[1:]:
         assert np.array_equal(x, results[0][0]) and iterations == results[0][1]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_sorted_tiles_equal_column_blocks(hip, torch_gpu, mixed):
+    """Column-sorted tiles stage every product at its block's position: the batch solves bitwise as it does on
+    the plain column-block tiles (tile partials are per tile, so the grids do not matter)."""
+    w = small_batch(seed=11)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.set_tile_rows(np.full(batch.systems, 64, dtype=np.int32))
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    results = []
+    for kernel in ("blocks", "sorted", "blocks"):
+        batch.set_tile_kernel(kernel)
+        assert batch.tile_kernel() == kernel
+        x = torch_gpu.zeros_like(b)
+        stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=mixed)
+        assert np.all(status == 1)
+        results.append((x.cpu().numpy(), [s.iterationCount for s in stats]))
+    for x, iterations in results[1:]:
+        assert np.array_equal(x, results[0][0]) and iterations == results[0][1]
+    assert np.all(per_system_residuals(w, results[1][0]) < 1e-10)
+    with pytest.raises(hip._lib.InvalidArgument):
+        batch.set_tile_kernel("vector")
 
 
 def test_batched_rejects_coupled_systems(hip, torch_gpu):
@@ -98,8 +126,8 @@ def test_full_batch_256_uneven_systems(hip, torch_gpu):
     residuals = per_system_residuals(w, x.cpu().numpy())
     assert np.all(residuals < 1e-10)
     print(f"batch: {w.matrix.rows} rows, {w.matrix.nnz} nnz, {summary.seconds:.3f} s, max iterations "
-          f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, column blocks "
-          f"{batch.column_blocks()}, tile rows "
+          f"{summary.stats.iterationCount}, refinements {int(refinements.sum())}, tile format "
+          f"{batch.tile_kernel()}, tile rows "
           f"{np.bincount(batch.tile_rows(), minlength=257)[[32, 64, 128, 256]].tolist()}")
 
 
