@@ -44,6 +44,10 @@ namespace {
 constexpr int kVectorThreads = 1024;  // update and direction blocks
 constexpr int kVectorBlocksMax = kVectorPartials;  // at most this many update partials
 constexpr int kFinishBlock = 1024;
+// Rows per thread of the vector kernels loaded before their partial sums: as
+// many as keep two 1024-thread blocks resident per compute unit (<= 64 VGPRs).
+template <class T>
+constexpr int aheadRows() { return sizeof(T) == sizeof(double) ? 2 : 3; }
 
 // Blocks of the update and direction kernels for n rows: about 4 rows per
 // thread for large systems; a smaller system still spreads over up to 256
@@ -162,6 +166,22 @@ __global__ __launch_bounds__(kVectorThreads) void updateKernel(int slot, int64_t
     __shared__ T lds[kVectorThreads / kWave];
     __shared__ T lds2[kVectorThreads / kWave];
     if (state->done) return;
+    constexpr int kAhead = aheadRows<TX>();
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    TX xs[kAhead];
+    T ps[kAhead], qs[kAhead], rs[kAhead], is[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {  // the first rows' loads are in flight during the partial sum
+        const int64_t i = first + k * stride;
+        if (i < n) {
+            xs[k] = x[i];
+            ps[k] = p[i];
+            qs[k] = q[i];
+            rs[k] = r[i];
+            is[k] = inverse[i];
+        }
+    }
     const T pq = sumPartialsAll<kVectorThreads, T>(productPartials, productCount, 0, lds);
     if (!(T(0) < pq)) {
         if (firstThread()) state->done = kNotSpd;
@@ -171,15 +191,20 @@ __global__ __launch_bounds__(kVectorThreads) void updateKernel(int slot, int64_t
     if (firstThread()) state->alpha = double(alpha);
     const TX alphaX = TX(alpha);
     T rr = T(0), rz = T(0);
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        x[i] = x[i] + alphaX * TX(p[i]);
-        const T residual = r[i] + (-alpha) * q[i];
+    const auto row = [&](int64_t i, TX xi, T pi, T qi, T ri, T ii) {
+        x[i] = xi + alphaX * TX(pi);
+        const T residual = ri + (-alpha) * qi;
         r[i] = residual;
-        const T z = residual * inverse[i];
+        const T z = residual * ii;
         rr += residual * residual;
         rz += residual * z;
+    };
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+        const int64_t i = first + k * stride;
+        if (i < n) row(i, xs[k], ps[k], qs[k], rs[k], is[k]);
     }
+    for (int64_t i = first + kAhead * stride; i < n; i += stride) row(i, x[i], p[i], q[i], r[i], inverse[i]);
     rr = blockSum<kVectorThreads>(rr, lds);
     rz = blockSum<kVectorThreads>(rz, lds2);
     if (threadIdx.x == 0) {
@@ -215,33 +240,53 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
     __shared__ float lds[kVectorThreads / kWave];
     __shared__ float lds2[kVectorThreads / kWave];
     if (state->done) return;
+    const bool fold = refineDue(*state);
+    constexpr int kAhead = aheadRows<float>();
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float ds[kAhead], ps[kAhead], qs[kAhead], rs[kAhead], is[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {  // the first rows' loads are in flight during the partial sum
+        const int64_t i = first + k * stride;
+        if (i < n) {
+            ds[k] = d[i];
+            ps[k] = p[i];
+            qs[k] = q[i];
+            rs[k] = r[i];
+            is[k] = inverse[i];
+        }
+    }
     const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
     if (!(0.0f < pq)) {
         if (firstThread()) state->done = kNotSpd;
         return;
     }
     const float alpha = float(state->rhos[slot]) / pq;
-    const bool fold = refineDue(*state);
     if (firstThread()) {
         state->alpha = double(alpha);
         state->refined[slot] = fold ? 1 : 0;
     }
     float rr = 0.0f, rz = 0.0f;
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float correction = d[i] + alpha * p[i];
+    const auto row = [&](int64_t i, float di, float pi, float qi, float ri, float ii) {
+        const float correction = di + alpha * pi;
         if (fold) {
             x[i] = x[i] + double(correction);
             d[i] = 0.0f;
         } else {
             d[i] = correction;
         }
-        const float residual = r[i] + (-alpha) * q[i];
+        const float residual = ri + (-alpha) * qi;
         r[i] = residual;
-        const float z = residual * inverse[i];
+        const float z = residual * ii;
         rr += residual * residual;
         rz += residual * z;
+    };
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+        const int64_t i = first + k * stride;
+        if (i < n) row(i, ds[k], ps[k], qs[k], rs[k], is[k]);
     }
+    for (int64_t i = first + kAhead * stride; i < n; i += stride) row(i, d[i], p[i], q[i], r[i], inverse[i]);
     rr = blockSum<kVectorThreads>(rr, lds);
     rz = blockSum<kVectorThreads>(rz, lds2);
     if (threadIdx.x == 0) {
@@ -295,6 +340,19 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
     __shared__ double lds64[kVectorThreads / kWave];
     SolverState& s = *state;
     if (s.done) return;
+    constexpr int kAhead = aheadRows<T>();
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    T ps[kAhead], rs[kAhead], is[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {  // the first rows' loads are in flight during the partial sums
+        const int64_t i = first + k * stride;
+        if (i < n) {
+            ps[k] = p[i];
+            rs[k] = r[i];
+            is[k] = inverse[i];
+        }
+    }
     const double rhoPrevious = s.rhos[slot];
     bool converged = false;
     double beta = 0.0;
@@ -356,9 +414,12 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
     }
     if (converged) return;
     const T scale = T(beta);
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-        p[i] = p[i] * scale + r[i] * inverse[i];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+        const int64_t i = first + k * stride;
+        if (i < n) p[i] = ps[k] * scale + rs[k] * is[k];
+    }
+    for (int64_t i = first + kAhead * stride; i < n; i += stride) p[i] = p[i] * scale + r[i] * inverse[i];
 }
 
 // Single block, once per solve: the start sums (r.r in FP64, r.z in TR) and
