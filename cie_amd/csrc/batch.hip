@@ -214,6 +214,51 @@ __global__ __launch_bounds__(kChunkRows) void batchUpdateKernel(BatchLayout layo
     }
 }
 
+// Mixed form: the FP32 correction d += alpha p, folded into the FP64 x (and
+// cleared) for a system that refines this trip; r -= alpha q; z = D^-1 r;
+// chunk partials r.r and r.z.
+__global__ __launch_bounds__(kChunkRows) void batchMixedUpdateKernel(BatchLayout layout, ActiveWork work,
+                                                                     double* __restrict__ x, float* __restrict__ d,
+                                                                     float* __restrict__ r, const float* __restrict__ p,
+                                                                     const float* __restrict__ q,
+                                                                     const float* __restrict__ inverse,
+                                                                     double* __restrict__ partials,
+                                                                     const SystemState* __restrict__ state) {
+    __shared__ float lds[kChunkRows / kWave];
+    __shared__ float lds2[kChunkRows / kWave];
+    const int items = work.count[2];
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int2 chunk = work.chunkList[item];
+        const int c = chunk.x;
+        const SystemState& s = state[chunk.y];
+        float rr = 0.0f, rz = 0.0f;
+        if (!s.done) {  // NOT_SPD may end a system between the product and this kernel
+            const float alpha = float(s.alpha);
+            const int row = layout.chunkRows[c] + int(threadIdx.x);
+            if (row < layout.chunkRows[c + 1]) {
+                const float correction = d[row] + alpha * p[row];
+                if (s.refine) {
+                    x[row] = x[row] + double(correction);
+                    d[row] = 0.0f;
+                } else {
+                    d[row] = correction;
+                }
+                const float residual = r[row] + (-alpha) * q[row];
+                r[row] = residual;
+                const float z = residual * inverse[row];
+                rr = residual * residual;
+                rz = residual * z;
+            }
+        }
+        rr = blockSum<kChunkRows>(rr, lds);
+        rz = blockSum<kChunkRows>(rz, lds2);
+        if (threadIdx.x == 0) {
+            partials[2 * c] = double(rr);
+            partials[2 * c + 1] = double(rz);
+        }
+    }
+}
+
 // p = beta p + D^-1 r
 template <class T>
 __global__ __launch_bounds__(kChunkRows) void batchDirectionKernel(BatchLayout layout, ActiveWork work,
@@ -285,6 +330,16 @@ __global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* _
             finished = true;
         } else {
             s.alpha = double(TR(s.rho) / AR);
+            if (settings.mixed) {
+                // Does this trip refine? The FP32 r.r the previous trip left fell by the refinement factor
+                // since the last refinement, looks converged, or the iteration cap is near. The update then
+                // folds the correction into x and the residual kernel recomputes r from it.
+                const double absolute = sqrt(s.rr);
+                const double relative = absolute / s.initialNorm;
+                s.refine = s.rr <= settings.updateThreshold * s.updateNorm ||
+                           floatLess(absolute, settings.absoluteTolerance) ||
+                           floatLess(relative, settings.relativeTolerance) || s.iteration + 1 >= settings.maxIterations;
+            }
         }
     } else if (kPhase == kPhaseUpdate) {
         const double absolute = sqrt(double(AR));
@@ -305,12 +360,7 @@ __global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* _
     } else if (kPhase == kPhaseUpdateMixed) {
         const double absolute = sqrt(double(AR));
         const double relative = absolute / s.initialNorm;
-        const bool refine = double(AR) <= settings.updateThreshold * s.updateNorm ||
-                            floatLess(absolute, settings.absoluteTolerance) ||
-                            floatLess(relative, settings.relativeTolerance) || s.iteration + 1 >= settings.maxIterations;
-        if (refine) {
-            s.refine = 1;
-        } else {
+        if (!s.refine) {  // (a refining trip advances in the refinement phase, from the true residual)
             const TR previous = TR(s.rho);
             s.rho = double(B);
             s.beta = double(B / previous);
@@ -688,7 +738,8 @@ constexpr int kProfiledTrips = 8;  // product launches averaged by the profiling
 
 template <class TR, int W>
 void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
-                      const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
+                      float* correction, const TR* inverse, const BatchSettings& settings, bool mixed,
+                      ProductProfile* profile) {
     cie_hip_context ctx = batch->context;
     cie_hip_csr m = batch->matrix;
     const hipStream_t stream = ctx->stream;
@@ -703,9 +754,15 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     if (profile) profile->end(ctx, mark);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
                        batch->tilePartials, batch->state, settings, batch->active);
-    hipLaunchKernelGGL((batchUpdateKernel<TR, double>),
-                       workGrid(batchUpdateKernel<TR, double>, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
-                       stream, layout, work, x, r, p, q, inverse, batch->chunkPartials, batch->state);
+    if constexpr (std::is_same_v<TR, float>) {  // the mixed form: FP32 correction, folded when refining
+        hipLaunchKernelGGL(batchMixedUpdateKernel,
+                           workGrid(batchMixedUpdateKernel, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
+                           stream, layout, work, x, correction, r, p, q, inverse, batch->chunkPartials, batch->state);
+    } else {
+        hipLaunchKernelGGL((batchUpdateKernel<TR, double>),
+                           workGrid(batchUpdateKernel<TR, double>, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
+                           stream, layout, work, x, r, p, q, inverse, batch->chunkPartials, batch->state);
+    }
     if (mixed) {
         hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdateMixed>), systems, tileBlock, 0, stream,
                            batch->systemChunks, batch->chunkPartials, batch->state, settings, batch->active);
@@ -728,9 +785,11 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
 // One trip with the batch's walker.
 template <class TR>
 void enqueueTrip(cie_hip_batch batch, const BatchLayout& layout, const double* b, double* x, TR* r, TR* p, TR* q,
-                 const TR* inverse, const BatchSettings& settings, bool mixed, ProductProfile* profile) {
+                 float* correction, const TR* inverse, const BatchSettings& settings, bool mixed,
+                 ProductProfile* profile) {
     withWalkerCode(walkerCode(batch), [&](auto w) {
-        enqueueIteration<TR, decltype(w)::value>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+        enqueueIteration<TR, decltype(w)::value>(batch, layout, b, x, r, p, q, correction, inverse, settings, mixed,
+                                                 profile);
     });
 }
 
@@ -749,15 +808,18 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     if constexpr (std::is_same_v<TR, float>) inverse = m->inverseDiagonal32;
     else inverse = m->inverseDiagonal64;
     const size_t vector = (sizeof(TR) * size_t(std::max<int64_t>(1, n)) + 255) / 256 * 256;
-    if (batch->workspaceBytes < 3 * vector) {
+    const size_t vector32 = (sizeof(float) * size_t(std::max<int64_t>(1, n)) + 255) / 256 * 256;
+    const size_t bytes = 3 * vector + (mixed ? vector32 : 0);
+    if (batch->workspaceBytes < bytes) {
         freeDevice(batch->workspace);
-        CIE_HIP_CHECK(hipMalloc(&batch->workspace, 3 * vector));
-        batch->workspaceBytes = 3 * vector;
+        CIE_HIP_CHECK(hipMalloc(&batch->workspace, bytes));
+        batch->workspaceBytes = bytes;
         releaseGraph(batch);
     }
     TR* r = reinterpret_cast<TR*>(batch->workspace);
     TR* p = reinterpret_cast<TR*>(static_cast<char*>(batch->workspace) + vector);
     TR* q = reinterpret_cast<TR*>(static_cast<char*>(batch->workspace) + 2 * vector);
+    float* correction = mixed ? reinterpret_cast<float*>(static_cast<char*>(batch->workspace) + 3 * vector) : nullptr;
 
     // Initial states and the active-system counter (uploaded, not memset: the
     // graph below never records a memset node).
@@ -774,6 +836,9 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
     hipEvent_t* const marks = events.event + 1;
     CIE_HIP_CHECK(hipEventRecord(start, ctx->stream));
     const ActiveWork& work = batch->running;
+    if (correction)  // a fill kernel, as everywhere in the batch's stream (no memset nodes near its graph)
+        hipLaunchKernelGGL((vectorKernel<float>), dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, int(kFill),
+                           n, correction, static_cast<const float*>(nullptr), 0.0f);
     launchSelect<kRunning>(batch, work);  // every system runs
     withWalkerCode(walkerCode(batch), [&](auto w) {
         constexpr int W = decltype(w)::value;
@@ -796,7 +861,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             hipGraph_t graph = nullptr;
             CIE_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueTrip<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, nullptr);
+                enqueueTrip<TR>(batch, layout, b, x, r, p, q, correction, inverse, settings, mixed, nullptr);
             const hipError_t captured = hipStreamEndCapture(ctx->stream, &graph);
             if (captured != hipSuccess) {
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph capture: ") + hipGetErrorString(captured));
@@ -822,7 +887,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
             CIE_HIP_CHECK(hipGraphLaunch(batch->graph, ctx->stream));
         } else {
             for (int i = 0; i < kGraphIterations; ++i)
-                enqueueTrip<TR>(batch, layout, b, x, r, p, q, inverse, settings, mixed, profile);
+                enqueueTrip<TR>(batch, layout, b, x, r, p, q, correction, inverse, settings, mixed, profile);
             CIE_HIP_CHECK(hipGetLastError());
         }
         ++count;
