@@ -127,15 +127,16 @@ def test_nested_product_operator(hip, torch_gpu):
 
 
 @pytest.mark.parametrize("order,n,blocks", [(1, 20, False), (2, 16, False), (3, 12, False), (4, 10, False),
-                                            (4, 32, False), (3, 12, True), (4, 32, True)])
+                                            (4, 32, False), (3, 12, True), (4, 32, True), (3, 12, "sorted"),
+                                            (4, 32, "sorted")])
 def test_pmg_matches_oracle(hip, torch_gpu, order, n, blocks):
-    """blocks: the finest grid's sweeps and updates walk 1 x 3 column blocks."""
+    """blocks: the finest grid's sweeps and updates walk 1 x 3 column blocks, plain or column-sorted."""
     csr, mask_values = _legendre(order, n)
     b = systems.uniform(1, csr.rows)
     x_ref, history, coarse = oracle.pmg(csr, mask_values, order, b)
     matrix = hip.CSRMatrix(csr)
     if blocks:
-        matrix.set_launch(64, "blocks", 3, 8)
+        matrix.set_launch(64, "sorted" if blocks == "sorted" else "blocks", 3, 8)
     pmg = hip.PMultigrid(matrix, hip.Mask(mask_values), order)
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
     info = pmg.solve(_dev(torch_gpu, b), x)
