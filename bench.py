@@ -124,8 +124,9 @@ def main():
     args = parser.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
-              "(the solve path is single-device; see the sharding note in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the solve path runs on "
+              "one device (a system would shard by contiguous row blocks with a halo exchange of the search "
+              "direction per product; batches by whole systems)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload not in ("pcg", "mixed-q4"):
