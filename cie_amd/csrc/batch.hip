@@ -748,10 +748,13 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     const TR* entries;
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
-    const int mark = profile ? profile->begin(ctx) : -1;
-    hipLaunchKernelGGL((batchProductKernel<TR, W>), workGrid(batchProductKernel<TR, W>, ctx, kTileBlock, batch->tileCount),
-                       tileBlock, 0, stream, layout, work, entries, p, q, batch->tilePartials);
-    if (profile) profile->end(ctx, mark);
+    const dim3 productGrid = workGrid(batchProductKernel<TR, W>, ctx, kTileBlock, batch->tileCount);
+    if (profile)  // eager launches only (a captured graph is never profiled)
+        profile->launch(ctx, batchProductKernel<TR, W>, productGrid, tileBlock, layout, work, entries,
+                        static_cast<const TR*>(p), q, batch->tilePartials);
+    else
+        hipLaunchKernelGGL((batchProductKernel<TR, W>), productGrid, tileBlock, 0, stream, layout, work, entries, p, q,
+                           batch->tilePartials);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
                        batch->tilePartials, batch->state, settings, batch->active);
     if constexpr (std::is_same_v<TR, float>) {  // the mixed form: FP32 correction, folded when refining

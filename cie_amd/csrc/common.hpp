@@ -1,6 +1,7 @@
 // Internal definitions shared by the HIP translation units of libcie_hip.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,18 +90,23 @@ struct cie_hip_context_s {
 };
 
 namespace ciehip {
-// With the context's profiling on, brackets up to kProfileEvents launches of
-// one kernel (the product of a solver loop) with HIP events; mean() is their
-// average duration over the first `samples` of them.
+// With the context's profiling on, times up to kProfileEvents launches of one
+// kernel (the product of a solver loop) with a pair of HIP events each,
+// recorded by the launch itself at the kernel's start and end (so the launch
+// latency before the kernel is not counted); mean() is their average duration
+// over the first `samples` of them. Further launches, and all launches with
+// profiling off, are plain.
 struct ProductProfile {
     int used = 0;
-    int begin(cie_hip_context ctx) {
-        if (!ctx->profiling || used >= cie_hip_context_s::kProfileEvents) return -1;
-        (void)hipEventRecord(ctx->profileEvents[2 * used], ctx->stream);
-        return used++;
-    }
-    void end(cie_hip_context ctx, int mark) {
-        if (mark >= 0) (void)hipEventRecord(ctx->profileEvents[2 * mark + 1], ctx->stream);
+    template <class Kernel, class... Args>
+    void launch(cie_hip_context ctx, Kernel kernel, dim3 grid, dim3 block, Args... args) {
+        if (ctx->profiling && used < cie_hip_context_s::kProfileEvents) {
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, ctx->profileEvents[2 * used],
+                                  ctx->profileEvents[2 * used + 1], 0, args...);
+            ++used;
+        } else {
+            hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+        }
     }
     double mean(cie_hip_context ctx, int samples) const {
         double total = 0.0;

@@ -531,10 +531,8 @@ int launchProduct(cie_hip_csr m, const T* entries, const CsrLaunch& launch, cons
         status = prepareLaunch(m, launch, productKernel<Walker>, source, grid);
         if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
         if (status != CIE_HIP_OK) return;
-        const int mark = profile.begin(ctx);
-        hipLaunchKernelGGL((productKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, source, entries,
-                           p, q, ctx->scratch.partials, ctx->scratch.state);
-        profile.end(ctx, mark);
+        profile.launch(ctx, productKernel<Walker>, dim3(grid), dim3(Walker::kBlock), source, entries, p, q,
+                       ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state));
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     if (status != CIE_HIP_OK) return status;
