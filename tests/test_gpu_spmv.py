@@ -232,16 +232,25 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
         y = space.makeVector(w.matrix.rows)
         hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
         assert np.all(np.abs(y.cpu().numpy() - expected) <= bound)
-    # column blocks (runs of 3 and 5 adjacent columns: 1 x 3 blocks are 13% padding)
-    for width, unroll in ((3, 16), (2, 8), (4, 4)):
-        matrix.set_launch(64, "blocks", width, unroll)
-        matrix.set_launch(32, "blocks", width, unroll)
-        y = space.makeVector(w.matrix.rows)
-        hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
-        assert np.all(np.abs(y.cpu().numpy() - expected) <= bound), (width, unroll)
-        y32 = space32.makeVector(w.matrix.rows)
-        hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
-        assert np.linalg.norm(y32.cpu().numpy() - expected) / np.linalg.norm(expected) < 1e-5
+    # column blocks (runs of 3 and 5 adjacent columns: 1 x 3 blocks are 13% padding), and the same blocks in
+    # column-sorted tiles, whose products are bitwise the plain blocks'
+    for width, unroll in ((3, 16), (2, 8), (4, 4), (3, 8)):
+        plain = {}
+        for kernel in ("blocks", "sorted"):
+            matrix.set_launch(64, kernel, width, unroll)
+            matrix.set_launch(32, kernel, width, unroll)
+            y = space.makeVector(w.matrix.rows)
+            hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
+            got64 = y.cpu().numpy()
+            assert np.all(np.abs(got64 - expected) <= bound), (kernel, width, unroll)
+            y32 = space32.makeVector(w.matrix.rows)
+            hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
+            got32 = y32.cpu().numpy()
+            assert np.linalg.norm(got32 - expected) / np.linalg.norm(expected) < 1e-5
+            if kernel == "blocks":
+                plain = {64: got64, 32: got32}
+            else:
+                assert np.array_equal(got64, plain[64]) and np.array_equal(got32, plain[32]), (width, unroll)
 
 
 def test_spmv_shape_error(hip, torch_gpu):
