@@ -114,6 +114,8 @@ def main():
     parser.add_argument("--cpu-iterations", type=int, default=20, help="CPU baseline sample (0 disables)")
     parser.add_argument("--cpu-threads", type=int, default=0)
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
+    parser.add_argument("--batch-tiles", choices=("auto", "tiled", "blocks", "sorted"), default="auto",
+                        help="batch workload: the tile format (default: the autotuner's)")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
     parser.add_argument("--workload", choices=("pcg", "mixed-q4", "batch", "adjoint", "assembly", "reorder", "pmg"),
@@ -132,6 +134,8 @@ def main():
     if args.workload not in ("pcg", "mixed-q4"):
         {"batch": bench_batch, "adjoint": bench_adjoint, "assembly": bench_assembly, "reorder": bench_reorder,
          "pmg": bench_pmg}[args.workload](args)
+        if args.calibration_copy:
+            _calibration_copy()
         return
 
     import torch
@@ -277,9 +281,17 @@ def main():
         result["cpu_baseline"] = cpu_baseline(workload, int(f.stats.iterationCount), args.cpu_iterations, threads)
     print(json.dumps(result), flush=True)
     if args.calibration_copy:
-        source = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
-        torch.empty_like(source).copy_(source)
-        torch.cuda.synchronize()
+        _calibration_copy()
+
+
+def _calibration_copy():
+    """The 1 GiB device-to-device copy that ends a run profiled for HBM traffic: FETCH_SIZE and WRITE_SIZE of
+    this copy scale every kernel's counters to bytes (cie_amd/profiling.py)."""
+    import torch
+
+    source = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    torch.empty_like(source).copy_(source)
+    torch.cuda.synchronize()
 
 
 # ---------------------------------------------------------------------------
@@ -336,11 +348,33 @@ def _line(args, metric, value, unit, dtype, config, roofline, cpu, extra=None):
     print(json.dumps(result), flush=True)
 
 
-def _roofline(kernel, bytes_per_launch, seconds, note):
+def _roofline(kernel, bytes_per_launch, seconds, note, traffic=None):
     achieved = bytes_per_launch / seconds / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": kernel,
             "bytes_per_launch": bytes_per_launch, "mean_launch_seconds": seconds, "bytes": note}
+
+
+def _walker(launch, value="double"):
+    """Kernel-name form of the SpMV walker a launch (kernel, width, unroll) selects."""
+    kernel, width, unroll = launch
+    if kernel == "tiled":
+        return f"BlockTiles<{value}, {value}, {unroll}, {width}>"
+    if kernel == "blocks":
+        return f"ColumnBlockTiles<{value}, {value}, {unroll}, 256, {width}>"
+    if kernel == "sorted":
+        return f"SortedBlockTiles<{value}, {value}, {unroll}, 256, {width}>"
+    return f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<{value}, {value}, {width}, {unroll}, 256>"
+
+
+def _profiled_traffic(profile, kernel):
+    """HBM bytes per dispatch of `kernel` (its short name) in the committed calibrated rocprofv3 summary
+    `profile` of this workload, else None."""
+    path = ROOT / "profiles" / profile
+    if not path.exists():
+        return None
+    entry = json.loads(path.read_text()).get("kernels", {}).get(kernel)
+    return None if entry is None else entry["traffic_bytes"]
 
 
 _BATCH_WALKER = {"tiled": 0, "blocks": 3, "sorted": -3}  # the walker code of the batch's tile format
@@ -362,6 +396,8 @@ def bench_batch(args):
     matrix = linalg.CSRMatrix(w.matrix, context)
     batch = linalg.Batch(matrix, offsets)
     batch.autotune(32, 5)
+    if args.batch_tiles != "auto":  # a fixed tile format (profiling passes that must see the same kernels)
+        batch.set_tile_kernel(args.batch_tiles)
     b = torch.from_numpy(w.rhs).cuda()
     x = torch.zeros_like(b)
     settings = Statistics(100000, 0.0, w.relative_tolerance)
@@ -413,7 +449,9 @@ def bench_batch(args):
                        "relative residual 1e-10 each", "rows": rows, "nnz": nnz, "systems": 256},
           _roofline(f"batchProductKernel<float, {_BATCH_WALKER[batch.tile_kernel()]}> (all systems active; tiles of "
                     f"{_BATCH_TILES[batch.tile_kernel()]})", algorithmic,
-                    profiled.productSeconds, "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix"),
+                    profiled.productSeconds, "B_SpMV(FP32 plane, int32 CSR) over the whole block-diagonal matrix",
+                    _profiled_traffic("batch_q2_256_traffic.json",
+                                      f"batchProductKernel<float, {_BATCH_WALKER[batch.tile_kernel()]}>")),
           {"value": sampled_seconds * total_work / sampled_work, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG (ConjugateGradients_impl.hpp:49-201) on the first {sampled} systems "
                      f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
@@ -474,7 +512,9 @@ def bench_adjoint(args):
           {"workload": f"3D Q2 Laplace {n}^3 hexahedra, k_e = exp(g_e), g ~ N(0, 0.5) seed 4, target from seed 5, "
                        f"b ~ U(-1,1) seed 1", "rows": rows, "nnz": nnz, "elements": int(problem.count)},
           _roofline(f"productKernel {list(problem.matrix.launch(64))} (q = A p) inside the forward solve", algorithmic,
-                    info.productSeconds, "B_SpMV(FP64, int32 CSR)"),
+                    info.productSeconds, "B_SpMV(FP64, int32 CSR)",
+                    _profiled_traffic("adjoint_q2_48_traffic.json",
+                                      f"productKernel<{_walker(problem.matrix.launch(64))} >")),
           {"value": secs * total_iterations / sample + gather, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG, {sample} iterations in {secs:.2f} s, scaled to the "
                      f"{total_iterations} forward + adjoint iterations, plus the oracle's element gradient "
@@ -518,7 +558,8 @@ def bench_assembly(args):
                        f"benchmark pattern", "rows": rows, "nnz": nnz, "contributions": contributions},
           _roofline("assembleSlotsKernel<uint16_t> (one launch per assembly)", algorithmic, kernel,
                     "12 nnz (FP64 + FP32 values written) + 4 (nnz + 1) (slot offsets) + 6 per contribution "
-                    "(element, local pair) + 8 E (k_e); K_e0 stays in cache"),
+                    "(element, local pair) + 8 E (k_e); K_e0 stays in cache",
+                    _profiled_traffic("assembly_q2_64_traffic.json", "assembleSlotsKernel<unsigned short>")),
           {"value": cpu, "unit": "s", "cores": 1, "kind": "port",
            "sample": "oracle_assemble (the Serial addContribution scatter, Assembler_impl.hpp:156-194) over all "
                      "elements, one thread"})
@@ -570,7 +611,8 @@ def bench_reorder(args):
            "rows": rows, "nnz": nnz},
           _roofline("permuteRowsKernel (one launch per permutation; the value above is the whole call, with its "
                     "host preparation, allocations and map uploads)", algorithmic, kernel,
-                    "(4 + 8 + 4) nnz read + written (columns, FP64, FP32) + 2 x 4 (N + 1) extents + 3 x 4 N maps"),
+                    "(4 + 8 + 4) nnz read + written (columns, FP64, FP32) + 2 x 4 (N + 1) extents + 3 x 4 N maps",
+                    _profiled_traffic("reorder_q2_64_traffic.json", "permuteRowsKernel")),
           {"value": cpu_permute, "unit": "s", "cores": 1, "kind": "port",
            "sample": f"oracle_reorder_csr (reorder.cpp:215-365) of the whole matrix, one thread; the oracle's RCM "
                      f"map took {cpu_map:.3f} s"},
@@ -635,7 +677,9 @@ def bench_pmg(args):
           {"workload": f"2D Q{order} integrated Legendre Laplace, {n}^2 elements, Dirichlet eliminated, "
                        f"b ~ U(-1,1) seed 1, x0 = 0", "rows": rows, "nnz": nnz},
           _roofline(f"jacobiSweepKernel {list(matrix.launch(64))} on the finest grid", algorithmic, sweep,
-                    "B_SpMV(FP64, int32 CSR) + 2 x 8 N (right-hand side and inverse diagonal)"),
+                    "B_SpMV(FP64, int32 CSR) + 2 x 8 N (right-hand side and inverse diagonal)",
+                    _profiled_traffic("pmg_q4_legendre_512_traffic.json",
+                                      f"jacobiSweepKernel<{_walker(matrix.launch(64))} >")),
           {"value": cpu_sample * info.cycles / sample_cycles, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle_pmg (solver.cpp:111-232 restated) for {sample_cycles} cycles in {cpu_sample:.2f} s, "
                      f"scaled to the {info.cycles} cycles of the solve"},
