@@ -236,11 +236,12 @@ int cie_hip_pcg_f64(cie_hip_context context, cie_hip_csr matrix, double inScale,
 
 /* Mixed-precision refined PCG (no reference counterpart; the reference only
  * instantiates all-FP32 CG, ConjugateGradients.cpp:10). The inner PCG keeps
- * matrix values, Krylov vectors and dot products in FP32; the solution is
- * accumulated in FP64, and the true residual b - A x (FP64 matrix plane) and
- * the convergence test are recomputed in FP64 on the device whenever the FP32
- * residual has fallen by refinementFactor, or looks converged. The matrix
- * needs both planes. Statistics report the FP64 true residual. */
+ * matrix values, Krylov vectors, its correction and dot products in FP32.
+ * Whenever the FP32 residual has fallen by refinementFactor (or looks
+ * converged, or the iteration cap is near), the correction is added into the
+ * FP64 solution, and the true residual b - A x (FP64 matrix plane) and the
+ * convergence test are recomputed in FP64 on the device. The matrix needs
+ * both planes. Statistics report the FP64 true residual. */
 int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* b,
                       double outScale, double* x, int64_t size, const cie_hip_mixed_settings* settings,
                       cie_hip_solve_info* info);
