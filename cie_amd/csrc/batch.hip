@@ -507,6 +507,7 @@ struct cie_hip_batch_s {
     SortedCopy sorted;                  // the sorted copy of the block form over the batch's tiles
     int graphKey[5] = {0, 0, 0, 0, 0};  // what the captured graph was built for
     const void* graphPointers[3] = {nullptr, nullptr, nullptr};  // b, x, inverse diagonal
+    BatchSettings graphSettings{};      // the settings its kernels were captured with (passed by value)
 };
 
 namespace {
@@ -857,8 +858,17 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
 
     if (useGraph) {
         const int key[5] = {mixed ? 1 : 0, settings.restart, int(sizeof(TR)), batch->tileCount, walkerCode(batch)};
-        const bool same = batch->graph && std::equal(key, key + 5, batch->graphKey) && batch->graphPointers[0] == b &&
-                          batch->graphPointers[1] == x && batch->graphPointers[2] == inverse;
+        // The finish kernels take the settings by value, so a graph captured
+        // with other tolerances, caps or refinement threshold is not reused.
+        const BatchSettings& g = batch->graphSettings;
+        const bool sameSettings = g.absoluteTolerance == settings.absoluteTolerance &&
+                                  g.relativeTolerance == settings.relativeTolerance &&
+                                  g.updateThreshold == settings.updateThreshold &&
+                                  g.maxIterations == settings.maxIterations && g.restart == settings.restart &&
+                                  g.mixed == settings.mixed;
+        const bool same = batch->graph && sameSettings && std::equal(key, key + 5, batch->graphKey) &&
+                          batch->graphPointers[0] == b && batch->graphPointers[1] == x &&
+                          batch->graphPointers[2] == inverse;
         if (!same) {
             releaseGraph(batch);
             hipGraph_t graph = nullptr;
@@ -876,6 +886,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
                 return setError(CIE_HIP_ERROR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(made));
             }
             std::copy(key, key + 5, batch->graphKey);
+            batch->graphSettings = settings;
             batch->graphPointers[0] = b;
             batch->graphPointers[1] = x;
             batch->graphPointers[2] = inverse;

@@ -53,6 +53,7 @@ struct SolverState {
     int32_t done;          // 0 running, else a SolveEnd value
     int32_t refined[2];    // mixed: 1 if the trip of that parity recomputed the true residual
     int32_t restart;       // mixed: 1 resets the search direction at each refinement
+    int32_t startExit;     // 1 if the initial residual met the tolerance (no loop trip ran)
 };
 
 enum SolveEnd : int32_t {

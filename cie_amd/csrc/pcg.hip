@@ -441,6 +441,7 @@ __global__ __launch_bounds__(kFinishBlock) void startFinishKernel(int count, con
     s.updateNorm = rr;
     if (floatLess(initial, s.absoluteTolerance) || floatLess(1.0, s.relativeTolerance)) {
         s.done = kConverged;
+        s.startExit = 1;
     } else if (!(TR(0) < rz)) {
         s.done = kBreakdown;
     } else {
@@ -754,11 +755,12 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     result.converged = final.done == kConverged;
     result.seconds = seconds;
     result.refinements = final.refinements;
-    result.spmvCount = final.iteration + (final.done == kConverged ? 1 : 0) + 1 + final.refinements;
+    // Loop trips that ran the product kernel: iteration (+1 for the converged
+    // trip); none when the start check ended the solve.
+    const uint64_t trips = final.startExit ? 0 : final.iteration + (final.done == kConverged ? 1 : 0);
+    result.spmvCount = trips + 1 + final.refinements;
     if (ctx->profiling) {
-        // Loop trips that ran the product kernel: iteration (+1 for the converged trip).
-        const int worked = int(std::min<uint64_t>(final.iteration + (final.done == kConverged ? 1 : 0),
-                                                  uint64_t(cie_hip_context_s::kProfileEvents)));
+        const int worked = int(std::min<uint64_t>(trips, uint64_t(cie_hip_context_s::kProfileEvents)));
         result.productSamples = uint64_t(std::min(worked, profile.used));
         result.productSeconds = profile.mean(ctx, worked);
     }
@@ -774,8 +776,13 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         if (status) return status;
         result.trueRelativeResidual = final.initialNorm > 0 ? norm / final.initialNorm : norm;
     }
-    status = writeOutput(ctx, q.inScale, q.outScale, q.x, solution, n);
-    if (status) return status;
+    // ConjugateGradients_impl.hpp:100-111 return before the output step of
+    // :190-198 when the initial residual already meets the tolerance: x keeps
+    // its value whatever inScale and outScale are.
+    if (!final.startExit) {
+        status = writeOutput(ctx, q.inScale, q.outScale, q.x, solution, n);
+        if (status) return status;
+    }
     CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (info) *info = result;
     return CIE_HIP_OK;

@@ -326,14 +326,20 @@ def makeDiagonalOperator(matrix, space):
     return DiagonalOperator(inverse, space)
 
 
-def _float_equal(left, right):
-    """cieutils/packages/maths/impl/Comparison_impl.hpp:49-56 (default tolerances)."""
-    norm = min(abs(left) + abs(right), sys.float_info.max)
-    return abs(left - right) < max(_DBL_MIN, _DBL_EPS * norm)
+def _float_equal(left, right, value=float):
+    """cieutils/packages/maths/impl/Comparison_impl.hpp:49-56 with the default tolerances of the value type
+    (numeric_limits<T>::min() and epsilon(), :36-40); `value` is float (double) or numpy.float32."""
+    if value is float:
+        norm = min(abs(left) + abs(right), sys.float_info.max)
+        return abs(left - right) < max(_DBL_MIN, _DBL_EPS * norm)
+    info = np.finfo(np.float32)
+    left, right = np.float32(left), np.float32(right)
+    norm = min(np.float32(abs(left) + abs(right)), info.max)
+    return bool(abs(np.float32(left - right)) < max(info.tiny, np.float32(info.eps * norm)))
 
 
-def _float_less(left, right):
-    return left < right and not _float_equal(left, right)
+def _float_less(left, right, value=float):
+    return bool(value(left) < value(right)) and not _float_equal(left, right, value)
 
 
 class IterativeSolver(LinearOperator):
@@ -372,7 +378,16 @@ class ConjugateGradients(IterativeSolver):
         if space.size(in_) != size:
             raise _lib.ShapeError(_lib.ERROR_SHAPE, f"incompatible vector sizes {space.size(in_)} != {size}")
         settings = self.getConfiguration()
-        stats = Statistics(0, sys.float_info.max, sys.float_info.max)
+        # The scalars are TS::Value: double, or float for ConjugateGradients<...<float>>
+        # (ConjugateGradients.cpp:10,16), whose sqrt, quotients and Comparison<float> round in float.
+        V = float if getattr(space, "_suffix", "f64") == "f64" else np.float32
+        sqrt = math.sqrt if V is float else (lambda value: np.sqrt(np.float32(value)))
+        less = lambda left, right: _float_less(left, right, V)  # noqa: E731
+        if V is not float:
+            settings = Statistics(settings.iterationCount, float(np.float32(settings.absoluteResidual)),
+                                  float(np.float32(settings.relativeResidual)))
+        maximum = sys.float_info.max if V is float else float(np.finfo(np.float32).max)
+        stats = Statistics(0, maximum, maximum)
         self.history = []
         solution = space.makeVector(size)
         space.assign(solution, out)
@@ -384,18 +399,19 @@ class ConjugateGradients(IterativeSolver):
             residual = space.makeVector(size)
             space.assign(residual, in_)
             self._lhs.product(1, solution, -1, residual)
-            residualNorm = space.innerProduct(residual, residual)
-            initialResidualNorm = math.sqrt(residualNorm)
-            stats.absoluteResidual = initialResidualNorm
+            residualNorm = V(space.innerProduct(residual, residual))
+            initialResidualNorm = V(sqrt(residualNorm))
+            stats.absoluteResidual = float(initialResidualNorm)
             stats.relativeResidual = 1.0
-            if _float_less(stats.absoluteResidual, settings.absoluteResidual) or _float_less(
-                    stats.relativeResidual, settings.relativeResidual):
+            if less(stats.absoluteResidual, settings.absoluteResidual) or less(stats.relativeResidual,
+                                                                              settings.relativeResidual):
+                # :100-111 return before the output step of :190-198: `out` keeps its value.
                 self.report(stats)
-                return self._finish(inScale, outScale, out, solution)
+                return
             if self._preconditioner is not None:
                 preconditioned = space.makeVector(size)
                 self._preconditioner.product(0, residual, 1, preconditioned)
-                preconditionedNorm = space.innerProduct(residual, preconditioned)
+                preconditionedNorm = V(space.innerProduct(residual, preconditioned))
                 space.assign(search, preconditioned)
                 if not 0 < preconditionedNorm:
                     raise _lib.Breakdown(_lib.ERROR_BREAKDOWN, "preconditioned norm vanished in iteration 0 (r^T @ z = 0)")
@@ -406,28 +422,29 @@ class ConjugateGradients(IterativeSolver):
                     raise _lib.Breakdown(_lib.ERROR_BREAKDOWN, "residual norm vanished in iteration 0 (r^T @ r = 0)")
             while stats.iterationCount < settings.iterationCount:
                 self._lhs.product(0, search, 1, searchProduct)
-                denominator = space.innerProduct(search, searchProduct)
+                denominator = V(space.innerProduct(search, searchProduct))
                 if not 0 < denominator:
                     raise _lib.NotPositiveDefinite(_lib.ERROR_NOT_SPD,
                                                    f"p^T @ A @ p vanished in iteration {stats.iterationCount}")
-                searchScale = preconditionedNorm / denominator
+                searchScale = V(preconditionedNorm / denominator)
                 space.add(solution, search, searchScale)
                 space.add(residual, searchProduct, -searchScale)
-                residualNorm = space.innerProduct(residual, residual)
+                residualNorm = V(space.innerProduct(residual, residual))
                 previous = preconditionedNorm
                 if preconditioned is not None:
                     self._preconditioner.product(0, residual, 1, preconditioned)
-                    preconditionedNorm = space.innerProduct(residual, preconditioned)
+                    preconditionedNorm = V(space.innerProduct(residual, preconditioned))
                 else:
                     preconditionedNorm = residualNorm
-                searchScale = preconditionedNorm / previous
+                searchScale = V(preconditionedNorm / previous)
                 space.scale(search, searchScale)
                 space.add(search, preconditioned if preconditioned is not None else residual, 1)
-                stats.absoluteResidual = math.sqrt(residualNorm)
-                stats.relativeResidual = stats.absoluteResidual / initialResidualNorm
+                absolute = V(sqrt(residualNorm))
+                stats.absoluteResidual = float(absolute)
+                stats.relativeResidual = float(V(absolute / initialResidualNorm))
                 self.history.append(stats.absoluteResidual)
-                if _float_less(stats.absoluteResidual, settings.absoluteResidual) or _float_less(
-                        stats.relativeResidual, settings.relativeResidual):
+                if less(stats.absoluteResidual, settings.absoluteResidual) or less(stats.relativeResidual,
+                                                                                  settings.relativeResidual):
                     break
                 stats.iterationCount += 1
         self.report(stats)

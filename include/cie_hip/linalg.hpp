@@ -460,8 +460,8 @@ public:
             stats.absoluteResidual = initialResidualNorm;
             stats.relativeResidual = 1;
             if (less(stats.absoluteResidual, settings.absoluteResidual) || less(stats.relativeResidual, settings.relativeResidual)) {
+                // :100-111 return before the output step of :190-198: `out` keeps its value.
                 this->report(stats);
-                finish(inScale, outScale, out, solution);
                 return;
             }
             if (_preconditioner) {

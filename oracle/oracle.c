@@ -466,8 +466,10 @@ static int pcg_core(pool_t* pool, const csr_operator_t* A, const double* inverse
         const double initialResidualNorm = sqrt(residualNorm);
         s.absoluteResidual = initialResidualNorm;
         s.relativeResidual = 1.0;
+        /* :100-111: the early exit returns before the output step of :190-198,
+         * so `x` keeps its value whatever inScale and outScale are. */
         if (float_less(s.absoluteResidual, absoluteTolerance) || float_less(s.relativeResidual, relativeTolerance))
-            goto finish;                                                          /* :100-111 */
+            goto done;
 
         if (precondition) {                                                       /* :114-123 */
             diagonal_product(pool, inverse, 0.0, residual, 1.0, preconditioned, buffer, n);
@@ -509,7 +511,6 @@ static int pcg_core(pool_t* pool, const csr_operator_t* A, const double* inverse
         }
     }
 
-finish:
     /* :190-198 */
     if (inScale == 1.0) {
         vec(pool, OP_ADD, x, solution, outScale, n);

@@ -6,8 +6,12 @@
 // usage: facade_test <input.bin> <output-prefix>
 //   input: int64 rows, int64 nnz, double tolerance, int32 extents[rows+1],
 //          int32 columns[nnz], double entries[nnz], double rhs[rows]
-//   writes <prefix>_{generic,fused,mixed}.bin (rows doubles) and prints one
-//   line per solver: name iterations absolute relative
+//   writes <prefix>_{generic,fused,mixed}.bin (rows doubles) and
+//   <prefix>_generic32.bin (rows floats: ConjugateGradients<HIPSpace<float>>,
+//   the reference's FP32 instantiation, ConjugateGradients.cpp:10,16, run to
+//   1e-6 as the driver's settings are) and prints one line per solver: name
+//   iterations absolute relative; then "early <name> <scales> <0|1>", 1 when
+//   an early exit (ConjugateGradients_impl.hpp:100-111) left x bit for bit.
 #include "cie_hip/linalg.hpp"
 
 #include <cstdio>
@@ -87,6 +91,50 @@ int main(int argc, char** argv) {
             write(prefix + (mixed ? "_mixed.bin" : "_fused.bin"), host);
             std::printf("%s %zu %.17g %.17g\n", mixed ? "mixed" : "fused", stats.iterationCount,
                         stats.absoluteResidual, stats.relativeResidual);
+        }
+        {
+            // The reference's FP32 instantiation over the same CSR (FP32 plane).
+            auto space32 = std::make_shared<HIPSpace<float>>(context);
+            auto lhs32 = std::make_shared<HIPCSROperator<int, float>>(matrix, space32);
+            auto preconditioner32 = std::make_shared<HIPDiagonalOperator<float>>(
+                makeDiagonalOperator<float>(*matrix, std::shared_ptr<const HIPSpace<float>>(space32)));
+            using Solver32 = ConjugateGradients<HIPSpace<float>>;
+            Solver32 solver(lhs32, space32, preconditioner32, Solver32::Statistics{20000, 0.0f, 1e-6f}, 0);
+            std::vector<float> rhs32(rhs.begin(), rhs.end()), host32(static_cast<std::size_t>(rows));
+            auto b32 = space32->makeVector(std::size_t(rows));
+            space32->assign(space32->view(b32), std::span<const float>(rhs32));
+            auto x = space32->makeVector(std::size_t(rows));
+            space32->fill(space32->view(x), 0.0f);
+            solver.product(0, space32->view(std::as_const(b32)), 1, space32->view(x));
+            const auto stats = solver.getStats().value();
+            space32->assign(std::span<float>(host32), space32->view(std::as_const(x)));
+            std::ofstream out(prefix + "_generic32.bin", std::ios::binary);
+            out.write(reinterpret_cast<const char*>(host32.data()), std::streamsize(sizeof(float) * host32.size()));
+            std::printf("generic32 %zu %.9g %.9g\n", stats.iterationCount, double(stats.absoluteResidual),
+                        double(stats.relativeResidual));
+        }
+        {
+            // Early exit: an absolute tolerance above any residual ends every solve at the start check,
+            // which returns before the output step (:190-198), so x keeps its bits whatever the scales.
+            const double scales[2][2] = {{1.0, 1.0}, {0.5, -2.0}};
+            for (const char* name : {"generic", "fused", "mixed"}) {
+                for (const auto& scale : scales) {
+                    auto x = space->makeVector(std::size_t(rows));
+                    std::vector<double> before(static_cast<std::size_t>(rows)), after(before.size());
+                    for (std::size_t i = 0; i < before.size(); ++i) before[i] = double(i % 17) - 8.0;
+                    space->assign(space->view(x), std::span<const double>(before));
+                    const Solver::Statistics early{100, 1e300, 0.0};
+                    if (std::string(name) == "generic") {
+                        Solver solver(lhs, space, preconditioner, early, 0);
+                        solver.product(scale[0], space->view(std::as_const(b)), scale[1], space->view(x));
+                    } else {
+                        FusedConjugateGradients solver(matrix, space, {100, 1e300, 0.0}, std::string(name) == "mixed");
+                        solver.product(scale[0], space->view(std::as_const(b)), scale[1], space->view(x));
+                    }
+                    space->assign(std::span<double>(after), space->view(std::as_const(x)));
+                    std::printf("early %s %g,%g %d\n", name, scale[0], scale[1], before == after ? 1 : 0);
+                }
+            }
         }
         // A missing diagonal raises like makeDiagonalOperator's CIE_THROW (DiagonalOperator.cpp:79-82).
         std::vector<int> badExtents{0, 1, 2}, badColumns{0, 0};

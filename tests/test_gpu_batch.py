@@ -82,6 +82,34 @@ def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_batched_graph_follows_new_settings(hip, torch_gpu, mixed):
+    """A captured trip takes the tolerances, the iteration cap and the refinement threshold by value: a
+    second graph solve of the same b and x with other settings must capture again, reach its own tolerance
+    and equal the eager solve with those settings bit for bit."""
+    w = small_batch(seed=13)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x = torch_gpu.zeros_like(b)
+    loose, _, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-6), mixed=mixed, graph=True)
+    assert all(s.relativeResidual < 1e-6 for s in loose)
+    runs = []
+    for graph in (True, False):
+        x = torch_gpu.zeros_like(b)
+        stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-12), mixed=mixed, graph=graph)
+        torch_gpu.cuda.synchronize()
+        assert np.all(status == 1)
+        assert all(s.relativeResidual < 1e-12 for s in stats)
+        runs.append((x.cpu().numpy(), [s.iterationCount for s in stats]))
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert sum(runs[0][1]) > sum(s.iterationCount for s in loose)
+    # and a cap: the graph captured at 20000 must not keep that cap
+    x = torch_gpu.zeros_like(b)
+    capped, status, _, _ = batch.solve(b, x, Statistics(5, 0.0, 1e-12), mixed=mixed, graph=True)
+    assert all(s.iterationCount <= 5 for s in capped) and not np.all(status == 1)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_batched_sorted_tiles_equal_column_blocks(hip, torch_gpu, mixed):
     """Column-sorted tiles stage every product at its block's position: the batch solves bitwise as it does on
     the plain column-block tiles (tile partials are per tile, so the grids do not matter)."""

@@ -40,3 +40,28 @@ def test_cpp_facade_matches_oracle(tmp_path, torch_gpu):
             assert abs(iterations - int(s_ref.iterationCount)) <= 1, (name, iterations, s_ref.iterationCount)
             assert error < 1e-10, (name, error)
     assert lines["missing-diagonal"] == [str(6)]
+    # ConjugateGradients<HIPSpace<float>> (ConjugateGradients.cpp:10,16) to relative 1e-6
+    iterations, relative = int(lines["generic32"][0]), float(lines["generic32"][2])
+    assert relative < 1e-6 and iterations > 0
+    x32 = np.fromfile(tmp_path / "x_generic32.bin", dtype=np.float32).astype(np.float64)
+    assert_fp32_solution_bound(w, x32, x_ref)
+    early = [line.split() for line in result.stdout.splitlines() if line.startswith("early ")]
+    assert len(early) == 6 and all(fields[3] == "1" for fields in early), early
+
+
+def five_point_condition(m):
+    """2-norm condition number of the 5-point Dirichlet Laplacian on an m x m interior grid (closed form):
+    eigenvalues 4 - 2cos(j pi/(m+1)) - 2cos(k pi/(m+1)), so cond = cot^2(pi / (2(m+1)))."""
+    return 1.0 / np.tan(np.pi / (2 * (m + 1))) ** 2
+
+
+def assert_fp32_solution_bound(w, x32, x_ref):
+    """The FP32 solution is bounded against the FP64 oracle by its own FP64 true residual:
+    ||x32 - x*|| / ||x*|| <= cond(A) * ||b - A x32|| / ||b||, and the oracle is within cond(A) * 1e-8."""
+    r = oracle.csr_product(w.matrix, 1.0, x32, -1.0, w.rhs.copy())
+    true32 = np.linalg.norm(r) / np.linalg.norm(w.rhs)
+    cond = five_point_condition(256)
+    assert true32 < cond * np.finfo(np.float32).eps, true32  # FP32 attainable accuracy is about eps32 * cond
+    error = np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref)
+    assert error <= cond * (true32 + 1e-8), (error, true32)
+    return error, true32

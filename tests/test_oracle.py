@@ -176,6 +176,27 @@ def test_pcg_iteration_count_rules():
     assert none.iterationCount == 0 and none.absoluteResidual == np.finfo(float).max
 
 
+@pytest.mark.parametrize("in_scale,out_scale", [(1.0, 1.0), (0.5, -2.0), (0.0, 3.0)])
+def test_pcg_early_exit_leaves_output_untouched(in_scale, out_scale):
+    """ConjugateGradients_impl.hpp:100-111 returns before the output step of :190-198: when the initial
+    guess already meets the tolerance, `out` keeps its bits whatever the scales are. Integer x* on the
+    integer 5-point stencil makes b = A x* exact, so r0 = 0 exactly."""
+    w = systems.five_point_2d(16)
+    x_star = np.random.default_rng(5).integers(-9, 10, w.matrix.rows).astype(np.float64)
+    b = oracle.csr_product(w.matrix, 0.0, x_star, 1.0, np.zeros(w.matrix.rows))
+    x, stats, _, _ = oracle.pcg(w.matrix, b, x0=x_star, absolute_tolerance=1e-12, relative_tolerance=1e-8,
+                                in_scale=in_scale, out_scale=out_scale)
+    assert np.array_equal(x, x_star)
+    assert stats.iterationCount == 0 and stats.absoluteResidual == 0.0 and stats.relativeResidual == 1.0
+    # one ulp off the exact guess: the loop runs and the output step blends as :190-198 say
+    x0 = x_star.copy()
+    x0[3] += 0.5
+    x, stats, _, _ = oracle.pcg(w.matrix, b, x0=x0, absolute_tolerance=1e-12, relative_tolerance=1e-12,
+                                in_scale=in_scale, out_scale=out_scale)
+    assert stats.iterationCount > 0
+    assert np.allclose(x, in_scale * x0 + out_scale * x_star, atol=1e-9)
+
+
 def test_pcg_errors():
     neg = csr_from_dense(-np.eye(4))
     with pytest.raises(oracle.OracleError) as breakdown:
