@@ -63,7 +63,10 @@ class SolveInfo(ctypes.Structure):
 
 
 class MixedSettings(ctypes.Structure):
-    _fields_ = [("settings", Statistics), ("refinementFactor", c_double), ("restart", c_int32), ("pad", c_int32)]
+    _fields_ = [("settings", Statistics), ("refinementFactor", c_double), ("restart", c_int32), ("flags", c_int32)]
+
+
+MIXED_PLAIN_FP32 = 1  # CIE_HIP_MIXED_PLAIN_FP32: no row-sum correction in the inner FP32 product
 
 
 class PmgInfo(ctypes.Structure):
@@ -165,6 +168,7 @@ _SIGNATURES = {
     # Jacobi
     "cie_hip_inverse_diagonal_f64": (c_int, [_P, _P, _P, POINTER(c_int64)]),
     "cie_hip_inverse_diagonal_f32": (c_int, [_P, _P, _P, POINTER(c_int64)]),
+    "cie_hip_csr_row_sum_correction": (c_int, [_P, _P, _P]),
     "cie_hip_diagonal_product_f64": (c_int, [_P, _P, c_double, _P, c_double, _P, c_int64]),
     "cie_hip_diagonal_product_f32": (c_int, [_P, _P, c_float, _P, c_float, _P, c_int64]),
     # solvers

@@ -65,6 +65,8 @@ struct BatchSettings {
     double absoluteTolerance, relativeTolerance, updateThreshold;
     uint64_t maxIterations;
     int32_t restart, mixed;
+    int32_t rowSumCorrection;  // mixed: the FP32 product adds c_i p_i (the FP32 plane's row-sum correction)
+    int32_t unused;
 };
 
 // A work list: the selected systems (in system order), the prefix sums of
@@ -143,11 +145,13 @@ __global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
     }
 }
 
-// q = A p; tile partial p.q.
+// q = A p; tile partial p.q. With `correction` (the mixed form), q_i gets
+// c_i p_i added: the FP32 plane's row-sum correction.
 template <class T, int W>
 __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout layout, ActiveWork work,
                                                                  const T* __restrict__ entries, const T* __restrict__ p,
-                                                                 T* __restrict__ q, double* __restrict__ partials) {
+                                                                 T* __restrict__ q, double* __restrict__ partials,
+                                                                 const T* __restrict__ correction) {
     __shared__ T stage[kTileEntries];
     __shared__ int32_t extents[kTileBlock + 1];
     __shared__ T lds[kTileBlock / kWave];
@@ -156,8 +160,10 @@ __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout lay
         const int t = work.tileList[item].x;
         T pq = T(0);
         auto epilogue = [&](int row, T sum) {
-            q[row] = sum;
-            pq += p[row] * sum;
+            const T pi = p[row];
+            const T value = correction ? sum + correction[row] * pi : sum;
+            q[row] = value;
+            pq += pi * value;
         };
         BatchTiles<T, W>::tile(t, layout.source, entries, p, epilogue, stage, extents);
         pq = blockSum<kTileBlock>(pq, lds);
@@ -750,12 +756,15 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
     const dim3 productGrid = workGrid(batchProductKernel<TR, W>, ctx, kTileBlock, batch->tileCount);
+    const TR* rowCorrection = nullptr;
+    if constexpr (std::is_same_v<TR, float>)
+        if (settings.rowSumCorrection) rowCorrection = m->rowCorrection32;
     if (profile)  // eager launches only (a captured graph is never profiled)
         profile->launch(ctx, batchProductKernel<TR, W>, productGrid, tileBlock, layout, work, entries,
-                        static_cast<const TR*>(p), q, batch->tilePartials);
+                        static_cast<const TR*>(p), q, batch->tilePartials, rowCorrection);
     else
         hipLaunchKernelGGL((batchProductKernel<TR, W>), productGrid, tileBlock, 0, stream, layout, work, entries, p, q,
-                           batch->tilePartials);
+                           batch->tilePartials, rowCorrection);
     hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseAlpha>), systems, tileBlock, 0, stream, batch->systemTiles,
                        batch->tilePartials, batch->state, settings, batch->active);
     if constexpr (std::is_same_v<TR, float>) {  // the mixed form: FP32 correction, folded when refining
@@ -865,7 +874,7 @@ int runBatch(cie_hip_batch batch, const double* b, double* x, const BatchSetting
                                   g.relativeTolerance == settings.relativeTolerance &&
                                   g.updateThreshold == settings.updateThreshold &&
                                   g.maxIterations == settings.maxIterations && g.restart == settings.restart &&
-                                  g.mixed == settings.mixed;
+                                  g.mixed == settings.mixed && g.rowSumCorrection == settings.rowSumCorrection;
         const bool same = batch->graph && sameSettings && std::equal(key, key + 5, batch->graphKey) &&
                           batch->graphPointers[0] == b && batch->graphPointers[1] == x &&
                           batch->graphPointers[2] == inverse;
@@ -1155,6 +1164,7 @@ int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const doub
     s.maxIterations = settings->settings.iterationCount;
     s.restart = settings->restart ? 1 : 0;
     s.mixed = mixed ? 1 : 0;
+    s.rowSumCorrection = mixed && (settings->flags & CIE_HIP_MIXED_PLAIN_FP32) == 0 ? 1 : 0;
     if (s.maxIterations == 0)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pcg_batched: iterationCount must be positive");
     double seconds = 0.0;

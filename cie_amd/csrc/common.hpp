@@ -200,6 +200,7 @@ struct cie_hip_csr_s {
     bool ownsIndices = false, ownsEntries64 = false;
     double* inverseDiagonal64 = nullptr;  // built lazily for the fused solvers
     float* inverseDiagonal32 = nullptr;
+    float* rowCorrection32 = nullptr;     // with inverseDiagonal32: sum_j (a_ij - fl32(a_ij)) per row (mixed solvers)
     double meanRowLength = 0.0;
     int maxRowLength = 0;
     CsrLaunch launch64, launch32;

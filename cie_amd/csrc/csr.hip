@@ -228,6 +228,7 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->entries32) (void)hipFree(m->entries32);
     if (m->inverseDiagonal64) (void)hipFree(m->inverseDiagonal64);
     if (m->inverseDiagonal32) (void)hipFree(m->inverseDiagonal32);
+    if (m->rowCorrection32) (void)hipFree(m->rowCorrection32);
     if (m->workspace) (void)hipFree(m->workspace);
     for (TileSet& set : m->tileSets) {
         if (set.tiles) (void)hipFree(set.tiles);
@@ -260,16 +261,23 @@ int buildPlane32(cie_hip_csr m) {
     return CIE_HIP_OK;
 }
 
+// Lanes per row of the Jacobi set-up kernel: FEM rows hold tens to hundreds of entries.
+constexpr int kDiagonalLanes = 16;
+
 template <class T>
-int inverseDiagonal(cie_hip_csr m, T* inverse, int64_t* failedRow) {
+int inverseDiagonal(cie_hip_csr m, T* inverse, int64_t* failedRow, float* correction = nullptr) {
     cie_hip_context ctx = m->context;
     unsigned long long* failure = reinterpret_cast<unsigned long long*>(ctx->scratch.scalars);
     unsigned long long* pinned = reinterpret_cast<unsigned long long*>(ctx->scratch.pinned);
     pinned[0] = pinned[1] = std::numeric_limits<unsigned long long>::max();
     CIE_HIP_CHECK(hipMemcpyAsync(failure, pinned, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
     if (m->rows) {
-        hipLaunchKernelGGL((inverseDiagonalKernel<T>), dim3(vectorGrid(m->rows)), dim3(kReduceBlock), 0, ctx->stream,
-                           int(m->rows), m->rowExtents, m->columnIndices, m->entries64, inverse, failure);
+        constexpr int groupsPerBlock = kReduceBlock / kDiagonalLanes;
+        const int64_t blocks = (m->rows + groupsPerBlock - 1) / groupsPerBlock;
+        const int grid = int(std::min<int64_t>(blocks, int64_t(ctx->computeUnits) * 32));
+        hipLaunchKernelGGL((diagonalKernel<T, kDiagonalLanes>), dim3(std::max(grid, 1)), dim3(kReduceBlock), 0,
+                           ctx->stream, int(m->rows), m->rowExtents, m->columnIndices, m->entries64, inverse,
+                           correction, failure);
         CIE_HIP_CHECK(hipGetLastError());
     }
     CIE_HIP_CHECK(hipMemcpyAsync(pinned, failure, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -489,15 +497,26 @@ int ensureInverseDiagonal(cie_hip_csr m, int precision) {
     if (m->rows != m->columns)
         return setError(CIE_HIP_ERROR_SHAPE, "the Jacobi preconditioner needs a square matrix");
     if (precision == 32) {
-        if (m->inverseDiagonal32) return CIE_HIP_OK;
-        float* inverse = nullptr;
-        CIE_HIP_CHECK(hipMalloc(&inverse, sizeof(float) * std::max<int64_t>(1, m->rows)));
-        const int status = inverseDiagonal<float>(m, inverse, nullptr);
+        // The FP32 inverse diagonal and, from the same pass, the row-sum
+        // correction of the FP32 plane (both used by the mixed solvers).
+        if (m->inverseDiagonal32 && m->rowCorrection32) return CIE_HIP_OK;
+        float *inverse = nullptr, *correction = nullptr;
+        const size_t bytes = sizeof(float) * size_t(std::max<int64_t>(1, m->rows));
+        CIE_HIP_CHECK(hipMalloc(&inverse, bytes));
+        if (hipMalloc(&correction, bytes) != hipSuccess) {
+            (void)hipFree(inverse);
+            return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "row-sum correction: out of device memory");
+        }
+        const int status = inverseDiagonal<float>(m, inverse, nullptr, correction);
         if (status != CIE_HIP_OK) {
             (void)hipFree(inverse);
+            (void)hipFree(correction);
             return status;
         }
+        if (m->inverseDiagonal32) (void)hipFree(m->inverseDiagonal32);
+        if (m->rowCorrection32) (void)hipFree(m->rowCorrection32);
         m->inverseDiagonal32 = inverse;
+        m->rowCorrection32 = correction;
         return CIE_HIP_OK;
     }
     if (m->inverseDiagonal64) return CIE_HIP_OK;
@@ -540,6 +559,10 @@ int entriesChanged(cie_hip_csr m) {
     if (m->inverseDiagonal32) {
         CIE_HIP_CHECK(hipFree(m->inverseDiagonal32));
         m->inverseDiagonal32 = nullptr;
+    }
+    if (m->rowCorrection32) {
+        CIE_HIP_CHECK(hipFree(m->rowCorrection32));
+        m->rowCorrection32 = nullptr;
     }
     return CIE_HIP_OK;
 }
@@ -1024,6 +1047,22 @@ int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, fl
     if (matrix->rows != matrix->columns) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_inverse_diagonal_f32: matrix is not square");
     CIE_HIP_CHECK(hipSetDevice(context->device));
     return inverseDiagonal<float>(matrix, inverse, failedRow);
+}
+
+int cie_hip_csr_row_sum_correction(cie_hip_context context, cie_hip_csr matrix, float* correction) {
+    if (!context || !matrix || (!correction && matrix->rows))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: null argument");
+    if (context != matrix->context)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: matrix of another context");
+    if (!matrix->entries64)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: matrix has no FP64 plane");
+    CIE_HIP_CHECK(hipSetDevice(context->device));
+    if (const int status = ensureInverseDiagonal(matrix, 32)) return status;
+    if (!matrix->rows) return CIE_HIP_OK;
+    CIE_HIP_CHECK(hipMemcpyAsync(correction, matrix->rowCorrection32, sizeof(float) * size_t(matrix->rows),
+                                 hipMemcpyDeviceToDevice, context->stream));
+    CIE_HIP_CHECK(hipStreamSynchronize(context->stream));
+    return CIE_HIP_OK;
 }
 
 }  // extern "C"

@@ -813,25 +813,47 @@ __global__ __launch_bounds__(kReduceBlock) void innerProductFinishKernel(int cou
 
 // makeDiagonalOperator: inverse = 1 / a_ii; the first failing row (zero or
 // missing diagonal) is recorded with atomicMin in failure[0] / failure[1].
-template <class T>
-__global__ __launch_bounds__(kReduceBlock) void inverseDiagonalKernel(int rows, const int32_t* __restrict__ rowExtents,
-                                                                      const int32_t* __restrict__ columnIndices,
-                                                                      const double* __restrict__ entries,
-                                                                      T* __restrict__ inverse,
-                                                                      unsigned long long* failure) {
-    const int stride = gridDim.x * blockDim.x;
-    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < rows; row += stride) {
-        // Columns are sorted: binary search for the diagonal.
-        int lo = rowExtents[row], hi = rowExtents[row + 1];
-        while (lo < hi) {
-            const int mid = (lo + hi) / 2;
-            if (columnIndices[mid] < row) lo = mid + 1; else hi = mid;
+// Jacobi set-up (DiagonalOperator.cpp:37-88), a group of G lanes per row: the
+// group reads the row's entries G at a time, coalesced, and finds the
+// diagonal among them (columns are unique per row). inverse[row] = 1 / a_ii;
+// the first row without a diagonal entry or with a zero there is recorded in
+// failure[1] / failure[0] (atomicMin of the row; no value is accumulated).
+// With `correction`, also the row-sum correction of the FP32 plane:
+// correction[row] = fl32(sum_j (a_ij - fl32(a_ij))), each difference exact in
+// FP64 (Sterbenz), summed in a fixed order. The FP32 plane plus this term on
+// the diagonal keeps the row sums of the FP64 matrix, so the smooth vectors
+// that FEM stiffness rows annihilate are annihilated by the FP32 operator too.
+template <class T, int G>
+__global__ __launch_bounds__(kReduceBlock) void diagonalKernel(int rows, const int32_t* __restrict__ rowExtents,
+                                                               const int32_t* __restrict__ columnIndices,
+                                                               const double* __restrict__ entries,
+                                                               T* __restrict__ inverse, float* __restrict__ correction,
+                                                               unsigned long long* failure) {
+    static_assert(G >= 1 && G <= kWave && (G & (G - 1)) == 0, "lane groups are powers of two within a wave");
+    const int lane = int(threadIdx.x) & (G - 1);
+    const int groupsPerBlock = kReduceBlock / G;
+    const int stride = int(gridDim.x) * groupsPerBlock;
+    for (int row = int(blockIdx.x) * groupsPerBlock + int(threadIdx.x) / G; row < rows; row += stride) {
+        const int begin = rowExtents[row], end = rowExtents[row + 1];
+        double diagonal = 0.0, residue = 0.0;
+        int found = 0;
+        for (int k = begin + lane; k < end; k += G) {
+            const double a = entries[k];
+            if (columnIndices[k] == row) {
+                diagonal = a;
+                found = 1;
+            }
+            residue += a - double(float(a));
         }
-        const bool found = lo < rowExtents[row + 1] && columnIndices[lo] == row;
-        const double diagonal = found ? entries[lo] : 0.0;
-        if (!found) atomicMin(failure + 1, (unsigned long long)row);
-        else if (diagonal == 0.0) atomicMin(failure + 0, (unsigned long long)row);
-        inverse[row] = found && diagonal != 0.0 ? T(1.0 / diagonal) : T(0);
+        found = groupSum<G>(found);          // 0 or 1: columns are unique per row
+        diagonal = groupSum<G>(diagonal);    // one lane holds it, the others add zeros
+        residue = groupSum<G>(residue);
+        if (lane == 0) {
+            if (!found) atomicMin(failure + 1, (unsigned long long)row);
+            else if (diagonal == 0.0) atomicMin(failure + 0, (unsigned long long)row);
+            inverse[row] = found && diagonal != 0.0 ? T(1.0 / diagonal) : T(0);
+            if (correction) correction[row] = float(residue);
+        }
     }
 }
 

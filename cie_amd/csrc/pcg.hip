@@ -110,19 +110,25 @@ __global__ __launch_bounds__(Walker::kBlock) void startKernel(
     }
 }
 
-// q = A p, partial p.q (ConjugateGradients_impl.hpp:136-143).
+// q = A p, partial p.q (ConjugateGradients_impl.hpp:136-143). With
+// `correction` (the mixed form's FP32 product), q_i = (A32 p)_i + c_i p_i:
+// the row-sum correction restores the FP64 row sums that rounding the values
+// to FP32 perturbs (all in FP32 arithmetic).
 template <class Walker>
 __global__ __launch_bounds__(Walker::kBlock) void productKernel(
     RowSource source, const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ p,
-    typename Walker::Value* __restrict__ q, double* __restrict__ partials, const SolverState* __restrict__ state) {
+    typename Walker::Value* __restrict__ q, double* __restrict__ partials, const SolverState* __restrict__ state,
+    const typename Walker::Value* __restrict__ correction) {
     using T = typename Walker::Value;
     constexpr int BS = Walker::kBlock;
     __shared__ T lds[BS / kWave];
     if (state->done) return;
     T pq = T(0);
     Walker::run(source, entries, p, [&](int row, T sum) {
-        q[row] = sum;
-        pq += p[row] * sum;
+        const T pi = p[row];
+        const T value = correction ? sum + correction[row] * pi : sum;
+        q[row] = value;
+        pq += pi * value;
     });
     pq = blockSum<BS>(pq, lds);
     if (threadIdx.x == 0) partials[2 * Walker::slot()] = double(pq);
@@ -522,7 +528,7 @@ int launchRefine(cie_hip_csr m, int slot, const double* x, const double* b, floa
 // update kernel reads them.
 template <class T>
 int launchProduct(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q,
-                  ProductProfile& profile, PartialSource* sums) {
+                  ProductProfile& profile, PartialSource* sums, const T* correction = nullptr) {
     cie_hip_context ctx = m->context;
     int grid = 0;
     int status = CIE_HIP_OK;
@@ -533,7 +539,7 @@ int launchProduct(cie_hip_csr m, const T* entries, const CsrLaunch& launch, cons
         if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
         if (status != CIE_HIP_OK) return;
         profile.launch(ctx, productKernel<Walker>, dim3(grid), dim3(Walker::kBlock), source, entries, p, q,
-                       ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state));
+                       ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state), correction);
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     if (status != CIE_HIP_OK) return status;
@@ -635,6 +641,7 @@ struct Request {
     double refinementFactor;
     int restart;
     bool trueResidual;
+    bool rowSumCorrection;  // mixed: apply the FP32 plane's row-sum correction in the product
 };
 
 int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_info* info) {
@@ -730,7 +737,8 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         status = runLoop(ctx, timer, [&]() -> int {
             const int slot = trip++ & 1;
             PartialSource sums{};
-            int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, &sums);
+            int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, &sums,
+                                         q.rowSumCorrection ? m->rowCorrection32 : nullptr);
             if (s) return s;
             hipLaunchKernelGGL(mixedUpdateKernel, vgrid, vblock, 0, ctx->stream, slot, n, solution, correction, r32,
                                p32, q32, m->inverseDiagonal32, sums.count, sums.partials, ctx->scratch.vectorPartials,
@@ -796,7 +804,7 @@ int cie_hip_pcg_f64(cie_hip_context context, cie_hip_csr matrix, double inScale,
                     double* x, int64_t size, const cie_hip_statistics* settings, int trueResidual,
                     cie_hip_solve_info* info) {
     if (!settings) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pcg_f64: null settings");
-    Request q{inScale, outScale, b, x, size, *settings, false, 0.0, 0, trueResidual != 0};
+    Request q{inScale, outScale, b, x, size, *settings, false, 0.0, 0, trueResidual != 0, false};
     return solve(context, matrix, q, info);
 }
 
@@ -805,7 +813,8 @@ int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScal
     if (!settings) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pcg_mixed: null settings");
     double factor = settings->refinementFactor;
     if (!(factor > 0.0 && factor < 1.0)) factor = 0.1;
-    Request q{inScale, outScale, b, x, size, settings->settings, true, factor, settings->restart ? 1 : 0, true};
+    Request q{inScale, outScale, b, x, size, settings->settings, true, factor, settings->restart ? 1 : 0, true,
+              (settings->flags & CIE_HIP_MIXED_PLAIN_FP32) == 0};
     return solve(context, matrix, q, info);
 }
 

@@ -179,6 +179,15 @@ class CSRMatrix:
                                             byref(handle)), "cie_hip_csr_reorder")
         return CSRMatrix._adopt(handle, self.context, self.rows, self.columns, self.nnz)
 
+    def row_sum_correction(self):
+        """The FP32 plane's row-sum correction that the mixed solvers' inner product adds on the diagonal:
+        fl32(sum_j (a_ij - fl32(a_ij))) per row (a device float32 tensor)."""
+        out = _torch().empty(self.rows, dtype=_torch().float32, device=f"cuda:{self.context.device}")
+        _sync_torch()
+        check(library().cie_hip_csr_row_sum_correction(self.context.handle, self.handle, _dptr(out)),
+              "cie_hip_csr_row_sum_correction")
+        return out
+
     def update_entries(self, entries):
         """Replace the FP64 entries (device tensor, same pattern); refreshes the FP32 plane."""
         _sync_torch()
@@ -486,15 +495,17 @@ class FusedPCG(IterativeSolver):
 class MixedPCG(IterativeSolver):
     """FP32 inner PCG with FP64 solution accumulation and FP64 residual refinement (cie_hip_pcg_mixed)."""
 
-    def __init__(self, matrix, settings=None, refinement_factor=0.1, restart=False):
+    def __init__(self, matrix, settings=None, refinement_factor=0.1, restart=False, row_sum_correction=True):
         super().__init__(settings)
         self.matrix = matrix
         self.refinement_factor = refinement_factor
         self.restart = restart
+        self.row_sum_correction = row_sum_correction  # False: the inner product is the plain FP32 plane
         self.info = None
 
     def product(self, inScale, in_, outScale, out):
-        mixed = MixedSettings(self.getConfiguration(), self.refinement_factor, 1 if self.restart else 0, 0)
+        flags = 0 if self.row_sum_correction else _lib.MIXED_PLAIN_FP32
+        mixed = MixedSettings(self.getConfiguration(), self.refinement_factor, 1 if self.restart else 0, flags)
         info = SolveInfo()
         _sync_torch()
         status = library().cie_hip_pcg_mixed(self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_),
@@ -833,13 +844,15 @@ class Batch:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         check(library().cie_hip_batch_set_tile_rows(self.handle, rows.ctypes.data_as(c_void_p)))
 
-    def solve(self, b, x, settings, mixed=True, graph=True, refinement_factor=0.1, restart=False):
+    def solve(self, b, x, settings, mixed=True, graph=True, refinement_factor=0.1, restart=False,
+              row_sum_correction=True):
         """Solve every system in place on x; returns (stats, status, refinements, summary)."""
         stats = (Statistics * self.systems)()
         status = np.zeros(self.systems, dtype=np.int32)
         refinements = np.zeros(self.systems, dtype=np.uint64)
         summary = SolveInfo()
-        mixed_settings = MixedSettings(settings, refinement_factor, 1 if restart else 0, 0)
+        mixed_settings = MixedSettings(settings, refinement_factor, 1 if restart else 0,
+                                       0 if row_sum_correction else _lib.MIXED_PLAIN_FP32)
         _sync_torch()
         code = library().cie_hip_pcg_batched(self.matrix.context.handle, self.handle, _dptr(b), _dptr(x),
                                              byref(mixed_settings), 1 if mixed else 0, 1 if graph else 0, stats,

@@ -78,8 +78,16 @@ typedef struct cie_hip_mixed_settings {
     cie_hip_statistics settings;   /* caps and FP64 tolerances, as for cie_hip_pcg_f64 */
     double refinementFactor;       /* refine when ||r_32|| <= factor * ||r|| at the last refinement (default 0.1) */
     int32_t restart;               /* 1: reset the search direction at each refinement (classic restarted IR) */
-    int32_t pad;
+    int32_t flags;                 /* CIE_HIP_MIXED_* bits, 0 for the default form */
 } cie_hip_mixed_settings;
+
+/* The inner FP32 product applies the FP32 plane's row-sum correction by
+ * default: q_i = (A32 p)_i + c_i p_i with c_i = sum_j (a_ij - fl32(a_ij)),
+ * built once per matrix, so the FP32 operator keeps the FP64 row sums (FEM
+ * stiffness rows sum to zero away from the boundary, and rounding the values
+ * perturbs exactly the smooth modes CG converges last). This flag turns it
+ * off: the inner product is then the plain FP32 plane. */
+#define CIE_HIP_MIXED_PLAIN_FP32 1
 
 /* ---- errors, devices, contexts ------------------------------------------ */
 const char* cie_hip_last_error(void);
@@ -217,6 +225,12 @@ int cie_hip_fill_f32(cie_hip_context context, float* target, int64_t size, float
  * diagonal; *failedRow (optional, host) receives that row. */
 int cie_hip_inverse_diagonal_f64(cie_hip_context context, cie_hip_csr matrix, double* inverse, int64_t* failedRow);
 int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, float* inverse, int64_t* failedRow);
+/* The row-sum correction of the FP32 plane that the mixed solvers' inner
+ * product adds on the diagonal (no reference counterpart):
+ * correction[i] = fl32(sum_j (a_ij - fl32(a_ij))), copied to `correction`
+ * (device, rows floats). Needs a square matrix with a diagonal in every row
+ * (it is built with the FP32 Jacobi inverse). */
+int cie_hip_csr_row_sum_correction(cie_hip_context context, cie_hip_csr matrix, float* correction);
 /* DiagonalOperator::product (DiagonalOperator.cpp:37-53): out = inScale*out + outScale*D^-1*in. */
 int cie_hip_diagonal_product_f64(cie_hip_context context, const double* inverse, double inScale, const double* in,
                                  double outScale, double* out, int64_t size);

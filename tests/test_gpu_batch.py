@@ -41,15 +41,17 @@ def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu, blocks):
         assert np.linalg.norm(got[a:c] - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
 
 
-@pytest.mark.parametrize("restart,blocks", [(False, 0), (True, 0), (False, 3)])
-def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks):
+@pytest.mark.parametrize("restart,blocks,correction", [(False, 0, True), (True, 0, True), (False, 3, True),
+                                                      (False, 3, False)])
+def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks, correction):
     w = small_batch(seed=8)
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
     batch.set_column_blocks(blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     x = torch_gpu.zeros_like(b)
-    stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, restart=restart)
+    stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, restart=restart,
+                                                row_sum_correction=correction)
     assert np.all(status == 1) and np.all(refinements >= 1)
     assert np.all(per_system_residuals(w, x.cpu().numpy()) < 1e-10)
     assert all(s.relativeResidual < 1e-10 for s in stats)

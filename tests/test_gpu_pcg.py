@@ -272,6 +272,8 @@ def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu, q2_64, launch):
     assert true_relative_residual(w.matrix, host, w.rhs) < 1e-10
     error_mixed = rel(host, x_ref)
     assert error_mixed < MIXED_SOLUTION_TOLERANCE
+    # with the FP32 plane's row-sum correction the inner solve needs the FP64 iteration count
+    assert int(mixed.getStats().iterationCount) <= int(fp64.getStats().iterationCount) + 6
     print(f"{launch} {matrix.launch(64)} {matrix.launch(32)}: fp64 {fp64.getStats().iterationCount} "
           f"iterations (oracle {s_ref.iterationCount}) error vs oracle {error64:.2e}; mixed "
           f"{mixed.getStats().iterationCount} iterations, {mixed.info.refinements} refinements, error vs oracle "
