@@ -141,6 +141,9 @@ def finite_difference(problem, g, u, elements, h=1e-6, tolerance=1e-13):
     device = problem.device
     k = np.exp(np.asarray(g, dtype=np.float64))
     r0 = problem.rhs.clone()
+    # The clone runs on torch's stream and the product on the library's: wait for the clone (an unordered
+    # read of r0 would start every correction system from whatever that memory held before).
+    torch.cuda.synchronize()
     check(library().cie_hip_csr_product_f64(problem.context.handle, matrix.handle, 1.0, _dptr(u), problem.rows, -1.0,
                                             _dptr(r0), problem.rows), "r0")
     problem.context.synchronize()

@@ -70,6 +70,7 @@ struct Scratch {
     int partialSlots = 0;
     double* level = nullptr;      // kLevelBlocks * 2 doubles: first level of a large partial reduction
     double* vectorPartials = nullptr;  // kVectorPartials * 2 doubles: partials of the solver's vector kernels
+    double* refinePartials = nullptr;  // kMaxPartials * 2 doubles: FP64 r.r partials of the mixed refinement
     double* scalars = nullptr;    // 64 doubles of device scalar results
     double* pinned = nullptr;     // 64 doubles of pinned host memory for readback
     SolverState* state = nullptr; // one solver state

@@ -91,6 +91,7 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     bool ok = hipMalloc(&s.partials, sizeof(double) * ciehip::kMaxPartials * 4) == hipSuccess
            && hipMalloc(&s.level, sizeof(double) * ciehip::kLevelBlocks * 2) == hipSuccess
            && hipMalloc(&s.vectorPartials, sizeof(double) * ciehip::kVectorPartials * 2) == hipSuccess
+           && hipMalloc(&s.refinePartials, sizeof(double) * ciehip::kMaxPartials * 2) == hipSuccess
            && hipMalloc(&s.scalars, sizeof(double) * 64) == hipSuccess
            && hipMalloc(&s.state, sizeof(ciehip::SolverState)) == hipSuccess
            && hipHostMalloc(&s.pinned, sizeof(double) * 64) == hipSuccess
@@ -111,6 +112,7 @@ int cie_hip_context_destroy(cie_hip_context context) {
     if (s.partials) (void)hipFree(s.partials);
     if (s.level) (void)hipFree(s.level);
     if (s.vectorPartials) (void)hipFree(s.vectorPartials);
+    if (s.refinePartials) (void)hipFree(s.refinePartials);
     if (s.scalars) (void)hipFree(s.scalars);
     if (s.state) (void)hipFree(s.state);
     if (s.pinned) (void)hipHostFree(s.pinned);

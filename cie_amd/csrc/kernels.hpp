@@ -13,6 +13,7 @@
 #include "common.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace ciehip {
 
@@ -182,8 +183,8 @@ struct VectorRows {
         const int32_t* __restrict__ rowExtents = source.rowExtents;
         const int32_t* __restrict__ columnIndices = source.columnIndices;
         const int chunk = xcdChunk();
-        const int rowBegin = min(source.rows, chunk * source.param);
-        const int rowEnd = min(source.rows, rowBegin + source.param);
+        const int rowBegin = int(min<int64_t>(source.rows, int64_t(chunk) * source.param));
+        const int rowEnd = int(min<int64_t>(source.rows, int64_t(rowBegin) + source.param));
         const int group = threadIdx.x / L;
         const int lane = threadIdx.x % L;
         for (int row0 = rowBegin; row0 < rowEnd; row0 += BS / L) {
@@ -195,6 +196,13 @@ struct VectorRows {
         }
     }
 };
+
+// Walkers over block tiles of whole consecutive rows (BlockTiles,
+// ColumnBlockTiles, SortedBlockTiles) expose tile(t, ...) and kCapacity.
+template <class W, class = void>
+struct IsTileWalker : std::false_type {};
+template <class W>
+struct IsTileWalker<W, std::void_t<decltype(W::kTiles)>> : std::bool_constant<W::kTiles> {};
 
 // Orders a wavefront's LDS writes before its later LDS reads of other lanes' data.
 __device__ __forceinline__ void stageFence() {
@@ -295,8 +303,8 @@ struct StagedRows {
         const int32_t* __restrict__ columnIndices = source.columnIndices;
         const int tiles = (rows + kTile - 1) / kTile;
         const int chunk = xcdChunk();
-        const int tileBegin = min(tiles, chunk * source.param);
-        const int tileEnd = min(tiles, tileBegin + source.param);
+        const int tileBegin = int(min<int64_t>(tiles, int64_t(chunk) * source.param));
+        const int tileEnd = int(min<int64_t>(tiles, int64_t(tileBegin) + source.param));
         const int wave = threadIdx.x / kWave;
         const int lane = threadIdx.x % kWave;
         for (int tile = tileBegin + wave; tile < tileEnd; tile += BS / kWave) {
@@ -325,6 +333,7 @@ struct StagedRows {
 // resident round), block b handles tiles b, b + grid, ... in turn.
 template <class TV, class TM, int U, int BS>
 struct BlockTiles {
+    static constexpr bool kTiles = true;  // walks tiles of whole rows: tile(t, ...) is public
     using Value = TV;
     using Matrix = TM;
     static constexpr int kBlock = BS;
@@ -479,6 +488,7 @@ struct ValueRun {  // W consecutive values; loads of it need only T's alignment
 
 template <class TV, class TM, int U, int BS, int W>
 struct ColumnBlockTiles {
+    static constexpr bool kTiles = true;  // walks tiles of whole rows: tile(t, ...) is public
     using Value = TV;
     using Matrix = TM;
     static constexpr int kBlock = BS;
@@ -649,6 +659,7 @@ struct ColumnBlockTiles {
 
 template <class TV, class TM, int U, int BS, int W>
 struct SortedBlockTiles {
+    static constexpr bool kTiles = true;  // walks tiles of whole rows: tile(t, ...) is public
     using Value = TV;
     using Matrix = TM;
     using Plain = ColumnBlockTiles<TV, TM, U, BS, W>;

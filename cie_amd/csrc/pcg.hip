@@ -134,6 +134,48 @@ __global__ __launch_bounds__(Walker::kBlock) void productKernel(
     if (threadIdx.x == 0) partials[2 * Walker::slot()] = double(pq);
 }
 
+// The same product for walkers over tiles of whole rows. Thread `tid` owns
+// row first + tid of the tile: it loads p and the correction of that row when
+// the tile starts (the loads are in flight while the tile streams), the
+// walker leaves each row sum in LDS, and thread `tid` then writes q and adds
+// its p.q term: coalesced stores, and no dependent load after the row sums.
+template <class Walker>
+__global__ __launch_bounds__(Walker::kBlock) void tileProductKernel(
+    RowSource source, const typename Walker::Matrix* __restrict__ entries, const typename Walker::Value* __restrict__ p,
+    typename Walker::Value* __restrict__ q, double* __restrict__ partials, const SolverState* __restrict__ state,
+    const typename Walker::Value* __restrict__ correction) {
+    using T = typename Walker::Value;
+    constexpr int BS = Walker::kBlock;
+    __shared__ T stage[Walker::kCapacity];
+    __shared__ int32_t extents[BS + 1];
+    __shared__ T rowSums[BS];
+    __shared__ T lds[BS / kWave];
+    if (state->done) return;
+    const int tid = threadIdx.x;
+    T pq = T(0);
+    for (int t = blockIdx.x; t < source.tileCount; t += gridDim.x) {
+        const int first = source.tiles[t].x;
+        const int rows = source.tiles[t + 1].x - first;  // at most BS rows per tile
+        T pi = T(0), ci = T(0);
+        if (tid < rows) {
+            pi = p[first + tid];
+            if (correction) ci = correction[first + tid];
+        }
+        auto stash = [&](int row, T sum) { rowSums[row - first] = sum; };
+        Walker::tile(t, source, entries, p, stash, stage, extents);
+        __syncthreads();
+        if (tid < rows) {
+            const T sum = rowSums[tid];
+            const T value = correction ? sum + ci * pi : sum;
+            q[first + tid] = value;
+            pq += pi * value;
+        }
+        __syncthreads();  // stage, extents and rowSums are reused by the next tile
+    }
+    pq = blockSum<BS>(pq, lds);
+    if (threadIdx.x == 0) partials[2 * blockIdx.x] = double(pq);
+}
+
 // First level of a sum over more than kLevelBlocks partials (one per tile of
 // a tiled SpMV): block b sums slots [b * per, min(count, b * per + per)) in a
 // fixed order, both values of a slot, in FP64. One compute unit cannot read
@@ -233,7 +275,11 @@ __device__ __forceinline__ bool refineDue(const SolverState& s) {
 // into the FP64 x (and cleared) on a trip that refines; r -= alpha q;
 // partials r.r and r.z. Every block takes the refinement decision from the
 // state the previous trip left (none of which this kernel writes) and block 0
-// records it for the refinement and direction kernels.
+// records it in the slot of this trip's parity. If the previous trip folded,
+// this trip's level kernel has replaced r by the rounded true residual
+// b - A x of the folded x, with its FP64 r.r in `refinePartials`: the
+// convergence test runs on that FP64 norm first (x is final if it passes),
+// and the refinement threshold restarts from it.
 __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, int64_t n, double* __restrict__ x,
                                                                     float* __restrict__ d, float* __restrict__ r,
                                                                     const float* __restrict__ p,
@@ -241,18 +287,22 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
                                                                     const float* __restrict__ inverse,
                                                                     int productCount,
                                                                     const double* __restrict__ productPartials,
+                                                                    int refineCount,
+                                                                    const double* __restrict__ refinePartials,
                                                                     double* __restrict__ partials,
                                                                     SolverState* __restrict__ state) {
     __shared__ float lds[kVectorThreads / kWave];
     __shared__ float lds2[kVectorThreads / kWave];
-    if (state->done) return;
-    const bool fold = refineDue(*state);
+    __shared__ double lds64[kVectorThreads / kWave];
+    SolverState& s = *state;
+    if (s.done) return;
+    const bool replaced = s.refined[slot ^ 1] != 0;
     constexpr int kAhead = aheadRows<float>();
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t first = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     float ds[kAhead], ps[kAhead], qs[kAhead], rs[kAhead], is[kAhead];
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) {  // the first rows' loads are in flight during the partial sum
+    for (int k = 0; k < kAhead; ++k) {  // the first rows' loads are in flight during the partial sums
         const int64_t i = first + k * stride;
         if (i < n) {
             ds[k] = d[i];
@@ -262,15 +312,38 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
             is[k] = inverse[i];
         }
     }
+    bool fold;
+    if (replaced) {
+        const double rr = sumPartialsAll<kVectorThreads, double>(refinePartials, refineCount, 0, lds64);
+        const double absolute = sqrt(rr);
+        const double relative = absolute / s.initialNorm;
+        const bool converged = floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance);
+        if (firstThread()) {
+            s.refinements += 1;
+            s.rr = rr;
+            s.absolute = absolute;
+            s.relative = relative;
+            s.updateNorm = rr;
+            // The trip that folded counts as the last one: its direction step already counted it.
+            if (converged) {
+                s.iteration -= 1;
+                s.done = kConverged;
+            }
+        }
+        if (converged) return;
+        fold = s.iteration + 1 >= s.maxIterations;  // just refined: only the iteration cap folds again
+    } else {
+        fold = refineDue(s);
+    }
     const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
     if (!(0.0f < pq)) {
-        if (firstThread()) state->done = kNotSpd;
+        if (firstThread()) s.done = kNotSpd;
         return;
     }
-    const float alpha = float(state->rhos[slot]) / pq;
+    const float alpha = float(s.rhos[slot]) / pq;
     if (firstThread()) {
-        state->alpha = double(alpha);
-        state->refined[slot] = fold ? 1 : 0;
+        s.alpha = double(alpha);
+        s.refined[slot] = fold ? 1 : 0;
     }
     float rr = 0.0f, rz = 0.0f;
     const auto row = [&](int64_t i, float di, float pi, float qi, float ri, float ii) {
@@ -301,49 +374,62 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
     }
 }
 
-// Mixed form, every trip: if the update decided to refine (and folded the
-// correction into x), r = b - A x with the FP64 matrix and x, and partials of
-// the FP64 r.r and the FP32 r.z; else nothing.
+// Mixed form, the kernel after the product, every trip. Blocks below
+// `levelBlocks` first sum the product's partials into `level` (the first level
+// of partialLevelKernel; none when the product wrote few enough partials).
+// Then, if the previous trip folded the correction into x (refined[previous]),
+// every block recomputes its rows of r = b - A x with the FP64 plane and x
+// (rounded into the FP32 r) and writes a partial of the FP64 r.r to
+// `refinePartials`; else it returns. The product ahead of it used only p, so
+// the true residual replaces the recursive one before this trip's update,
+// which tests convergence on its FP64 norm.
 template <class Walker>
-__global__ __launch_bounds__(Walker::kBlock) void refineKernel(
-    int slot, RowSource source, const double* __restrict__ entries, const double* __restrict__ x,
-    const double* __restrict__ b, float* __restrict__ r, const float* __restrict__ inverse,
-    double* __restrict__ partials, SolverState* __restrict__ state) {
+__global__ __launch_bounds__(Walker::kBlock) void levelRefineKernel(
+    int levelBlocks, int levelCount, int levelPer, const double* __restrict__ productPartials,
+    double* __restrict__ level, int previous, RowSource source, const double* __restrict__ entries,
+    const double* __restrict__ x, const double* __restrict__ b, float* __restrict__ r,
+    double* __restrict__ refinePartials, const SolverState* __restrict__ state) {
     constexpr int BS = Walker::kBlock;
-    __shared__ double ldsRR[BS / kWave];
-    __shared__ float ldsRZ[BS / kWave];
-    if (state->done || !state->refined[slot]) return;
+    static_assert(BS == kReduceBlock, "the level sums use the reduction block size");
+    __shared__ double ldsA[BS / kWave];
+    __shared__ double ldsB[BS / kWave];
+    if (state->done) return;
+    if (int(blockIdx.x) < levelBlocks) {
+        const int begin = int(blockIdx.x) * levelPer;
+        const int end = min(levelCount, begin + levelPer);
+        double a = 0.0, c = 0.0;
+        for (int i = begin + int(threadIdx.x); i < end; i += BS) {
+            a += productPartials[2 * i];
+            c += productPartials[2 * i + 1];
+        }
+        a = blockSum<BS>(a, ldsA);
+        c = blockSum<BS>(c, ldsB);
+        if (threadIdx.x == 0) {
+            level[2 * blockIdx.x] = a;
+            level[2 * blockIdx.x + 1] = c;
+        }
+    }
+    if (!state->refined[previous]) return;
     double rr = 0.0;
-    float rz = 0.0f;
     Walker::run(source, entries, x, [&](int row, double sum) {
         const double t = b[row] - sum;
-        const float residual = float(t);
-        const float z = residual * inverse[row];
-        r[row] = residual;
+        r[row] = float(t);
         rr += t * t;
-        rz += residual * z;
     });
-    const int own = Walker::slot();
-    rr = blockSum<BS>(rr, ldsRR);
-    rz = blockSum<BS>(rz, ldsRZ);
-    if (threadIdx.x == 0) {
-        partials[2 * own] = rr;
-        partials[2 * own + 1] = double(rz);
-    }
+    rr = blockSum<BS>(rr, ldsA);
+    if (threadIdx.x == 0) refinePartials[2 * Walker::slot()] = rr;
 }
 
 // beta, the convergence test and the iteration count (:158-185), then
 // p = beta p + D^-1 r (:168-170: scale, then add). kMixed: the FP32 recurrence
-// step, or after a refinement the FP64 test on the true residual and the
-// restart of the recurrence from it.
+// step only (the mixed form tests convergence on FP64 true residuals, in the
+// update kernel).
 template <class T, bool kMixed>
 __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int64_t n, T* __restrict__ p,
                                                                   const T* __restrict__ r, const T* __restrict__ inverse,
                                                                   int updateCount, const double* __restrict__ updatePartials,
-                                                                  int refineCount, const double* __restrict__ refinePartials,
                                                                   SolverState* __restrict__ state) {
     __shared__ T ldsT[kVectorThreads / kWave];
-    __shared__ double lds64[kVectorThreads / kWave];
     SolverState& s = *state;
     if (s.done) return;
     constexpr int kAhead = aheadRows<T>();
@@ -382,32 +468,13 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
                 if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
             }
         }
-    } else if (s.refined[slot]) {
-        const double rr = sumPartialsAll<kVectorThreads, double>(refinePartials, refineCount, 0, lds64);
-        const float rz = sumPartialsAll<kVectorThreads, float>(refinePartials, refineCount, 1, ldsT);
-        const double absolute = sqrt(rr);
-        const double relative = absolute / s.initialNorm;
-        converged = floatLess(absolute, s.absoluteTolerance) || floatLess(relative, s.relativeTolerance);
-        beta = s.restart ? 0.0 : double(rz / float(rhoPrevious));
-        if (firstThread()) {
-            s.refinements += 1;
-            s.rr = rr;
-            s.absolute = absolute;
-            s.relative = relative;
-            if (converged) {
-                s.done = kConverged;
-            } else {
-                s.rhos[slot ^ 1] = double(rz);
-                s.beta = beta;
-                s.updateNorm = rr;
-                s.iteration += 1;
-                if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
-            }
-        }
     } else {
+        // The FP32 recurrence step. A trip that folded the correction into x
+        // resets the search direction here when `restart` is set; its true
+        // residual replaces r on the next trip (levelRefineKernel).
         const float rr = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 0, ldsT);
         const float rz = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 1, ldsT);
-        beta = double(rz / float(rhoPrevious));
+        beta = s.restart && s.refined[slot] ? 0.0 : double(rz / float(rhoPrevious));
         if (firstThread()) {
             const double absolute = sqrt(double(rr));
             s.rhos[slot ^ 1] = double(rz);
@@ -416,6 +483,7 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
             s.absolute = absolute;
             s.relative = absolute / s.initialNorm;
             s.iteration += 1;
+            if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
         }
     }
     if (converged) return;
@@ -501,21 +569,39 @@ int launchStart(cie_hip_csr m, const double* x, const double* b, TR* r, const TR
     return CIE_HIP_OK;
 }
 
-// Mixed form: the refinement kernel (resident-round grid: it usually exits at
-// once). Returns its partial count in `count`.
-int launchRefine(cie_hip_csr m, int slot, const double* x, const double* b, float* r, int* count) {
+// Mixed form: the kernel after the product (levelRefineKernel) with the
+// FP64 walker on a resident-round grid (it usually only sums the product's
+// partials). `products` are the product's partials; returns where the update
+// kernel reads them in `sums` and the refinement partials' count in `count`.
+int launchLevelRefine(cie_hip_csr m, int previous, int products, const double* x, const double* b, float* r,
+                      PartialSource* sums, int* count) {
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
+    int levelBlocks = 0, per = 0;
+    if (products > kLevelBlocks) {
+        per = (products + kLevelBlocks - 1) / kLevelBlocks;
+        levelBlocks = (products + per - 1) / per;
+        *sums = {ctx->scratch.level, levelBlocks};
+    } else {
+        *sums = {ctx->scratch.partials, products};
+    }
     int grid = 0;
     int status = CIE_HIP_OK;
     const bool ok = withWalker<double, double>(launch, [&](auto walker) {
         using Walker = decltype(walker);
         RowSource source;
-        status = prepareLaunch(m, launch, refineKernel<Walker>, source, grid, true);
-        if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+        status = prepareLaunch(m, launch, levelRefineKernel<Walker>, source, grid, true);
         if (status != CIE_HIP_OK) return;
-        hipLaunchKernelGGL((refineKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, slot, source,
-                           m->entries64, x, b, r, m->inverseDiagonal32, ctx->scratch.partials, ctx->scratch.state);
+        // A larger grid only adds blocks without rows: the tile walkers stride
+        // over tiles, the row-chunk walkers' extra chunks lie past the last row.
+        grid = std::max(grid, levelBlocks);
+        if (grid > kMaxPartials) {
+            status = setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "refinement grid exceeds its partial slots");
+            return;
+        }
+        hipLaunchKernelGGL((levelRefineKernel<Walker>), dim3(grid), dim3(Walker::kBlock), 0, ctx->stream, levelBlocks,
+                           products, per, ctx->scratch.partials, ctx->scratch.level, previous, source, m->entries64, x,
+                           b, r, ctx->scratch.refinePartials, static_cast<const SolverState*>(ctx->scratch.state));
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     *count = grid;
@@ -526,24 +612,36 @@ int launchRefine(cie_hip_csr m, int slot, const double* x, const double* b, floa
 // solver stream (profiling on).
 // q = A p with its p.q partials (first-level sum included); returns where the
 // update kernel reads them.
+// `sums` null: the caller sums the partials (the mixed form's level kernel);
+// `*partialCount` receives their number.
 template <class T>
 int launchProduct(cie_hip_csr m, const T* entries, const CsrLaunch& launch, const T* p, T* q,
-                  ProductProfile& profile, PartialSource* sums, const T* correction = nullptr) {
+                  ProductProfile& profile, PartialSource* sums, const T* correction = nullptr,
+                  int* partialCount = nullptr) {
     cie_hip_context ctx = m->context;
     int grid = 0;
     int status = CIE_HIP_OK;
     const bool ok = withWalker<T, T>(launch, [&](auto walker) {
         using Walker = decltype(walker);
         RowSource source;
-        status = prepareLaunch(m, launch, productKernel<Walker>, source, grid);
-        if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
-        if (status != CIE_HIP_OK) return;
-        profile.launch(ctx, productKernel<Walker>, dim3(grid), dim3(Walker::kBlock), source, entries, p, q,
-                       ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state), correction);
+        if constexpr (IsTileWalker<Walker>::value) {
+            status = prepareLaunch(m, launch, tileProductKernel<Walker>, source, grid);
+            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+            if (status != CIE_HIP_OK) return;
+            profile.launch(ctx, tileProductKernel<Walker>, dim3(grid), dim3(Walker::kBlock), source, entries, p, q,
+                           ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state), correction);
+        } else {
+            status = prepareLaunch(m, launch, productKernel<Walker>, source, grid);
+            if (status == CIE_HIP_OK) status = ensurePartials(ctx, grid);
+            if (status != CIE_HIP_OK) return;
+            profile.launch(ctx, productKernel<Walker>, dim3(grid), dim3(Walker::kBlock), source, entries, p, q,
+                           ctx->scratch.partials, static_cast<const SolverState*>(ctx->scratch.state), correction);
+        }
     });
     if (!ok) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "unsupported SpMV launch shape");
     if (status != CIE_HIP_OK) return status;
-    *sums = levelPartials(ctx, grid);
+    if (partialCount) *partialCount = grid;
+    if (sums) *sums = levelPartials(ctx, grid);
     return CIE_HIP_OK;
 }
 
@@ -722,8 +820,7 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
                                p64, q64, m->inverseDiagonal64, sums.count, sums.partials, ctx->scratch.vectorPartials,
                                ctx->scratch.state);
             hipLaunchKernelGGL((directionKernel<double, false>), vgrid, vblock, 0, ctx->stream, slot, n, p64, r64,
-                               m->inverseDiagonal64, updateCount, ctx->scratch.vectorPartials, 0,
-                               static_cast<const double*>(nullptr), ctx->scratch.state);
+                               m->inverseDiagonal64, updateCount, ctx->scratch.vectorPartials, ctx->scratch.state);
             return CIE_HIP_OK;
         }, &seconds);
     } else {
@@ -734,21 +831,24 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         if (status) return status;
         hipLaunchKernelGGL((vectorKernel<float>), dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, int(kFill),
                            n, correction, static_cast<const float*>(nullptr), 0.0f);
+        // One trip: q = A32 p (+ c p); the product's level sums with, after a
+        // trip that folded, the FP64 true residual replacing r; the update
+        // (FP64 convergence test first when r was replaced); the direction.
         status = runLoop(ctx, timer, [&]() -> int {
             const int slot = trip++ & 1;
+            int products = 0;
+            int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, nullptr,
+                                         q.rowSumCorrection ? m->rowCorrection32 : nullptr, &products);
+            if (s) return s;
             PartialSource sums{};
-            int s = launchProduct<float>(m, m->entries32, m->launch32, p32, q32, profile, &sums,
-                                         q.rowSumCorrection ? m->rowCorrection32 : nullptr);
+            int refineCount = 0;
+            s = launchLevelRefine(m, slot ^ 1, products, solution, q.b, r32, &sums, &refineCount);
             if (s) return s;
             hipLaunchKernelGGL(mixedUpdateKernel, vgrid, vblock, 0, ctx->stream, slot, n, solution, correction, r32,
-                               p32, q32, m->inverseDiagonal32, sums.count, sums.partials, ctx->scratch.vectorPartials,
-                               ctx->scratch.state);
-            int refineCount = 0;
-            s = launchRefine(m, slot, solution, q.b, r32, &refineCount);
-            if (s) return s;
+                               p32, q32, m->inverseDiagonal32, sums.count, sums.partials, refineCount,
+                               ctx->scratch.refinePartials, ctx->scratch.vectorPartials, ctx->scratch.state);
             hipLaunchKernelGGL((directionKernel<float, true>), vgrid, vblock, 0, ctx->stream, slot, n, p32, r32,
-                               m->inverseDiagonal32, updateCount, ctx->scratch.vectorPartials, refineCount,
-                               ctx->scratch.partials, ctx->scratch.state);
+                               m->inverseDiagonal32, updateCount, ctx->scratch.vectorPartials, ctx->scratch.state);
             return CIE_HIP_OK;
         }, &seconds);
     }
@@ -764,8 +864,10 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     result.seconds = seconds;
     result.refinements = final.refinements;
     // Loop trips that ran the product kernel: iteration (+1 for the converged
-    // trip); none when the start check ended the solve.
-    const uint64_t trips = final.startExit ? 0 : final.iteration + (final.done == kConverged ? 1 : 0);
+    // trip; the mixed form detects convergence in the trip after the one that
+    // folded, +2); none when the start check ended the solve.
+    const uint64_t converging = final.done == kConverged ? (q.mixed ? 2 : 1) : 0;
+    const uint64_t trips = final.startExit ? 0 : final.iteration + converging;
     result.spmvCount = trips + 1 + final.refinements;
     if (ctx->profiling) {
         const int worked = int(std::min<uint64_t>(trips, uint64_t(cie_hip_context_s::kProfileEvents)));
@@ -783,6 +885,11 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
         status = trueResidualNorm(m, q.b, solution, temp64, &norm);
         if (status) return status;
         result.trueRelativeResidual = final.initialNorm > 0 ? norm / final.initialNorm : norm;
+        if (q.mixed && final.done == kMaxIterations) {
+            // Stopped by the cap: report the FP64 true residual of the returned x, not the FP32 recursion's.
+            result.stats.absoluteResidual = norm;
+            result.stats.relativeResidual = result.trueRelativeResidual;
+        }
     }
     // ConjugateGradients_impl.hpp:100-111 return before the output step of
     // :190-198 when the initial residual already meets the tolerance: x keeps

@@ -188,6 +188,29 @@ def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart, tiled):
     assert rel(got, x_ref) < 1e-6
 
 
+@pytest.mark.parametrize("restart", [False, True])
+def test_mixed_pcg_iteration_cap_and_cadence(hip, torch_gpu, restart):
+    """The mixed form stops at the cap with x holding every correction (the last trip folds) and reports the
+    FP64 true residual of that x; refinements recompute r = b - A x on the trip after the fold."""
+    w = systems.five_point_2d(48)
+    matrix = hip.CSRMatrix(w.matrix)
+    b = to_dev(torch_gpu, w.rhs)
+    solver = hip.MixedPCG(matrix, Statistics(25, 0.0, 1e-30), restart=restart)
+    x = torch_gpu.zeros_like(b)
+    solver.product(0, b, 1, x)
+    info = solver.info
+    assert solver.getStats().iterationCount == 25 and info.converged == 0
+    truth = true_relative_residual(w.matrix, x.cpu().numpy(), w.rhs)
+    assert solver.getStats().relativeResidual == pytest.approx(truth, rel=1e-9)
+    assert info.trueRelativeResidual == pytest.approx(truth, rel=1e-9)
+    # the same 25 trips uncapped reach the same x up to the rounding of the fold
+    longer = hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), restart=restart)
+    x2 = torch_gpu.zeros_like(b)
+    longer.product(0, b, 1, x2)
+    assert longer.info.converged == 1 and longer.info.trueRelativeResidual < 1e-10
+    assert longer.info.refinements >= 2
+
+
 def test_fused_pcg_errors(hip, torch_gpu):
     # r^T z <= 0 before the loop (ConjugateGradients_impl.hpp:119-123)
     neg = systems.CSR(4, 4, np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32), -np.ones(4))
