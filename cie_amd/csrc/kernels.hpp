@@ -822,8 +822,6 @@ __global__ __launch_bounds__(kReduceBlock) void innerProductFinishKernel(int cou
     if (threadIdx.x == 0) *result = total;
 }
 
-// makeDiagonalOperator: inverse = 1 / a_ii; the first failing row (zero or
-// missing diagonal) is recorded with atomicMin in failure[0] / failure[1].
 // Jacobi set-up (DiagonalOperator.cpp:37-88), a group of G lanes per row: the
 // group reads the row's entries G at a time, coalesced, and finds the
 // diagonal among them (columns are unique per row). inverse[row] = 1 / a_ii;
