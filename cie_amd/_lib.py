@@ -154,6 +154,7 @@ _SIGNATURES = {
     "cie_hip_csr_product_f32": (c_int, [_P, _P, c_float, _P, c_int64, c_float, _P, c_int64]),
     # spaces
     "cie_hip_inner_product_f64": (c_int, [_P, _P, _P, c_int64, POINTER(c_double)]),
+    "cie_hip_inner_product_f64_static": (c_int, [_P, _P, _P, c_int64, _P]),
     "cie_hip_scale_f64": (c_int, [_P, _P, _P, c_int64, c_double]),
     "cie_hip_scale_uniform_f64": (c_int, [_P, _P, c_int64, c_double]),
     "cie_hip_add_f64": (c_int, [_P, _P, _P, c_int64, c_double]),

@@ -260,4 +260,7 @@ int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t*
 // Grows the context's partials to at least `slots` slots of 4 doubles. A
 // growth waits for the stream first (queued kernels may use the old buffer).
 int ensurePartials(cie_hip_context context, int slots);
+// This device's address of the static scratch of the no-allocation dot
+// product (looked up on the first call per device; the device must be current).
+int staticScratchAddress(int device, double** address);
 }  // namespace ciehip

@@ -69,6 +69,9 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     if (device < 0 || device >= devices)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_context_create: device index out of range");
     CIE_HIP_CHECK(hipSetDevice(device));
+    // Set-up of this device's static scratch (the no-allocation dot product's
+    // partials): its address, looked up once per device with the device current.
+    if (const int status = ciehip::staticScratchAddress(device, nullptr)) return status;
     auto* ctx = new cie_hip_context_s;
     ctx->device = device;
     hipDeviceProp_t properties;

@@ -218,6 +218,20 @@ int cie_hip_scale_uniform_f32(cie_hip_context context, float* target, int64_t si
 int cie_hip_add_f32(cie_hip_context context, float* target, const float* source, int64_t size, float scale);
 int cie_hip_assign_f32(cie_hip_context context, float* target, const float* source, int64_t size);
 int cie_hip_fill_f32(cie_hip_context context, float* target, int64_t size, float value);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory or wait on the host (code under HIP graph capture, tight serving
+ * loops). The deterministic dot product of cie_hip_inner_product_f64, the
+ * same two kernels and summation order (bitwise the same value), with its
+ * block partials in a static device array of 2048 doubles (16 KiB per device,
+ * held until the process ends, its address looked up when a context is
+ * created on the device) instead of the context's scratch, and the result
+ * written to device memory: *result (device) = left . right. It is
+ * asynchronous on the context's stream and calls no allocation, memset or
+ * synchronization. Every size fits (the grid is capped at 2048 blocks); a
+ * launch that would need more partials is rejected. Calls that overlap on one
+ * device share the array and race: serialize them (one stream, or events). */
+int cie_hip_inner_product_f64_static(cie_hip_context context, const double* left, const double* right,
+                                     int64_t size, double* result);
 
 /* ---- Jacobi preconditioner (solvers/src/DiagonalOperator.cpp) ------------ */
 /* makeDiagonalOperator: inverse[i] = 1 / a_ii. Fails like the CPU version
