@@ -194,6 +194,9 @@ def main():
                              "for the other workloads the sample itself); 0 disables the CPU leg")
     parser.add_argument("--cpu-threads", type=int, default=0)
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
+    parser.add_argument("--launch", default=None,
+                        help="pcg / mixed-q4: pin the SpMV launches instead of autotuning (profiling runs): "
+                             "KERNEL,WIDTH,UNROLL for both precisions, or FP64/FP32 (e.g. sorted,3,4/sorted,3,8)")
     parser.add_argument("--batch-tiles", choices=("auto", "tiled", "blocks", "sorted"), default="auto",
                         help="batch workload: the tile format (default: the autotuner's)")
     parser.add_argument("--calibration-copy", action="store_true",
@@ -250,11 +253,21 @@ def main():
     # Set-up beside the solve: the upload, then the autotuner, which builds every derived format it times
     # (narrow columns, column blocks, sorted tiles) and frees the ones no launch keeps.
     t0 = time.perf_counter()
-    tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
+    if args.launch:  # a pinned launch (profiling runs), built but not timed against others
+        specs = args.launch.split("/")
+        for precision, spec in zip((64, 32), specs if len(specs) == 2 else specs * 2):
+            kernel, width, unroll = spec.split(",")
+            matrix.set_launch(precision, kernel, int(width), int(unroll))
+        tune64 = time_spmv(torch, linalg, matrix, 64, repetitions=3)
+    else:
+        tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
     context.synchronize()
     autotune64_seconds = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
+    if args.launch:
+        tune32 = time_spmv(torch, linalg, matrix, 32, repetitions=3)
+    else:
+        tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
     context.synchronize()
     autotune32_seconds = time.perf_counter() - t0
     b = torch.from_numpy(workload.rhs).to("cuda:0")
@@ -328,6 +341,9 @@ def main():
     product_seconds = statistics.mean(i.productSeconds for i in mixed_infos if i.productSamples) \
         if any(i.productSamples for i in mixed_infos) else spmv32
     achieved = csr32 / product_seconds / 1e9
+    traffic = measured_traffic(matrix.launch(32), f"spmv_{'q4' if q4 else 'q2'}_{args.elements}_traffic.json")
+    rocprof_mean = _profiled_mean_seconds(f"pcg_{'q4' if q4 else 'q2'}_{args.elements}_kernel_stats.md",
+                                          f"tileProductKernel<{_walker(matrix.launch(32), 'float')} >")
 
     result = {
         "metric": "time-to-1e-10 residual (s) + SpMV HBM GB/s, FP64 vs mixed-refined PCG, 1 GPU",
@@ -378,12 +394,18 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(matrix.launch(32), f"spmv_{'q4' if q4 else 'q2'}_{args.elements}_traffic.json"),
-            "kernel": f"productKernel {list(matrix.launch(32))} (q = A p, FP32 plane, fused p.q) inside the mixed solve",
+            "traffic": traffic,
+            "measured_GBps": None if traffic is None or rocprof_mean is None else traffic / rocprof_mean / 1e9,
+            "rocprof_mean_launch_seconds": rocprof_mean,
+            "measured_note": "measured_GBps: calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch over the "
+                             "rocprofv3 mean launch time, both from the committed profiles of this workload and "
+                             "launch (null when the autotuner picked a launch they do not cover)",
+            "kernel": f"tileProductKernel {list(matrix.launch(32))} (q = A p, FP32 plane, fused p.q) inside the mixed solve",
             "bytes_per_launch": csr32, "bytes_rule": "int32 CSR: (4 + 4) nnz + 4 (N + 1) + 2 * 4 N",
             "mean_launch_seconds": product_seconds,
             "format_bytes_per_launch": bytes32, "format_GBps": bytes32 / product_seconds / 1e9,
         },
+        "setup_launch": "pinned " + args.launch if args.launch else "autotuned",
     }
     if cpu is not None:
         result["cpu_baseline"] = cpu
@@ -483,6 +505,19 @@ def _profiled_traffic(profile, kernel):
         return None
     entry = json.loads(path.read_text()).get("kernels", {}).get(kernel)
     return None if entry is None else entry["traffic_bytes"]
+
+
+def _profiled_mean_seconds(stats, kernel):
+    """Mean duration in seconds of `kernel` (its short name) in the committed rocprofv3 --kernel-trace --stats
+    summary `stats` of this workload (a markdown table, mean in microseconds), else None."""
+    path = ROOT / "profiles" / stats
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) >= 5 and cells[1] == f"`{kernel}`":
+            return float(cells[3]) * 1e-6
+    return None
 
 
 _BATCH_WALKER = {"tiled": 0, "blocks": 3, "sorted": -3}  # the walker code of the batch's tile format
@@ -816,7 +851,8 @@ def measured_traffic(launch, profile):
         walker = f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<float, float, {width}, {unroll}, 256> >"
     kernels = json.loads(path.read_text()).get("kernels", {})
     # the in-solve product kernel; else the standalone SpMV of the same shape (the autotuner times every shape)
-    entry = kernels.get(f"productKernel<{walker}") or kernels.get(f"csrProductKernel<{walker}")
+    entry = (kernels.get(f"tileProductKernel<{walker}") or kernels.get(f"productKernel<{walker}")
+             or kernels.get(f"csrProductKernel<{walker}"))
     return None if entry is None else entry["traffic_bytes"]
 
 

@@ -1,6 +1,8 @@
 // Device CSR matrices, SpMV (LinearOperator::product), the Jacobi inverse
 // diagonal and the SpMV launch autotuner.
 #include "dispatch.hpp"
+
+#include <climits>
 #include "kernels.hpp"
 
 #include <limits>
@@ -58,29 +60,60 @@ __global__ __launch_bounds__(kReduceBlock) void narrowColumnsKernel(int64_t nnz,
 // ---- column-block form ------------------------------------------------------
 // A row's entries are covered by 1 x width blocks: each block starts at the
 // column of the row's first entry not yet covered (or at columns - width, when
-// that would pass the last column) and covers [start, start + width). One
-// thread per row walks its entries in the count and structure passes (the
-// same walk, so they agree block for block); the structure pass also records
-// each block's first entry, so refilling the values after the entries change
-// is one coalesced pass with a thread per block.
+// that would pass the last column) and covers [start, start + width). A group
+// of kCoverLanes lanes walks each row: it loads the row's columns
+// kCoverLanes at a time (coalesced), and within each load the next block
+// starts at the first lane whose column lies past the current block (a ballot
+// and a find-first-set), so the walk visits each entry once. The count and
+// structure passes run the same walk, so they agree block for block; the
+// structure pass also records each block's first entry, so refilling the
+// values after the entries change is one coalesced pass with a thread per block.
+
+constexpr int kCoverLanes = 16;
 
 __device__ __forceinline__ int blockStart(int column, int width, int columns) {
     return column + width > columns ? columns - width : column;
+}
+
+// Calls emit(block index within the row, start column, first entry) for every
+// block of `row`, in order, from the group's first lane; returns the count.
+template <class Emit>
+__device__ __forceinline__ int coverRow(int row, const int32_t* __restrict__ extents,
+                                        const int32_t* __restrict__ columnIndices, int width, int columns, int lane,
+                                        int groupShift, Emit&& emit) {
+    const int begin = extents[row], end = extents[row + 1];
+    int limit = INT_MIN;  // columns below this are covered (start + width of the last block)
+    int count = 0;
+    for (int base = begin; base < end; base += kCoverLanes) {
+        const int k = base + lane;
+        const int column = k < end ? columnIndices[k] : INT_MAX;
+        for (;;) {
+            const uint64_t open = __ballot(k < end && column >= limit);
+            const uint32_t mine = uint32_t(open >> groupShift) & ((1u << kCoverLanes) - 1);
+            if (!mine) break;
+            const int first = __ffs(int(mine)) - 1;
+            const int next = __shfl(column, (groupShift & (kWave - 1)) + first, kWave);
+            const int start = blockStart(next, width, columns);
+            if (lane == 0) emit(count, start, base + first);
+            ++count;
+            limit = start + width;
+        }
+    }
+    return count;
 }
 
 __global__ __launch_bounds__(kReduceBlock) void columnBlockCountKernel(int rows, const int32_t* __restrict__ extents,
                                                                        const int32_t* __restrict__ columnIndices,
                                                                        int width, int columns,
                                                                        int32_t* __restrict__ counts) {
-    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
-        const int end = extents[row + 1];
-        int count = 0;
-        for (int k = extents[row]; k < end;) {
-            const int start = blockStart(columnIndices[k], width, columns);
-            ++count;
-            while (k < end && columnIndices[k] < start + width) ++k;
-        }
-        counts[row] = count;
+    const int lane = int(threadIdx.x) % kCoverLanes;
+    const int groupShift = int(threadIdx.x) % kWave - lane;  // the group's first lane in the wavefront
+    const int groupsPerBlock = kReduceBlock / kCoverLanes;
+    const int stride = int(gridDim.x) * groupsPerBlock;
+    for (int row = int(blockIdx.x) * groupsPerBlock + int(threadIdx.x) / kCoverLanes; row < rows; row += stride) {
+        const int count = coverRow(row, extents, columnIndices, width, columns, lane, groupShift,
+                                   [](int, int, int) {});
+        if (lane == 0) counts[row] = count;
     }
 }
 
@@ -93,15 +126,16 @@ __global__ __launch_bounds__(kReduceBlock) void columnBlockStartsKernel(int rows
                                                                         const int32_t* __restrict__ blockExtents,
                                                                         int32_t* __restrict__ blockColumns,
                                                                         int32_t* __restrict__ entryBegin) {
-    for (int row = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); row < rows; row += int(gridDim.x) * kReduceBlock) {
-        const int end = extents[row + 1];
-        int b = blockExtents[row];
-        for (int k = extents[row]; k < end;) {
-            const int start = blockStart(columnIndices[k], width, columns);
-            entryBegin[b] = k;
-            blockColumns[b++] = start;
-            while (k < end && columnIndices[k] < start + width) ++k;
-        }
+    const int lane = int(threadIdx.x) % kCoverLanes;
+    const int groupShift = int(threadIdx.x) % kWave - lane;
+    const int groupsPerBlock = kReduceBlock / kCoverLanes;
+    const int stride = int(gridDim.x) * groupsPerBlock;
+    for (int row = int(blockIdx.x) * groupsPerBlock + int(threadIdx.x) / kCoverLanes; row < rows; row += stride) {
+        const int b = blockExtents[row];
+        coverRow(row, extents, columnIndices, width, columns, lane, groupShift, [&](int i, int start, int entry) {
+            entryBegin[b + i] = entry;
+            blockColumns[b + i] = start;
+        });
     }
 }
 
@@ -191,6 +225,13 @@ __global__ __launch_bounds__(kReduceBlock) void sortedValuesKernel(const int2* _
         const int64_t source = int64_t(begin) + int64_t(keys[i] & kSortedPositionMask);
         sorted[v] = values[source * width + w];
     }
+}
+
+// Grid of the block-cover kernels: a group of lanes per row, rows strided over a few rounds of blocks.
+int coverGrid(cie_hip_context ctx, int rows) {
+    constexpr int groupsPerBlock = kReduceBlock / kCoverLanes;
+    const int64_t blocks = (int64_t(rows) + groupsPerBlock - 1) / groupsPerBlock;
+    return int(std::max<int64_t>(1, std::min<int64_t>(blocks, int64_t(std::max(1, ctx->computeUnits)) * 32)));
 }
 
 template <class T>
@@ -591,7 +632,7 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
         bool ok = hipMalloc(&counts, sizeof(int32_t) * size_t(std::max(1, rows))) == hipSuccess &&
                   hipMalloc(&extents, sizeof(int32_t) * (size_t(rows) + 1)) == hipSuccess;
         if (ok && rows) {
-            hipLaunchKernelGGL(columnBlockCountKernel, dim3(vectorGrid(rows)), dim3(kReduceBlock), 0, ctx->stream, rows,
+            hipLaunchKernelGGL(columnBlockCountKernel, dim3(coverGrid(ctx, rows)), dim3(kReduceBlock), 0, ctx->stream, rows,
                                m->rowExtents, m->columnIndices, width, columns, counts);
             ok = hipGetLastError() == hipSuccess &&
                  hipMemcpyAsync(hostCounts.data(), counts, sizeof(int32_t) * size_t(rows), hipMemcpyDeviceToHost,
@@ -618,7 +659,7 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
              hipMemcpyAsync(extents, hostExtents.data(), sizeof(int32_t) * (size_t(rows) + 1), hipMemcpyHostToDevice,
                             ctx->stream) == hipSuccess;
         if (ok && rows) {
-            hipLaunchKernelGGL(columnBlockStartsKernel, dim3(vectorGrid(rows)), dim3(kReduceBlock), 0, ctx->stream,
+            hipLaunchKernelGGL(columnBlockStartsKernel, dim3(coverGrid(ctx, rows)), dim3(kReduceBlock), 0, ctx->stream,
                                rows, m->rowExtents, m->columnIndices, width, columns, extents, starts, entryBegin);
             ok = hipGetLastError() == hipSuccess;
         }

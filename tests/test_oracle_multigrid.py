@@ -185,3 +185,57 @@ def test_pmg_order_one_solves_exactly_and_higher_orders_track_only_the_masked_re
     x, history, _ = oracle.pmg(csr, systems.ansatz_mask(2, 3, 6), 3, b)
     true = np.linalg.norm(b - csr.to_scipy() @ x) / np.linalg.norm(b)
     assert history[-1] / np.linalg.norm(b) <= 1e-6 and true > 1e-3
+
+
+def test_pmg_order_one_cycles_are_restarted_cg_to_one_half():
+    """Order 1 has one grid, the coarse CG (solver.cpp:146-170) with relative tolerance 0.5 from a zero update:
+    the first cycle is the oracle's PCG to 0.5 on b, and the next starts it again on the tracked residual
+    r = b - A u (product(1, update, -1, residual), solver.cpp:219)."""
+    csr = systems.lagrange(2, 1, 10, basis="legendre")
+    mask = systems.ansatz_mask(2, 1, 10)
+    b = systems.uniform(7, csr.rows)
+    x1, history1, coarse1 = oracle.pmg(csr, mask, 1, b, max_cycles=1)
+    u1, s1, _, _ = oracle.pcg(csr, b, relative_tolerance=0.5, max_iterations=5000)
+    assert np.array_equal(x1, u1) and list(coarse1) == [s1.iterationCount]
+    r1 = oracle.csr_product(csr, 1.0, u1, -1.0, b.copy())
+    assert history1[0] == pytest.approx(np.linalg.norm(r1), rel=1e-14)
+    x2, _, coarse2 = oracle.pmg(csr, mask, 1, b, max_cycles=2)
+    u2, s2, _, _ = oracle.pcg(csr, r1, relative_tolerance=0.5, max_iterations=5000)
+    assert np.allclose(x2, u1 + u2, rtol=0, atol=1e-15 * np.abs(u1 + u2).max())
+    assert list(coarse2) == [s1.iterationCount, s2.iterationCount]
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_pmg_is_scale_invariant_and_linear_in_b(order):
+    """Power-of-two scalings are exact: the cycle on (2^k A, 2^k b) returns the same x bit for bit (Jacobi
+    divides by the scaled diagonal, CG's alpha and beta are ratios), and on (A, 2^k b) it returns 2^k x; the
+    coarse iteration counts do not change."""
+    csr = systems.lagrange(2, order, 5, basis="legendre")
+    mask = systems.ansatz_mask(2, order, 5)
+    b = systems.uniform(3, csr.rows)
+    x, history, coarse = oracle.pmg(csr, mask, order, b)
+    scaled = systems.CSR(csr.rows, csr.columns, csr.row_extents, csr.column_indices, 8.0 * csr.entries)
+    xs, hs, cs = oracle.pmg(scaled, mask, order, 8.0 * b)
+    assert np.array_equal(xs, x) and np.array_equal(hs, 8.0 * history) and np.array_equal(cs, coarse)
+    xb, hb, cb = oracle.pmg(csr, mask, order, 0.25 * b)
+    assert np.array_equal(xb, 0.25 * x) and np.array_equal(hb, 0.25 * history) and np.array_equal(cb, coarse)
+
+
+def test_pmg_jacobi_grid_on_an_uncoupled_diagonal_block():
+    """Order 2 on A = blockdiag(A_c, D_f) with the coarse rows masked 1 and the fine rows masked 2: the fine rows
+    couple to nothing, so the coarse CG never touches them and each visit of the Jacobi grid (6 sweeps of
+    relaxation 2/3 from zero, solver.cpp:188-195) multiplies their residual by (1 - 2/3)^6 exactly in exact
+    arithmetic; a cycle visits that grid twice (down and up, solver.cpp:210-226)."""
+    coarse = systems.poisson5(6)
+    nc = coarse.rows
+    d_f = np.array([1.0, 2.5, 4.0, 0.75, 3.0, 8.0])
+    a = scipy.sparse.block_diag([coarse.to_scipy(), scipy.sparse.diags(d_f)]).tocsr()
+    a.sort_indices()
+    csr = systems.CSR(a.shape[0], a.shape[1], a.indptr.astype(np.int32), a.indices.astype(np.int32), a.data)
+    mask = np.concatenate([np.ones(nc), 2 * np.ones(len(d_f))]).astype(np.uint16)
+    b = systems.uniform(11, csr.rows)
+    for cycles in (1, 2):
+        x, history, _ = oracle.pmg(csr, mask, 2, b, max_cycles=cycles)
+        # r = b - A x cancels to about eps |b| here, so the ratio is known to about 1e-4 of (1/3)^12
+        ratio = (b[nc:] - d_f * x[nc:]) / b[nc:]
+        assert np.allclose(ratio, (1.0 / 3.0) ** (12 * cycles), rtol=1e-3, atol=0)
