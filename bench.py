@@ -283,6 +283,32 @@ def main():
         solver.product(0, b, 1, x)
         return solver.info
 
+    # Launch selection by solve time: the autotuner times the standalone SpMV; the solves run the product
+    # kernel with the fused p.q, whose best tile size can differ. Each precision keeps whichever of its
+    # autotuned launch and the sorted 1 x 3 tiles of 1024, 2048 and 4096 rows solves fastest (FP64 first:
+    # the mixed solve's start and refinement steps use the FP64 launch).
+    selection = {}
+    t0 = time.perf_counter()
+    if not args.launch:
+        for precision, solver in ((64, fp64), (32, mixed)):
+            candidates = [tuple(matrix.launch(precision))]
+            candidates += [c for c in (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16)) if c not in candidates]
+            times = {}
+            for candidate in candidates:
+                matrix.set_launch(precision, *candidate)
+                solve(solver)  # builds the candidate's tiles
+                torch.cuda.synchronize()
+                start = time.perf_counter()
+                solve(solver)
+                torch.cuda.synchronize()
+                times[candidate] = time.perf_counter() - start
+            best = min(times, key=times.get)
+            matrix.set_launch(precision, *best)
+            selection[f"fp{precision}"] = {",".join(map(str, c)): t for c, t in times.items()}
+            print(f"[bench] fp{precision} solve launch {best} ({times[best]:.4f} s of "
+                  f"{', '.join(f'{t:.4f}' for t in times.values())})", file=sys.stderr, flush=True)
+    selection_seconds = time.perf_counter() - t0
+
     for _ in range(args.warmup):
         solve(mixed)
         solve(fp64)
@@ -378,10 +404,12 @@ def main():
                         "tolerance; error_vs_oracle_tight: to that solution continued to an absolute residual "
                         "of 1e-14 ||b||; null when the CPU leg is off"),
         "setup": {"upload_seconds": upload_seconds, "autotune_fp64_seconds": autotune64_seconds,
-                  "autotune_fp32_seconds": autotune32_seconds,
+                  "autotune_fp32_seconds": autotune32_seconds, "solve_selection_seconds": selection_seconds,
                   "seconds": upload_seconds + autotune64_seconds + autotune32_seconds,
                   "note": "once per matrix: host-to-device copy of the CSR, then the autotuner (builds and times "
-                          "every candidate format and launch); not part of value"},
+                          "every candidate format and launch); not part of value. solve_selection_seconds: the "
+                          "bench's own choice among a few launches by timing whole solves (two per candidate)"},
+        "solve_launch_selection": selection or "pinned",
         "speedup_mixed_vs_fp64": fp64_wall / mixed_wall,
         "spmv": {
             "fp64_seconds": spmv64, "fp64_GBps": csr64 / spmv64 / 1e9, "fp64_bytes": csr64,
@@ -405,7 +433,7 @@ def main():
             "mean_launch_seconds": product_seconds,
             "format_bytes_per_launch": bytes32, "format_GBps": bytes32 / product_seconds / 1e9,
         },
-        "setup_launch": "pinned " + args.launch if args.launch else "autotuned",
+        "setup_launch": "pinned " + args.launch if args.launch else "autotuned, then selected by solve time",
     }
     if cpu is not None:
         result["cpu_baseline"] = cpu
@@ -508,15 +536,15 @@ def _profiled_traffic(profile, kernel):
 
 
 def _profiled_mean_seconds(stats, kernel):
-    """Mean duration in seconds of `kernel` (its short name) in the committed rocprofv3 --kernel-trace --stats
-    summary `stats` of this workload (a markdown table, mean in microseconds), else None."""
+    """Mean duration in seconds of the working launches of `kernel` (its short name) in the committed rocprofv3
+    --kernel-trace --stats summary `stats` of this workload (a markdown table, means in microseconds), else None."""
     path = ROOT / "profiles" / stats
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) >= 5 and cells[1] == f"`{kernel}`":
-            return float(cells[3]) * 1e-6
+        if len(cells) >= 8 and cells[1] == f"`{kernel}`" and cells[5]:
+            return float(cells[5]) * 1e-6  # the mean of the long launches (trips past convergence left out)
     return None
 
 

@@ -611,6 +611,7 @@ int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
     source.sortedKeys = nullptr;
     source.sortedValues64 = nullptr;
     source.sortedValues32 = nullptr;
+    source.sortedBlocks = 0;
     if (batch->blockWidth && batch->sortedTiles) {
         status = ensureSortedCopy(m, batch->tiles, batch->tileCount, kTileEntries, batch->sorted);
         if (status != CIE_HIP_OK) return status;
@@ -618,6 +619,7 @@ int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
         source.sortedKeys = batch->sorted.keys;
         source.sortedValues64 = batch->sorted.values64;
         source.sortedValues32 = batch->sorted.values32;
+        source.sortedBlocks = m->nnz;
     }
     source.blockValues64 = batch->blockWidth ? m->entries64 : nullptr;
     source.blockValues32 = batch->blockWidth ? m->entries32 : nullptr;

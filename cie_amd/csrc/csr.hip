@@ -209,21 +209,28 @@ __global__ __launch_bounds__(kSortThreads) void sortedKeysKernel(const int2* __r
 }
 
 // The width values of each sorted block, copied from its block: one block of
-// threads per tile, one thread per value in sorted order (coalesced writes,
-// reads within the tile's own span); tiles walked unsorted are skipped.
+// threads per tile, one thread per value in sorted order (coalesced writes
+// within each plane, reads within the tile's own span); tiles walked unsorted
+// are skipped. `blocks` sorted blocks in all: runs longer than 16 bytes go to
+// two planes (sortedHead).
 template <class T>
 __global__ __launch_bounds__(kReduceBlock) void sortedValuesKernel(const int2* __restrict__ tiles, int capacity,
-                                                                   int width, const uint32_t* __restrict__ keys,
+                                                                   int width, int64_t blocks,
+                                                                   const uint32_t* __restrict__ keys,
                                                                    const T* __restrict__ values,
                                                                    T* __restrict__ sorted) {
     const int t = blockIdx.x;
     const int begin = tiles[t].y, end = tiles[t + 1].y;
     if (end - begin > capacity) return;
+    const int head = sortedHead(width, int(sizeof(T)));
+    T* __restrict__ tail = sorted + blocks * head;
     const int64_t first = int64_t(begin) * width, last = int64_t(end) * width;
     for (int64_t v = first + threadIdx.x; v < last; v += kReduceBlock) {
         const int64_t i = v / width, w = v - i * width;
         const int64_t source = int64_t(begin) + int64_t(keys[i] & kSortedPositionMask);
-        sorted[v] = values[source * width + w];
+        const T value = values[source * width + w];
+        if (w < head) sorted[i * head + w] = value;
+        else tail[i * (width - head) + (w - head)] = value;
     }
 }
 
@@ -456,12 +463,12 @@ int ensureSortedCopy(cie_hip_csr blocks, const int2* tiles, int tileCount, int c
     if (copy.version != blocks->filledVersion) {
         if (tileCount && blocks->nnz) {
             hipLaunchKernelGGL((sortedValuesKernel<double>), dim3(tileCount), dim3(kReduceBlock), 0, ctx->stream, tiles,
-                               capacity, width, copy.keys, blocks->entries64, copy.values64);
+                               capacity, width, int64_t(blocks->nnz), copy.keys, blocks->entries64, copy.values64);
             CIE_HIP_CHECK(hipGetLastError());
             if (copy.values32) {
                 hipLaunchKernelGGL((sortedValuesKernel<float>), dim3(tileCount), dim3(kReduceBlock), 0, ctx->stream,
-                                   tiles, capacity, width, copy.keys, static_cast<const float*>(blocks->entries32),
-                                   copy.values32);
+                                   tiles, capacity, width, int64_t(blocks->nnz), copy.keys,
+                                   static_cast<const float*>(blocks->entries32), copy.values32);
                 CIE_HIP_CHECK(hipGetLastError());
             }
         }

@@ -171,6 +171,7 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     source.sortedKeys = nullptr;
     source.sortedValues64 = nullptr;
     source.sortedValues32 = nullptr;
+    source.sortedBlocks = 0;
     const int64_t rows = m->rows;
     if (launch.kernel == CIE_HIP_SPMV_SORTED) {  // the column-block form with column-sorted tiles
         cie_hip_csr blocks = nullptr;
@@ -189,6 +190,7 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
         source.sortedKeys = set->sorted.keys;
         source.sortedValues64 = set->sorted.values64;
         source.sortedValues32 = set->sorted.values32;
+        source.sortedBlocks = blocks->nnz;
         source.param = 1;
         grid = std::max(1, residentRound ? std::min(set->count, cap) : set->count);
     } else if (launch.kernel == CIE_HIP_SPMV_BLOCKS) {  // the tiled walker over the column-block form

@@ -9,8 +9,8 @@
 //         abs = sqrt(rr), rel = abs / ||r0||; stop on Comparison::less.
 // One loop trip is three streaming kernels (four in the mixed form), and the
 // scalars never leave the device:
-//   product:   q = A p, one p.q partial per block (then, for tens of
-//              thousands of tiles, a first-level sum of those partials);
+//   product:   q = A p, one p.q partial per block (then, for more than 4096
+//              tiles, a first-level sum of those partials);
 //   update:    every block sums the p.q partials itself, in one fixed order,
 //              so all blocks hold the same alpha; x += alpha p, r -= alpha q,
 //              one partial of r.r and r.z per block;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
     } else {
         fold = refineDue(s);
     }
-    const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
+    const float pq = float(sumPartialsAll<kVectorThreads, double>(productPartials, productCount, 0, lds64));
     if (!(0.0f < pq)) {
         if (firstThread()) s.done = kNotSpd;
         return;
@@ -529,15 +529,19 @@ __global__ __launch_bounds__(kFinishBlock) void startFinishKernel(int count, con
 
 size_t alignUp(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
-// Partials of a product or start launch as its consumer reads them: at most
-// kLevelBlocks slots (a first-level sum is launched when there are more).
+// Partials of a product or start launch as their consumers (blocks of 1024
+// threads) read them: up to kDirectPartials slots directly, four loads per
+// thread at most; a first-level sum into kLevelBlocks slots above that.
 struct PartialSource {
     const double* partials;
     int count;
 };
 
+constexpr int kDirectPartials = 4 * kVectorThreads;
+static_assert(kFinishBlock == kVectorThreads, "the start sums read the partials as the update kernels do");
+
 PartialSource levelPartials(cie_hip_context ctx, int count) {
-    if (count <= kLevelBlocks) return {ctx->scratch.partials, count};
+    if (count <= kDirectPartials) return {ctx->scratch.partials, count};
     const int per = (count + kLevelBlocks - 1) / kLevelBlocks;
     const int blocks = (count + per - 1) / per;
     hipLaunchKernelGGL(partialLevelKernel, dim3(blocks), dim3(kReduceBlock), 0, ctx->stream, count, per,
@@ -578,7 +582,7 @@ int launchLevelRefine(cie_hip_csr m, int previous, int products, const double* x
     cie_hip_context ctx = m->context;
     const CsrLaunch& launch = m->launch64;
     int levelBlocks = 0, per = 0;
-    if (products > kLevelBlocks) {
+    if (products > kDirectPartials) {
         per = (products + kLevelBlocks - 1) / kLevelBlocks;
         levelBlocks = (products + per - 1) / per;
         *sums = {ctx->scratch.level, levelBlocks};

@@ -29,12 +29,40 @@ def _short(name):
     return name.split("(", 1)[0][:100]
 
 
+LONG_FRACTION = 0.25  # a launch "long" if it ran at least this fraction of the kernel's longest launch
+
+
+def _long_launches(directory):
+    """Per kernel name, from the --kernel-trace CSV beside the stats: (count, mean ns) of the launches that ran at
+    least LONG_FRACTION of that kernel's longest one, and the VGPR count. The solver loops enqueue a few trips past
+    convergence whose kernels return at once; this mean leaves those out."""
+    files = glob.glob(str(Path(directory) / "*kernel_trace.csv"))
+    if not files:
+        return {}
+    durations, vgprs = defaultdict(list), {}
+    for row in csv.DictReader(open(files[0])):
+        durations[row["Kernel_Name"]].append(int(row["End_Timestamp"]) - int(row["Start_Timestamp"]))
+        vgprs[row["Kernel_Name"]] = row.get("VGPR_Count", "")
+    result = {}
+    for name, values in durations.items():
+        longest = max(values)
+        kept = [v for v in values if v >= LONG_FRACTION * longest]
+        result[name] = (len(kept), sum(kept) / len(kept), vgprs[name])
+    return result
+
+
 def kernel_stats(path, out, command):
     rows = list(csv.DictReader(open(path)))
-    lines = [f"# Kernel time: `{command}`", "", "| kernel | calls | mean (us) | total share |", "|---|---:|---:|---:|"]
-    for r in rows[:20]:
-        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
-                     f"{float(r['Percentage']):.1f}% |")
+    long = _long_launches(Path(path).parent)
+    lines = [f"# Kernel time: `{command}`", "",
+             f"long calls / long mean: the launches that ran at least {LONG_FRACTION:g} of the kernel's longest "
+             "(the solver loops enqueue a few trips past convergence whose kernels return at once).", "",
+             "| kernel | calls | mean (us) | long calls | long mean (us) | VGPRs | total share |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows[:24]:
+        count, mean, vgpr = long.get(r["Name"], ("", None, ""))
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {count} | "
+                     f"{'' if mean is None else f'{mean / 1e3:.1f}'} | {vgpr} | {float(r['Percentage']):.1f}% |")
     Path(out).write_text("\n".join(lines) + "\n")
 
 
