@@ -682,10 +682,12 @@ def bench_adjoint(args):
           "s", "f64",
           {"workload": f"3D Q2 Laplace {n}^3 hexahedra, k_e = exp(g_e), g ~ N(0, 0.5) seed 4, target from seed 5, "
                        f"b ~ U(-1,1) seed 1", "rows": rows, "nnz": nnz, "elements": int(problem.count)},
-          _roofline(f"productKernel {list(problem.matrix.launch(64))} (q = A p) inside the forward solve", algorithmic,
-                    info.productSeconds, "B_SpMV(FP64, int32 CSR)",
-                    _profiled_traffic("adjoint_q2_48_traffic.json",
-                                      f"productKernel<{_walker(problem.matrix.launch(64))} >")),
+          _roofline(f"tileProductKernel {list(problem.matrix.launch(64))} (q = A p) inside the forward solve",
+                    algorithmic, info.productSeconds, "B_SpMV(FP64, int32 CSR)",
+                    next((t for t in (_profiled_traffic("adjoint_q2_48_traffic.json",
+                                                        f"{form}<{_walker(problem.matrix.launch(64))} >")
+                                      for form in ("tileProductKernel", "productKernel", "csrProductKernel"))
+                          if t is not None), None)),
           {"value": secs * total_iterations / sample + gather, "unit": "s", "cores": threads, "kind": "port",
            "sample": f"oracle FP64 Jacobi-PCG, {sample} iterations in {secs:.2f} s, scaled to the "
                      f"{total_iterations} forward + adjoint iterations, plus the oracle's element gradient "
