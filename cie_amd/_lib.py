@@ -67,6 +67,7 @@ class MixedSettings(ctypes.Structure):
 
 
 MIXED_PLAIN_FP32 = 1  # CIE_HIP_MIXED_PLAIN_FP32: no row-sum correction in the inner FP32 product
+DEFAULT_REFINEMENT_FACTOR = 0.03  # CIE_HIP_DEFAULT_REFINEMENT_FACTOR
 
 
 class PmgInfo(ctypes.Structure):

@@ -1161,7 +1161,7 @@ int cie_hip_pcg_batched(cie_hip_context context, cie_hip_batch batch, const doub
     s.absoluteTolerance = settings->settings.absoluteResidual;
     s.relativeTolerance = settings->settings.relativeResidual;
     double factor = settings->refinementFactor;
-    if (!(factor > 0.0 && factor < 1.0)) factor = 0.1;
+    if (!(factor > 0.0 && factor < 1.0)) factor = CIE_HIP_DEFAULT_REFINEMENT_FACTOR;
     s.updateThreshold = factor * factor;
     s.maxIterations = settings->settings.iterationCount;
     s.restart = settings->restart ? 1 : 0;

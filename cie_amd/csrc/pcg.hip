@@ -923,7 +923,7 @@ int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScal
                       double* x, int64_t size, const cie_hip_mixed_settings* settings, cie_hip_solve_info* info) {
     if (!settings) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pcg_mixed: null settings");
     double factor = settings->refinementFactor;
-    if (!(factor > 0.0 && factor < 1.0)) factor = 0.1;
+    if (!(factor > 0.0 && factor < 1.0)) factor = CIE_HIP_DEFAULT_REFINEMENT_FACTOR;
     Request q{inScale, outScale, b, x, size, settings->settings, true, factor, settings->restart ? 1 : 0, true,
               (settings->flags & CIE_HIP_MIXED_PLAIN_FP32) == 0};
     return solve(context, matrix, q, info);

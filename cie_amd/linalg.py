@@ -495,7 +495,8 @@ class FusedPCG(IterativeSolver):
 class MixedPCG(IterativeSolver):
     """FP32 inner PCG with FP64 solution accumulation and FP64 residual refinement (cie_hip_pcg_mixed)."""
 
-    def __init__(self, matrix, settings=None, refinement_factor=0.1, restart=False, row_sum_correction=True):
+    def __init__(self, matrix, settings=None, refinement_factor=_lib.DEFAULT_REFINEMENT_FACTOR, restart=False,
+                 row_sum_correction=True):
         super().__init__(settings)
         self.matrix = matrix
         self.refinement_factor = refinement_factor
@@ -844,8 +845,8 @@ class Batch:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         check(library().cie_hip_batch_set_tile_rows(self.handle, rows.ctypes.data_as(c_void_p)))
 
-    def solve(self, b, x, settings, mixed=True, graph=True, refinement_factor=0.1, restart=False,
-              row_sum_correction=True):
+    def solve(self, b, x, settings, mixed=True, graph=True, refinement_factor=_lib.DEFAULT_REFINEMENT_FACTOR,
+              restart=False, row_sum_correction=True):
         """Solve every system in place on x; returns (stats, status, refinements, summary)."""
         stats = (Statistics * self.systems)()
         status = np.zeros(self.systems, dtype=np.int32)

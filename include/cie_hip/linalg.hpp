@@ -551,7 +551,8 @@ public:
     using typename Base::Statistics;
 
     FusedConjugateGradients(std::shared_ptr<const HIPCSRMatrix> matrix, std::shared_ptr<Space> space,
-                            Statistics settings = {}, bool mixedPrecision = false, double refinementFactor = 0.1)
+                            Statistics settings = {}, bool mixedPrecision = false,
+                            double refinementFactor = CIE_HIP_DEFAULT_REFINEMENT_FACTOR)
         : Base(settings), _matrix(std::move(matrix)), _space(std::move(space)), _mixed(mixedPrecision),
           _refinementFactor(refinementFactor) {}
 

@@ -33,3 +33,11 @@ def test_status_strings_and_no_device_path():
     count = _lib.ctypes.c_int(-1)
     assert lib.cie_hip_device_count(_lib.byref(count)) == 0
     assert count.value >= 0
+
+
+def test_binding_constants_match_the_header():
+    """The constants the Python mirror restates equal the header's macros."""
+    header = (Path(__file__).resolve().parent.parent / "include" / "cie_hip.h").read_text()
+    macros = dict(re.findall(r"#define\s+(CIE_HIP_\w+)\s+([-\w.]+)", header))
+    assert float(macros["CIE_HIP_DEFAULT_REFINEMENT_FACTOR"]) == _lib.DEFAULT_REFINEMENT_FACTOR
+    assert int(macros["CIE_HIP_MIXED_PLAIN_FP32"]) == _lib.MIXED_PLAIN_FP32

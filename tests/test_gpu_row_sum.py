@@ -48,16 +48,17 @@ def test_row_sum_correction_plane(hip, torch_gpu, dim, order, n):
 
 
 def test_mixed_iterations_with_and_without_correction(hip, torch_gpu):
-    """3D Q2 32^3 (250,047 rows): with the correction the mixed solve needs about the FP64 solve's iteration
-    count (a CPU model of the same arithmetic gives 200 against 198 loop trips; measured 196 FP64, 201 mixed),
-    without it about 11% more (220)."""
+    """3D Q2 32^3 (250,047 rows), refinement factor 0.1: with the correction the mixed solve needs about the
+    FP64 solve's iteration count (a CPU model of the same arithmetic gives 200 against 198 loop trips; measured
+    196 FP64, 201 mixed), without it about 11% more (220)."""
     w = systems.q2_laplace_3d(32)
     matrix = hip.CSRMatrix(w.matrix)
     b = torch_gpu.from_numpy(w.rhs).to("cuda:0")
     runs = {}
     for label, solver in (("fp64", hip.FusedPCG(matrix, Statistics(20000, 0.0, 1e-10))),
-                          ("corrected", hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10))),
-                          ("plain", hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), row_sum_correction=False))):
+                          ("corrected", hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), refinement_factor=0.1)),
+                          ("plain", hip.MixedPCG(matrix, Statistics(20000, 0.0, 1e-10), refinement_factor=0.1,
+                                                 row_sum_correction=False))):
         x = torch_gpu.zeros_like(b)
         solver.product(0, b, 1, x)
         assert solver.info.converged == 1 and solver.info.trueRelativeResidual < (2e-10 if label == "fp64" else 1e-10)
