@@ -146,7 +146,11 @@ __global__ __launch_bounds__(kTileBlock) void batchResidualKernel(
 }
 
 // q = A p; tile partial p.q. With `correction` (the mixed form), q_i gets
-// c_i p_i added: the FP32 plane's row-sum correction.
+// c_i p_i added: the FP32 plane's row-sum correction. A tile holds at most
+// kTileBlock rows, and thread `tid` owns row first + tid: it loads p (and the
+// correction) of that row when the tile starts, so the loads are in flight
+// while the tile streams; the walker leaves each row sum in LDS, and the
+// owning thread writes q (coalesced) and adds its p.q term.
 template <class T, int W>
 __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout layout, ActiveWork work,
                                                                  const T* __restrict__ entries, const T* __restrict__ p,
@@ -154,21 +158,35 @@ __global__ __launch_bounds__(kTileBlock) void batchProductKernel(BatchLayout lay
                                                                  const T* __restrict__ correction) {
     __shared__ T stage[kTileEntries];
     __shared__ int32_t extents[kTileBlock + 1];
+    __shared__ T rowSums[kTileBlock];
     __shared__ T lds[kTileBlock / kWave];
     const int items = work.count[1];
+    const int tid = threadIdx.x;
     for (int item = blockIdx.x; item < items; item += gridDim.x) {
         const int t = work.tileList[item].x;
-        T pq = T(0);
-        auto epilogue = [&](int row, T sum) {
-            const T pi = p[row];
-            const T value = correction ? sum + correction[row] * pi : sum;
-            q[row] = value;
-            pq += pi * value;
-        };
-        BatchTiles<T, W>::tile(t, layout.source, entries, p, epilogue, stage, extents);
-        pq = blockSum<kTileBlock>(pq, lds);
-        if (threadIdx.x == 0) partials[2 * t] = double(pq);
+        const int first = layout.source.tiles[t].x;
+        const int rows = layout.source.tiles[t + 1].x - first;
+        T pi = T(0), ci = T(0);
+        if (tid < rows) {
+            pi = p[first + tid];
+            if constexpr (sizeof(T) == sizeof(float))  // only the FP32 plane has a correction
+                if (correction) ci = correction[first + tid];
+        }
+        auto stash = [&](int row, T sum) { rowSums[row - first] = sum; };
+        BatchTiles<T, W>::tile(t, layout.source, entries, p, stash, stage, extents);
         __syncthreads();
+        T pq = T(0);
+        if (tid < rows) {
+            const T sum = rowSums[tid];
+            T value = sum;
+            if constexpr (sizeof(T) == sizeof(float))
+                if (correction) value = sum + ci * pi;
+            q[first + tid] = value;
+            pq = pi * value;
+        }
+        pq = blockSum<kTileBlock>(pq, lds);
+        if (tid == 0) partials[2 * t] = double(pq);
+        __syncthreads();  // stage, extents and rowSums are reused by the next item
     }
 }
 
