@@ -628,6 +628,8 @@ def bench_batch(args):
                      f"in {sampled_seconds:.1f} s, scaled by nonzeros x FP64 iterations to all 256"},
           {"device_seconds": summary.seconds, "max_iterations": int(iterations.max()),
            "tile_format": batch.tile_kernel(),
+           "tile_rows": {str(r): int(c) for r, c in zip((32, 64, 128, 256),
+                                                        np.bincount(batch.tile_rows(), minlength=257)[[32, 64, 128, 256]])},
            "refinements": int(refinements.sum()), "fp64": {"device_seconds": fp64_summary.seconds,
                                                            "max_iterations": int(fp64_iterations.max())}})
 

@@ -1071,6 +1071,25 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
             codes.push_back(-kBatchBlockWidth);
         }
     }
+    // The time of `tiles` tiles from `t0` on (one launch, the best of `repetitions`), or a negative status.
+    const auto timeTiles = [&](int code, const BatchLayout& layout, int t0, int tiles, float* best) -> int {
+        *best = std::numeric_limits<float>::infinity();
+        for (int pass = 0; pass < 2; ++pass) {
+            (void)hipEventRecord(start, ctx->stream);
+            for (int r = 0; r < (pass ? repetitions : 1); ++r)
+                launchTileRange(code, precision, tiles, ctx->stream, layout.source, t0, m, buffer);
+            if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess)
+                return setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, start, stop);
+            if (pass) *best = ms / float(repetitions);
+        }
+        return CIE_HIP_OK;
+    };
+    // Candidates per format: one cap for every system (32, 64, 128 or 256 rows), and each system's own fastest
+    // cap timed alone. A small system alone runs fastest on small tiles (more blocks for the GPU), but the batch
+    // runs every system at once, where fewer, fuller tiles win; so every candidate is judged by the time of the
+    // whole batch's product in one launch.
     double bestTotal = std::numeric_limits<double>::infinity();
     int bestCode = 0;
     std::vector<int32_t> bestChoice(size_t(batch->systems), kDefaultTileRows);
@@ -1078,56 +1097,53 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     for (int code : codes) {
         status = setBlockWidth(batch, std::abs(code), code < 0);
         if (status) break;
-        std::vector<double> best(size_t(batch->systems), std::numeric_limits<double>::infinity());
+        std::vector<float> best(size_t(batch->systems), std::numeric_limits<float>::infinity());
         std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
+        std::vector<std::vector<int32_t>> candidates;
         bool skipped = false;
-        for (int rows : {32, 64, 128, 256}) {
-            batch->tileRows.assign(size_t(batch->systems), rows);
-            status = buildWork(batch);
-            if (status) break;
+        const auto judge = [&](const std::vector<int32_t>& rows, bool perSystem) -> int {
+            batch->tileRows = rows;
+            int result = buildWork(batch);
+            if (result) return result;
             BatchLayout layout{};
-            status = batchLayout(batch, &layout);
-            if (status && code < 0 && batch->sorted.state < 0) {  // tiles too wide for the sorted keys
-                status = CIE_HIP_OK;
+            result = batchLayout(batch, &layout);
+            if (result && code < 0 && batch->sorted.state < 0) {  // tiles too wide for the sorted keys
                 skipped = true;
-                break;
+                return CIE_HIP_OK;
             }
-            if (status) break;
+            if (result) return result;
+            float whole = 0.0f;
+            result = timeTiles(code, layout, 0, batch->tileCount, &whole);
+            if (result) return result;
+            if (whole < bestTotal) {
+                bestTotal = whole;
+                bestCode = code;
+                bestChoice = rows;
+            }
+            if (!perSystem) return CIE_HIP_OK;
             std::vector<int32_t> systemTiles(size_t(batch->systems) + 1);
             if (hipMemcpy(systemTiles.data(), batch->systemTiles, sizeof(int32_t) * systemTiles.size(),
-                          hipMemcpyDeviceToHost) != hipSuccess) {
-                status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: reading the tile layout failed");
-                break;
-            }
-            for (int s = 0; s < batch->systems && status == CIE_HIP_OK; ++s) {
-                const int t0 = systemTiles[s], tiles = systemTiles[s + 1] - t0;
-                for (int pass = 0; pass < 2; ++pass) {
-                    (void)hipEventRecord(start, ctx->stream);
-                    for (int r = 0; r < (pass ? repetitions : 1); ++r)
-                        launchTileRange(code, precision, tiles, ctx->stream, layout.source, t0, m, buffer);
-                    if (hipEventRecord(stop, ctx->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess) {
-                        status = setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: timing failed");
-                        break;
-                    }
-                    float ms = 0.0f;
-                    (void)hipEventElapsedTime(&ms, start, stop);
-                    if (pass && ms < best[s]) {
-                        best[s] = ms;
-                        choice[s] = rows;
-                    }
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return setError(CIE_HIP_ERROR_HIP, "cie_hip_batch_autotune: reading the tile layout failed");
+            for (int sys = 0; sys < batch->systems; ++sys) {
+                float ms = 0.0f;
+                result = timeTiles(code, layout, systemTiles[sys], systemTiles[sys + 1] - systemTiles[sys], &ms);
+                if (result) return result;
+                if (ms < best[sys]) {
+                    best[sys] = ms;
+                    choice[sys] = rows[0];
                 }
             }
-            if (status) break;
+            return CIE_HIP_OK;
+        };
+        for (int rows : {32, 64, 128, 256}) {
+            status = judge(std::vector<int32_t>(size_t(batch->systems), rows), true);
+            if (status || skipped) break;
         }
         if (status) break;
         if (skipped) continue;
-        double total = 0.0;
-        for (double t : best) total += t;
-        if (total < bestTotal) {
-            bestTotal = total;
-            bestCode = code;
-            bestChoice = choice;
-        }
+        status = judge(choice, false);
+        if (status) break;
     }
     (void)hipFree(buffer);
     if (status) return status;

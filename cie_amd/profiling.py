@@ -2,6 +2,7 @@
 
     python -m cie_amd.profiling stats <kernel_stats.csv> <out.md> "<command>"
     python -m cie_amd.profiling traffic <fetch dir> <write dir> <out.json> <out.md> "<command>"
+    python -m cie_amd.profiling merge <out.json> <out.md> "<command>" <traffic.json> ...
 
 ``stats`` turns a --kernel-trace --stats CSV into a short table.
 ``traffic`` reads two --pmc runs (FETCH_SIZE, then WRITE_SIZE; both in KiB per
@@ -119,11 +120,35 @@ def traffic(fetch_dir, write_dir, out_json, out_md, command):
     Path(out_md).write_text("\n".join(lines) + "\n")
 
 
+def merge(out_json, out_md, command, *inputs):
+    """One traffic summary from several (profiled runs that each pinned another launch of the same workload):
+    every kernel's entry from the first input that has it."""
+    kernels, calibration = {}, None
+    for path in inputs:
+        data = json.loads(Path(path).read_text())
+        calibration = calibration or data["calibration"]
+        for name, entry in data["kernels"].items():
+            kernels.setdefault(name, entry)
+    result = {"command": command, "calibration": calibration, "kernels": kernels}
+    Path(out_json).write_text(json.dumps(result, indent=1) + "\n")
+    lines = [f"# HBM traffic per dispatch: `{command}`", "",
+             f"Calibration: a {CALIBRATION_BYTES} B device copy read FETCH_SIZE x{calibration['fetch_scale']:.3f}, "
+             f"WRITE_SIZE x{calibration['write_scale']:.3f} of its true bytes in the first run (each run is scaled "
+             "by its own copy); the figures below are scaled by those factors.", "",
+             "| kernel | fetch (MB) | write (MB) | traffic (MB) |", "|---|---:|---:|---:|"]
+    for name, k in sorted(kernels.items(), key=lambda kv: (kv[0].startswith("csrProductKernel"), kv[0])):
+        lines.append(f"| `{name}` | {k['fetch_bytes'] / 1e6:.1f} | {k['write_bytes'] / 1e6:.1f} | "
+                     f"{k['traffic_bytes'] / 1e6:.1f} |")
+    Path(out_md).write_text("\n".join(lines) + "\n")
+
+
 def main(argv):
     if len(argv) >= 4 and argv[0] == "stats":
         kernel_stats(argv[1], argv[2], argv[3])
     elif len(argv) >= 6 and argv[0] == "traffic":
         traffic(argv[1], argv[2], argv[3], argv[4], argv[5])
+    elif len(argv) >= 5 and argv[0] == "merge":
+        merge(argv[1], argv[2], argv[3], *argv[4:])
     else:
         print(__doc__)
         return 2
