@@ -36,7 +36,9 @@ def _dptr(tensor):
 class LagrangeProblem:
     """A structured Lagrange Laplace problem K(exp(g)) u = f, device resident."""
 
-    def __init__(self, dim, order, elements, rhs, target, tolerance=1e-11, context=None, threads=0):
+    def __init__(self, dim, order, elements, rhs, target, tolerance=1e-11, context=None, threads=0, mixed=False):
+        """mixed: the forward and adjoint solves by the mixed-refined PCG (FP32 inner iterations, stopped on the
+        FP64 true residual at the same tolerance) instead of the FP64 PCG."""
         torch = _torch()
         self.dim, self.order, self.elements = dim, order, elements
         self.context = context or linalg.Context.default()
@@ -51,6 +53,7 @@ class LagrangeProblem:
         self.rhs = torch.as_tensor(np.asarray(rhs, dtype=np.float64), device=self.device)
         self.target = torch.as_tensor(target, dtype=torch.float64, device=self.device)
         self.tolerance = tolerance
+        self.mixed = mixed
         self.matrix = None
         self.host_matrix = None
         self.assembly = None
@@ -71,10 +74,12 @@ class LagrangeProblem:
         self.assembly.assemble(self.element_matrix, torch.from_numpy(k).to(self.device))
         return self.matrix
 
-    def solve(self, b, x0=None, tolerance=None, absolute=0.0):
+    def solve(self, b, x0=None, tolerance=None, absolute=0.0, mixed=None):
+        """K x = b from x0 (zero), by the FP64 PCG or, with `mixed` (default: the problem's), the mixed one."""
         torch = _torch()
         x = torch.zeros_like(b) if x0 is None else x0.clone()
-        solver = linalg.FusedPCG(self.matrix, Statistics(100000, absolute, tolerance or self.tolerance))
+        solver_class = linalg.MixedPCG if (self.mixed if mixed is None else mixed) else linalg.FusedPCG
+        solver = solver_class(self.matrix, Statistics(100000, absolute, tolerance or self.tolerance))
         solver.product(0, b, 1, x)
         return x, solver.info
 
@@ -170,7 +175,7 @@ def finite_difference(problem, g, u, elements, h=1e-6, tolerance=1e-13):
             rhs = r0.clone()
             rows = torch.from_numpy(d[keep].astype(np.int64)).to(device)
             rhs[rows] -= torch.from_numpy(c * local[keep]).to(device)
-            delta, _ = problem.solve(rhs, tolerance=tolerance)
+            delta, _ = problem.solve(rhs, tolerance=tolerance, mixed=False)
             deltas[sign] = delta
             problem.assemble(g)  # restore K(g) exactly (the assembly is deterministic)
         plus, minus = deltas[+1], deltas[-1]

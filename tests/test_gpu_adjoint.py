@@ -61,6 +61,27 @@ def test_autograd_matches_cpu_reference(torch_gpu, hip, blocks):
     assert np.linalg.norm(grad - grad_ref) <= 1e-8 * np.linalg.norm(grad_ref)
 
 
+def test_mixed_solves_give_the_fp64_gradient(torch_gpu, hip):
+    """Forward and adjoint solves by the mixed-refined PCG stop on the FP64 true residual, so the gradient agrees
+    with the FP64-solve gradient to the error both leave at that tolerance, and with the CPU reference."""
+    dim, order, n = 3, 2, 10
+    _, _, count, _ = systems.lagrange_size(dim, order, n)
+    g = systems.normal(4, np.sqrt(0.5), count)
+    gt = systems.normal(5, np.sqrt(0.5), count)
+    rhs = systems.uniform(1, systems.lagrange_size(dim, order, n)[0])
+    grad_ref, j_ref, ut = cpu_reference_gradient(dim, order, n, g, gt, rhs)
+    grads = {}
+    for mixed in (False, True):
+        problem = adjoint.LagrangeProblem(dim, order, n, rhs, ut, tolerance=1e-13, mixed=mixed)
+        g_t = torch_gpu.tensor(g, dtype=torch_gpu.float64, requires_grad=True)
+        j = adjoint.Objective.apply(g_t, problem)
+        j.backward()
+        assert abs(j.item() - j_ref) <= 1e-9 * j_ref
+        grads[mixed] = g_t.grad.cpu().numpy()
+    assert np.linalg.norm(grads[True] - grads[False]) <= 1e-8 * np.linalg.norm(grads[False])
+    assert np.linalg.norm(grads[True] - grad_ref) <= 1e-8 * np.linalg.norm(grad_ref)
+
+
 def test_full_size_gradient_against_finite_differences(torch_gpu, hip):
     """3D Q2, 48^3 hexahedra (857,375 DOFs, 110,592 elements): forward and adjoint FP64 PCG to 1e-11,
     gradient over all elements, central FD with h = 1e-6 on 64 random elements."""
