@@ -648,6 +648,7 @@ def bench_adjoint(args):
     problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
     problem.assemble(gt)
     problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted"))
+    problem.matrix.autotune(32, 5, kernels=("tiled", "blocks", "sorted"))  # the mixed-solve variant's FP32 plane
     ut, _ = problem.solve(problem.rhs)
     problem.target = ut
     context = problem.context
