@@ -217,6 +217,7 @@ _SIGNATURES = {
     "cie_hip_pmg_create": (c_int, [_P, _P, c_int, POINTER(_P)]),
     "cie_hip_pmg_destroy": (c_int, [_P]),
     "cie_hip_pmg_solve": (c_int, [_P, _P, _P, c_int64, c_double, c_uint64, POINTER(PmgInfo)]),
+    "cie_hip_pmg_grid": (c_int, [_P, c_int, POINTER(_P)]),
     # MatrixMarket files
     "cie_hip_mm_write_csr_f64": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),
     "cie_hip_mm_write_csr_f32": (c_int, [c_char_p, c_int64, c_int64, c_int64, _P, _P, _P, c_int]),

@@ -508,6 +508,13 @@ int cie_hip_pmg_destroy(cie_hip_pmg plan) {
     return CIE_HIP_OK;
 }
 
+int cie_hip_pmg_grid(cie_hip_pmg plan, int grid, cie_hip_csr* matrix) {
+    if (!plan || !matrix || grid < -1 || grid >= int(plan->grids.size()))
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pmg_grid: invalid arguments");
+    *matrix = grid < 0 ? plan->coarse : plan->grids[size_t(grid)];
+    return CIE_HIP_OK;
+}
+
 int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size, double relativeTolerance,
                       uint64_t maxCycles, cie_hip_pmg_info* info) {
     if (info) *info = cie_hip_pmg_info{};

@@ -193,10 +193,19 @@ class CSRMatrix:
         _sync_torch()
         check(library().cie_hip_csr_update_entries(self.handle, _dptr(entries)), "cie_hip_csr_update_entries")
 
+    @classmethod
+    def _borrow(cls, handle, context, owner):
+        """A view of a matrix that `owner` holds (kept alive by the view, never destroyed by it)."""
+        rows, columns, nnz = c_int64(), c_int64(), c_int64()
+        check(library().cie_hip_csr_shape(handle, byref(rows), byref(columns), byref(nnz)), "cie_hip_csr_shape")
+        matrix = cls._adopt(handle, context, rows.value, columns.value, nnz.value)
+        matrix._owner = owner
+        return matrix
+
     def close(self):
-        if self.handle:
+        if self.handle and getattr(self, "_owner", None) is None:
             library().cie_hip_csr_destroy(self.handle)
-            self.handle = c_void_p()
+        self.handle = c_void_p()
 
     def __del__(self):
         try:
@@ -643,6 +652,13 @@ class PMultigrid:
         check(library().cie_hip_pmg_create(matrix.handle, mask.handle, self.order, byref(self.handle)),
               "cie_hip_pmg_create")
         self.info = None
+
+    def grid(self, index):
+        """The matrix of grid `index` (-1: the renumbered coarse matrix; None if there is none), a view whose
+        launch may be set or autotuned (cie_hip_pmg_grid)."""
+        handle = c_void_p()
+        check(library().cie_hip_pmg_grid(self.handle, int(index), byref(handle)), "cie_hip_pmg_grid")
+        return None if not handle.value else CSRMatrix._borrow(handle, self.matrix.context, self)
 
     def solve(self, b, x, relative_tolerance=1e-6, max_cycles=10000):
         """Cycles from x (expected zero, as in the driver) until ||r|| <= relative_tolerance ||b||."""

@@ -148,6 +148,38 @@ def test_pmg_matches_oracle(hip, torch_gpu, order, n, blocks):
     assert np.linalg.norm(got - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
 
 
+@pytest.mark.parametrize("launch", [("blocks", 2, 4), ("sorted", 3, 8), ("tiled", 256, 16)])
+def test_pmg_grid_launches(hip, torch_gpu, launch):
+    """Every grid's matrix is reachable (the finest grid, where the mask keeps every row, is the plan's own
+    matrix; -1 the renumbered coarse matrix), and a launch set on the masked grids and the coarse matrix changes
+    only summation orders: the oracle's cycle and coarse CG counts, x to 1e-9."""
+    order, n = 3, 6
+    csr, mask_values = _legendre(order, n)
+    b = systems.uniform(1, csr.rows)
+    x_ref, history, coarse = oracle.pmg(csr, mask_values, order, b)
+    matrix = hip.CSRMatrix(csr)
+    pmg = hip.PMultigrid(matrix, hip.Mask(mask_values), order)
+    grids = [pmg.grid(g) for g in range(order)]
+    assert grids[order - 1].handle.value == matrix.handle.value  # mask < order + 1 keeps every row
+    assert all(g.rows == csr.rows and g.columns == csr.columns for g in grids)
+    assert [g.nnz for g in grids] == sorted(g.nnz for g in grids) and grids[0].nnz < csr.nnz
+    coarse_matrix = pmg.grid(-1)
+    assert coarse_matrix.rows == int(np.count_nonzero(mask_values < 2))
+    for g in grids[:order - 1] + [coarse_matrix]:
+        g.set_launch(64, *launch)
+        assert tuple(g.launch(64)) == launch
+    x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
+    info = pmg.solve(_dev(torch_gpu, b), x)
+    assert info.converged == 1 and info.cycles == len(history) and info.coarseIterations == int(coarse.sum())
+    assert np.linalg.norm(x.cpu().numpy() - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+    with pytest.raises(hip._lib.InvalidArgument):
+        pmg.grid(order)
+    del grids, coarse_matrix  # views: dropping them leaves the plan's matrices alone
+    x.zero_()
+    again = pmg.solve(_dev(torch_gpu, b), x)
+    assert again.cycles == info.cycles
+
+
 def test_pmg_rejects_bad_input(hip, torch_gpu):
     csr, mask_values = _legendre(2, 4)
     with pytest.raises(ShapeError):
