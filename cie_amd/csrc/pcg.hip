@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
     } else {
         fold = refineDue(s);
     }
-    const float pq = float(sumPartialsAll<kVectorThreads, double>(productPartials, productCount, 0, lds64));
+    const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
     if (!(0.0f < pq)) {
         if (firstThread()) s.done = kNotSpd;
         return;
@@ -375,8 +375,9 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
 }
 
 // Mixed form, the kernel after the product, every trip. Blocks below
-// `levelBlocks` first sum the product's partials into `level` (the first level
-// of partialLevelKernel; none when the product wrote few enough partials).
+// `levelBlocks` first sum the product's FP32 p.q partials into `level`, in
+// FP32 (the first level of the sum; none when the product wrote few enough
+// partials).
 // Then, if the previous trip folded the correction into x (refined[previous]),
 // every block recomputes its rows of r = b - A x with the FP64 plane and x
 // (rounded into the FP32 r) and writes a partial of the FP64 r.r to
@@ -392,22 +393,15 @@ __global__ __launch_bounds__(Walker::kBlock) void levelRefineKernel(
     constexpr int BS = Walker::kBlock;
     static_assert(BS == kReduceBlock, "the level sums use the reduction block size");
     __shared__ double ldsA[BS / kWave];
-    __shared__ double ldsB[BS / kWave];
+    __shared__ float ldsF[BS / kWave];
     if (state->done) return;
-    if (int(blockIdx.x) < levelBlocks) {
+    if (int(blockIdx.x) < levelBlocks) {  // the FP32 p.q partials, summed in FP32
         const int begin = int(blockIdx.x) * levelPer;
         const int end = min(levelCount, begin + levelPer);
-        double a = 0.0, c = 0.0;
-        for (int i = begin + int(threadIdx.x); i < end; i += BS) {
-            a += productPartials[2 * i];
-            c += productPartials[2 * i + 1];
-        }
-        a = blockSum<BS>(a, ldsA);
-        c = blockSum<BS>(c, ldsB);
-        if (threadIdx.x == 0) {
-            level[2 * blockIdx.x] = a;
-            level[2 * blockIdx.x + 1] = c;
-        }
+        float a = 0.0f;
+        for (int i = begin + int(threadIdx.x); i < end; i += BS) a += float(productPartials[2 * i]);
+        a = blockSum<BS>(a, ldsF);
+        if (threadIdx.x == 0) level[2 * blockIdx.x] = double(a);
     }
     if (!state->refined[previous]) return;
     double rr = 0.0;
