@@ -59,6 +59,8 @@ class SolveInfo(ctypes.Structure):
         ("spmvCount", c_uint64),
         ("productSeconds", c_double),
         ("productSamples", c_uint64),
+        ("initialResidual", c_double),
+        ("reportedIterations", c_uint64),
     ]
 
 
@@ -66,6 +68,8 @@ class MixedSettings(ctypes.Structure):
     _fields_ = [("settings", Statistics), ("refinementFactor", c_double), ("restart", c_int32), ("flags", c_int32)]
 
 
+REPORT_TERMINATION = 0  # CIE_HIP_REPORT_TERMINATION (IterativeSolver::ReportType::Termination)
+REPORT_ITERATION = 1    # CIE_HIP_REPORT_ITERATION (IterativeSolver::ReportType::Iteration)
 MIXED_PLAIN_FP32 = 1  # CIE_HIP_MIXED_PLAIN_FP32: no row-sum correction in the inner FP32 product
 DEFAULT_REFINEMENT_FACTOR = 0.03  # CIE_HIP_DEFAULT_REFINEMENT_FACTOR
 
@@ -138,6 +142,9 @@ _SIGNATURES = {
     "cie_hip_context_stream": (c_int, [_P, POINTER(_P)]),
     "cie_hip_context_set_profiling": (c_int, [_P, c_int]),
     "cie_hip_context_kernel_seconds": (c_int, [_P, POINTER(c_double)]),
+    "cie_hip_context_set_residual_history": (c_int, [_P, _P, c_int64]),
+    "cie_hip_iteration_report": (c_int, [c_int, c_int, POINTER(Statistics), POINTER(Statistics), ctypes.c_char_p,
+                                         ctypes.c_size_t, POINTER(ctypes.c_size_t)]),
     "cie_hip_malloc": (c_int, [_P, ctypes.c_size_t, POINTER(_P)]),
     "cie_hip_free": (c_int, [_P, _P]),
     "cie_hip_memcpy": (c_int, [_P, _P, _P, ctypes.c_size_t, c_int]),

@@ -54,6 +54,8 @@ struct SolverState {
     int32_t refined[2];    // mixed: 1 if the trip of that parity recomputed the true residual
     int32_t restart;       // mixed: 1 resets the search direction at each refinement
     int32_t startExit;     // 1 if the initial residual met the tolerance (no loop trip ran)
+    double* history;       // caller's residual history (device), or null: ||r|| of loop trip k in [k]
+    uint64_t historyCapacity;
 };
 
 enum SolveEnd : int32_t {
@@ -89,6 +91,8 @@ struct cie_hip_context_s {
     static constexpr int kProfileEvents = 64;
     hipEvent_t profileEvents[2 * kProfileEvents] = {};
     double kernelSeconds = 0.0;  // profiling on: the last call's main kernel (cie_hip_context_kernel_seconds)
+    double* history = nullptr;   // residual history of the fused solvers (cie_hip_context_set_residual_history)
+    int64_t historyCapacity = 0;
 };
 
 namespace ciehip {

@@ -150,6 +150,15 @@ int cie_hip_context_set_profiling(cie_hip_context context, int enable) {
     return CIE_HIP_OK;
 }
 
+int cie_hip_context_set_residual_history(cie_hip_context context, double* history, int64_t capacity) {
+    if (!context) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_context_set_residual_history: null context");
+    if (history && capacity < 0)
+        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_context_set_residual_history: negative capacity");
+    context->history = history && capacity > 0 ? history : nullptr;
+    context->historyCapacity = context->history ? capacity : 0;
+    return CIE_HIP_OK;
+}
+
 int cie_hip_context_kernel_seconds(cie_hip_context context, double* seconds) {
     if (!context || !seconds) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_context_kernel_seconds: null argument");
     *seconds = context->kernelSeconds;

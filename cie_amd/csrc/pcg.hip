@@ -77,6 +77,11 @@ __device__ __forceinline__ TR sumPartialsAll(const double* __restrict__ partials
 
 __device__ __forceinline__ bool firstThread() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
+// The residual history's entry of loop trip `trip` (one thread, when recording).
+__device__ __forceinline__ void recordResidual(const SolverState& s, uint64_t trip, double absolute) {
+    if (s.history && trip < s.historyCapacity) s.history[trip] = absolute;
+}
+
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
@@ -324,6 +329,8 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
             s.absolute = absolute;
             s.relative = relative;
             s.updateNorm = rr;
+            // The trip that folded (its direction step counted it) reports the FP64 residual.
+            recordResidual(s, s.iteration - 1, absolute);
             // The trip that folded counts as the last one: its direction step already counted it.
             if (converged) {
                 s.iteration -= 1;
@@ -455,6 +462,7 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
             s.rr = double(rr);
             s.absolute = absolute;
             s.relative = relative;
+            recordResidual(s, s.iteration, absolute);
             if (converged) {
                 s.done = kConverged;
             } else {
@@ -476,6 +484,7 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
             s.rr = double(rr);
             s.absolute = absolute;
             s.relative = absolute / s.initialNorm;
+            recordResidual(s, s.iteration, absolute);
             s.iteration += 1;
             if (s.iteration >= s.maxIterations) s.done = kMaxIterations;
         }
@@ -786,6 +795,8 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     init.maxIterations = q.settings.iterationCount;
     init.updateThreshold = q.refinementFactor * q.refinementFactor;
     init.restart = q.restart;
+    init.history = ctx->history;
+    init.historyCapacity = uint64_t(ctx->historyCapacity);
     *ctx->scratch.pinnedState = init;
     CIE_HIP_CHECK(hipMemcpyAsync(ctx->scratch.state, ctx->scratch.pinnedState, sizeof(SolverState),
                                  hipMemcpyHostToDevice, ctx->stream));
@@ -867,6 +878,8 @@ int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_in
     const uint64_t converging = final.done == kConverged ? (q.mixed ? 2 : 1) : 0;
     const uint64_t trips = final.startExit ? 0 : final.iteration + converging;
     result.spmvCount = trips + 1 + final.refinements;
+    result.initialResidual = final.initialNorm;
+    result.reportedIterations = final.startExit ? 0 : final.iteration + (final.done == kConverged ? 1 : 0);
     if (ctx->profiling) {
         const int worked = int(std::min<uint64_t>(trips, uint64_t(cie_hip_context_s::kProfileEvents)));
         result.productSamples = uint64_t(std::min(worked, profile.used));

@@ -17,6 +17,7 @@ Vectors are torch tensors on a HIP device; torch only provides the memory.
 Every compute call goes through libcie_hip.so.
 """
 import ctypes
+import logging
 import math
 import sys
 from ctypes import byref, c_double, c_float, c_int, c_int64, c_void_p
@@ -360,10 +361,41 @@ def _float_less(left, right, value=float):
     return bool(value(left) < value(right)) and not _float_equal(left, right, value)
 
 
+def makeIterationReport(verbosity, reportType, status, settings):
+    """IterativeSolver::makeIterationReport (linalg/packages/solvers/src/IterativeSolver.cpp:17-58), formatted
+    by cie_hip_iteration_report: the table header at iteration 0, a coloured row on termination, on convergence
+    or (verbosity >= 3) on every iteration, the closing rule once converged; empty below verbosity 2."""
+    lib = library()
+    length = ctypes.c_size_t(0)
+    check(lib.cie_hip_iteration_report(verbosity, reportType, byref(status), byref(settings), None, 0, byref(length)),
+          "cie_hip_iteration_report")
+    buffer = ctypes.create_string_buffer(length.value + 1)
+    check(lib.cie_hip_iteration_report(verbosity, reportType, byref(status), byref(settings), buffer, len(buffer),
+                                       byref(length)), "cie_hip_iteration_report")
+    return buffer.value.decode()
+
+
+# The reports go to this logger, one record per non-empty report, as the
+# reference logs them into its "ConjugateGradients" log block
+# (ConjugateGradients_impl.hpp:72-74); each solver also keeps them in `reports`.
+REPORT_LOG = logging.getLogger("cie_amd.ConjugateGradients")
+
+# Loop trips whose residual the fused solvers record for their reports; the
+# report of the final trip always comes from the solver's statistics.
+HISTORY_CAPACITY = 1 << 20
+
+
 class IterativeSolver(LinearOperator):
     def __init__(self, settings=None):
         self._settings = settings or Statistics()
         self._results = None
+        self.reports = []
+
+    def _log_report(self, verbosity, reportType, stats, settings):
+        text = makeIterationReport(verbosity, reportType, stats, settings)
+        if text:
+            self.reports.append(text)
+            REPORT_LOG.info(text)
 
     def getConfiguration(self):
         return self._settings
@@ -407,6 +439,7 @@ class ConjugateGradients(IterativeSolver):
         maximum = sys.float_info.max if V is float else float(np.finfo(np.float32).max)
         stats = Statistics(0, maximum, maximum)
         self.history = []
+        self.reports = []
         solution = space.makeVector(size)
         space.assign(solution, out)
         if settings.iterationCount:
@@ -424,6 +457,7 @@ class ConjugateGradients(IterativeSolver):
             if less(stats.absoluteResidual, settings.absoluteResidual) or less(stats.relativeResidual,
                                                                               settings.relativeResidual):
                 # :100-111 return before the output step of :190-198: `out` keeps its value.
+                self._log_report(self._verbosity, _lib.REPORT_TERMINATION, stats, settings)
                 self.report(stats)
                 return
             if self._preconditioner is not None:
@@ -461,6 +495,7 @@ class ConjugateGradients(IterativeSolver):
                 stats.absoluteResidual = float(absolute)
                 stats.relativeResidual = float(V(absolute / initialResidualNorm))
                 self.history.append(stats.absoluteResidual)
+                self._log_report(self._verbosity, _lib.REPORT_ITERATION, stats, settings)
                 if less(stats.absoluteResidual, settings.absoluteResidual) or less(stats.relativeResidual,
                                                                                   settings.relativeResidual):
                     break
@@ -480,49 +515,92 @@ class ConjugateGradients(IterativeSolver):
             space.add(out, solution, outScale)
 
 
-class FusedPCG(IterativeSolver):
-    """Jacobi-PCG in three fused kernels per iteration (cie_hip_pcg_f64)."""
+class _FusedSolver(IterativeSolver):
+    """A fused solver behind the IterativeSolver face; with verbosity >= 2 it records each loop trip's residual
+    on the device (cie_hip_context_set_residual_history) and logs the reference's per-iteration reports from it
+    after the solve (ConjugateGradients_impl.hpp:100-111 for the start check, :175-182 for each trip)."""
 
-    def __init__(self, matrix, settings=None, true_residual=True):
+    def __init__(self, matrix, settings=None, verbosity=0):
         super().__init__(settings)
         self.matrix = matrix
-        self.true_residual = true_residual
+        self.verbosity = verbosity
         self.info = None
+        self.history = None
+
+    def _solve(self, call, name):
+        settings = self.getConfiguration()
+        info = SolveInfo()
+        lib = library()
+        context = self.matrix.context.handle
+        recorded = None
+        if self.verbosity >= 2 and settings.iterationCount:
+            recorded = _torch().empty(max(1, min(int(settings.iterationCount), HISTORY_CAPACITY)), dtype=_torch().float64,
+                                      device=f"cuda:{self.matrix.context.device}")
+            check(lib.cie_hip_context_set_residual_history(context, _dptr(recorded), recorded.numel()),
+                  "cie_hip_context_set_residual_history")
+        _sync_torch()
+        try:
+            status = call(lib, info)
+        finally:
+            if recorded is not None:
+                lib.cie_hip_context_set_residual_history(context, None, 0)
+        self.info = info
+        self.reports = []
+        rows = int(info.reportedIterations)
+        self.history = recorded[:min(rows, recorded.numel())].cpu().numpy() if recorded is not None else None
+        if recorded is not None:
+            self._log_reports(info, settings)
+        check(status, name)
+        self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+
+    def _log_reports(self, info, settings):
+        rows = int(info.reportedIterations)
+        if rows == 0:
+            if info.converged:  # the start check ended the solve: one Termination report
+                self._log_report(self.verbosity, _lib.REPORT_TERMINATION,
+                                 Statistics(0, info.initialResidual, 1.0), settings)
+            return
+        for trip in range(rows):
+            if trip == rows - 1:  # the solver's own final statistics (the mixed form's FP64 residual at a cap)
+                status = Statistics(trip, info.stats.absoluteResidual, info.stats.relativeResidual)
+            elif trip < len(self.history):
+                absolute = float(self.history[trip])
+                status = Statistics(trip, absolute, absolute / info.initialResidual)
+            else:
+                continue  # beyond the recorded trips
+            self._log_report(self.verbosity, _lib.REPORT_ITERATION, status, settings)
+
+
+class FusedPCG(_FusedSolver):
+    """Jacobi-PCG in three fused kernels per iteration (cie_hip_pcg_f64)."""
+
+    def __init__(self, matrix, settings=None, true_residual=True, verbosity=0):
+        super().__init__(matrix, settings, verbosity)
+        self.true_residual = true_residual
 
     def product(self, inScale, in_, outScale, out):
         settings = self.getConfiguration()
-        info = SolveInfo()
-        _sync_torch()
-        status = library().cie_hip_pcg_f64(self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_),
-                                           outScale, _dptr(out), out.numel(), byref(settings),
-                                           1 if self.true_residual else 0, byref(info))
-        self.info = info
-        check(status, "cie_hip_pcg_f64")
-        self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+        self._solve(lambda lib, info: lib.cie_hip_pcg_f64(
+            self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_), outScale, _dptr(out), out.numel(),
+            byref(settings), 1 if self.true_residual else 0, byref(info)), "cie_hip_pcg_f64")
 
 
-class MixedPCG(IterativeSolver):
+class MixedPCG(_FusedSolver):
     """FP32 inner PCG with FP64 solution accumulation and FP64 residual refinement (cie_hip_pcg_mixed)."""
 
     def __init__(self, matrix, settings=None, refinement_factor=_lib.DEFAULT_REFINEMENT_FACTOR, restart=False,
-                 row_sum_correction=True):
-        super().__init__(settings)
-        self.matrix = matrix
+                 row_sum_correction=True, verbosity=0):
+        super().__init__(matrix, settings, verbosity)
         self.refinement_factor = refinement_factor
         self.restart = restart
         self.row_sum_correction = row_sum_correction  # False: the inner product is the plain FP32 plane
-        self.info = None
 
     def product(self, inScale, in_, outScale, out):
         flags = 0 if self.row_sum_correction else _lib.MIXED_PLAIN_FP32
         mixed = MixedSettings(self.getConfiguration(), self.refinement_factor, 1 if self.restart else 0, flags)
-        info = SolveInfo()
-        _sync_torch()
-        status = library().cie_hip_pcg_mixed(self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_),
-                                             outScale, _dptr(out), out.numel(), byref(mixed), byref(info))
-        self.info = info
-        check(status, "cie_hip_pcg_mixed")
-        self.report(Statistics(info.stats.iterationCount, info.stats.absoluteResidual, info.stats.relativeResidual))
+        self._solve(lambda lib, info: lib.cie_hip_pcg_mixed(
+            self.matrix.context.handle, self.matrix.handle, inScale, _dptr(in_), outScale, _dptr(out), out.numel(),
+            byref(mixed), byref(info)), "cie_hip_pcg_mixed")
 
 
 class Mask:

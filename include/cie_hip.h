@@ -71,6 +71,11 @@ typedef struct cie_hip_solve_info {
     uint64_t spmvCount;            /* SpMV launches that did work */
     double productSeconds;         /* profiling on: mean duration of the loop's SpMV (q = A p) kernel */
     uint64_t productSamples;       /* profiling on: launches that mean is taken over */
+    double initialResidual;        /* ||b - A x0||, the denominator of every relative residual */
+    uint64_t reportedIterations;   /* loop trips the reference's solver reports (one row each at verbosity 3,
+                                      ConjugateGradients_impl.hpp:175-182): iterationCount + 1 on convergence
+                                      in the loop, iterationCount otherwise, 0 when the start check ended the
+                                      solve (that report is one Termination row at iteration 0) */
 } cie_hip_solve_info;
 
 /* Settings of the mixed-precision refined PCG. */
@@ -117,6 +122,19 @@ int cie_hip_context_set_profiling(cie_hip_context context, int enable);
  * main kernel of the last cie_hip_assemble, cie_hip_element_stiffness or
  * cie_hip_csr_reorder call on this context. */
 int cie_hip_context_kernel_seconds(cie_hip_context context, double* seconds);
+/* Residual history of the fused solvers (cie_hip_pcg_f64, cie_hip_pcg_mixed) on
+ * this context, for the reference's per-iteration report
+ * (ConjugateGradients_impl.hpp:175-182 calls IterativeSolver::makeIterationReport
+ * after every loop trip). history: device memory of `capacity` doubles, or NULL
+ * to stop recording. Loop trip k (reported as iteration k) stores the absolute
+ * residual its report shows in history[k], for k < capacity; the relative
+ * residual is that over cie_hip_solve_info.initialResidual. In the mixed form
+ * it is the FP32 recurrence's ||r||, replaced by the FP64 ||b - A x|| on a
+ * trip whose correction was folded into x; an FP32 residual that passes the
+ * tolerance triggers the FP64 test on the next trip, so its row can read as
+ * converged one row before the last. The buffer is the caller's and must
+ * outlive the solves that write it. */
+int cie_hip_context_set_residual_history(cie_hip_context context, double* history, int64_t capacity);
 
 /* Device memory for callers without an allocator of their own
  * (cieutils/packages/concurrency/inc/sycl.hpp:35-70 DeviceMemory). */
@@ -281,6 +299,20 @@ int cie_hip_pcg_f64(cie_hip_context context, cie_hip_csr matrix, double inScale,
 int cie_hip_pcg_mixed(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* b,
                       double outScale, double* x, int64_t size, const cie_hip_mixed_settings* settings,
                       cie_hip_solve_info* info);
+
+/* IterativeSolver<TS>::makeIterationReport (linalg/packages/solvers/src/
+ * IterativeSolver.cpp:17-58): the report text for one solver state. At
+ * verbosity >= 2: the table header when status->iterationCount is 0; a row
+ * (iteration, absolute and relative residual, each in a 24-bit ANSI colour:
+ * green below its tolerance, orange not) for a Termination report, a converged
+ * state, or every Iteration report at verbosity >= 3; the closing rule once
+ * converged. Below verbosity 2 the text is empty. Writes at most capacity - 1
+ * characters and a NUL into out (out may be NULL with capacity 0); *length
+ * receives the full length. */
+#define CIE_HIP_REPORT_TERMINATION 0
+#define CIE_HIP_REPORT_ITERATION 1
+int cie_hip_iteration_report(int verbosity, int reportType, const cie_hip_statistics* status,
+                             const cie_hip_statistics* settings, char* out, size_t capacity, size_t* length);
 
 /* ---- Batched solves (no reference counterpart) ---------------------------- */
 /* Many independent SPD systems of uneven size, stored as the diagonal blocks

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <concepts>
 #include <cstddef>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <optional>
@@ -58,7 +59,35 @@ inline void check(int status, const char* what) {
     if (status != CIE_HIP_OK)
         throw HIPError(status, std::string(what) + ": " + cie_hip_last_error());
 }
+
+// The report text of IterativeSolver::makeIterationReport (src/IterativeSolver.cpp:17-58) for one solver
+// state, formatted by cie_hip_iteration_report.
+inline std::string iterationReport(int verbosity, int reportType, std::size_t iteration, double absolute,
+                                   double relative, const cie_hip_statistics& settings) {
+    const cie_hip_statistics status{iteration, absolute, relative};
+    std::size_t length = 0;
+    check(cie_hip_iteration_report(verbosity, reportType, &status, &settings, nullptr, 0, &length),
+          "cie_hip_iteration_report");
+    std::string text(length, '\0');
+    check(cie_hip_iteration_report(verbosity, reportType, &status, &settings, text.data(), length + 1, &length),
+          "cie_hip_iteration_report");
+    return text;
+}
+
+// Where the solvers of this header log their reports (the reference logs them into a "ConjugateGradients"
+// log block, ConjugateGradients_impl.hpp:72-74): one line per non-empty report, std::clog unless set.
+inline std::ostream*& reportStream() {
+    static std::ostream* stream = &std::clog;
+    return stream;
+}
+
+inline void logReport(const std::string& report) {
+    if (!report.empty() && reportStream()) *reportStream() << report << '\n';
+}
 }  // namespace hipdetail
+
+// Sends the solvers' per-iteration reports to `stream` (nullptr: drop them).
+inline void setReportStream(std::ostream* stream) { hipdetail::reportStream() = stream; }
 
 #ifndef CIE_HIP_WITH_REFERENCE_HEADERS
 
@@ -127,6 +156,19 @@ public:
     std::optional<Statistics> getStats() const noexcept { return _results; }
 
 protected:
+    enum class ReportType { Termination, Iteration };
+
+    // IterativeSolver::makeIterationReport (src/IterativeSolver.cpp:17-58).
+    std::string& makeIterationReport(std::string& output, int verbosity, ReportType reportType,
+                                     const Statistics& status, const Statistics& settings) const {
+        output = hipdetail::iterationReport(
+            verbosity, reportType == ReportType::Termination ? CIE_HIP_REPORT_TERMINATION : CIE_HIP_REPORT_ITERATION,
+            status.iterationCount, double(status.absoluteResidual), double(status.relativeResidual),
+            cie_hip_statistics{settings.iterationCount, double(settings.absoluteResidual),
+                               double(settings.relativeResidual)});
+        return output;
+    }
+
     void report(Statistics stats) noexcept { _results = stats; }
 
 private:
@@ -444,6 +486,7 @@ public:
                                                     std::to_string(n));
         const Statistics settings = this->getConfiguration();
         Statistics stats{0, std::numeric_limits<Value>::max(), std::numeric_limits<Value>::max()};
+        std::string report;
         auto solution = space.makeVector(n);
         space.assign(space.view(solution), out);
         if (settings.iterationCount) {
@@ -461,6 +504,8 @@ public:
             stats.relativeResidual = 1;
             if (less(stats.absoluteResidual, settings.absoluteResidual) || less(stats.relativeResidual, settings.relativeResidual)) {
                 // :100-111 return before the output step of :190-198: `out` keeps its value.
+                hipdetail::logReport(this->makeIterationReport(report, _verbosity, Base::ReportType::Termination,
+                                                               stats, settings));
                 this->report(stats);
                 return;
             }
@@ -502,6 +547,8 @@ public:
                           1);
                 stats.absoluteResidual = std::sqrt(residualNorm);
                 stats.relativeResidual = stats.absoluteResidual / initialResidualNorm;
+                hipdetail::logReport(this->makeIterationReport(report, _verbosity, Base::ReportType::Iteration, stats,
+                                                               settings));
                 if (less(stats.absoluteResidual, settings.absoluteResidual) || less(stats.relativeResidual, settings.relativeResidual))
                     break;
             }
@@ -542,34 +589,51 @@ private:
 // Fused Jacobi-PCG (cie_hip_pcg_f64) or mixed refined PCG (cie_hip_pcg_mixed)
 // behind the same LinearOperator / IterativeSolver face: product(0, b, 1, x)
 // solves A x = b with x as the initial guess, getStats() as for
-// ConjugateGradients. The preconditioner is always Jacobi.
+// ConjugateGradients. The preconditioner is always Jacobi. With verbosity >= 2
+// the solve records each loop trip's residual on the device and logs the
+// reference's per-iteration reports from it afterwards (the start check's
+// Termination report, ConjugateGradients_impl.hpp:100-111, or one report per
+// trip, :175-182; the last trip's from the solver's statistics).
 class FusedConjugateGradients : public IterativeSolver<HIPSpace<double>> {
     using Base = IterativeSolver<HIPSpace<double>>;
     using Space = HIPSpace<double>;
 
 public:
     using typename Base::Statistics;
+    static constexpr std::size_t kHistoryCapacity = std::size_t(1) << 20;  // trips recorded for the reports
 
     FusedConjugateGradients(std::shared_ptr<const HIPCSRMatrix> matrix, std::shared_ptr<Space> space,
                             Statistics settings = {}, bool mixedPrecision = false,
-                            double refinementFactor = CIE_HIP_DEFAULT_REFINEMENT_FACTOR)
+                            double refinementFactor = CIE_HIP_DEFAULT_REFINEMENT_FACTOR, int verbosity = 0)
         : Base(settings), _matrix(std::move(matrix)), _space(std::move(space)), _mixed(mixedPrecision),
-          _refinementFactor(refinementFactor) {}
+          _refinementFactor(refinementFactor), _verbosity(verbosity) {}
 
     void product(double inScale, Space::ConstVectorView in, double outScale, Space::VectorView out) override {
         const Statistics settings = this->getConfiguration();
         const cie_hip_statistics s{settings.iterationCount, settings.absoluteResidual, settings.relativeResidual};
         const cie_hip_context ctx = _space->getContext()->get();
+        std::optional<Space::Vector> history;
+        if (_verbosity >= 2 && settings.iterationCount) {
+            history = _space->makeVector(std::min<std::size_t>(settings.iterationCount, kHistoryCapacity));
+            hipdetail::check(cie_hip_context_set_residual_history(ctx, history->get(), std::int64_t(history->size())),
+                             "cie_hip_context_set_residual_history");
+        }
+        int status;
         if (_mixed) {
             const cie_hip_mixed_settings mixed{s, _refinementFactor, 0, 0};
-            hipdetail::check(cie_hip_pcg_mixed(ctx, _matrix->get(), inScale, in.get(), outScale, out.get(),
-                                               std::int64_t(out.size()), &mixed, &_info),
-                             "FusedConjugateGradients::product");
+            status = cie_hip_pcg_mixed(ctx, _matrix->get(), inScale, in.get(), outScale, out.get(),
+                                       std::int64_t(out.size()), &mixed, &_info);
         } else {
-            hipdetail::check(cie_hip_pcg_f64(ctx, _matrix->get(), inScale, in.get(), outScale, out.get(),
-                                             std::int64_t(out.size()), &s, 1, &_info),
-                             "FusedConjugateGradients::product");
+            status = cie_hip_pcg_f64(ctx, _matrix->get(), inScale, in.get(), outScale, out.get(),
+                                     std::int64_t(out.size()), &s, 1, &_info);
         }
+        if (history) {
+            const std::string message = status == CIE_HIP_OK ? std::string() : cie_hip_last_error();
+            cie_hip_context_set_residual_history(ctx, nullptr, 0);
+            logReports(*history, s);
+            if (status != CIE_HIP_OK) throw HIPError(status, "FusedConjugateGradients::product: " + message);
+        }
+        hipdetail::check(status, "FusedConjugateGradients::product");
         this->report(Statistics{std::size_t(_info.stats.iterationCount), _info.stats.absoluteResidual,
                                 _info.stats.relativeResidual});
     }
@@ -577,10 +641,34 @@ public:
     const cie_hip_solve_info& info() const noexcept { return _info; }
 
 private:
+    void logReports(const Space::Vector& history, const cie_hip_statistics& settings) const {
+        const std::size_t rows = std::size_t(_info.reportedIterations);
+        if (rows == 0) {
+            if (_info.converged)  // the start check ended the solve
+                hipdetail::logReport(hipdetail::iterationReport(_verbosity, CIE_HIP_REPORT_TERMINATION, 0,
+                                                                _info.initialResidual, 1.0, settings));
+            return;
+        }
+        std::vector<double> recorded(std::min(rows, history.size()));
+        _space->assign(std::span<double>(recorded), _space->view(history));  // the first recorded.size() entries
+        for (std::size_t trip = 0; trip < rows; ++trip) {
+            if (trip + 1 == rows) {
+                hipdetail::logReport(hipdetail::iterationReport(_verbosity, CIE_HIP_REPORT_ITERATION, trip,
+                                                                _info.stats.absoluteResidual,
+                                                                _info.stats.relativeResidual, settings));
+            } else if (trip < recorded.size()) {
+                hipdetail::logReport(hipdetail::iterationReport(_verbosity, CIE_HIP_REPORT_ITERATION, trip,
+                                                                recorded[trip], recorded[trip] / _info.initialResidual,
+                                                                settings));
+            }
+        }
+    }
+
     std::shared_ptr<const HIPCSRMatrix> _matrix;
     std::shared_ptr<Space> _space;
     bool _mixed;
     double _refinementFactor;
+    int _verbosity;
     cie_hip_solve_info _info{};
 };
 

@@ -11,7 +11,9 @@
 //   the reference's FP32 instantiation, ConjugateGradients.cpp:10,16, run to
 //   1e-6 as the driver's settings are) and prints one line per solver: name
 //   iterations absolute relative; then "early <name> <scales> <0|1>", 1 when
-//   an early exit (ConjugateGradients_impl.hpp:100-111) left x bit for bit.
+//   an early exit (ConjugateGradients_impl.hpp:100-111) left x bit for bit;
+//   and <prefix>_reports_{generic,fused}.txt, the solvers' per-iteration
+//   reports at verbosity 3, one per line.
 #include "cie_hip/linalg.hpp"
 
 #include <cstdio>
@@ -134,6 +136,24 @@ int main(int argc, char** argv) {
                     space->assign(std::span<double>(after), space->view(std::as_const(x)));
                     std::printf("early %s %g,%g %d\n", name, scale[0], scale[1], before == after ? 1 : 0);
                 }
+            }
+        }
+        {
+            // Per-iteration reports at verbosity 3 (ConjugateGradients_impl.hpp:175-182), one file per solver.
+            for (const std::string name : {"generic", "fused"}) {
+                std::ofstream log(prefix + "_reports_" + name + ".txt", std::ios::binary);
+                setReportStream(&log);
+                auto x = space->makeVector(std::size_t(rows));
+                space->fill(space->view(x), 0.0);
+                if (name == "generic") {
+                    Solver solver(lhs, space, preconditioner, settings, 3);
+                    solver.product(0, space->view(std::as_const(b)), 1, space->view(x));
+                } else {
+                    FusedConjugateGradients solver(matrix, space, {20000, 0.0, tolerance}, false,
+                                                   CIE_HIP_DEFAULT_REFINEMENT_FACTOR, 3);
+                    solver.product(0, space->view(std::as_const(b)), 1, space->view(x));
+                }
+                setReportStream(&std::clog);
             }
         }
         // A missing diagonal raises like makeDiagonalOperator's CIE_THROW (DiagonalOperator.cpp:79-82).

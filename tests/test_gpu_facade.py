@@ -47,6 +47,32 @@ def test_cpp_facade_matches_oracle(tmp_path, torch_gpu):
     assert_fp32_solution_bound(w, x32, x_ref)
     early = [line.split() for line in result.stdout.splitlines() if line.startswith("early ")]
     assert len(early) == 6 and all(fields[3] == "1" for fields in early), early
+    check_reports_match_python(w, tmp_path)
+
+
+def check_reports_match_python(w, tmp_path):
+    """The C++ face's reports at verbosity 3 are the Python mirror's, character for character: both run the same
+    deterministic kernels on the same system and format through cie_hip_iteration_report."""
+    import torch
+
+    from cie_amd import linalg
+    from cie_amd._lib import Statistics
+
+    matrix = linalg.CSRMatrix(w.matrix)
+    space = linalg.HIPSpace()
+    b = torch.from_numpy(w.rhs).to("cuda:0")
+    solvers = {
+        "generic": linalg.ConjugateGradients(linalg.HIPCSROperator(matrix, space), space,
+                                             linalg.makeDiagonalOperator(matrix, space), Statistics(20000, 0.0, 1e-8),
+                                             verbosity=3),
+        "fused": linalg.FusedPCG(matrix, Statistics(20000, 0.0, 1e-8), verbosity=3),
+    }
+    for name, solver in solvers.items():
+        x = torch.zeros_like(b)
+        solver.product(0, b, 1, x)
+        text = (tmp_path / f"x_reports_{name}.txt").read_text()
+        assert len(solver.reports) == int(solver.getStats().iterationCount) + 1, name
+        assert text == "".join(report + "\n" for report in solver.reports), name
 
 
 def five_point_condition(m):
