@@ -46,7 +46,7 @@ def index_bytes(matrix, precision):
         return matrix.column_bytes()
     if kernel == "blocks":
         return matrix.column_blocks(width)[1]
-    return 4  # int32 columns, or the 32-bit key per sorted block
+    return 4  # int32 columns, or the 32-bit key per sorted or paired block
 
 
 def format_bytes(matrix, precision):
@@ -55,6 +55,10 @@ def format_bytes(matrix, precision):
     y once (tile descriptors, 8-12 B per tile of thousands of blocks, are not counted for any kernel)."""
     value = 8 if precision == 64 else 4
     kernel, width, _ = matrix.launch(precision)
+    if kernel == "paired":  # wide and narrow blocks, each with its 32-bit key
+        wide, narrow = matrix.paired_blocks(width)
+        return ((width // 10) * value + 4) * wide + ((width % 10) * value + 4) * narrow + 4 * (matrix.rows + 1) \
+            + 2 * value * matrix.rows
     if kernel in ("blocks", "sorted"):
         blocks, column = matrix.column_blocks(width)
         column = 4 if kernel == "sorted" else column
@@ -260,14 +264,14 @@ def main():
             matrix.set_launch(precision, kernel, int(width), int(unroll))
         tune64 = time_spmv(torch, linalg, matrix, 64, repetitions=3)
     else:
-        tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
+        tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted", "paired"))
     context.synchronize()
     autotune64_seconds = time.perf_counter() - t0
     t0 = time.perf_counter()
     if args.launch:
         tune32 = time_spmv(torch, linalg, matrix, 32, repetitions=3)
     else:
-        tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted"))
+        tune32 = matrix.autotune(32, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted", "paired"))
     context.synchronize()
     autotune32_seconds = time.perf_counter() - t0
     b = torch.from_numpy(workload.rhs).to("cuda:0")
@@ -285,14 +289,16 @@ def main():
 
     # Launch selection by solve time: the autotuner times the standalone SpMV; the solves run the product
     # kernel with the fused p.q, whose best tile size can differ. Each precision keeps whichever of its
-    # autotuned launch and the sorted 1 x 3 tiles of 1024, 2048 and 4096 rows solves fastest (FP64 first:
-    # the mixed solve's start and refinement steps use the FP64 launch).
+    # autotuned launch, the sorted 1 x 3 tiles of 1024, 2048 and 4096 blocks and the paired 1 x 5 / 1 x 3
+    # tiles of 512 and 1024 blocks per width solves fastest (FP64 first: the mixed solve's start and
+    # refinement steps use the FP64 launch).
     selection = {}
     t0 = time.perf_counter()
     if not args.launch:
         for precision, solver in ((64, fp64), (32, mixed)):
             candidates = [tuple(matrix.launch(precision))]
-            candidates += [c for c in (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16)) if c not in candidates]
+            candidates += [c for c in (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16), ("paired", 53, 2),
+                                       ("paired", 53, 4)) if c not in candidates]
             times = {}
             for candidate in candidates:
                 matrix.set_launch(precision, *candidate)
@@ -522,6 +528,8 @@ def _walker(launch, value="double"):
         return f"ColumnBlockTiles<{value}, {value}, {unroll}, 256, {width}>"
     if kernel == "sorted":
         return f"SortedBlockTiles<{value}, {value}, {unroll}, 256, {width}>"
+    if kernel == "paired":
+        return f"PairedBlockTiles<{value}, {value}, {unroll}, 256, {width // 10}, {width % 10}>"
     return f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<{value}, {value}, {width}, {unroll}, 256>"
 
 
@@ -647,8 +655,8 @@ def bench_adjoint(args):
     g, gt = w.extra["log_conductivity"], w.extra["target_log_conductivity"]
     problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
     problem.assemble(gt)
-    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted"))
-    problem.matrix.autotune(32, 5, kernels=("tiled", "blocks", "sorted"))  # the mixed-solve variant's FP32 plane
+    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))
+    problem.matrix.autotune(32, 5, kernels=("tiled", "blocks", "sorted", "paired"))  # the mixed-solve variant's FP32 plane
     ut, _ = problem.solve(problem.rhs)
     problem.target = ut
     context = problem.context
@@ -829,7 +837,7 @@ def bench_pmg(args):
     rows, nnz = csr.rows, csr.nnz
     context = linalg.Context.default()
     matrix = linalg.CSRMatrix(csr, context)
-    matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted"))  # the finest grid's sweeps and updates use this launch
+    matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))  # the finest grid's sweeps and updates use this launch
     mask = linalg.Mask(mask_values, context)
     t0 = time.perf_counter()
     pmg = linalg.PMultigrid(matrix, mask, order)
@@ -859,7 +867,8 @@ def bench_pmg(args):
     t0 = time.perf_counter()
     # The masked grids and the coarse matrix keep entry tiles: selected the same way (pmg.grid(g).set_launch),
     # no format took more than 1% off the cycle, inside the run-to-run spread.
-    selection = {"finest": select(matrix, (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16)))}
+    selection = {"finest": select(matrix, (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16), ("paired", 53, 2),
+                                           ("paired", 53, 4)))}
     selection_seconds = time.perf_counter() - t0
     for _ in range(args.warmup):
         x.zero_()
@@ -923,6 +932,8 @@ def measured_traffic(launch, profile):
         walker = f"ColumnBlockTiles<float, float, {unroll}, 256, {width}> >"
     elif kernel == "sorted":
         walker = f"SortedBlockTiles<float, float, {unroll}, 256, {width}> >"
+    elif kernel == "paired":
+        walker = f"PairedBlockTiles<float, float, {unroll}, 256, {width // 10}, {width % 10}> >"
     else:
         walker = f"{'StagedRows' if kernel == 'staged' else 'VectorRows'}<float, float, {width}, {unroll}, 256> >"
     kernels = json.loads(path.read_text()).get("kernels", {})

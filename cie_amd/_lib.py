@@ -33,6 +33,7 @@ SPMV_STAGED = 1
 SPMV_TILED = 2
 SPMV_BLOCKS = 3
 SPMV_SORTED = 4
+SPMV_PAIRED = 5
 REORDER_NONE, REORDER_CUTHILL_MCKEE, REORDER_REVERSE_CUTHILL_MCKEE = 0, 1, 2
 PLANE_F64 = 1
 PLANE_F32 = 2
@@ -155,6 +156,7 @@ _SIGNATURES = {
     "cie_hip_csr_update_entries": (c_int, [_P, _P]),
     "cie_hip_csr_column_bytes": (c_int, [_P, POINTER(c_int)]),
     "cie_hip_csr_column_blocks": (c_int, [_P, c_int, POINTER(c_int64), POINTER(c_int)]),
+    "cie_hip_csr_paired_blocks": (c_int, [_P, c_int, POINTER(c_int64), POINTER(c_int64)]),
     "cie_hip_csr_set_launch": (c_int, [_P, c_int, c_int, c_int, c_int]),
     "cie_hip_csr_get_launch": (c_int, [_P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "cie_hip_csr_autotune": (c_int, [_P, c_int, c_int, c_int, POINTER(c_double)]),

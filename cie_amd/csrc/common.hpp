@@ -195,6 +195,30 @@ struct TileSet {
     SortedCopy sorted;      // of a column-block matrix, built on first use by a sorted launch
 };
 
+// A matrix covered by 1 x wide and 1 x narrow column blocks, in tiles of
+// whole rows with each width's blocks column-sorted within the tile (see
+// PairedBlockTiles), for one tile capacity (unroll blocks per lane and
+// width). Built on the device on first use; the values are refilled whenever
+// the matrix's entries change.
+struct PairedForm {
+    int wide = 0, narrow = 0, unroll = 0;
+    int state = 0;                   // 1 built, -1 the matrix does not fit the form (error says why)
+    std::string error;
+    int64_t blocks[2] = {0, 0};      // wide, narrow
+    int tileCount = 0;
+    int32_t* extents = nullptr;      // per row: its first block, both widths in row order (rows + 1)
+    int2* tiles = nullptr;           // tileCount + 1: {first row, first block}
+    int2* begins = nullptr;          // tileCount + 1: {first wide block, first narrow block}
+    int2* bases = nullptr;           // per tile: smallest wide and smallest narrow block column
+    uint32_t* keys = nullptr;        // the wide blocks' keys, then the narrow blocks', sorted within tiles
+    int32_t* sortedStart = nullptr;  // per sorted block: first column, first CSR entry and entry count
+    int32_t* sortedEntry = nullptr;  //   (the refill reads the CSR through them)
+    uint8_t* sortedCount = nullptr;
+    double* values64 = nullptr;      // wide planes, then narrow planes (PlaneRun layout)
+    float* values32 = nullptr;
+    uint64_t version = 0;            // the matrix's valuesVersion the values hold
+};
+
 struct cie_hip_csr_s {
     cie_hip_context context = nullptr;
     int64_t rows = 0, columns = 0, nnz = 0;
@@ -228,6 +252,7 @@ struct cie_hip_csr_s {
     int32_t* blockEntryBegin = nullptr;// of a block matrix: each block's first entry of the parent (blocks + 1)
     int blockUsers = 0;                // of a block matrix: holders (batches) that keep it from being freed
     uint64_t filledVersion = 0;        // of a block matrix: the parent's valuesVersion its values hold
+    std::vector<PairedForm*> pairedForms;  // built on first use by paired launches
 };
 
 namespace ciehip {
@@ -248,6 +273,14 @@ void releaseSortedCopy(cie_hip_context context, SortedCopy& copy);
 // The tiles of `capacity` blocks of a column-block matrix with their sorted
 // copy (see ensureSortedCopy).
 int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles);
+// The paired block form of `matrix` for widths wide > narrow and `unroll`
+// blocks per lane and width, values current (built on first use, refilled
+// after the entries changed). Fails with CIE_HIP_ERROR_INVALID_ARGUMENT when
+// a row holds more blocks of one width than a tile, or a tile's blocks of one
+// width span more columns than the keys hold.
+int ensurePairedForm(cie_hip_csr matrix, int wide, int narrow, int unroll, const PairedForm** form);
+// Frees a paired form's device arrays and the form; waits for the stream.
+void releasePairedForm(cie_hip_context context, PairedForm* form);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.
 int ensureNarrowColumns(cie_hip_csr matrix);

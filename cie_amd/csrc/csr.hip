@@ -286,6 +286,7 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->narrowBases) (void)hipFree(m->narrowBases);
     if (m->blockEntryBegin) (void)hipFree(m->blockEntryBegin);
     for (cie_hip_csr blocks : m->columnBlocks) releaseMatrix(blocks);
+    for (PairedForm* form : m->pairedForms) releasePairedForm(m->context, form);
     delete m;
 }
 
@@ -707,9 +708,21 @@ int ensureColumnBlocks(cie_hip_csr m, int width, cie_hip_csr* out) {
     return CIE_HIP_OK;
 }
 
-// Frees the column-block forms no launch of the matrix uses, and the sorted
-// copies of tile capacities no sorted launch uses (rebuilt if asked for again).
+// Frees the column-block forms no launch of the matrix uses, the sorted
+// copies of tile capacities no sorted launch uses and the paired forms no
+// paired launch uses (all rebuilt if asked for again; paired forms that do not
+// fit the matrix are kept, they hold no memory).
 void releaseUnusedColumnBlocks(cie_hip_csr m) {
+    std::vector<PairedForm*> kept;
+    for (PairedForm* form : m->pairedForms) {
+        const auto uses = [form](const CsrLaunch& launch) {
+            return launch.kernel == CIE_HIP_SPMV_PAIRED && pairedWide(launch.width) == form->wide &&
+                   pairedNarrow(launch.width) == form->narrow && launch.unroll == form->unroll;
+        };
+        if (form->state < 0 || uses(m->launch64) || uses(m->launch32)) kept.push_back(form);
+        else releasePairedForm(m->context, form);
+    }
+    m->pairedForms = std::move(kept);
     for (int width = 2; width <= cie_hip_csr_s::kMaxBlockWidth; ++width) {
         cie_hip_csr blocks = m->columnBlocks[width];
         if (!blocks) continue;
@@ -872,6 +885,11 @@ int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int wi
     } else if (kernel == CIE_HIP_SPMV_TILED) {
         if (!validTiles(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: tiled width must be 256, unroll 4, 8 or 16");
+    } else if (kernel == CIE_HIP_SPMV_PAIRED) {
+        if (!validPaired(width, unroll))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: paired width must be 53 or 32, unroll 2 or 4");
+        if (matrix->columns < pairedWide(width))
+            return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: the matrix has fewer columns than the wide block");
     } else if (kernel == CIE_HIP_SPMV_BLOCKS || kernel == CIE_HIP_SPMV_SORTED) {
         if (!validBlocks(width, unroll))
             return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_set_launch: block width must be 2, 3 or 4, unroll 4, 8 or 16");
@@ -910,6 +928,21 @@ int cie_hip_csr_column_blocks(cie_hip_csr matrix, int width, int64_t* blocks, in
     return CIE_HIP_OK;
 }
 
+int cie_hip_csr_paired_blocks(cie_hip_csr matrix, int width, int64_t* wideBlocks, int64_t* narrowBlocks) {
+    if (!matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_paired_blocks: null matrix");
+    if (!validPaired(width, 4)) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_paired_blocks: paired width must be 53 or 32");
+    CIE_HIP_CHECK(hipSetDevice(matrix->context->device));
+    int unroll = 4;
+    for (const CsrLaunch* launch : {&matrix->launch64, &matrix->launch32})
+        if (launch->kernel == CIE_HIP_SPMV_PAIRED && launch->width == width) unroll = launch->unroll;
+    const PairedForm* form = nullptr;
+    const int status = ensurePairedForm(matrix, pairedWide(width), pairedNarrow(width), unroll, &form);
+    if (status != CIE_HIP_OK) return status;
+    if (wideBlocks) *wideBlocks = form->blocks[0];
+    if (narrowBlocks) *narrowBlocks = form->blocks[1];
+    return CIE_HIP_OK;
+}
+
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll) {
     if (!matrix || (precision != 32 && precision != 64))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_get_launch: invalid matrix or precision");
@@ -926,7 +959,7 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
     if (precision == 32 && !matrix->entries32)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: matrix has no FP32 plane");
     if (!(kernels & ((1 << CIE_HIP_SPMV_VECTOR) | (1 << CIE_HIP_SPMV_STAGED) | (1 << CIE_HIP_SPMV_TILED) |
-                     (1 << CIE_HIP_SPMV_BLOCKS) | (1 << CIE_HIP_SPMV_SORTED))))
+                     (1 << CIE_HIP_SPMV_BLOCKS) | (1 << CIE_HIP_SPMV_SORTED) | (1 << CIE_HIP_SPMV_PAIRED))))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_autotune: empty kernel mask");
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
@@ -982,6 +1015,26 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
                     candidates.push_back(c);
                 }
             }
+        }
+    }
+    if (kernels & (1 << CIE_HIP_SPMV_PAIRED)) {
+        // Each supported pair and unroll whose form fits the matrix.
+        for (const auto& [width, unroll] : std::vector<std::pair<int, int>>{
+#define CIEHIP_PAIRED_ENTRY(W1, W2, U) {10 * W1 + W2, U},
+                 CIEHIP_PAIRED(CIEHIP_PAIRED_ENTRY)
+#undef CIEHIP_PAIRED_ENTRY
+             }) {
+            if (matrix->columns < pairedWide(width)) continue;
+            const PairedForm* form = nullptr;
+            const int status = ensurePairedForm(matrix, pairedWide(width), pairedNarrow(width), unroll, &form);
+            if (status == CIE_HIP_ERROR_INVALID_ARGUMENT) continue;  // the matrix does not fit this form
+            if (status != CIE_HIP_OK) return status;
+            CsrLaunch c;
+            c.kernel = CIE_HIP_SPMV_PAIRED;
+            c.width = width;
+            c.unroll = unroll;
+            c.block = kSpmvBlock;
+            candidates.push_back(c);
         }
     }
     if (kernels & (1 << CIE_HIP_SPMV_VECTOR)) {

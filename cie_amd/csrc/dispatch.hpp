@@ -77,9 +77,36 @@ inline bool validBlocks(int width, int unroll) {
     }
 }
 
+// Wide width, narrow width and blocks per lane and width of the paired-tile
+// kernel; its launch width is 10 * wide + narrow.
+#define CIEHIP_PAIRED(X) X(5, 3, 2) X(5, 3, 4) X(3, 2, 2) X(3, 2, 4)
+
+inline bool validPaired(int width, int unroll) {
+    switch (width * 32 + unroll) {
+#define CIEHIP_PAIRED_VALID(W1, W2, U) case (10 * W1 + W2) * 32 + U: return true;
+        CIEHIP_PAIRED(CIEHIP_PAIRED_VALID)
+#undef CIEHIP_PAIRED_VALID
+    default: return false;
+    }
+}
+
+inline int pairedWide(int width) { return width / 10; }
+inline int pairedNarrow(int width) { return width % 10; }
+
 // Calls f(Walker{}) with the row walker the launch selects; false if unsupported.
 template <class TV, class TM, class F>
 bool withWalker(const CsrLaunch& launch, F&& f) {
+    if (launch.kernel == CIE_HIP_SPMV_PAIRED) {
+        switch (launch.width * 32 + launch.unroll) {
+#define CIEHIP_PAIRED_CASE(W1, W2, U)                                  \
+    case (10 * W1 + W2) * 32 + U:                                      \
+        f(PairedBlockTiles<TV, TM, U, kSpmvBlock, W1, W2>{});          \
+        return true;
+            CIEHIP_PAIRED(CIEHIP_PAIRED_CASE)
+#undef CIEHIP_PAIRED_CASE
+        default: return false;
+        }
+    }
     if (launch.kernel == CIE_HIP_SPMV_SORTED) {
         switch (launch.width * 32 + launch.unroll) {
 #define CIEHIP_SORTED_CASE(W, U)                                  \
@@ -172,8 +199,29 @@ int prepareLaunch(cie_hip_csr m, const CsrLaunch& launch, Kernel kernel, RowSour
     source.sortedValues64 = nullptr;
     source.sortedValues32 = nullptr;
     source.sortedBlocks = 0;
+    source.pairedBegins = nullptr;
+    source.pairedBases = nullptr;
+    source.pairedWide = 0;
     const int64_t rows = m->rows;
-    if (launch.kernel == CIE_HIP_SPMV_SORTED) {  // the column-block form with column-sorted tiles
+    if (launch.kernel == CIE_HIP_SPMV_PAIRED) {  // wide and narrow column blocks in column-sorted tiles
+        const PairedForm* form = nullptr;
+        const int status = ensurePairedForm(m, pairedWide(launch.width), pairedNarrow(launch.width), launch.unroll,
+                                            &form);
+        if (status != CIE_HIP_OK) return status;
+        source.rowExtents = form->extents;
+        source.columnIndices = nullptr;
+        source.tiles = form->tiles;
+        source.tileCount = form->tileCount;
+        source.sortedKeys = form->keys;
+        source.sortedValues64 = form->values64;
+        source.sortedValues32 = form->values32;
+        source.sortedBlocks = form->blocks[0] + form->blocks[1];
+        source.pairedBegins = form->begins;
+        source.pairedBases = form->bases;
+        source.pairedWide = form->blocks[0];
+        source.param = 1;
+        grid = std::max(1, residentRound ? std::min(form->tileCount, cap) : form->tileCount);
+    } else if (launch.kernel == CIE_HIP_SPMV_SORTED) {  // the column-block form with column-sorted tiles
         cie_hip_csr blocks = nullptr;
         int status = ensureColumnBlocks(m, launch.width, &blocks);
         if (status != CIE_HIP_OK) return status;

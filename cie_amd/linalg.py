@@ -38,7 +38,7 @@ def _torch():
 
 
 _KERNEL_NAMES = {_lib.SPMV_VECTOR: "vector", _lib.SPMV_STAGED: "staged", _lib.SPMV_TILED: "tiled",
-                 _lib.SPMV_BLOCKS: "blocks", _lib.SPMV_SORTED: "sorted"}
+                 _lib.SPMV_BLOCKS: "blocks", _lib.SPMV_SORTED: "sorted", _lib.SPMV_PAIRED: "paired"}
 
 
 def _kernel_code(name):
@@ -134,13 +134,21 @@ class CSRMatrix:
               "cie_hip_csr_column_blocks")
         return blocks.value, value.value
 
+    def paired_blocks(self, width):
+        """(wide blocks, narrow blocks) of the paired form with width 10 * wide + narrow (53 or 32)."""
+        wide, narrow = c_int64(), c_int64()
+        check(library().cie_hip_csr_paired_blocks(self.handle, width, byref(wide), byref(narrow)),
+              "cie_hip_csr_paired_blocks")
+        return wide.value, narrow.value
+
     def set_launch(self, precision, kernel, width, unroll):
         code = _kernel_code(kernel)
         check(library().cie_hip_csr_set_launch(self.handle, precision, code, width, unroll), "cie_hip_csr_set_launch")
 
-    def autotune(self, precision=64, repetitions=10, kernels=("tiled", "blocks", "sorted", "staged", "vector")):
-        """Time the candidate launches of `kernels` ("staged", "vector", "tiled", "blocks", "sorted") and keep the
-        fastest."""
+    def autotune(self, precision=64, repetitions=10,
+                 kernels=("tiled", "blocks", "sorted", "paired", "staged", "vector")):
+        """Time the candidate launches of `kernels` ("staged", "vector", "tiled", "blocks", "sorted", "paired") and
+        keep the fastest."""
         mask = sum(1 << _kernel_code(k) for k in kernels)
         best = c_double()
         check(library().cie_hip_csr_autotune(self.handle, precision, mask, repetitions, byref(best)),

@@ -200,6 +200,20 @@ int cie_hip_csr_update_entries(cie_hip_csr matrix, const double* entries);
  *     tile's block columns to span fewer than 2^20 columns (else the launch
  *     fails and the autotuner skips it). */
 #define CIE_HIP_SPMV_SORTED 4
+/*   CIE_HIP_SPMV_PAIRED: rows covered by 1 x wide and 1 x narrow blocks of
+ *     adjacent columns (a block is wide where a narrow one would leave
+ *     entries of the wide span to a further block), in tiles of whole rows of
+ *     at most 256 rows and 256 * unroll blocks of each width, each width's
+ *     blocks column-sorted within the tile with the sorted tiles' 32-bit key;
+ *     width = 10 * wide + narrow: 53 (1 x 5 and 1 x 3) or 32 (1 x 3 and
+ *     1 x 2), unroll 2 or 4. Rows whose entries come in runs of both lengths
+ *     (3D Q2 stiffness: runs of 5 and 3 columns) need no padding: 16.6% fewer
+ *     FP32 bytes than 1 x 3 blocks on 3D Q2 64^3 (14% in FP64). A matrix with
+ *     a row of more than 256 * unroll blocks of one width, or a tile whose
+ *     blocks of one width span 2^20 columns or more, does not fit
+ *     (CIE_HIP_ERROR_INVALID_ARGUMENT at the first product); the autotuner
+ *     skips it there. */
+#define CIE_HIP_SPMV_PAIRED 5
 /* The tiled kernel reads a narrow copy of the column indices when every run of
  * 4096 consecutive entries spans at most 65535 columns (banded FEM matrices in
  * their natural node order do): 16-bit offsets from each run's smallest
@@ -213,6 +227,12 @@ int cie_hip_csr_column_bytes(cie_hip_csr matrix, int* bytes);
  * *bytes the bytes per block column it reads (2 or 4, by the rule above). An
  * introspection call with no counterpart in the reference. */
 int cie_hip_csr_column_blocks(cie_hip_csr matrix, int width, int64_t* blocks, int* bytes);
+/* The block counts of the paired form with width = 10 * wide + narrow (built
+ * on first use, with unroll 4 unless a launch of the matrix uses that pair):
+ * blocks of each width (the form's bytes per product are
+ * (wide * v + 4) * wideBlocks + (narrow * v + 4) * narrowBlocks for v-byte
+ * values, plus row extents, x and y). */
+int cie_hip_csr_paired_blocks(cie_hip_csr matrix, int width, int64_t* wideBlocks, int64_t* narrowBlocks);
 int cie_hip_csr_set_launch(cie_hip_csr matrix, int precision, int kernel, int width, int unroll);
 int cie_hip_csr_get_launch(cie_hip_csr matrix, int precision, int* kernel, int* width, int* unroll);
 int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int repetitions, double* bestSeconds);

@@ -266,17 +266,22 @@ def q2_64():
 MIXED_SOLUTION_TOLERANCE = 1e-8
 
 
-@pytest.mark.parametrize("launch", ["sorted", "sorted-4/8", "autotuned"])
+@pytest.mark.parametrize("launch", ["sorted", "sorted-4/8", "paired-2/4", "paired-4/4", "autotuned"])
 def test_full_size_q2_64_fp64_and_mixed(hip, torch_gpu, q2_64, launch):
     """3D Q2 64^3: the FP64 solution equals the oracle's (threaded CPU Jacobi-PCG) to 1e-10 relative with the
     iteration count within one; the mixed solution reaches FP64 true relative residual 1e-10 and agrees with
     the oracle within the refinement tolerance. "sorted" pins sorted 1 x 3 tiles of 2048 blocks for both
-    precisions, "sorted-4/8" the launches the benchmark selects (FP64 tiles of 1024 blocks, FP32 of 2048)."""
+    precisions, "sorted-4/8" FP64 tiles of 1024 blocks and FP32 of 2048, "paired-U/V" paired 1 x 5 / 1 x 3
+    tiles of 256 U (FP64) and 256 V (FP32) blocks per width, the launches the benchmark chooses among."""
     w, x_ref, s_ref = q2_64
     matrix = hip.CSRMatrix(w.matrix)
     if launch.startswith("sorted"):
         matrix.set_launch(64, "sorted", 3, 4 if launch == "sorted-4/8" else 8)
         matrix.set_launch(32, "sorted", 3, 8)
+    elif launch.startswith("paired"):
+        unroll64, unroll32 = map(int, launch.split("-")[1].split("/"))
+        matrix.set_launch(64, "paired", 53, unroll64)
+        matrix.set_launch(32, "paired", 53, unroll32)
     else:
         matrix.autotune(64, 3)
         matrix.autotune(32, 3)

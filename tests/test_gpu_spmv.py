@@ -252,6 +252,16 @@ def test_spmv_full_size_q2_64(hip, torch_gpu):
                 plain = {64: got64, 32: got32}
             else:
                 assert np.array_equal(got64, plain[64]) and np.array_equal(got32, plain[32]), (width, unroll)
+    # paired 1 x 5 / 1 x 3 tiles (the runs of 5 and 3 without padding) and 1 x 3 / 1 x 2 tiles
+    for width, unroll in ((53, 2), (53, 4), (32, 4)):
+        matrix.set_launch(64, "paired", width, unroll)
+        matrix.set_launch(32, "paired", width, unroll)
+        y = space.makeVector(w.matrix.rows)
+        hip.HIPCSROperator(matrix, space).product(0, to_dev(torch_gpu, w.rhs), 1, y)
+        assert np.all(np.abs(y.cpu().numpy() - expected) <= bound), ("paired", width, unroll)
+        y32 = space32.makeVector(w.matrix.rows)
+        hip.HIPCSROperator(matrix, space32).product(0, to_dev(torch_gpu, w.rhs, torch_gpu.float32), 1, y32)
+        assert np.all(np.abs(y32.cpu().numpy() - expected) <= bound32), ("paired", width, unroll)
 
 
 def test_spmv_shape_error(hip, torch_gpu):
