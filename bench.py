@@ -66,6 +66,14 @@ def format_bytes(matrix, precision):
     return spmv_bytes(matrix.rows, matrix.nnz, value, index_bytes(matrix, precision))
 
 
+def pin_launches(matrix, spec):
+    """Sets the launches of --launch: KERNEL,WIDTH,UNROLL for both precisions, or FP64/FP32."""
+    specs = spec.split("/")
+    for precision, one in zip((64, 32), specs if len(specs) == 2 else specs * 2):
+        kernel, width, unroll = one.split(",")
+        matrix.set_launch(precision, kernel, int(width), int(unroll))
+
+
 def time_spmv(torch, linalg, matrix, precision, repetitions=50):
     import ctypes
 
@@ -199,7 +207,8 @@ def main():
     parser.add_argument("--cpu-threads", type=int, default=0)
     parser.add_argument("--no-profile", action="store_true", help="skip the in-solve SpMV event timing")
     parser.add_argument("--launch", default=None,
-                        help="pcg / mixed-q4: pin the SpMV launches instead of autotuning (profiling runs): "
+                        help="pcg / mixed-q4 / adjoint / pmg: pin the SpMV launches instead of autotuning and "
+                             "selecting them (profiling runs): "
                              "KERNEL,WIDTH,UNROLL for both precisions, or FP64/FP32 (e.g. sorted,3,4/sorted,3,8)")
     parser.add_argument("--batch-tiles", choices=("auto", "tiled", "blocks", "sorted"), default="auto",
                         help="batch workload: the tile format (default: the autotuner's)")
@@ -258,10 +267,7 @@ def main():
     # (narrow columns, column blocks, sorted tiles) and frees the ones no launch keeps.
     t0 = time.perf_counter()
     if args.launch:  # a pinned launch (profiling runs), built but not timed against others
-        specs = args.launch.split("/")
-        for precision, spec in zip((64, 32), specs if len(specs) == 2 else specs * 2):
-            kernel, width, unroll = spec.split(",")
-            matrix.set_launch(precision, kernel, int(width), int(unroll))
+        pin_launches(matrix, args.launch)
         tune64 = time_spmv(torch, linalg, matrix, 64, repetitions=3)
     else:
         tune64 = matrix.autotune(64, 10, kernels=("staged", "vector", "tiled", "blocks", "sorted", "paired"))
@@ -655,8 +661,11 @@ def bench_adjoint(args):
     g, gt = w.extra["log_conductivity"], w.extra["target_log_conductivity"]
     problem = adjoint.LagrangeProblem(3, 2, n, w.rhs, np.zeros(w.matrix.rows), tolerance=w.relative_tolerance)
     problem.assemble(gt)
-    problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))
-    problem.matrix.autotune(32, 5, kernels=("tiled", "blocks", "sorted", "paired"))  # the mixed-solve variant's FP32 plane
+    if args.launch:  # pinned (profiling runs)
+        pin_launches(problem.matrix, args.launch)
+    else:
+        problem.matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))
+        problem.matrix.autotune(32, 5, kernels=("tiled", "blocks", "sorted", "paired"))  # the mixed variant's FP32 plane
     ut, _ = problem.solve(problem.rhs)
     problem.target = ut
     context = problem.context
@@ -837,7 +846,10 @@ def bench_pmg(args):
     rows, nnz = csr.rows, csr.nnz
     context = linalg.Context.default()
     matrix = linalg.CSRMatrix(csr, context)
-    matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))  # the finest grid's sweeps and updates use this launch
+    if args.launch:  # pinned (profiling runs)
+        pin_launches(matrix, args.launch)
+    else:  # the finest grid's sweeps and updates use this launch
+        matrix.autotune(64, 5, kernels=("tiled", "blocks", "sorted", "paired"))
     mask = linalg.Mask(mask_values, context)
     t0 = time.perf_counter()
     pmg = linalg.PMultigrid(matrix, mask, order)
@@ -867,8 +879,9 @@ def bench_pmg(args):
     t0 = time.perf_counter()
     # The masked grids and the coarse matrix keep entry tiles: selected the same way (pmg.grid(g).set_launch),
     # no format took more than 1% off the cycle, inside the run-to-run spread.
-    selection = {"finest": select(matrix, (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16), ("paired", 53, 2),
-                                           ("paired", 53, 4)))}
+    selection = "pinned " + args.launch if args.launch else {
+        "finest": select(matrix, (("sorted", 3, 4), ("sorted", 3, 8), ("sorted", 3, 16), ("paired", 53, 2),
+                                  ("paired", 53, 4)))}
     selection_seconds = time.perf_counter() - t0
     for _ in range(args.warmup):
         x.zero_()
