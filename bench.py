@@ -210,7 +210,7 @@ def main():
                         help="pcg / mixed-q4 / adjoint / pmg: pin the SpMV launches instead of autotuning and "
                              "selecting them (profiling runs): "
                              "KERNEL,WIDTH,UNROLL for both precisions, or FP64/FP32 (e.g. sorted,3,4/sorted,3,8)")
-    parser.add_argument("--batch-tiles", choices=("auto", "tiled", "blocks", "sorted"), default="auto",
+    parser.add_argument("--batch-tiles", choices=("auto", "tiled", "blocks", "sorted", "paired"), default="auto",
                         help="batch workload: the tile format (default: the autotuner's)")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="end with a 1 GiB device-to-device copy (scale reference for rocprofv3 byte counters)")
@@ -562,8 +562,9 @@ def _profiled_mean_seconds(stats, kernel):
     return None
 
 
-_BATCH_WALKER = {"tiled": 0, "blocks": 3, "sorted": -3}  # the walker code of the batch's tile format
-_BATCH_TILES = {"tiled": "entries", "blocks": "1 x 3 column blocks", "sorted": "1 x 3 column blocks, column-sorted"}
+_BATCH_WALKER = {"tiled": 0, "blocks": 3, "sorted": -3, "paired": 53}  # the walker code of the batch's tile format
+_BATCH_TILES = {"tiled": "entries", "blocks": "1 x 3 column blocks", "sorted": "1 x 3 column blocks, column-sorted",
+                "paired": "paired 1 x 5 / 1 x 3 column blocks, column-sorted"}
 
 
 def bench_batch(args):

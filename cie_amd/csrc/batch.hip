@@ -46,14 +46,34 @@ constexpr int kGraphIterations = 8;
 constexpr int kDefaultTileRows = 256;
 
 constexpr int kBatchBlockWidth = 3;  // column-block width of a batch that walks blocks
+constexpr int kBatchPaired = 53;     // walker code of paired 1 x 5 / 1 x 3 tiles
+// Blocks per lane and width of paired tiles: a tile holds at most 256 rows, and
+// 2D rows hold few blocks (about 2 of each width on 2D Q2), so 512 per width
+// fill the tiles the row cap allows.
+constexpr int kBatchPairedUnroll = 2;
 
-// The tile walker of a batch: matrix entries (W = 0), 1 x W column blocks
-// (W > 0) or 1 x -W column blocks in column-sorted tiles (W < 0).
+// The tile walker of a batch by its code: matrix entries (0), 1 x 3 column
+// blocks (3), the same blocks in column-sorted tiles (-3), or paired 1 x 5 /
+// 1 x 3 tiles (kBatchPaired).
 template <class T, int W>
-using BatchTiles = std::conditional_t<
-    W == 0, BlockTiles<T, T, kTileUnroll, kTileBlock>,
-    std::conditional_t<(W > 0), ColumnBlockTiles<T, T, kTileUnroll, kTileBlock, (W > 0 ? W : 2)>,
-                       SortedBlockTiles<T, T, kTileUnroll, kTileBlock, (W < 0 ? -W : 2)>>>;
+struct BatchWalker {
+    using Type = BlockTiles<T, T, kTileUnroll, kTileBlock>;
+};
+template <class T>
+struct BatchWalker<T, kBatchBlockWidth> {
+    using Type = ColumnBlockTiles<T, T, kTileUnroll, kTileBlock, kBatchBlockWidth>;
+};
+template <class T>
+struct BatchWalker<T, -kBatchBlockWidth> {
+    using Type = SortedBlockTiles<T, T, kTileUnroll, kTileBlock, kBatchBlockWidth>;
+};
+template <class T>
+struct BatchWalker<T, kBatchPaired> {
+    using Type = PairedBlockTiles<T, T, kBatchPairedUnroll, kTileBlock, kBatchPaired / 10, kBatchPaired % 10>;
+};
+template <class T, int W>
+using BatchTiles = typename BatchWalker<T, W>::Type;
+static_assert(BatchTiles<double, kBatchPaired>::kCapacity <= kTileEntries, "paired tiles stage in the tile's LDS");
 
 struct SystemState {
     double rho, alpha, beta, rr, initialNorm, absolute, relative, updateNorm;
@@ -529,6 +549,8 @@ struct cie_hip_batch_s {
     int blockWidth = 0;                 // 0: tiles of entries; kBatchBlockWidth: tiles of column blocks
     bool sortedTiles = false;           // tiles of column blocks stored in column order (blockWidth > 0)
     SortedCopy sorted;                  // the sorted copy of the block form over the batch's tiles
+    bool pairedTiles = false;           // paired 1 x 5 / 1 x 3 tiles (blockWidth 0)
+    PairedForm* paired = nullptr;       // the paired form cut into the batch's tiles
     int graphKey[5] = {0, 0, 0, 0, 0};  // what the captured graph was built for
     const void* graphPointers[3] = {nullptr, nullptr, nullptr};  // b, x, inverse diagonal
     BatchSettings graphSettings{};      // the settings its kernels were captured with (passed by value)
@@ -543,6 +565,7 @@ void releaseGraph(cie_hip_batch batch) {
 
 // The walker code of the batch's tile format (the W of BatchTiles).
 int walkerCode(const cie_hip_batch_s* batch) {
+    if (batch->pairedTiles) return kBatchPaired;
     return batch->blockWidth ? (batch->sortedTiles ? -batch->blockWidth : batch->blockWidth) : 0;
 }
 
@@ -554,43 +577,67 @@ void freeDevice(T*& pointer) {
 
 // Cuts every system into block tiles (<= kTileEntries entries, <= tileRows[s]
 // rows; a longer row is a tile of its own) and into chunks of kChunkRows rows.
+// Paired tiles are cut by the paired form's builder with the same rules
+// (<= tileRows[s] rows, <= 256 * kBatchPairedUnroll blocks of each width;
+// a matrix with a longer row does not fit).
 int buildWork(cie_hip_batch batch) {
-    // Tiles are cut over the rows' entries, or over their column blocks (as many per row as the block form holds).
-    cie_hip_csr form = batch->matrix;
-    if (batch->blockWidth) {
-        const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &form);
-        if (status != CIE_HIP_OK) return status;
-    }
-    const int32_t* extents = form->hostExtents.data();
     std::vector<int2> tiles;
     std::vector<int32_t> systemTiles(size_t(batch->systems) + 1, 0);
     std::vector<int32_t> chunkRows;
     std::vector<int32_t> systemChunks(size_t(batch->systems) + 1, 0);
-    for (int s = 0; s < batch->systems; ++s) {
-        systemTiles[s] = int32_t(tiles.size());
-        systemChunks[s] = int32_t(chunkRows.size());
-        const int lo = int(batch->offsets[s]), hi = int(batch->offsets[s + 1]);
-        const int cap = batch->tileRows[s];
-        int row = lo;
-        while (row < hi) {
-            const int first = row;
-            if (extents[row + 1] - extents[row] > kTileEntries) {
-                ++row;
-            } else {
-                while (row < hi && row - first < cap && extents[row + 1] - extents[first] <= kTileEntries) ++row;
-            }
-            tiles.push_back(make_int2(first, extents[first]));
-        }
-        for (int first = lo; first < hi; first += kChunkRows) chunkRows.push_back(first);
-    }
     const int rows = int(batch->offsets[batch->systems]);
-    systemTiles[batch->systems] = int32_t(tiles.size());
+    if (batch->pairedTiles) {
+        releasePairedForm(batch->context, batch->paired);
+        batch->paired = new PairedForm;
+        batch->paired->wide = kBatchPaired / 10;
+        batch->paired->narrow = kBatchPaired % 10;
+        batch->paired->unroll = kBatchPairedUnroll;
+        const int status = buildPairedForm(batch->matrix, *batch->paired,
+                                           PairedCut{batch->offsets.data(), batch->tileRows.data(), batch->systems,
+                                                     &systemTiles});
+        if (status != CIE_HIP_OK) return status;
+        if (batch->paired->state < 0) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, batch->paired->error);
+        batch->tileCount = batch->paired->tileCount;
+    } else {
+        // Tiles are cut over the rows' entries, or over their column blocks (as many per row as the block form holds).
+        cie_hip_csr form = batch->matrix;
+        if (batch->blockWidth) {
+            const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &form);
+            if (status != CIE_HIP_OK) return status;
+        }
+        const int32_t* extents = form->hostExtents.data();
+        for (int s = 0; s < batch->systems; ++s) {
+            systemTiles[s] = int32_t(tiles.size());
+            const int lo = int(batch->offsets[s]), hi = int(batch->offsets[s + 1]);
+            const int cap = batch->tileRows[s];
+            int row = lo;
+            while (row < hi) {
+                const int first = row;
+                if (extents[row + 1] - extents[row] > kTileEntries) {
+                    ++row;
+                } else {
+                    while (row < hi && row - first < cap && extents[row + 1] - extents[first] <= kTileEntries) ++row;
+                }
+                tiles.push_back(make_int2(first, extents[first]));
+            }
+        }
+        systemTiles[batch->systems] = int32_t(tiles.size());
+        batch->tileCount = int(tiles.size());
+        tiles.push_back(make_int2(rows, extents[rows]));
+    }
+    for (int s = 0; s < batch->systems; ++s) {
+        systemChunks[s] = int32_t(chunkRows.size());
+        for (int first = int(batch->offsets[s]); first < int(batch->offsets[s + 1]); first += kChunkRows)
+            chunkRows.push_back(first);
+    }
     systemChunks[batch->systems] = int32_t(chunkRows.size());
-    batch->tileCount = int(tiles.size());
     batch->chunkCount = int(chunkRows.size());
-    tiles.push_back(make_int2(rows, extents[rows]));
     chunkRows.push_back(rows);
     releaseSortedCopy(batch->context, batch->sorted);  // it follows the tiles
+    if (!batch->pairedTiles) {
+        releasePairedForm(batch->context, batch->paired);
+        batch->paired = nullptr;
+    }
     freeDevice(batch->tiles);
     freeDevice(batch->chunkRows);
     freeDevice(batch->tilePartials);
@@ -601,11 +648,13 @@ int buildWork(cie_hip_batch batch) {
         CIE_HIP_CHECK(hipMalloc(&work->tileList, sizeof(int2) * std::max(1, batch->tileCount)));
         CIE_HIP_CHECK(hipMalloc(&work->chunkList, sizeof(int2) * std::max(1, batch->chunkCount)));
     }
-    CIE_HIP_CHECK(hipMalloc(&batch->tiles, sizeof(int2) * tiles.size()));
+    if (!tiles.empty()) {  // paired tiles live in the paired form
+        CIE_HIP_CHECK(hipMalloc(&batch->tiles, sizeof(int2) * tiles.size()));
+        CIE_HIP_CHECK(hipMemcpy(batch->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+    }
     CIE_HIP_CHECK(hipMalloc(&batch->chunkRows, sizeof(int32_t) * chunkRows.size()));
     CIE_HIP_CHECK(hipMalloc(&batch->tilePartials, 2 * sizeof(double) * std::max(1, batch->tileCount)));
     CIE_HIP_CHECK(hipMalloc(&batch->chunkPartials, 2 * sizeof(double) * std::max(1, batch->chunkCount)));
-    CIE_HIP_CHECK(hipMemcpy(batch->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
     CIE_HIP_CHECK(hipMemcpy(batch->chunkRows, chunkRows.data(), sizeof(int32_t) * chunkRows.size(),
                             hipMemcpyHostToDevice));
     CIE_HIP_CHECK(hipMemcpy(batch->systemTiles, systemTiles.data(), sizeof(int32_t) * systemTiles.size(),
@@ -618,6 +667,29 @@ int buildWork(cie_hip_batch batch) {
 
 int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
     cie_hip_csr m = batch->matrix;
+    if (batch->pairedTiles) {  // the paired form, its values current
+        PairedForm& form = *batch->paired;
+        const int status = refreshPairedValues(m, form);
+        if (status != CIE_HIP_OK) return status;
+        RowSource& source = layout->source;
+        source = RowSource{};
+        source.rows = int(m->rows);
+        source.param = 1;
+        source.rowExtents = form.extents;
+        source.tiles = form.tiles;
+        source.tileCount = form.tileCount;
+        source.sortedKeys = form.keys;
+        source.sortedValues64 = form.values64;
+        source.sortedValues32 = form.values32;
+        source.sortedBlocks = form.blocks[0] + form.blocks[1];
+        source.pairedBegins = form.begins;
+        source.pairedBases = form.bases;
+        source.pairedWide = form.blocks[0];
+        layout->systemTiles = batch->systemTiles;
+        layout->systemChunks = batch->systemChunks;
+        layout->chunkRows = batch->chunkRows;
+        return CIE_HIP_OK;
+    }
     if (batch->blockWidth) {  // the block form, its values current
         const int status = ensureColumnBlocks(batch->matrix, batch->blockWidth, &m);
         if (status != CIE_HIP_OK) return status;
@@ -630,6 +702,9 @@ int batchLayout(cie_hip_batch batch, BatchLayout* layout) {
     source.sortedValues64 = nullptr;
     source.sortedValues32 = nullptr;
     source.sortedBlocks = 0;
+    source.pairedBegins = nullptr;
+    source.pairedBases = nullptr;
+    source.pairedWide = 0;
     if (batch->blockWidth && batch->sortedTiles) {
         status = ensureSortedCopy(m, batch->tiles, batch->tileCount, kTileEntries, batch->sorted);
         if (status != CIE_HIP_OK) return status;
@@ -679,6 +754,7 @@ void releaseBatch(cie_hip_batch batch) {
     }
     releaseGraph(batch);
     releaseSortedCopy(batch->context, batch->sorted);
+    releasePairedForm(batch->context, batch->paired);
     freeDevice(batch->tiles);
     freeDevice(batch->systemTiles);
     freeDevice(batch->chunkRows);
@@ -706,12 +782,18 @@ void releaseBatch(cie_hip_batch batch) {
 // autotuner keeps it.
 int setBlockWidth(cie_hip_batch batch, int width, bool sorted = false) {
     sorted = sorted && width;
+    bool rebuild = false;
+    if (batch->pairedTiles) {  // leaving paired tiles: re-cut
+        batch->pairedTiles = false;
+        releaseGraph(batch);
+        rebuild = true;
+    }
     if (sorted != batch->sortedTiles) {
         releaseSortedCopy(batch->context, batch->sorted);
         batch->sortedTiles = sorted;
         releaseGraph(batch);
     }
-    if (width == batch->blockWidth) return CIE_HIP_OK;
+    if (width == batch->blockWidth) return rebuild ? buildWork(batch) : CIE_HIP_OK;
     cie_hip_csr m = batch->matrix;
     if (width) {
         cie_hip_csr form = nullptr;
@@ -724,11 +806,37 @@ int setBlockWidth(cie_hip_batch batch, int width, bool sorted = false) {
     return buildWork(batch);  // the tiles always match the walker
 }
 
+// Switches the batch to paired 1 x 5 / 1 x 3 tiles (built over the batch's
+// own tile cut); fails with CIE_HIP_ERROR_INVALID_ARGUMENT where the matrix
+// does not fit the paired form (the batch then keeps entry tiles).
+int setPairedTiles(cie_hip_batch batch) {
+    if (batch->pairedTiles) return CIE_HIP_OK;
+    int status = setBlockWidth(batch, 0, false);
+    if (status != CIE_HIP_OK) return status;
+    batch->pairedTiles = true;
+    releaseGraph(batch);
+    status = buildWork(batch);
+    if (status != CIE_HIP_OK) {
+        const std::string message = cie_hip_last_error();
+        batch->pairedTiles = false;
+        if (buildWork(batch) != CIE_HIP_OK) return CIE_HIP_ERROR_HIP;
+        return setError(status, message);
+    }
+    return CIE_HIP_OK;
+}
+
+// The batch in the tile format of walker code `code`.
+int setTileFormat(cie_hip_batch batch, int code) {
+    if (code == kBatchPaired) return setPairedTiles(batch);
+    return setBlockWidth(batch, std::abs(code), code < 0);
+}
+
 // Calls f(integral_constant<W>) for the batch's walker code.
 template <class F>
 void withWalkerCode(int code, F&& f) {
     if (code == kBatchBlockWidth) f(std::integral_constant<int, kBatchBlockWidth>{});
     else if (code == -kBatchBlockWidth) f(std::integral_constant<int, -kBatchBlockWidth>{});
+    else if (code == kBatchPaired) f(std::integral_constant<int, kBatchPaired>{});
     else f(std::integral_constant<int, 0>{});
 }
 
@@ -1071,6 +1179,7 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
             codes.push_back(-kBatchBlockWidth);
         }
     }
+    if (m->columns >= kBatchPaired / 10) codes.push_back(kBatchPaired);
     // The time of `tiles` tiles from `t0` on (one launch, the best of `repetitions`), or a negative status.
     const auto timeTiles = [&](int code, const BatchLayout& layout, int t0, int tiles, float* best) -> int {
         *best = std::numeric_limits<float>::infinity();
@@ -1095,7 +1204,11 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     std::vector<int32_t> bestChoice(size_t(batch->systems), kDefaultTileRows);
     int status = CIE_HIP_OK;
     for (int code : codes) {
-        status = setBlockWidth(batch, std::abs(code), code < 0);
+        status = setTileFormat(batch, code);
+        if (status == CIE_HIP_ERROR_INVALID_ARGUMENT && code == kBatchPaired) {  // does not fit the paired form
+            status = CIE_HIP_OK;
+            continue;
+        }
         if (status) break;
         std::vector<float> best(size_t(batch->systems), std::numeric_limits<float>::infinity());
         std::vector<int32_t> choice(size_t(batch->systems), kDefaultTileRows);
@@ -1104,6 +1217,10 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
         const auto judge = [&](const std::vector<int32_t>& rows, bool perSystem) -> int {
             batch->tileRows = rows;
             int result = buildWork(batch);
+            if (result && code == kBatchPaired && batch->paired && batch->paired->state < 0) {  // does not fit
+                skipped = true;
+                return CIE_HIP_OK;
+            }
             if (result) return result;
             BatchLayout layout{};
             result = batchLayout(batch, &layout);
@@ -1147,25 +1264,31 @@ int cie_hip_batch_autotune(cie_hip_batch batch, int precision, int repetitions) 
     }
     (void)hipFree(buffer);
     if (status) return status;
-    status = setBlockWidth(batch, std::abs(bestCode), bestCode < 0);
-    if (status) return status;
     batch->tileRows = bestChoice;
+    status = setTileFormat(batch, bestCode);
+    if (status) return status;
     return buildWork(batch);
 }
 
 int cie_hip_batch_get_tile_kernel(cie_hip_batch batch, int* kernel) {
     if (!batch || !kernel) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_get_tile_kernel: null argument");
-    *kernel = !batch->blockWidth ? CIE_HIP_SPMV_TILED : batch->sortedTiles ? CIE_HIP_SPMV_SORTED : CIE_HIP_SPMV_BLOCKS;
+    *kernel = batch->pairedTiles ? CIE_HIP_SPMV_PAIRED
+              : !batch->blockWidth ? CIE_HIP_SPMV_TILED
+              : batch->sortedTiles ? CIE_HIP_SPMV_SORTED
+                                   : CIE_HIP_SPMV_BLOCKS;
     return CIE_HIP_OK;
 }
 
 int cie_hip_batch_set_tile_kernel(cie_hip_batch batch, int kernel) {
-    if (!batch || (kernel != CIE_HIP_SPMV_TILED && kernel != CIE_HIP_SPMV_BLOCKS && kernel != CIE_HIP_SPMV_SORTED))
+    if (!batch || (kernel != CIE_HIP_SPMV_TILED && kernel != CIE_HIP_SPMV_BLOCKS && kernel != CIE_HIP_SPMV_SORTED &&
+                   kernel != CIE_HIP_SPMV_PAIRED))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT,
-                        "cie_hip_batch_set_tile_kernel: kernel must be CIE_HIP_SPMV_TILED, _BLOCKS or _SORTED");
-    if (kernel != CIE_HIP_SPMV_TILED && batch->matrix->columns < kBatchBlockWidth)
+                        "cie_hip_batch_set_tile_kernel: kernel must be CIE_HIP_SPMV_TILED, _BLOCKS, _SORTED or _PAIRED");
+    const int needed = kernel == CIE_HIP_SPMV_PAIRED ? kBatchPaired / 10 : kBatchBlockWidth;
+    if (kernel != CIE_HIP_SPMV_TILED && batch->matrix->columns < needed)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_set_tile_kernel: fewer columns than a block");
     CIE_HIP_CHECK(hipSetDevice(batch->context->device));
+    if (kernel == CIE_HIP_SPMV_PAIRED) return setPairedTiles(batch);
     return setBlockWidth(batch, kernel == CIE_HIP_SPMV_TILED ? 0 : kBatchBlockWidth, kernel == CIE_HIP_SPMV_SORTED);
 }
 

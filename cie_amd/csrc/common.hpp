@@ -281,6 +281,21 @@ int ensureSortedTiles(cie_hip_csr blocks, int capacity, const TileSet** tiles);
 int ensurePairedForm(cie_hip_csr matrix, int wide, int narrow, int unroll, const PairedForm** form);
 // Frees a paired form's device arrays and the form; waits for the stream.
 void releasePairedForm(cie_hip_context context, PairedForm* form);
+// How a paired form's tiles are cut: never across the row offsets of
+// `systems` systems (null: one system of all rows), at most rowCaps[s] rows
+// per tile (null: 256); `systemTiles` (if given) receives the first tile of
+// each system and the tile count.
+struct PairedCut {
+    const int64_t* offsets = nullptr;
+    const int32_t* rowCaps = nullptr;
+    int systems = 1;
+    std::vector<int32_t>* systemTiles = nullptr;
+};
+// Builds a paired form (form.wide, narrow and unroll set) with its tiles cut
+// as asked; a matrix that does not fit leaves form.state -1 and form.error.
+int buildPairedForm(cie_hip_csr matrix, PairedForm& form, const PairedCut& cut);
+// Refills a built paired form's values if the matrix's entries changed.
+int refreshPairedValues(cie_hip_csr matrix, PairedForm& form);
 // Builds the narrow (16-bit) column plane on first use if every run of
 // entries fits; matrix->narrowState tells whether it did.
 int ensureNarrowColumns(cie_hip_csr matrix);

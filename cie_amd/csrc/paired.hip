@@ -271,9 +271,13 @@ int refill(cie_hip_csr m, PairedForm& form) {
     return CIE_HIP_OK;
 }
 
-// Builds the structure of `form` (cover, tiles, sort); on a matrix that does
-// not fit, leaves state -1 and the reason in form.error.
-int build(cie_hip_csr m, PairedForm& form) {
+}  // namespace
+
+namespace ciehip {
+
+// Builds the structure of `form` (cover, tiles cut as `cut` asks, sort); on a
+// matrix that does not fit, leaves state -1 and the reason in form.error.
+int buildPairedForm(cie_hip_csr m, PairedForm& form, const PairedCut& cut) {
     cie_hip_context ctx = m->context;
     const int rows = int(m->rows), columns = int(m->columns);
     const int perWidth = kSpmvBlock * form.unroll;
@@ -310,17 +314,25 @@ int build(cie_hip_csr m, PairedForm& form) {
     form.blocks[1] = extents[1][size_t(rows)];
     const int64_t total = form.blocks[0] + form.blocks[1];
 
-    // Tiles: consecutive rows within the row and per-width block capacities.
+    // Tiles: consecutive rows of one system, within the row cap and the per-width block capacities.
     std::vector<int2> tiles, begins;
-    for (int row = 0; row < rows;) {
-        const int first = row;
-        while (row < rows && row - first < kSpmvBlock &&
-               extents[0][size_t(row) + 1] - extents[0][size_t(first)] <= perWidth &&
-               extents[1][size_t(row) + 1] - extents[1][size_t(first)] <= perWidth)
-            ++row;
-        tiles.push_back(make_int2(first, merged[size_t(first)]));
-        begins.push_back(make_int2(extents[0][size_t(first)], extents[1][size_t(first)]));
+    if (cut.systemTiles) cut.systemTiles->assign(size_t(cut.systems) + 1, 0);
+    for (int system = 0; system < cut.systems; ++system) {
+        const int lo = cut.offsets ? int(cut.offsets[system]) : 0;
+        const int hi = cut.offsets ? int(cut.offsets[system + 1]) : rows;
+        const int cap = cut.rowCaps ? std::min(cut.rowCaps[system], kSpmvBlock) : kSpmvBlock;
+        if (cut.systemTiles) (*cut.systemTiles)[size_t(system)] = int32_t(tiles.size());
+        for (int row = lo; row < hi;) {
+            const int first = row;
+            while (row < hi && row - first < cap &&
+                   extents[0][size_t(row) + 1] - extents[0][size_t(first)] <= perWidth &&
+                   extents[1][size_t(row) + 1] - extents[1][size_t(first)] <= perWidth)
+                ++row;
+            tiles.push_back(make_int2(first, merged[size_t(first)]));
+            begins.push_back(make_int2(extents[0][size_t(first)], extents[1][size_t(first)]));
+        }
     }
+    if (cut.systemTiles) (*cut.systemTiles)[size_t(cut.systems)] = int32_t(tiles.size());
     tiles.push_back(make_int2(rows, merged[size_t(rows)]));
     begins.push_back(make_int2(extents[0][size_t(rows)], extents[1][size_t(rows)]));
     form.tileCount = int(tiles.size()) - 1;
@@ -393,9 +405,14 @@ int build(cie_hip_csr m, PairedForm& form) {
     return CIE_HIP_OK;
 }
 
-}  // namespace
-
-namespace ciehip {
+int refreshPairedValues(cie_hip_csr m, PairedForm& form) {
+    if (m->entries32 && !form.values32) {  // an FP32 plane added after the form was built
+        CIE_HIP_CHECK(hipMalloc(&form.values32, sizeof(float) * size_t(std::max<int64_t>(
+                                                    1, form.blocks[0] * form.wide + form.blocks[1] * form.narrow))));
+        form.version = std::numeric_limits<uint64_t>::max();
+    }
+    return form.version == m->valuesVersion ? CIE_HIP_OK : refill(m, form);
+}
 
 void releasePairedForm(cie_hip_context context, PairedForm* form) {
     if (!form) return;
@@ -428,22 +445,15 @@ int ensurePairedForm(cie_hip_csr m, int wide, int narrow, int unroll, const Pair
     }
     if (form->state < 0) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, form->error);
     if (form->state == 0) {
-        const int status = build(m, *form);
+        const int status = buildPairedForm(m, *form, PairedCut{});
         if (status != CIE_HIP_OK) {
             *form = PairedForm{wide, narrow, unroll};  // nothing kept: built again on the next use
             return status;
         }
         if (form->state < 0) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, form->error);
     }
-    if (m->entries32 && !form->values32) {  // an FP32 plane added after the form was built
-        CIE_HIP_CHECK(hipMalloc(&form->values32, sizeof(float) * size_t(std::max<int64_t>(
-                                                      1, form->blocks[0] * wide + form->blocks[1] * narrow))));
-        form->version = std::numeric_limits<uint64_t>::max();
-    }
-    if (form->version != m->valuesVersion) {
-        const int status = refill(m, *form);
-        if (status != CIE_HIP_OK) return status;
-    }
+    const int status = refreshPairedValues(m, *form);
+    if (status != CIE_HIP_OK) return status;
     *out = form;
     return CIE_HIP_OK;
 }

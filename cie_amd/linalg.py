@@ -930,7 +930,8 @@ class Batch:
         check(library().cie_hip_batch_set_column_blocks(self.handle, int(width)), "cie_hip_batch_set_column_blocks")
 
     def tile_kernel(self):
-        """Tile format of the batch's products: "tiled" (entries), "blocks" or "sorted" (1 x 3 column blocks)."""
+        """Tile format of the batch's products: "tiled" (entries), "blocks" or "sorted" (1 x 3 column blocks),
+        "paired" (1 x 5 and 1 x 3 column blocks)."""
         kernel = c_int()
         check(library().cie_hip_batch_get_tile_kernel(self.handle, byref(kernel)), "cie_hip_batch_get_tile_kernel")
         return _KERNEL_NAMES[kernel.value]

@@ -349,11 +349,14 @@ int cie_hip_iteration_report(int verbosity, int reportType, const cie_hip_statis
  * chunks of 1024 rows of one system. After every step the finished systems are
  * dropped from the work lists on the device. The tiles hold matrix entries
  * (CIE_HIP_SPMV_TILED), 1 x 3 column blocks (CIE_HIP_SPMV_BLOCKS, at most 2048
- * per tile) or the same blocks in column-sorted tiles (CIE_HIP_SPMV_SORTED,
- * results bitwise those of BLOCKS): cie_hip_batch_autotune times them where
- * the blocks' padding is small and keeps the fastest;
- * cie_hip_batch_set_tile_kernel sets the format, cie_hip_batch_set_column_blocks
- * (width 3, or 0 for entries) the unsorted ones. */
+ * per tile), the same blocks in column-sorted tiles (CIE_HIP_SPMV_SORTED,
+ * results bitwise those of BLOCKS) or paired 1 x 5 / 1 x 3 tiles
+ * (CIE_HIP_SPMV_PAIRED, at most 512 blocks of each width per tile; a matrix
+ * with a longer row does not fit): cie_hip_batch_autotune times them where
+ * the blocks' padding is small (paired tiles wherever they fit) and keeps the
+ * fastest; cie_hip_batch_set_tile_kernel sets the format,
+ * cie_hip_batch_set_column_blocks (width 3, or 0 for entries) the unsorted
+ * ones. */
 typedef struct cie_hip_batch_s* cie_hip_batch;
 int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t systems, const int64_t* offsets,
                          cie_hip_batch* batch);

@@ -13,19 +13,28 @@ def small_batch(count=12, seed=7):
     return systems.q2_quad_batch(count=count, seed=seed, smallest=3, largest=24)
 
 
+def set_format(batch, blocks):
+    """0 / 3: entry or 1 x 3 column-block tiles; "sorted" / "paired": those tile kernels."""
+    if blocks in ("sorted", "paired"):
+        batch.set_tile_kernel(blocks)
+        assert batch.tile_kernel() == blocks
+    else:
+        batch.set_column_blocks(blocks)
+        assert batch.column_blocks() == blocks
+
+
 def per_system_residuals(w, x):
     r = oracle.csr_product(w.matrix, 1.0, x, -1.0, w.rhs.copy(), threads=8)
     offsets = w.extra["offsets"]
     return np.array([np.linalg.norm(r[a:b]) / np.linalg.norm(w.rhs[a:b]) for a, b in zip(offsets[:-1], offsets[1:])])
 
 
-@pytest.mark.parametrize("blocks", [0, 3])
+@pytest.mark.parametrize("blocks", [0, 3, "paired"])
 def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu, blocks):
     w = small_batch()
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
-    batch.set_column_blocks(blocks)
-    assert batch.column_blocks() == blocks
+    set_format(batch, blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     x = torch_gpu.zeros_like(b)
     stats, status, _, summary = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=False)
@@ -42,12 +51,12 @@ def test_batched_fp64_matches_oracle_per_system(hip, torch_gpu, blocks):
 
 
 @pytest.mark.parametrize("restart,blocks,correction", [(False, 0, True), (True, 0, True), (False, 3, True),
-                                                      (False, 3, False)])
+                                                      (False, 3, False), (False, "paired", True)])
 def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks, correction):
     w = small_batch(seed=8)
     matrix = hip.CSRMatrix(w.matrix)
     batch = hip.Batch(matrix, w.extra["offsets"])
-    batch.set_column_blocks(blocks)
+    set_format(batch, blocks)
     b = torch_gpu.from_numpy(w.rhs).cuda()
     x = torch_gpu.zeros_like(b)
     stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True, restart=restart,
@@ -57,7 +66,7 @@ def test_batched_mixed_reaches_fp64_tolerance(hip, torch_gpu, restart, blocks, c
     assert all(s.relativeResidual < 1e-10 for s in stats)
 
 
-@pytest.mark.parametrize("blocks", [None, 0, 3, "sorted"])
+@pytest.mark.parametrize("blocks", [None, 0, 3, "sorted", "paired"])
 def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
     w = small_batch(seed=9)
     matrix = hip.CSRMatrix(w.matrix)
@@ -65,12 +74,11 @@ def test_batched_graph_replays_match_eager(hip, torch_gpu, blocks):
     batch.autotune(32, 2)
     assert set(batch.tile_rows().tolist()) <= {32, 64, 128, 256}
     assert batch.column_blocks() in (0, 3)
-    assert batch.tile_kernel() in ("tiled", "blocks", "sorted")
-    if blocks == "sorted":
-        batch.set_tile_kernel("sorted")
-        assert batch.tile_kernel() == "sorted" and batch.column_blocks() == 3
-    elif blocks is not None:
-        batch.set_column_blocks(blocks)
+    assert batch.tile_kernel() in ("tiled", "blocks", "sorted", "paired")
+    if blocks is not None:
+        set_format(batch, blocks)
+        if blocks == "sorted":
+            assert batch.column_blocks() == 3
     b = torch_gpu.from_numpy(w.rhs).cuda()
     results = []
     for graph in (False, True, True, True):   # the graph is captured once and replayed on later calls
@@ -133,6 +141,33 @@ def test_batched_sorted_tiles_equal_column_blocks(hip, torch_gpu, mixed):
     assert np.all(per_system_residuals(w, results[1][0]) < 1e-10)
     with pytest.raises(hip._lib.InvalidArgument):
         batch.set_tile_kernel("vector")
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_paired_tiles_switch_and_agree(hip, torch_gpu, mixed):
+    """Paired 1 x 5 / 1 x 3 tiles, cut per system like the other formats: switching to them and back leaves
+    every other format's results bitwise as they were; the paired solves are bitwise reproducible, reach the
+    tolerance per system and agree with the column-block solves to the tolerance's accuracy."""
+    w = small_batch(seed=12)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    batch.set_tile_rows(np.full(batch.systems, 64, dtype=np.int32))
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    results = {}
+    for kernel in ("blocks", "paired", "tiled", "paired", "blocks"):
+        batch.set_tile_kernel(kernel)
+        assert batch.tile_kernel() == kernel
+        assert batch.tile_rows().tolist() == [64] * batch.systems
+        x = torch_gpu.zeros_like(b)
+        stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=mixed)
+        assert np.all(status == 1)
+        got = (x.cpu().numpy(), [s.iterationCount for s in stats])
+        if kernel in results:
+            assert np.array_equal(got[0], results[kernel][0]) and got[1] == results[kernel][1], kernel
+        results[kernel] = got
+    assert np.all(per_system_residuals(w, results["paired"][0]) < 1e-10)
+    reference = results["blocks"][0]
+    assert np.linalg.norm(results["paired"][0] - reference) <= 1e-8 * np.linalg.norm(reference)
 
 
 def test_batched_rejects_coupled_systems(hip, torch_gpu):
