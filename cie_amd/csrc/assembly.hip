@@ -111,32 +111,71 @@ __global__ __launch_bounds__(kAssemblyBlock) void slotPairsKernel(
     }
 }
 
-constexpr int kSlotTile = 2048;  // contributions per tile (at most kAssemblyBlock slots)
+constexpr int kSlotTile = 2048;   // contributions per tile
+constexpr int kTileSlots = 1024;  // slots per tile (four per lane)
 
 // One block per tile of consecutive slots: the lanes stream the tile's
 // contributions (coalesced), stage each product scale_e * K_e[a][b] in LDS,
-// and after one barrier the lane of each slot adds its products from zero in
-// their order; no two lanes share a slot.
+// and after one barrier each lane adds, for each of its slots (tid, tid + 256,
+// ...), that slot's products from zero in their order; no two lanes share a
+// slot. A slot holds about 1.4 contributions on a 3D Q2 mesh, so a tile of
+// whole slots reaches its contribution capacity only with a few slots per lane.
 template <class TLocal>
 __global__ __launch_bounds__(kAssemblyBlock) void assembleSlotsKernel(
     const int2* __restrict__ tiles, const int32_t* __restrict__ slotBegin, const int32_t* __restrict__ elements,
     const TLocal* __restrict__ locals, const double* __restrict__ elementMatrices, int64_t elementStride,
     const double* __restrict__ scale, double* __restrict__ entries64, float* __restrict__ entries32) {
+    constexpr int kPerLane = kSlotTile / kAssemblyBlock;  // contributions per lane at most
+    constexpr int kSlotsPerLane = kTileSlots / kAssemblyBlock;
     __shared__ double stage[kSlotTile];
     const int2 tile = tiles[blockIdx.x], next = tiles[blockIdx.x + 1];  // {first slot, first contribution}
-    for (int p = tile.y + int(threadIdx.x); p < next.y; p += kAssemblyBlock) {
-        const int element = elements[p];
-        const double factor = scale ? scale[element] : 1.0;
-        stage[p - tile.y] = factor * elementMatrices[int64_t(element) * elementStride + int64_t(locals[p])];
+    const int tid = int(threadIdx.x);
+    // The lane's slot ranges, in flight while the contributions stream.
+    int begin[kSlotsPerLane], end[kSlotsPerLane];
+#pragma unroll
+    for (int j = 0; j < kSlotsPerLane; ++j) {
+        const int slot = tile.x + tid + kAssemblyBlock * j;
+        begin[j] = end[j] = 0;
+        if (slot < next.x) {
+            begin[j] = slotBegin[slot];
+            end[j] = slotBegin[slot + 1];
+        }
     }
+    // All of the lane's contribution loads first, then their dependent scale and K_e loads (masked lanes load
+    // nothing), so a lane's loads overlap instead of waiting on each other.
+    int element[kPerLane], local[kPerLane];
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+        const int p = tile.y + tid + kAssemblyBlock * u;
+        element[u] = 0;
+        local[u] = 0;
+        if (p < next.y) {
+            element[u] = elements[p];
+            local[u] = int(locals[p]);
+        }
+    }
+    double product[kPerLane];
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+        product[u] = 0.0;
+        if (tile.y + tid + kAssemblyBlock * u < next.y) {
+            const double factor = scale ? scale[element[u]] : 1.0;
+            product[u] = factor * elementMatrices[int64_t(element[u]) * elementStride + int64_t(local[u])];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u)
+        if (tile.y + tid + kAssemblyBlock * u < next.y) stage[tid + kAssemblyBlock * u] = product[u];
     __syncthreads();
-    const int slot = tile.x + int(threadIdx.x);
-    if (slot < next.x) {
-        double sum = 0.0;
-        const int last = slotBegin[slot + 1];
-        for (int p = slotBegin[slot]; p < last; ++p) sum += stage[p - tile.y];
-        entries64[slot] = sum;
-        if (entries32) entries32[slot] = float(sum);
+#pragma unroll
+    for (int j = 0; j < kSlotsPerLane; ++j) {
+        const int slot = tile.x + tid + kAssemblyBlock * j;
+        if (slot < next.x) {
+            double sum = 0.0;
+            for (int p = begin[j]; p < end[j]; ++p) sum += stage[p - tile.y];
+            entries64[slot] = sum;
+            if (entries32) entries32[slot] = float(sum);
+        }
     }
 }
 
@@ -363,12 +402,12 @@ int cie_hip_assembly_create(cie_hip_csr matrix, int64_t elements, int32_t nodesP
     }
     ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (!ok) return bail(CIE_HIP_ERROR_HIP, "cie_hip_assembly_create: writing slot contributions failed");
-    // Tiles of whole consecutive slots: at most kSlotTile contributions and kAssemblyBlock slots.
+    // Tiles of whole consecutive slots: at most kSlotTile contributions and kTileSlots slots.
     std::vector<int2> tiles;
     for (int64_t slot = 0; slot < nnz;) {
         tiles.push_back(make_int2(int(slot), hostBegin[size_t(slot)]));
         int64_t last = slot;
-        while (last < nnz && last - slot < kAssemblyBlock &&
+        while (last < nnz && last - slot < kTileSlots &&
                int64_t(hostBegin[size_t(last) + 1]) - hostBegin[size_t(slot)] <= kSlotTile)
             ++last;
         if (last == slot)
