@@ -402,13 +402,17 @@ def main():
             "parallelism": "single GPU",
         },
         "mixed": {
-            "seconds": mixed_wall, "device_seconds": m.seconds, "iterations": int(m.stats.iterationCount),
+            "seconds": mixed_wall, "device_seconds": m.seconds,
+            "device_seconds_median": statistics.median(i.seconds for i in mixed_infos),
+            "iterations": int(m.stats.iterationCount),
             "refinements": int(m.refinements), "relative_residual_fp64": m.stats.relativeResidual,
             "true_relative_residual": m.trueRelativeResidual,
             "error_vs_oracle_tight": error(mixed_x, oracle_tight), "error_vs_oracle": error(mixed_x, oracle_x),
         },
         "fp64": {
-            "seconds": fp64_wall, "device_seconds": f.seconds, "iterations": int(f.stats.iterationCount),
+            "seconds": fp64_wall, "device_seconds": f.seconds,
+            "device_seconds_median": statistics.median(i.seconds for i in fp64_infos),
+            "iterations": int(f.stats.iterationCount),
             "relative_residual": f.stats.relativeResidual, "true_relative_residual": f.trueRelativeResidual,
             "error_vs_oracle_tight": error(fp64_x, oracle_tight), "error_vs_oracle": error(fp64_x, oracle_x),
         },
