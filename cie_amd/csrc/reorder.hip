@@ -13,8 +13,9 @@
 //
 // The matrix permutation (reorder.cpp:215-365) runs on the device: one
 // wavefront per row of the result gathers the old row and renumbers its
-// columns; a row of up to 128 entries is sorted by a bitonic network in
-// registers (two keys per lane), a longer one places every entry at its rank
+// columns; a row of up to 64 (128) entries is sorted by a bitonic network in
+// registers, one (two) 32-bit keys per lane (64-bit keys from 2^25 rows on),
+// a longer one places every entry at its rank
 // among the row's new columns (counted out of LDS up to kRankEntries entries,
 // out of L2 beyond). Both value planes follow their entries, so the result is
 // bitwise the reordered input.
@@ -35,17 +36,18 @@ constexpr int kRankEntries = 1024;
 // Sorts the 128 keys held two per lane (positions lane and lane + 64) of one
 // wavefront ascending: a bitonic network, partners across lanes exchanged by
 // __shfl_xor, the stride-64 step inside each lane.
-__device__ __forceinline__ void bitonicSort128(uint64_t& low, uint64_t& high, int lane) {
+template <class Key>
+__device__ __forceinline__ void bitonicSort128(Key& low, Key& high, int lane) {
     for (int k = 2; k <= 128; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             if (j == 64) {
                 // positions lane and lane + 64: ascending iff ((lane & k) == 0), here k == 128
-                const uint64_t a = low, b = high;
+                const Key a = low, b = high;
                 low = a < b ? a : b;
                 high = a < b ? b : a;
             } else {
-                const uint64_t otherLow = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(low), j));
-                const uint64_t otherHigh = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(high), j));
+                const Key otherLow = __shfl_xor(low, j);
+                const Key otherHigh = __shfl_xor(high, j);
                 const bool lower = (lane & j) == 0;  // this lane holds the smaller position of the pair
                 const bool ascendingLow = (lane & k) == 0, ascendingHigh = ((lane + 64) & k) == 0;
                 const bool keepMinLow = lower == ascendingLow, keepMinHigh = lower == ascendingHigh;
@@ -54,6 +56,52 @@ __device__ __forceinline__ void bitonicSort128(uint64_t& low, uint64_t& high, in
             }
         }
     }
+}
+
+// Sorts the 64 keys held one per lane of one wavefront ascending (bitonic).
+template <class Key>
+__device__ __forceinline__ void bitonicSort64(Key& key, int lane) {
+    for (int k = 2; k <= 64; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const Key other = __shfl_xor(key, j);
+            const bool keepMin = ((lane & j) == 0) == ((lane & k) == 0);
+            key = keepMin ? (key < other ? key : other) : (key < other ? other : key);
+        }
+    }
+}
+
+// A row of up to 128 entries: sorts (new column, position) keys in registers
+// and writes the row in that order. Key is uint32_t ((column << 7) | position,
+// for fewer than 2^25 rows) or uint64_t ((column << 32) | position).
+template <class Key>
+__device__ __forceinline__ void permuteShortRow(int lane, int begin, int length, int out,
+                                                const int32_t* __restrict__ target,
+                                                const int32_t* __restrict__ oldColumns,
+                                                const double* __restrict__ oldEntries64,
+                                                const float* __restrict__ oldEntries32,
+                                                int32_t* __restrict__ newColumns, double* __restrict__ newEntries64,
+                                                float* __restrict__ newEntries32) {
+    constexpr int kShift = sizeof(Key) == 4 ? 7 : 32;
+    constexpr Key kPad = ~Key(0);
+    const auto key = [&](int k) { return (Key(uint32_t(target[oldColumns[begin + k]])) << kShift) | Key(k); };
+    const auto place = [&](int k, Key sorted) {
+        const int from = begin + int(sorted & ((Key(1) << kShift) - 1));
+        newColumns[out + k] = int32_t(sorted >> kShift);
+        newEntries64[out + k] = oldEntries64[from];
+        if (newEntries32) newEntries32[out + k] = oldEntries32[from];
+    };
+    if (length <= 64) {  // uniform over the wavefront
+        Key one = lane < length ? key(lane) : kPad;
+        bitonicSort64(one, lane);
+        if (lane < length) place(lane, one);
+        return;
+    }
+    const int k0 = lane, k1 = lane + 64;
+    Key low = k0 < length ? key(k0) : kPad;
+    Key high = k1 < length ? key(k1) : kPad;
+    bitonicSort128(low, high, lane);
+    if (k0 < length) place(k0, low);
+    if (k1 < length) place(k1, high);
 }
 
 __global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
@@ -72,23 +120,12 @@ __global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
     const int out = newExtents[row];
     if (length <= 128) {  // uniform over the wavefront
         // keys (new column, position in the old row): distinct, and equal columns keep their order
-        const uint64_t pad = ~uint64_t(0);
-        const int k0 = lane, k1 = lane + 64;
-        uint64_t low = k0 < length ? (uint64_t(uint32_t(target[oldColumns[begin + k0]])) << 32) | uint32_t(k0) : pad;
-        uint64_t high = k1 < length ? (uint64_t(uint32_t(target[oldColumns[begin + k1]])) << 32) | uint32_t(k1) : pad;
-        bitonicSort128(low, high, lane);
-        if (k0 < length) {
-            const int from = begin + int(uint32_t(low));
-            newColumns[out + k0] = int32_t(low >> 32);
-            newEntries64[out + k0] = oldEntries64[from];
-            if (newEntries32) newEntries32[out + k0] = oldEntries32[from];
-        }
-        if (k1 < length) {
-            const int from = begin + int(uint32_t(high));
-            newColumns[out + k1] = int32_t(high >> 32);
-            newEntries64[out + k1] = oldEntries64[from];
-            if (newEntries32) newEntries32[out + k1] = oldEntries32[from];
-        }
+        if (rows < (1 << 25))
+            permuteShortRow<uint32_t>(lane, begin, length, out, target, oldColumns, oldEntries64, oldEntries32,
+                                      newColumns, newEntries64, newEntries32);
+        else
+            permuteShortRow<uint64_t>(lane, begin, length, out, target, oldColumns, oldEntries64, oldEntries32,
+                                      newColumns, newEntries64, newEntries32);
         return;
     }
     int32_t* lds = staged[wave];
