@@ -9,7 +9,7 @@
 //     kTileEntries entries and rowsPerTile[s] rows: the same block-tile
 //     kernel as the single-system SpMV, one workgroup per tile);
 //   * chunks of at most kChunkRows rows of one system for the vector kernels
-//     (one row per thread).
+//     (kChunkRowsPerThread rows per thread).
 // Each tile or chunk writes its share of its system's dot products; one block
 // per system then sums that system's partials in order and advances its
 // scalars (rho, alpha, beta, residuals, flags), kept on the device.
@@ -40,7 +40,9 @@ namespace {
 constexpr int kTileBlock = 256;
 constexpr int kTileUnroll = 8;
 constexpr int kTileEntries = kTileBlock * kTileUnroll;  // entries per tile at most
-constexpr int kChunkRows = 1024;                         // rows per vector chunk (one per thread)
+constexpr int kChunkThreads = 256;                       // threads per vector chunk
+constexpr int kChunkRowsPerThread = 4;                   // rows per thread, kChunkThreads apart
+constexpr int kChunkRows = kChunkThreads * kChunkRowsPerThread;  // rows per vector chunk
 constexpr int kCompactThreads = 1024;
 constexpr int kGraphIterations = 8;
 constexpr int kDefaultTileRows = 256;
@@ -221,36 +223,70 @@ __global__ __launch_bounds__(kTileBlock) void batchTileRangeKernel(RowSource sou
     BatchTiles<T, W>::tile(firstTile + int(blockIdx.x), source, entries, p, epilogue, stage, extents);
 }
 
+// The vector kernels walk chunks of kChunkRows rows of one system: a block
+// of kChunkThreads threads, each owning kChunkRowsPerThread rows
+// kChunkThreads apart (coalesced). A thread issues all its rows' loads as
+// soon as the chunk's row range is known, before it reads the system's state,
+// so one block keeps several loads per thread in flight behind a single
+// dependent chain (item -> chunk rows -> data).
+struct ChunkRows {
+    int first, end;
+    __device__ __forceinline__ int row(int k) const { return first + int(threadIdx.x) + k * kChunkThreads; }
+    __device__ __forceinline__ bool has(int k) const { return row(k) < end; }
+};
+
+__device__ __forceinline__ ChunkRows chunkRowsOf(const BatchLayout& layout, int c) {
+    return {layout.chunkRows[c], layout.chunkRows[c + 1]};
+}
+
 // x += alpha p; r -= alpha q; z = D^-1 r; chunk partials r.r and r.z.
 template <class T, class TX>
-__global__ __launch_bounds__(kChunkRows) void batchUpdateKernel(BatchLayout layout, ActiveWork work,
-                                                                TX* __restrict__ x, T* __restrict__ r,
-                                                                const T* __restrict__ p, const T* __restrict__ q,
-                                                                const T* __restrict__ inverse,
-                                                                double* __restrict__ partials,
-                                                                const SystemState* __restrict__ state) {
-    __shared__ T lds[kChunkRows / kWave];
-    __shared__ T lds2[kChunkRows / kWave];
+__global__ __launch_bounds__(kChunkThreads) void batchUpdateKernel(BatchLayout layout, ActiveWork work,
+                                                                   TX* __restrict__ x, T* __restrict__ r,
+                                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                                   const T* __restrict__ inverse,
+                                                                   double* __restrict__ partials,
+                                                                   const SystemState* __restrict__ state) {
+    constexpr int R = kChunkRowsPerThread;
+    __shared__ T lds[kChunkThreads / kWave];
+    __shared__ T lds2[kChunkThreads / kWave];
     const int items = work.count[2];
     for (int item = blockIdx.x; item < items; item += gridDim.x) {
         const int2 chunk = work.chunkList[item];
         const int c = chunk.x;
+        const ChunkRows rows = chunkRowsOf(layout, c);
+        TX xs[R];
+        T ps[R], qs[R], rs[R], is[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (rows.has(k)) {
+                const int i = rows.row(k);
+                xs[k] = x[i];
+                ps[k] = p[i];
+                qs[k] = q[i];
+                rs[k] = r[i];
+                is[k] = inverse[i];
+            }
+        }
         const SystemState& s = state[chunk.y];
         T rr = T(0), rz = T(0);
         if (!s.done) {  // NOT_SPD may end a system between the product and this kernel
             const T alpha = T(s.alpha);
-            const int row = layout.chunkRows[c] + int(threadIdx.x);
-            if (row < layout.chunkRows[c + 1]) {
-                x[row] = x[row] + TX(alpha) * TX(p[row]);
-                const T residual = r[row] + (-alpha) * q[row];
-                r[row] = residual;
-                const T z = residual * inverse[row];
-                rr = residual * residual;
-                rz = residual * z;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (rows.has(k)) {
+                    const int i = rows.row(k);
+                    x[i] = xs[k] + TX(alpha) * TX(ps[k]);
+                    const T residual = rs[k] + (-alpha) * qs[k];
+                    r[i] = residual;
+                    const T z = residual * is[k];
+                    rr += residual * residual;
+                    rz += residual * z;
+                }
             }
         }
-        rr = blockSum<kChunkRows>(rr, lds);
-        rz = blockSum<kChunkRows>(rz, lds2);
+        rr = blockSum<kChunkThreads>(rr, lds);
+        rz = blockSum<kChunkThreads>(rz, lds2);
         if (threadIdx.x == 0) {
             partials[2 * c] = double(rr);
             partials[2 * c + 1] = double(rz);
@@ -261,41 +297,60 @@ __global__ __launch_bounds__(kChunkRows) void batchUpdateKernel(BatchLayout layo
 // Mixed form: the FP32 correction d += alpha p, folded into the FP64 x (and
 // cleared) for a system that refines this trip; r -= alpha q; z = D^-1 r;
 // chunk partials r.r and r.z.
-__global__ __launch_bounds__(kChunkRows) void batchMixedUpdateKernel(BatchLayout layout, ActiveWork work,
-                                                                     double* __restrict__ x, float* __restrict__ d,
-                                                                     float* __restrict__ r, const float* __restrict__ p,
-                                                                     const float* __restrict__ q,
-                                                                     const float* __restrict__ inverse,
-                                                                     double* __restrict__ partials,
-                                                                     const SystemState* __restrict__ state) {
-    __shared__ float lds[kChunkRows / kWave];
-    __shared__ float lds2[kChunkRows / kWave];
+__global__ __launch_bounds__(kChunkThreads) void batchMixedUpdateKernel(BatchLayout layout, ActiveWork work,
+                                                                        double* __restrict__ x, float* __restrict__ d,
+                                                                        float* __restrict__ r,
+                                                                        const float* __restrict__ p,
+                                                                        const float* __restrict__ q,
+                                                                        const float* __restrict__ inverse,
+                                                                        double* __restrict__ partials,
+                                                                        const SystemState* __restrict__ state) {
+    constexpr int R = kChunkRowsPerThread;
+    __shared__ float lds[kChunkThreads / kWave];
+    __shared__ float lds2[kChunkThreads / kWave];
     const int items = work.count[2];
     for (int item = blockIdx.x; item < items; item += gridDim.x) {
         const int2 chunk = work.chunkList[item];
         const int c = chunk.x;
+        const ChunkRows rows = chunkRowsOf(layout, c);
+        float ds[R], ps[R], qs[R], rs[R], is[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (rows.has(k)) {
+                const int i = rows.row(k);
+                ds[k] = d[i];
+                ps[k] = p[i];
+                qs[k] = q[i];
+                rs[k] = r[i];
+                is[k] = inverse[i];
+            }
+        }
         const SystemState& s = state[chunk.y];
         float rr = 0.0f, rz = 0.0f;
         if (!s.done) {  // NOT_SPD may end a system between the product and this kernel
             const float alpha = float(s.alpha);
-            const int row = layout.chunkRows[c] + int(threadIdx.x);
-            if (row < layout.chunkRows[c + 1]) {
-                const float correction = d[row] + alpha * p[row];
-                if (s.refine) {
-                    x[row] = x[row] + double(correction);
-                    d[row] = 0.0f;
-                } else {
-                    d[row] = correction;
+            const bool refine = s.refine;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (rows.has(k)) {
+                    const int i = rows.row(k);
+                    const float correction = ds[k] + alpha * ps[k];
+                    if (refine) {
+                        x[i] = x[i] + double(correction);
+                        d[i] = 0.0f;
+                    } else {
+                        d[i] = correction;
+                    }
+                    const float residual = rs[k] + (-alpha) * qs[k];
+                    r[i] = residual;
+                    const float z = residual * is[k];
+                    rr += residual * residual;
+                    rz += residual * z;
                 }
-                const float residual = r[row] + (-alpha) * q[row];
-                r[row] = residual;
-                const float z = residual * inverse[row];
-                rr = residual * residual;
-                rz = residual * z;
             }
         }
-        rr = blockSum<kChunkRows>(rr, lds);
-        rz = blockSum<kChunkRows>(rz, lds2);
+        rr = blockSum<kChunkThreads>(rr, lds);
+        rz = blockSum<kChunkThreads>(rz, lds2);
         if (threadIdx.x == 0) {
             partials[2 * c] = double(rr);
             partials[2 * c + 1] = double(rz);
@@ -305,19 +360,31 @@ __global__ __launch_bounds__(kChunkRows) void batchMixedUpdateKernel(BatchLayout
 
 // p = beta p + D^-1 r
 template <class T>
-__global__ __launch_bounds__(kChunkRows) void batchDirectionKernel(BatchLayout layout, ActiveWork work,
-                                                                   T* __restrict__ p, const T* __restrict__ r,
-                                                                   const T* __restrict__ inverse,
-                                                                   const SystemState* __restrict__ state) {
+__global__ __launch_bounds__(kChunkThreads) void batchDirectionKernel(BatchLayout layout, ActiveWork work,
+                                                                      T* __restrict__ p, const T* __restrict__ r,
+                                                                      const T* __restrict__ inverse,
+                                                                      const SystemState* __restrict__ state) {
+    constexpr int R = kChunkRowsPerThread;
     const int items = work.count[2];
     for (int item = blockIdx.x; item < items; item += gridDim.x) {
         const int2 chunk = work.chunkList[item];
+        const ChunkRows rows = chunkRowsOf(layout, chunk.x);
+        T ps[R], rs[R], is[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (rows.has(k)) {
+                const int i = rows.row(k);
+                ps[k] = p[i];
+                rs[k] = r[i];
+                is[k] = inverse[i];
+            }
+        }
         const SystemState& s = state[chunk.y];
         if (s.done) continue;
-        const int c = chunk.x;
         const T beta = T(s.beta);
-        const int row = layout.chunkRows[c] + int(threadIdx.x);
-        if (row < layout.chunkRows[c + 1]) p[row] = p[row] * beta + r[row] * inverse[row];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (rows.has(k)) p[rows.row(k)] = ps[k] * beta + rs[k] * is[k];
     }
 }
 
@@ -879,7 +946,7 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
     cie_hip_csr m = batch->matrix;
     const hipStream_t stream = ctx->stream;
     const ActiveWork& work = batch->running;
-    const dim3 tileBlock(kTileBlock), chunkBlock(kChunkRows), systems(batch->systems);
+    const dim3 tileBlock(kTileBlock), chunkBlock(kChunkThreads), systems(batch->systems);
     const TR* entries;
     if constexpr (std::is_same_v<TR, float>) entries = m->entries32;
     else entries = m->entries64;
@@ -897,11 +964,11 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
                        batch->tilePartials, batch->state, settings, batch->active);
     if constexpr (std::is_same_v<TR, float>) {  // the mixed form: FP32 correction, folded when refining
         hipLaunchKernelGGL(batchMixedUpdateKernel,
-                           workGrid(batchMixedUpdateKernel, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
+                           workGrid(batchMixedUpdateKernel, ctx, kChunkThreads, batch->chunkCount), chunkBlock, 0,
                            stream, layout, work, x, correction, r, p, q, inverse, batch->chunkPartials, batch->state);
     } else {
         hipLaunchKernelGGL((batchUpdateKernel<TR, double>),
-                           workGrid(batchUpdateKernel<TR, double>, ctx, kChunkRows, batch->chunkCount), chunkBlock, 0,
+                           workGrid(batchUpdateKernel<TR, double>, ctx, kChunkThreads, batch->chunkCount), chunkBlock, 0,
                            stream, layout, work, x, r, p, q, inverse, batch->chunkPartials, batch->state);
     }
     if (mixed) {
@@ -918,7 +985,7 @@ void enqueueIteration(cie_hip_batch batch, const BatchLayout& layout, const doub
         hipLaunchKernelGGL((batchFinishKernel<TR, kPhaseUpdate>), systems, tileBlock, 0, stream, batch->systemChunks,
                            batch->chunkPartials, batch->state, settings, batch->active);
     }
-    hipLaunchKernelGGL((batchDirectionKernel<TR>), workGrid(batchDirectionKernel<TR>, ctx, kChunkRows, batch->chunkCount),
+    hipLaunchKernelGGL((batchDirectionKernel<TR>), workGrid(batchDirectionKernel<TR>, ctx, kChunkThreads, batch->chunkCount),
                        chunkBlock, 0, stream, layout, work, p, r, inverse, batch->state);
     launchSelect<kRunning>(batch, work);
 }

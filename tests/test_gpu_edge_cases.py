@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from cie_amd import systems
-from cie_amd._lib import library
+from cie_amd._lib import InvalidArgument, library
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,15 @@ def test_products_of_a_matrix_without_entries(hip, torch_gpu, rows):
     matrix = hip.CSRMatrix(csr)
     rng = np.random.default_rng(rows)
     x, y0 = rng.standard_normal(rows), rng.standard_normal(rows)
-    for kernel, width, unroll in (("tiled", 256, 4), ("tiled", 256, 16), ("staged", 512, 4), ("vector", 4, 4)):
+    launches = [("tiled", 256, 4), ("tiled", 256, 16), ("staged", 512, 4), ("vector", 4, 4)]
+    block_launches = [("blocks", 3, 4), ("sorted", 3, 8), ("paired", 53, 2), ("paired", 32, 4)]
+    if rows >= 5:  # block forms need at least as many columns as their (wide) block
+        launches += block_launches
+    else:
+        for kernel, width, unroll in block_launches:
+            with pytest.raises(InvalidArgument):
+                matrix.set_launch(64, kernel, width, unroll)
+    for kernel, width, unroll in launches:
         matrix.set_launch(64, kernel, width, unroll)
         op = hip.HIPCSROperator(matrix, hip.HIPSpace())
         for in_scale, out_scale in SCALES:
