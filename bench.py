@@ -825,7 +825,7 @@ def bench_reorder(args):
           {"workload": f"3D Q2 Laplace {n}^3 hexahedra, rows numbered at random (seed 11), reverse Cuthill-McKee",
            "rows": rows, "nnz": nnz},
           _roofline("permuteRowsKernel (one launch per permutation; the value above is the whole call, with its "
-                    "host preparation, allocations and map uploads)", algorithmic, kernel,
+                    "allocations, map upload, device preparation of the maps and row extents, and extents download)", algorithmic, kernel,
                     "(4 + 8 + 4) nnz read + written (columns, FP64, FP32) + 2 x 4 (N + 1) extents + 3 x 4 N maps",
                     _profiled_traffic("reorder_q2_64_traffic.json", "permuteRowsKernel")),
           {"value": cpu_permute, "unit": "s", "cores": 1, "kind": "port",

@@ -11,14 +11,15 @@
 // starts the next component; reverse Cuthill-McKee reverses the result.
 // map[new] = old.
 //
-// The matrix permutation (reorder.cpp:215-365) runs on the device: one
-// wavefront per row of the result gathers the old row and renumbers its
-// columns; a row of up to 64 (128) entries is sorted by a bitonic network in
-// registers, one (two) 32-bit keys per lane (64-bit keys from 2^25 rows on),
-// a longer one places every entry at its rank
-// among the row's new columns (counted out of LDS up to kRankEntries entries,
-// out of L2 beyond). Both value planes follow their entries, so the result is
-// bitwise the reordered input.
+// The matrix permutation (reorder.cpp:215-365) runs on the device. The map
+// is uploaded, inverted and checked there, and the new row extents are
+// scanned there. Then one wavefront per row of the result gathers the old row
+// and renumbers its columns; a row of up to 64 (128) entries is sorted by a
+// bitonic network in registers, one (two) 32-bit keys per lane (64-bit keys
+// from 2^25 rows on), a longer one places every entry at its rank among the
+// row's new columns (counted out of LDS up to kRankEntries entries, out of L2
+// beyond). Both value planes follow their entries, so the result is bitwise
+// the reordered input.
 #include "dispatch.hpp"
 
 #include <algorithm>
@@ -150,6 +151,108 @@ __global__ __launch_bounds__(kPermuteBlock) void permuteRowsKernel(
     }
 }
 
+// Preparation of a matrix permutation on the device, from the uploaded map:
+// the inverse map, a check that the map is a permutation, and the new row
+// extents (each new row's length, then an inclusive scan).
+
+// inverse[map[i]] = i; an index outside [0, rows) is flagged. inverse starts
+// at -1, so after this kernel a slot still at -1 means the map repeats an
+// index (rows indices in range that miss a slot must hit another twice).
+__global__ __launch_bounds__(kReduceBlock) void invertMapKernel(int rows, const int32_t* __restrict__ map,
+                                                                int32_t* __restrict__ inverse,
+                                                                int* __restrict__ invalid) {
+    for (int i = int(blockIdx.x) * kReduceBlock + int(threadIdx.x); i < rows; i += int(gridDim.x) * kReduceBlock) {
+        const int32_t j = map[i];
+        if (j < 0 || j >= rows) {
+            atomicOr(invalid, 1);
+            continue;
+        }
+        inverse[j] = i;
+    }
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;  // consecutive rows per thread
+constexpr int kScanRows = kScanThreads * kScanItems;
+
+// Inclusive scan of the kScanThreads values of a block (Hillis-Steele in LDS).
+__device__ __forceinline__ int blockInclusiveScan(int value, int* lds) {
+    const int tid = threadIdx.x;
+    lds[tid] = value;
+    __syncthreads();
+    for (int offset = 1; offset < kScanThreads; offset *= 2) {
+        const int add = tid >= offset ? lds[tid - offset] : 0;
+        __syncthreads();
+        lds[tid] += add;
+        __syncthreads();
+    }
+    const int result = lds[tid];
+    __syncthreads();  // lds is reused by the caller's next scan
+    return result;
+}
+
+// New row i is old row source[i]: its length goes to extents[i + 1], scanned
+// within the block's kScanRows rows; the block's total goes to totals[block].
+// Flags a slot of `inverse` left at -1 and a source index outside [0, rows)
+// (then the length counts as 0).
+__global__ __launch_bounds__(kScanThreads) void newExtentsKernel(int rows, const int32_t* __restrict__ source,
+                                                                 const int32_t* __restrict__ inverse,
+                                                                 const int32_t* __restrict__ oldExtents,
+                                                                 int32_t* __restrict__ extents,
+                                                                 int32_t* __restrict__ totals,
+                                                                 int* __restrict__ invalid) {
+    __shared__ int lds[kScanThreads];
+    const int first = int(blockIdx.x) * kScanRows + int(threadIdx.x) * kScanItems;
+    int lengths[kScanItems];
+    int sum = 0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = first + k;
+        int length = 0;
+        if (i < rows) {
+            bad |= inverse[i] < 0;
+            const int32_t old = source[i];
+            if (old >= 0 && old < rows) length = oldExtents[old + 1] - oldExtents[old];
+            else bad = true;
+        }
+        sum += length;
+        lengths[k] = sum;
+    }
+    if (bad) atomicOr(invalid, 1);
+    const int before = blockInclusiveScan(sum, lds) - sum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (first + k < rows) extents[first + k + 1] = before + lengths[k];
+    if (threadIdx.x == kScanThreads - 1) totals[blockIdx.x] = before + sum;
+    if (blockIdx.x == 0 && threadIdx.x == 0) extents[0] = 0;
+}
+
+// One block: totals[b] becomes the exclusive prefix of the block totals.
+__global__ __launch_bounds__(kScanThreads) void scanTotalsKernel(int blocks, int32_t* __restrict__ totals) {
+    __shared__ int lds[kScanThreads];
+    const int per = (blocks + kScanThreads - 1) / kScanThreads;
+    const int first = int(threadIdx.x) * per;
+    int sum = 0;
+    for (int b = first; b < min(blocks, first + per); ++b) sum += totals[b];
+    int running = blockInclusiveScan(sum, lds) - sum;
+    for (int b = first; b < min(blocks, first + per); ++b) {
+        const int total = totals[b];
+        totals[b] = running;
+        running += total;
+    }
+}
+
+// Adds the prefix of the blocks before to each block's scanned rows.
+__global__ __launch_bounds__(kScanThreads) void addTotalsKernel(int rows, const int32_t* __restrict__ totals,
+                                                                int32_t* __restrict__ extents) {
+    const int offset = totals[blockIdx.x];
+    if (offset == 0) return;
+    const int first = int(blockIdx.x) * kScanRows;
+    for (int i = first + int(threadIdx.x); i < min(rows, first + kScanRows); i += kScanThreads)
+        extents[i + 1] += offset;
+}
+
 // out[i] = in[map[i]] (gather) or out[map[i]] = in[i] (scatter); an index
 // outside [0, n) is skipped and flagged.
 template <typename T>
@@ -255,25 +358,13 @@ int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie
     if (matrix->rows != matrix->columns)
         return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_reorder: the matrix must be square");
     const int64_t rows = matrix->rows, nnz = matrix->nnz;
-    // source[new] = old row, target[old] = new index (reorder.cpp:273-274).
-    std::vector<int32_t> inverse(size_t(rows), -1);
-    for (int64_t i = 0; i < rows; ++i) {
-        if (map[i] < 0 || map[i] >= rows || inverse[size_t(map[i])] >= 0)
-            return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_reorder: the map is not a permutation");
-        inverse[size_t(map[i])] = int32_t(i);
-    }
-    const std::vector<int32_t> forward(map, map + rows);
-    const std::vector<int32_t>& source = reverse ? inverse : forward;
-    const std::vector<int32_t>& target = reverse ? forward : inverse;
-    const std::vector<int32_t>& oldExtents = matrix->hostExtents;
-    std::vector<int32_t> newExtents(size_t(rows) + 1, 0);
-    for (int64_t i = 0; i < rows; ++i) {
-        const int32_t old = source[size_t(i)];
-        newExtents[size_t(i) + 1] = newExtents[size_t(i)] + (oldExtents[size_t(old) + 1] - oldExtents[size_t(old)]);
-    }
     cie_hip_context ctx = matrix->context;
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
-    int32_t *dExtents = nullptr, *dColumns = nullptr, *dSource = nullptr, *dTarget = nullptr;
+    // On the device: the map, its inverse, a check that the map is a
+    // permutation, and the new row extents. source[new] = old row,
+    // target[old] = new index (reorder.cpp:273-274).
+    const int scanBlocks = int((rows + kScanRows - 1) / kScanRows);
+    int32_t *dExtents = nullptr, *dColumns = nullptr, *dMap = nullptr, *dInverse = nullptr, *dTotals = nullptr;
     double* dEntries64 = nullptr;
     float* dEntries32 = nullptr;
     auto release = [&]() {
@@ -283,28 +374,63 @@ int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie
         if (dEntries32) (void)hipFree(dEntries32);
     };
     auto releaseMaps = [&]() {
-        if (dSource) (void)hipFree(dSource);
-        if (dTarget) (void)hipFree(dTarget);
+        if (dMap) (void)hipFree(dMap);
+        if (dInverse) (void)hipFree(dInverse);
+        if (dTotals) (void)hipFree(dTotals);
     };
     const size_t count = size_t(std::max<int64_t>(1, nnz));
+    const size_t mapCount = size_t(std::max<int64_t>(1, rows));
     bool ok = hipMalloc(&dExtents, sizeof(int32_t) * (size_t(rows) + 1)) == hipSuccess &&
               hipMalloc(&dColumns, sizeof(int32_t) * count) == hipSuccess &&
               hipMalloc(&dEntries64, sizeof(double) * count) == hipSuccess &&
               (!matrix->entries32 || hipMalloc(&dEntries32, sizeof(float) * count) == hipSuccess) &&
-              hipMalloc(&dSource, sizeof(int32_t) * size_t(std::max<int64_t>(1, rows))) == hipSuccess &&
-              hipMalloc(&dTarget, sizeof(int32_t) * size_t(std::max<int64_t>(1, rows))) == hipSuccess;
+              hipMalloc(&dMap, sizeof(int32_t) * mapCount) == hipSuccess &&
+              hipMalloc(&dInverse, sizeof(int32_t) * mapCount) == hipSuccess &&
+              hipMalloc(&dTotals, sizeof(int32_t) * size_t(std::max(1, scanBlocks))) == hipSuccess;
     if (!ok) {
         release();
         releaseMaps();
         return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_csr_reorder: device allocation failed");
     }
-    ok = hipMemcpyAsync(dExtents, newExtents.data(), sizeof(int32_t) * (size_t(rows) + 1), hipMemcpyHostToDevice,
-                        ctx->stream) == hipSuccess &&
-         (!rows || (hipMemcpyAsync(dSource, source.data(), sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice,
-                                   ctx->stream) == hipSuccess &&
-                    hipMemcpyAsync(dTarget, target.data(), sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice,
-                                   ctx->stream) == hipSuccess));
+    std::vector<int32_t> newExtents(size_t(rows) + 1, 0);
+    int* invalid = reinterpret_cast<int*>(ctx->scratch.scalars + 24);
+    int* pinned = reinterpret_cast<int*>(ctx->scratch.pinned + 24);
+    const int32_t* dSource = reverse ? dInverse : dMap;
+    const int32_t* dTarget = reverse ? dMap : dInverse;
+    ok = hipMemsetAsync(invalid, 0, sizeof(int), ctx->stream) == hipSuccess;
     if (ok && rows) {
+        ok = hipMemcpyAsync(dMap, map, sizeof(int32_t) * size_t(rows), hipMemcpyHostToDevice, ctx->stream) ==
+                 hipSuccess &&
+             hipMemsetAsync(dInverse, 0xff, sizeof(int32_t) * size_t(rows), ctx->stream) == hipSuccess;  // -1
+        if (ok) {
+            hipLaunchKernelGGL(invertMapKernel, dim3(unsigned(vectorGrid(rows))), dim3(kReduceBlock), 0, ctx->stream,
+                               int(rows), dMap, dInverse, invalid);
+            hipLaunchKernelGGL(newExtentsKernel, dim3(unsigned(scanBlocks)), dim3(kScanThreads), 0, ctx->stream,
+                               int(rows), dSource, dInverse, matrix->rowExtents, dExtents, dTotals, invalid);
+            hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kScanThreads), 0, ctx->stream, scanBlocks, dTotals);
+            hipLaunchKernelGGL(addTotalsKernel, dim3(unsigned(scanBlocks)), dim3(kScanThreads), 0, ctx->stream,
+                               int(rows), dTotals, dExtents);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        ok = ok && hipMemcpyAsync(newExtents.data(), dExtents, sizeof(int32_t) * (size_t(rows) + 1),
+                                  hipMemcpyDeviceToHost, ctx->stream) == hipSuccess;
+    } else if (ok) {
+        ok = hipMemcpyAsync(dExtents, newExtents.data(), sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) ==
+             hipSuccess;
+    }
+    ok = ok && hipMemcpyAsync(pinned, invalid, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) {
+        release();
+        releaseMaps();
+        return setError(CIE_HIP_ERROR_HIP, "cie_hip_csr_reorder: preparing the permutation failed");
+    }
+    if (*pinned) {
+        release();
+        releaseMaps();
+        return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_reorder: the map is not a permutation");
+    }
+    if (rows) {
         const KernelTimer timer(ctx);
         hipLaunchKernelGGL(permuteRowsKernel, dim3(unsigned((rows + kPermuteWaves - 1) / kPermuteWaves)),
                            dim3(kPermuteBlock), 0, ctx->stream, int(rows), dSource, dTarget, matrix->rowExtents,
@@ -318,7 +444,7 @@ int cie_hip_csr_reorder(cie_hip_csr matrix, const int32_t* map, int reverse, cie
     releaseMaps();
     if (!ok) {
         release();
-        return setError(CIE_HIP_ERROR_HIP, "cie_hip_csr_reorder: upload or permutation failed");
+        return setError(CIE_HIP_ERROR_HIP, "cie_hip_csr_reorder: permutation failed");
     }
     return adoptMatrix(ctx, rows, rows, dExtents, dColumns, dEntries64, dEntries32, std::move(newExtents), reordered);
 }

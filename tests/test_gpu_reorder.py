@@ -89,6 +89,33 @@ def test_csr_reorder_matches_oracle_bitwise(hip, torch_gpu, reverse):
         hip.CSRMatrix(cases[0]).reorder(np.zeros(cases[0].rows, dtype=np.int32))
 
 
+@pytest.mark.parametrize("reverse", [False, True])
+def test_csr_reorder_prepares_maps_of_several_scan_blocks(hip, torch_gpu, reverse):
+    """The new row extents are scanned on the device in runs of 4096 rows: a matrix of 3 * 4096 + 17 rows
+    with rows of 0 to 40 entries, against the oracle bitwise; maps that are not permutations (an index out of
+    range at either end, one index twice) are refused in both directions."""
+    rng = np.random.default_rng(21)
+    rows = 3 * 4096 + 17
+    lengths = rng.integers(0, 41, rows)
+    ext = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(rows, n, replace=False)) for n in lengths]).astype(np.int32)
+    csr = systems.CSR(rows, rows, ext, cols, rng.standard_normal(int(ext[-1])))
+    mapping = rng.permutation(rows).astype(np.int32)
+    matrix = hip.CSRMatrix(csr)
+    got = matrix.reorder(mapping, reverse=reverse).copy_out()
+    extents, columns, entries = oracle.reorder_csr(csr, mapping, reverse=reverse)
+    assert np.array_equal(got.row_extents, extents)
+    assert np.array_equal(got.column_indices, columns)
+    assert np.array_equal(got.entries, entries)
+    for position, value in ((5, -1), (rows - 1, rows), (4100, int(mapping[9000]))):
+        bad = mapping.copy()
+        bad[position] = value
+        with pytest.raises(ShapeError):
+            matrix.reorder(bad, reverse=reverse)
+    again = matrix.reorder(mapping, reverse=reverse).copy_out()  # a refused map leaves nothing behind
+    assert np.array_equal(again.column_indices, columns)
+
+
 def test_vector_reorder_gather_scatter_and_errors(hip, torch_gpu):
     rng = np.random.default_rng(8)
     n = 100003
