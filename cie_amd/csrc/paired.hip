@@ -3,9 +3,10 @@
 // tiles of whole rows, each width's blocks column-sorted within its tile.
 //
 // Cover. A row is walked in column order by a group of kCoverLanes lanes (the
-// row's columns loaded kCoverLanes at a time, the next block starting at the
-// first lane whose column lies past the blocks so far: a ballot and a
-// find-first-set). A block starts at the column of the first entry not yet
+// row's columns loaded kCoverLanes at a time with the next kCoverLanes as a
+// look-ahead, the next block starting at the first lane whose column lies
+// past the blocks so far: a ballot and a find-first-set; its width and entry
+// count are ballots over the look-ahead window). A block starts at the column of the first entry not yet
 // covered (moved left to columns - width at the last columns) and is wide
 // when an entry of [start + narrow, start + wide) follows, that is when a
 // narrow block would leave part of the wide span to a further block. The
@@ -42,35 +43,43 @@ constexpr int kPairedSortMax = 2048;  // blocks of one width per tile at most (k
 constexpr int kMaxPairedWidth = 8;
 
 // Calls emit(wide, index in row, start column, first entry, entries) for
-// every block of `row`, in column order, from the group's first lane.
+// every block of `row`, in column order, from the group's first lane. The
+// group holds a window of 2 * kCoverLanes columns (lane l: entries base + l
+// and base + kCoverLanes + l); a block's entries lie within `wide` <=
+// kMaxPairedWidth positions of its first one, inside the window, so the
+// width and entry count come from ballots over the window, not loads.
 template <class Emit>
 __device__ __forceinline__ void pairedCover(int row, const int32_t* __restrict__ extents,
                                             const int32_t* __restrict__ columnIndices, int wide, int narrow,
                                             int columns, int lane, int groupShift, Emit&& emit) {
+    static_assert(kMaxPairedWidth <= kCoverLanes, "a block's entries must fit the window past its first lane");
     const int begin = extents[row], end = extents[row + 1];
     int limit = INT_MIN;  // columns below this are covered
     int index = 0;
+    const uint32_t groupMask = (1u << kCoverLanes) - 1;
+    auto groupBits = [&](bool predicate) { return uint32_t(__ballot(predicate) >> groupShift) & groupMask; };
     for (int base = begin; base < end; base += kCoverLanes) {
         const int k = base + lane;
         const int column = k < end ? columnIndices[k] : INT_MAX;
+        const int ahead = k + kCoverLanes < end ? columnIndices[k + kCoverLanes] : INT_MAX;
+        const int high = lane + kCoverLanes;  // window position of `ahead`
         for (;;) {
-            const uint64_t open = __ballot(k < end && column >= limit);
-            const uint32_t mine = uint32_t(open >> groupShift) & ((1u << kCoverLanes) - 1);
+            const uint32_t mine = groupBits(k < end && column >= limit);
             if (!mine) break;
             const int firstLane = __ffs(int(mine)) - 1;
             const int next = __shfl(column, (groupShift & (kWave - 1)) + firstLane, kWave);
             const int entry = base + firstLane;
-            // Columns rise along the row, so the entries of [next, next + wide) are the next few.
-            bool isWide = false;
-            for (int j = 1; j < wide && entry + j < end; ++j) {
-                const int c = columnIndices[entry + j];
-                if (c >= next + wide) break;
-                if (c >= next + narrow) isWide = true;
-            }
+            // Columns rise along the row, so the entries of [next, next + wide) are the next few positions.
+            auto inWideTail = [&](int position, int c) {
+                return position > firstLane && position < firstLane + wide && c >= next + narrow && c < next + wide;
+            };
+            const bool isWide = (groupBits(inWideTail(lane, column)) | groupBits(inWideTail(high, ahead))) != 0;
             const int width = isWide ? wide : narrow;
             const int start = next + width > columns ? columns - width : next;
-            int count = 1;
-            while (count < width && entry + count < end && columnIndices[entry + count] < start + width) ++count;
+            auto inBlock = [&](int position, int c) {
+                return position >= firstLane && position < firstLane + width && c < start + width;
+            };
+            const int count = __popc(groupBits(inBlock(lane, column))) + __popc(groupBits(inBlock(high, ahead)));
             if (lane == 0) emit(isWide, index, start, entry, count);
             ++index;
             limit = start + width;
