@@ -22,10 +22,12 @@
 // all the tile's blocks in row order), its first column, first CSR entry and
 // entry count (the refill reads the CSR through them).
 //
-// Values. One thread per sorted block gathers its entries into the block's
-// width values (zero where the row has no entry) and writes them to the value
-// planes of its width (PlaneRun layout), for the FP64 plane and, when the
-// matrix has one, the FP32 plane; rerun whenever the entries change.
+// Values. One workgroup per tile visits the tile's blocks in row order (so
+// the CSR is read in order), gathers each block's entries into its width
+// values (zero where the row has no entry), stages them in LDS at the block's
+// sorted slot and writes each plane's run of the tile coalesced (PlaneRun
+// layout); once for the FP64 plane and, when the matrix has one, once for
+// the FP32 plane; rerun whenever the entries change.
 #include "dispatch.hpp"
 #include "kernels.hpp"
 
@@ -41,6 +43,8 @@ constexpr int kCoverLanes = 16;
 constexpr int kSortThreads = 1024;
 constexpr int kPairedSortMax = 2048;  // blocks of one width per tile at most (kSpmvBlock * 8)
 constexpr int kMaxPairedWidth = 8;
+constexpr int kValuesThreads = 256;
+constexpr int kStageValues = 5120;  // values staged per pass: 1024 blocks of 1 x 5 (40 KB in FP64)
 
 // Calls emit(wide, index in row, start column, first entry, entries) for
 // every block of `row`, in column order, from the group's first lane. The
@@ -202,27 +206,68 @@ __global__ __launch_bounds__(kSortThreads) void pairedSortKernel(const int2* __r
     }
 }
 
-// The values of `blocks` sorted blocks of one width (slots offset ..
-// offset + blocks) into that width's planes, one thread per block.
+// The values of one tile's blocks, one precision, into their planes: one
+// workgroup per tile. The blocks are visited in row order (their positions in
+// the tile, found through the sorted keys), so consecutive lanes read
+// consecutive CSR entries. Each width's values are staged in LDS at their
+// sorted slots, kStageValues values per pass, and each plane's run of the
+// tile is then written out coalesced.
 template <class T>
-__global__ __launch_bounds__(kReduceBlock) void pairedValuesKernel(int64_t blocks, int width, int64_t offset,
-                                                                   const int32_t* __restrict__ sortedStart,
-                                                                   const int32_t* __restrict__ sortedEntry,
-                                                                   const uint8_t* __restrict__ sortedCount,
-                                                                   const int32_t* __restrict__ columnIndices,
-                                                                   const T* __restrict__ entries,
-                                                                   T* __restrict__ planes) {
-    constexpr int kChunk = 16 / int(sizeof(T));
-    for (int64_t s = int64_t(blockIdx.x) * kReduceBlock + threadIdx.x; s < blocks;
-         s += int64_t(gridDim.x) * kReduceBlock) {
-        const int64_t slot = offset + s;
-        const int start = sortedStart[slot], entry = sortedEntry[slot], count = sortedCount[slot];
-        T v[kMaxPairedWidth] = {};
-        for (int j = 0; j < count; ++j) v[columnIndices[entry + j] - start] += entries[entry + j];
-        for (int p = 0; p * kChunk < width; ++p) {
-            const int length = min(kChunk, width - p * kChunk);
-            T* __restrict__ plane = planes + blocks * (int64_t(p) * kChunk) + s * length;
-            for (int j = 0; j < length; ++j) plane[j] = v[p * kChunk + j];
+__global__ __launch_bounds__(kValuesThreads) void pairedValuesKernel(
+    const int2* __restrict__ begins, const uint32_t* __restrict__ keys, const int32_t* __restrict__ sortedStart,
+    const int32_t* __restrict__ sortedEntry, const uint8_t* __restrict__ sortedCount, int64_t wideBlocks,
+    int64_t narrowBlocks, int wide, int narrow, const int32_t* __restrict__ columnIndices,
+    const T* __restrict__ entries, T* __restrict__ planes) {
+    constexpr int kChunk = 16 / int(sizeof(T));  // values per plane (PlaneRun)
+    __shared__ int16_t order[2 * kPairedSortMax];  // position in the tile -> the tile's wide, then narrow, slots
+    __shared__ T stage[kStageValues];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int2 head = begins[t], next = begins[t + 1];
+    const int wideCount = next.x - head.x, total = wideCount + (next.y - head.y);
+    for (int i = tid; i < total; i += kValuesThreads) {
+        const int64_t slot = i < wideCount ? int64_t(head.x) + i : wideBlocks + head.y + (i - wideCount);
+        order[keys[slot] & kSortedPositionMask] = int16_t(i);
+    }
+    __syncthreads();
+    for (int w = 0; w < 2; ++w) {
+        const bool isWide = w == 0;
+        const int width = isWide ? wide : narrow;
+        const int64_t blocks = isWide ? wideBlocks : narrowBlocks;
+        const int first = isWide ? 0 : wideCount;             // the width's slots among the tile's
+        const int count = isWide ? wideCount : total - wideCount;
+        const int64_t begin = isWide ? head.x : head.y;       // the tile's first slot within the width
+        T* __restrict__ widthPlanes = planes + (isWide ? 0 : wideBlocks * wide);
+        const int perPass = kStageValues / width;
+        for (int pass = 0; pass < count; pass += perPass) {
+            const int n = min(perPass, count - pass);
+            for (int position = tid; position < total; position += kValuesThreads) {
+                const int k = order[position] - first - pass;  // slot within this pass
+                if (k < 0 || k >= n) continue;
+                const int64_t slot = (isWide ? 0 : wideBlocks) + begin + pass + k;
+                const int start = sortedStart[slot], entry = sortedEntry[slot], held = sortedCount[slot];
+                T v[kMaxPairedWidth] = {};
+                for (int j = 0; j < held; ++j) {
+                    const int c = columnIndices[entry + j] - start;
+                    const T e = entries[entry + j];
+#pragma unroll
+                    for (int u = 0; u < kMaxPairedWidth; ++u)  // register indices only
+                        if (c == u) v[u] += e;
+                }
+#pragma unroll
+                for (int u = 0; u < kMaxPairedWidth; ++u)
+                    if (u < width) stage[k * width + u] = v[u];
+            }
+            __syncthreads();
+            for (int p = 0; p * kChunk < width; ++p) {
+                const int length = min(kChunk, width - p * kChunk);
+                T* __restrict__ run = widthPlanes + blocks * (int64_t(p) * kChunk) + (begin + pass) * length;
+                for (int i = tid; i < n * length; i += kValuesThreads) {
+                    const int k = i / length, j = i - k * length;
+                    run[i] = stage[k * width + p * kChunk + j];
+                }
+            }
+            __syncthreads();  // the stage is reused by the next pass
         }
     }
 }
@@ -255,26 +300,20 @@ int coverGrid(cie_hip_context ctx, int rows) {
 
 int refill(cie_hip_csr m, PairedForm& form) {
     cie_hip_context ctx = m->context;
-    const int widths[2] = {form.wide, form.narrow};
-    int64_t offset = 0;
-    for (int w = 0; w < 2; ++w) {
-        const int64_t blocks = form.blocks[w];
-        if (blocks) {
-            const dim3 grid(unsigned(vectorGrid(blocks)));
-            hipLaunchKernelGGL((pairedValuesKernel<double>), grid, dim3(kReduceBlock), 0, ctx->stream, blocks,
-                               widths[w], offset, form.sortedStart, form.sortedEntry, form.sortedCount,
-                               m->columnIndices, static_cast<const double*>(m->entries64),
-                               form.values64 + form.blocks[0] * (w ? form.wide : 0));
+    if (form.tileCount && form.blocks[0] + form.blocks[1]) {
+        const dim3 grid(unsigned(form.tileCount)), block(kValuesThreads);
+        hipLaunchKernelGGL((pairedValuesKernel<double>), grid, block, 0, ctx->stream, form.begins, form.keys,
+                           form.sortedStart, form.sortedEntry, form.sortedCount, form.blocks[0], form.blocks[1],
+                           form.wide, form.narrow, m->columnIndices, static_cast<const double*>(m->entries64),
+                           form.values64);
+        CIE_HIP_CHECK(hipGetLastError());
+        if (form.values32) {
+            hipLaunchKernelGGL((pairedValuesKernel<float>), grid, block, 0, ctx->stream, form.begins, form.keys,
+                               form.sortedStart, form.sortedEntry, form.sortedCount, form.blocks[0], form.blocks[1],
+                               form.wide, form.narrow, m->columnIndices, static_cast<const float*>(m->entries32),
+                               form.values32);
             CIE_HIP_CHECK(hipGetLastError());
-            if (form.values32) {
-                hipLaunchKernelGGL((pairedValuesKernel<float>), grid, dim3(kReduceBlock), 0, ctx->stream, blocks,
-                                   widths[w], offset, form.sortedStart, form.sortedEntry, form.sortedCount,
-                                   m->columnIndices, static_cast<const float*>(m->entries32),
-                                   form.values32 + form.blocks[0] * (w ? form.wide : 0));
-                CIE_HIP_CHECK(hipGetLastError());
-            }
         }
-        offset += blocks;
     }
     form.version = m->valuesVersion;
     return CIE_HIP_OK;
