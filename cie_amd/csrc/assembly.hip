@@ -289,7 +289,9 @@ void releasePlan(cie_hip_assembly plan) {
     if (plan->pairElements) (void)hipFree(plan->pairElements);
     if (plan->pairLocals) (void)hipFree(plan->pairLocals);
     if (plan->tiles) (void)hipFree(plan->tiles);
+    cie_hip_csr matrix = plan->matrix;
     delete plan;
+    if (matrix) dropMatrix(matrix);
 }
 
 }  // namespace
@@ -326,6 +328,7 @@ int cie_hip_assembly_create(cie_hip_csr matrix, int64_t elements, int32_t nodesP
     CIE_HIP_CHECK(hipSetDevice(ctx->device));
     auto* p = new cie_hip_assembly_s;
     p->matrix = matrix;
+    holdMatrix(matrix);
     p->elements = elements;
     p->nodes = nodesPerElement;
     auto bail = [&](int status, const std::string& message) {

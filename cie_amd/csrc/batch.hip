@@ -841,7 +841,11 @@ void releaseBatch(cie_hip_batch batch) {
     if (batch->pinnedActive) (void)hipHostFree(batch->pinnedActive);
     if (batch->pinnedStates) (void)hipHostFree(batch->pinnedStates);
     freeDevice(batch->workspace);
+    cie_hip_context context = batch->context;
+    cie_hip_csr matrix = batch->matrix;
     delete batch;
+    if (matrix) dropMatrix(matrix);
+    if (context) dropContext(context);
 }
 
 // Switches the batch between tiles of entries (0) and of column blocks,
@@ -1140,6 +1144,7 @@ int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t sy
     *batch = nullptr;
     if (!context || !matrix || systems < 1 || !offsets)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_batch_create: invalid arguments");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_batch_create");
     if (matrix->rows != matrix->columns || offsets[0] != 0 || offsets[systems] != matrix->rows)
         return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_batch_create: offsets must cover the square matrix's rows");
     for (int s = 0; s < systems; ++s)
@@ -1151,6 +1156,8 @@ int cie_hip_batch_create(cie_hip_context context, cie_hip_csr matrix, int32_t sy
     auto* b = new cie_hip_batch_s;
     b->context = context;
     b->matrix = matrix;
+    holdContext(context);
+    holdMatrix(matrix);
     b->systems = systems;
     b->offsets.assign(offsets, offsets + systems + 1);
     b->tileRows.assign(size_t(systems), kDefaultTileRows);

@@ -23,6 +23,17 @@ constexpr int kXcds = 8;           // MI355X accelerator complex dies (blocks ar
 // Error state: the C ABI never throws; each failure sets a thread-local message.
 int setError(int status, const std::string& message);
 
+// A matrix, mask or batch belongs to the context it was created on: its
+// derived forms, workspace and solver scratch live there, and its work is
+// ordered on that context's stream. A call that names another context is
+// refused (CIE_HIP_ERROR_INVALID_ARGUMENT).
+#define CIE_HIP_REQUIRE_CONTEXT(context, owner, name)                                   \
+    do {                                                                                  \
+        if ((context) != (owner))                                                         \
+            return ::ciehip::setError(CIE_HIP_ERROR_INVALID_ARGUMENT,                     \
+                                      std::string(name) + ": an argument belongs to another context"); \
+    } while (0)
+
 #define CIE_HIP_CHECK(call)                                                               \
     do {                                                                                  \
         hipError_t cieHipError_ = (call);                                                 \
@@ -81,7 +92,18 @@ struct Scratch {
 
 }  // namespace ciehip
 
+// Lifetime of the library's objects: a context is held by every object created
+// on it, a matrix by the batches, assembly and multigrid plans over it, a mask
+// by the multigrid plans over it. Destroying an object that is still held only
+// marks it; it is released with its last holder, so objects may be destroyed
+// in any order.
+struct Holds {
+    int holders = 0;
+    bool destroyRequested = false;
+};
+
 struct cie_hip_context_s {
+    Holds holds;
     int device = 0;
     hipStream_t stream = nullptr;
     bool ownsStream = false;
@@ -220,6 +242,7 @@ struct PairedForm {
 };
 
 struct cie_hip_csr_s {
+    Holds holds;
     cie_hip_context context = nullptr;
     int64_t rows = 0, columns = 0, nnz = 0;
     int32_t* rowExtents = nullptr;
@@ -254,6 +277,16 @@ struct cie_hip_csr_s {
     uint64_t filledVersion = 0;        // of a block matrix: the parent's valuesVersion its values hold
     std::vector<PairedForm*> pairedForms;  // built on first use by paired launches
 };
+
+namespace ciehip {
+// Holds (see Holds): hold on creation of a holder, drop on its release; a
+// drop that ends the holds of an object whose destruction was asked for
+// releases it.
+void holdContext(cie_hip_context context);
+void dropContext(cie_hip_context context);
+void holdMatrix(cie_hip_csr matrix);
+void dropMatrix(cie_hip_csr matrix);
+}  // namespace ciehip
 
 namespace ciehip {
 // Returns workspace of at least `bytes` bytes owned by the matrix handle.

@@ -107,8 +107,11 @@ int cie_hip_context_create(int device, void* stream, cie_hip_context* context) {
     return CIE_HIP_OK;
 }
 
-int cie_hip_context_destroy(cie_hip_context context) {
-    if (!context) return CIE_HIP_OK;
+}  // extern "C"
+
+namespace {
+
+void destroyContext(cie_hip_context context) {
     (void)hipSetDevice(context->device);
     if (context->stream) (void)hipStreamSynchronize(context->stream);
     auto& s = context->scratch;
@@ -124,6 +127,29 @@ int cie_hip_context_destroy(cie_hip_context context) {
         if (event) (void)hipEventDestroy(event);
     if (context->ownsStream && context->stream) (void)hipStreamDestroy(context->stream);
     delete context;
+}
+
+}  // namespace
+
+namespace ciehip {
+
+void holdContext(cie_hip_context context) { ++context->holds.holders; }
+
+void dropContext(cie_hip_context context) {
+    if (--context->holds.holders == 0 && context->holds.destroyRequested) destroyContext(context);
+}
+
+}  // namespace ciehip
+
+extern "C" {
+
+int cie_hip_context_destroy(cie_hip_context context) {
+    if (!context || context->holds.destroyRequested) return CIE_HIP_OK;
+    if (context->holds.holders > 0) {
+        context->holds.destroyRequested = true;  // released with the last object created on it
+        return CIE_HIP_OK;
+    }
+    destroyContext(context);
     return CIE_HIP_OK;
 }
 

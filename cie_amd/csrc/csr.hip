@@ -287,7 +287,9 @@ void releaseMatrix(cie_hip_csr m) {
     if (m->blockEntryBegin) (void)hipFree(m->blockEntryBegin);
     for (cie_hip_csr blocks : m->columnBlocks) releaseMatrix(blocks);
     for (PairedForm* form : m->pairedForms) releasePairedForm(m->context, form);
+    cie_hip_context context = m->context;
     delete m;
+    if (context) dropContext(context);
 }
 
 int rowStatistics(cie_hip_csr m, const int32_t* hostExtents) {
@@ -345,6 +347,12 @@ int inverseDiagonal(cie_hip_csr m, T* inverse, int64_t* failedRow, float* correc
 }  // namespace
 
 namespace ciehip {
+
+void holdMatrix(cie_hip_csr m) { ++m->holds.holders; }
+
+void dropMatrix(cie_hip_csr m) {
+    if (--m->holds.holders == 0 && m->holds.destroyRequested) releaseMatrix(m);
+}
 
 int residentBlocks(const void* kernel, int block) {
     static std::mutex mutex;
@@ -584,6 +592,7 @@ int adoptMatrix(cie_hip_context context, int64_t rows, int64_t columns, int32_t*
                 double* entries64, float* entries32, std::vector<int32_t> hostExtents, cie_hip_csr* matrix) {
     auto* m = new cie_hip_csr_s;
     m->context = context;
+    holdContext(context);
     m->rows = rows;
     m->columns = columns;
     m->nnz = hostExtents[size_t(rows)];
@@ -789,6 +798,7 @@ int cie_hip_csr_create(cie_hip_context context, int64_t rows, int64_t columns, i
 
     auto* m = new cie_hip_csr_s;
     m->context = context;
+    holdContext(context);
     m->rows = rows;
     m->columns = columns;
     m->nnz = nnz;
@@ -841,6 +851,11 @@ int cie_hip_csr_copy_out(cie_hip_csr matrix, int32_t* rowExtents, int32_t* colum
 }
 
 int cie_hip_csr_destroy(cie_hip_csr matrix) {
+    if (!matrix || matrix->holds.destroyRequested) return CIE_HIP_OK;
+    if (matrix->holds.holders > 0) {
+        matrix->holds.destroyRequested = true;  // released with the last batch or plan over it
+        return CIE_HIP_OK;
+    }
     releaseMatrix(matrix);
     return CIE_HIP_OK;
 }
@@ -1109,6 +1124,7 @@ int cie_hip_csr_autotune(cie_hip_csr matrix, int precision, int kernels, int rep
 int cie_hip_csr_product_f64(cie_hip_context context, cie_hip_csr matrix, double inScale, const double* in,
                             int64_t inSize, double outScale, double* out, int64_t outSize) {
     if (!context || !matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f64: null handle");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_csr_product_f64");
     if (matrix->rows != outSize || matrix->columns != inSize)
         return setError(CIE_HIP_ERROR_SHAPE, "Incompatible matrix-vector product: [" + std::to_string(matrix->rows) + "x"
                                                  + std::to_string(matrix->columns) + "] * [" + std::to_string(inSize)
@@ -1123,6 +1139,7 @@ int cie_hip_csr_product_f64(cie_hip_context context, cie_hip_csr matrix, double 
 int cie_hip_csr_product_f32(cie_hip_context context, cie_hip_csr matrix, float inScale, const float* in,
                             int64_t inSize, float outScale, float* out, int64_t outSize) {
     if (!context || !matrix) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_product_f32: null handle");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_csr_product_f32");
     if (matrix->rows != outSize || matrix->columns != inSize)
         return setError(CIE_HIP_ERROR_SHAPE, "Incompatible matrix-vector product: [" + std::to_string(matrix->rows) + "x"
                                                  + std::to_string(matrix->columns) + "] * [" + std::to_string(inSize)
@@ -1137,6 +1154,7 @@ int cie_hip_csr_product_f32(cie_hip_context context, cie_hip_csr matrix, float i
 int cie_hip_inverse_diagonal_f64(cie_hip_context context, cie_hip_csr matrix, double* inverse, int64_t* failedRow) {
     if (!context || !matrix || (!inverse && matrix->rows))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_inverse_diagonal_f64: null argument");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_inverse_diagonal_f64");
     if (matrix->rows != matrix->columns) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_inverse_diagonal_f64: matrix is not square");
     CIE_HIP_CHECK(hipSetDevice(context->device));
     return inverseDiagonal<double>(matrix, inverse, failedRow);
@@ -1145,6 +1163,7 @@ int cie_hip_inverse_diagonal_f64(cie_hip_context context, cie_hip_csr matrix, do
 int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, float* inverse, int64_t* failedRow) {
     if (!context || !matrix || (!inverse && matrix->rows))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_inverse_diagonal_f32: null argument");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_inverse_diagonal_f32");
     if (matrix->rows != matrix->columns) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_inverse_diagonal_f32: matrix is not square");
     CIE_HIP_CHECK(hipSetDevice(context->device));
     return inverseDiagonal<float>(matrix, inverse, failedRow);
@@ -1153,8 +1172,7 @@ int cie_hip_inverse_diagonal_f32(cie_hip_context context, cie_hip_csr matrix, fl
 int cie_hip_csr_row_sum_correction(cie_hip_context context, cie_hip_csr matrix, float* correction) {
     if (!context || !matrix || (!correction && matrix->rows))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: null argument");
-    if (context != matrix->context)
-        return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: matrix of another context");
+    CIE_HIP_REQUIRE_CONTEXT(context, matrix->context, "cie_hip_csr_row_sum_correction");
     if (!matrix->entries64)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_row_sum_correction: matrix has no FP64 plane");
     CIE_HIP_CHECK(hipSetDevice(context->device));

@@ -23,6 +23,7 @@
 using namespace ciehip;
 
 struct cie_hip_mask_s {
+    Holds holds;
     cie_hip_context context = nullptr;
     int64_t size = 0;
     uint16_t* values = nullptr;     // device
@@ -295,6 +296,8 @@ int substitution(cie_hip_context ctx, cie_hip_csr m, const cie_hip_mask_s* mask,
                  const char* name) {
     if (!ctx || !m || (size && (!in || !out)))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid arguments");
+    CIE_HIP_REQUIRE_CONTEXT(ctx, m->context, name);
+    if (mask) CIE_HIP_REQUIRE_CONTEXT(ctx, mask->context, name);
     if (const int s = checkVectors(m, size, name)) return s;
     if (mask && mask->size != size) return setError(CIE_HIP_ERROR_SHAPE, std::string(name) + ": mask size differs");
     if (inScale != 0.0 && iterations != 1)  // MaskedJacobiOperator.cpp:64-65, ApproximateSubstitution.cpp:40-41
@@ -327,7 +330,7 @@ int substitution(cie_hip_context ctx, cie_hip_csr m, const cie_hip_mask_s* mask,
 
 struct cie_hip_pmg_s {
     cie_hip_csr matrix = nullptr;
-    const cie_hip_mask_s* mask = nullptr;
+    cie_hip_mask_s* mask = nullptr;   // held by the plan
     int order = 0;
     std::vector<cie_hip_csr> grids;   // grid g: the matrix masked to mask < g + 2 (owned unless it is `matrix`)
     cie_hip_csr coarse = nullptr;      // mask < 2, rows and columns renumbered
@@ -338,6 +341,14 @@ struct cie_hip_pmg_s {
 
 namespace {
 
+void releaseMask(cie_hip_mask mask) {
+    cie_hip_context context = mask->context;
+    (void)hipSetDevice(context->device);
+    if (mask->values) (void)hipFree(mask->values);
+    delete mask;
+    dropContext(context);
+}
+
 void releasePmg(cie_hip_pmg plan) {
     if (!plan) return;
     for (cie_hip_csr g : plan->grids)
@@ -345,7 +356,11 @@ void releasePmg(cie_hip_pmg plan) {
     if (plan->coarse) cie_hip_csr_destroy(plan->coarse);
     if (plan->coarseIndex) (void)hipFree(plan->coarseIndex);
     if (plan->work) (void)hipFree(plan->work);
+    cie_hip_csr matrix = plan->matrix;
+    cie_hip_mask mask = plan->mask;
     delete plan;
+    if (matrix) dropMatrix(matrix);
+    if (mask && --mask->holds.holders == 0 && mask->holds.destroyRequested) releaseMask(mask);
 }
 
 }  // namespace
@@ -360,6 +375,7 @@ int cie_hip_mask_create(cie_hip_context context, const uint16_t* values, int64_t
     CIE_HIP_CHECK(hipSetDevice(context->device));
     auto* m = new cie_hip_mask_s;
     m->context = context;
+    holdContext(context);
     m->size = size;
     m->host.assign(values, values + size);
     m->largest = size ? *std::max_element(values, values + size) : 0;
@@ -367,6 +383,7 @@ int cie_hip_mask_create(cie_hip_context context, const uint16_t* values, int64_t
         (size && hipMemcpy(m->values, values, sizeof(uint16_t) * size_t(size), hipMemcpyHostToDevice) != hipSuccess)) {
         if (m->values) (void)hipFree(m->values);
         delete m;
+        dropContext(context);
         return setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_mask_create: upload failed");
     }
     *mask = m;
@@ -374,10 +391,12 @@ int cie_hip_mask_create(cie_hip_context context, const uint16_t* values, int64_t
 }
 
 int cie_hip_mask_destroy(cie_hip_mask mask) {
-    if (!mask) return CIE_HIP_OK;
-    (void)hipSetDevice(mask->context->device);
-    if (mask->values) (void)hipFree(mask->values);
-    delete mask;
+    if (!mask || mask->holds.destroyRequested) return CIE_HIP_OK;
+    if (mask->holds.holders > 0) {
+        mask->holds.destroyRequested = true;  // released with the last multigrid plan over it
+        return CIE_HIP_OK;
+    }
+    releaseMask(mask);
     return CIE_HIP_OK;
 }
 
@@ -385,6 +404,7 @@ int cie_hip_csr_masked(cie_hip_csr matrix, cie_hip_mask mask, uint16_t threshold
     if (!masked) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_masked: null output");
     *masked = nullptr;
     if (!matrix || !mask) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_csr_masked: null argument");
+    CIE_HIP_REQUIRE_CONTEXT(mask->context, matrix->context, "cie_hip_csr_masked");
     if (matrix->rows != matrix->columns || mask->size != matrix->rows)
         return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_csr_masked: the mask needs one value per row of a square matrix");
     std::vector<int32_t> renumber(size_t(matrix->rows));
@@ -396,6 +416,7 @@ int cie_hip_masked_identity_f64(cie_hip_context context, cie_hip_mask mask, uint
                                 const double* in, double outScale, double* out, int64_t size) {
     if (!context || !mask || (size && (!in || !out)))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_masked_identity_f64: invalid arguments");
+    CIE_HIP_REQUIRE_CONTEXT(context, mask->context, "cie_hip_masked_identity_f64");
     if (mask->size != size) return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_masked_identity_f64: mask size differs");
     if (!size) return CIE_HIP_OK;
     CIE_HIP_CHECK(hipSetDevice(context->device));
@@ -411,6 +432,7 @@ int cie_hip_jacobi_f64(cie_hip_context context, cie_hip_csr lhs, const double* i
                        int64_t size) {
     if (!context || !lhs || (size && (!inverseDiagonal || !in || !out)))
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_jacobi_f64: invalid arguments");
+    CIE_HIP_REQUIRE_CONTEXT(context, lhs->context, "cie_hip_jacobi_f64");
     if (const int s = checkVectors(lhs, size, "cie_hip_jacobi_f64")) return s;
     if (!size) return CIE_HIP_OK;
     CIE_HIP_CHECK(hipSetDevice(context->device));
@@ -461,12 +483,15 @@ int cie_hip_pmg_create(cie_hip_csr matrix, cie_hip_mask mask, int order, cie_hip
     *plan = nullptr;
     if (!matrix || !mask || order < 1 || order > 65533)
         return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "cie_hip_pmg_create: invalid arguments");
+    CIE_HIP_REQUIRE_CONTEXT(mask->context, matrix->context, "cie_hip_pmg_create");
     if (matrix->rows != matrix->columns || mask->size != matrix->rows)
         return setError(CIE_HIP_ERROR_SHAPE, "cie_hip_pmg_create: the mask needs one value per row of a square matrix");
     if (const int s = ensureInverseDiagonal(matrix, 64)) return s;
     auto* p = new cie_hip_pmg_s;
     p->matrix = matrix;
     p->mask = mask;
+    holdMatrix(matrix);
+    ++mask->holds.holders;
     p->order = order;
     auto bail = [&](int status) {
         releasePmg(p);

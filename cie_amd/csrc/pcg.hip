@@ -751,6 +751,7 @@ struct Request {
 
 int solve(cie_hip_context ctx, cie_hip_csr m, const Request& q, cie_hip_solve_info* info) {
     if (!ctx || !m) return setError(CIE_HIP_ERROR_INVALID_ARGUMENT, "pcg: null handle");
+    CIE_HIP_REQUIRE_CONTEXT(ctx, m->context, "pcg");
     if (m->rows != m->columns || q.size != m->rows)
         return setError(CIE_HIP_ERROR_SHAPE, "incompatible vector sizes " + std::to_string(q.size) + " != "
                                                  + std::to_string(m->rows));

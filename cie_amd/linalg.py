@@ -20,6 +20,7 @@ import ctypes
 import logging
 import math
 import sys
+import weakref
 from ctypes import byref, c_double, c_float, c_int, c_int64, c_void_p
 
 import numpy as np
@@ -50,14 +51,25 @@ def _kernel_code(name):
 
 
 class Context:
-    """One device + one HIP stream (cie_hip_context)."""
+    """One device + one HIP stream (cie_hip_context).
+
+    The objects created on a context (matrices, masks, batches, assembly and multigrid plans) hold it; closing
+    the context first closes those still open, newest first, so the order in which the garbage collector
+    finalizes them does not matter."""
 
     _defaults = {}
 
     def __init__(self, device=0, stream=None):
         self.device = device
         self.handle = c_void_p()
+        self._dependents = []  # weak references, in creation order
         check(library().cie_hip_context_create(device, stream, byref(self.handle)), "cie_hip_context_create")
+
+    def _register(self, obj):
+        """Records an object whose library handle lives on this context."""
+        if len(self._dependents) >= 256 and len(self._dependents) % 256 == 0:
+            self._dependents = [ref for ref in self._dependents if ref() is not None]
+        self._dependents.append(weakref.ref(obj))
 
     @classmethod
     def default(cls, device=0):
@@ -70,6 +82,11 @@ class Context:
 
     def close(self):
         if self.handle:
+            for ref in reversed(getattr(self, "_dependents", [])):
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._dependents = []
             library().cie_hip_context_destroy(self.handle)
             self.handle = c_void_p()
 
@@ -108,6 +125,7 @@ class CSRMatrix:
                                            extents.ctypes.data_as(c_void_p), cols.ctypes.data_as(c_void_p),
                                            vals.ctypes.data_as(c_void_p), _lib.HOST, planes, byref(self.handle)),
               "cie_hip_csr_create")
+        self.context._register(self)
 
     def rowCount(self):
         return self.rows
@@ -160,6 +178,7 @@ class CSRMatrix:
         matrix = cls.__new__(cls)
         matrix.context, matrix.handle = context, handle
         matrix.rows, matrix.columns, matrix.nnz = rows, columns, nnz
+        context._register(matrix)
         return matrix
 
     def copy_out(self):
@@ -620,6 +639,7 @@ class Mask:
         self.handle = c_void_p()
         check(library().cie_hip_mask_create(self.context.handle, self.values.ctypes.data_as(c_void_p),
                                             self.values.size, byref(self.handle)), "cie_hip_mask_create")
+        self.context._register(self)
 
     def close(self):
         if self.handle:
@@ -737,6 +757,7 @@ class PMultigrid:
         self.handle = c_void_p()
         check(library().cie_hip_pmg_create(matrix.handle, mask.handle, self.order, byref(self.handle)),
               "cie_hip_pmg_create")
+        matrix.context._register(self)
         self.info = None
 
     def grid(self, index):
@@ -852,6 +873,7 @@ class Assembly:
         check(library().cie_hip_assembly_create(matrix.handle, self.elements, self.nodes,
                                                 dofs.ctypes.data_as(c_void_p), byref(self.handle)),
               "cie_hip_assembly_create")
+        matrix.context._register(self)
 
     def assemble(self, element_matrices, scale=None):
         """Overwrite the matrix values with sum_e scale[e] K_e. element_matrices (device tensor) is one
@@ -916,6 +938,7 @@ class Batch:
         check(library().cie_hip_batch_create(matrix.context.handle, matrix.handle, self.systems,
                                              self.offsets.ctypes.data_as(c_void_p), byref(self.handle)),
               "cie_hip_batch_create")
+        matrix.context._register(self)
 
     def autotune(self, precision=32, repetitions=3):
         check(library().cie_hip_batch_autotune(self.handle, precision, repetitions), "cie_hip_batch_autotune")

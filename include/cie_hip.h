@@ -15,6 +15,10 @@
  *   - All work of a context is ordered on its one HIP stream. Functions that
  *     return a host value (inner products, solver statistics) synchronise it.
  *   - One context is not thread-safe; separate contexts are independent.
+ *   - A matrix, mask, batch, assembly or multigrid plan belongs to the context
+ *     it was created on (its derived forms, workspace and solver scratch live
+ *     there). A call that passes a context together with such an object must
+ *     pass that context; another one is CIE_HIP_ERROR_INVALID_ARGUMENT.
  */
 #ifndef CIE_HIP_H
 #define CIE_HIP_H
@@ -111,6 +115,8 @@ int cie_hip_device_count(int* count);
  * existing hipStream_t (borrowed, e.g. torch's current stream) or NULL to
  * create one. Replaces the sycl::queue held by SYCLSpace (SYCLSpace.hpp:128). */
 int cie_hip_context_create(int device, void* stream, cie_hip_context* context);
+/* Destroy the objects created on a context (matrices, masks, batches,
+ * assembly and multigrid plans) before the context itself. */
 int cie_hip_context_destroy(cie_hip_context context);
 int cie_hip_context_synchronize(cie_hip_context context);
 int cie_hip_context_stream(cie_hip_context context, void** stream);
