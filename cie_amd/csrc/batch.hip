@@ -79,6 +79,7 @@ static_assert(BatchTiles<double, kBatchPaired>::kCapacity <= kTileEntries, "pair
 
 struct SystemState {
     double rho, alpha, beta, rr, initialNorm, absolute, relative, updateNorm;
+    double recursiveRho;  // mixed: r^T z of the recursive residual on the trip that folds
     uint64_t iteration, refinements;
     int32_t done, refine;
 };
@@ -436,21 +437,26 @@ __global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* _
         }
         finished = s.done != kRunning;
     } else if (kPhase == kPhaseAlpha) {
-        if (!(TR(0) < AR)) {
+        // Mixed: does this trip refine? The FP32 r.r the previous trip left fell by the refinement factor since
+        // the last refinement, looks converged, or the iteration cap is near. The update then folds the
+        // correction into x and the residual kernel recomputes r from it.
+        bool refine = false;
+        if (settings.mixed) {
+            const double absolute = sqrt(s.rr);
+            const double relative = absolute / s.initialNorm;
+            refine = s.rr <= settings.updateThreshold * s.updateNorm ||
+                     floatLess(absolute, settings.absoluteTolerance) ||
+                     floatLess(relative, settings.relativeTolerance) || s.iteration + 1 >= settings.maxIterations;
+        }
+        // p.q = 0 on a refining trip: the FP32 recurrence solved the system exactly (r, then p, became
+        // zero); the trip folds without a step and the true residual decides. Any other p.q <= 0 ends it.
+        const bool vanished = AR == TR(0) && refine;
+        if (!(TR(0) < AR) && !vanished) {
             s.done = kNotSpd;
             finished = true;
         } else {
-            s.alpha = double(TR(s.rho) / AR);
-            if (settings.mixed) {
-                // Does this trip refine? The FP32 r.r the previous trip left fell by the refinement factor
-                // since the last refinement, looks converged, or the iteration cap is near. The update then
-                // folds the correction into x and the residual kernel recomputes r from it.
-                const double absolute = sqrt(s.rr);
-                const double relative = absolute / s.initialNorm;
-                s.refine = s.rr <= settings.updateThreshold * s.updateNorm ||
-                           floatLess(absolute, settings.absoluteTolerance) ||
-                           floatLess(relative, settings.relativeTolerance) || s.iteration + 1 >= settings.maxIterations;
-            }
+            s.alpha = vanished ? 0.0 : double(TR(s.rho) / AR);
+            if (settings.mixed) s.refine = refine;
         }
     } else if (kPhase == kPhaseUpdate) {
         const double absolute = sqrt(double(AR));
@@ -471,10 +477,12 @@ __global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* _
     } else if (kPhase == kPhaseUpdateMixed) {
         const double absolute = sqrt(double(AR));
         const double relative = absolute / s.initialNorm;
-        if (!s.refine) {  // (a refining trip advances in the refinement phase, from the true residual)
+        if (s.refine) {  // a refining trip advances in the refinement phase, from the true residual
+            s.recursiveRho = double(B);
+        } else {
             const TR previous = TR(s.rho);
             s.rho = double(B);
-            s.beta = double(B / previous);
+            s.beta = previous == TR(0) ? 0.0 : double(B / previous);  // a vanished residual restarts p
             s.rr = double(AR);
             s.absolute = absolute;
             s.relative = relative;
@@ -491,7 +499,9 @@ __global__ __launch_bounds__(kTileBlock) void batchFinishKernel(const int32_t* _
         } else {
             const TR previous = TR(s.rho);
             s.rho = double(B);
-            s.beta = settings.restart ? 0.0 : double(B / previous);
+            // A replaced residual far above the recursive one restarts the direction (kRestartGap).
+            const bool gap = double(B) > kRestartGap * s.recursiveRho;
+            s.beta = settings.restart || previous == TR(0) || gap ? 0.0 : double(B / previous);
             s.updateNorm = A64;
             s.iteration += 1;
             if (s.iteration >= settings.maxIterations) s.done = kMaxIterations;
@@ -571,6 +581,81 @@ __global__ __launch_bounds__(kTileBlock) void batchExpandKernel(const int32_t* _
         const int k = locate(i, work.chunkPrefix, selected);
         const int s = work.systems[k];
         work.chunkList[i] = make_int2(systemChunks[s] + (i - work.chunkPrefix[k]), s);
+    }
+}
+
+// The two steps above in one kernel, for batches of at most kSelectSystems
+// systems: every block ranks the selected systems itself (their prefix sums
+// of tiles and chunks, in LDS; the order and sums are the compaction's) and
+// writes its share of the items; block 0 writes the counts.
+constexpr int kSelectSystems = 4 * kTileBlock;
+
+template <int kSelect>
+__global__ __launch_bounds__(kTileBlock) void batchSelectKernel(int systems, const SystemState* __restrict__ state,
+                                                                const int32_t* __restrict__ systemTiles,
+                                                                const int32_t* __restrict__ systemChunks,
+                                                                ActiveWork work) {
+    __shared__ int32_t selectedSystems[kSelectSystems];
+    __shared__ int32_t tilePrefix[kSelectSystems + 1], chunkPrefix[kSelectSystems + 1];
+    __shared__ int32_t scanRun[kTileBlock], scanTiles[kTileBlock], scanChunks[kTileBlock];
+    const int tid = threadIdx.x;
+    const int per = (systems + kTileBlock - 1) / kTileBlock;  // at most 4
+    const int lo = min(systems, tid * per), hi = min(systems, lo + per);
+    auto selected = [&](int s) { return !state[s].done && (kSelect == kRunning || state[s].refine); };
+    int run = 0, tiles = 0, chunks = 0;
+    for (int s = lo; s < hi; ++s) {
+        if (!selected(s)) continue;
+        run += 1;
+        tiles += systemTiles[s + 1] - systemTiles[s];
+        chunks += systemChunks[s + 1] - systemChunks[s];
+    }
+    scanRun[tid] = run;
+    scanTiles[tid] = tiles;
+    scanChunks[tid] = chunks;
+    __syncthreads();
+    for (int offset = 1; offset < kTileBlock; offset *= 2) {  // inclusive Hillis-Steele scan
+        const int a = tid >= offset ? scanRun[tid - offset] : 0;
+        const int b = tid >= offset ? scanTiles[tid - offset] : 0;
+        const int c = tid >= offset ? scanChunks[tid - offset] : 0;
+        __syncthreads();
+        scanRun[tid] += a;
+        scanTiles[tid] += b;
+        scanChunks[tid] += c;
+        __syncthreads();
+    }
+    int k = scanRun[tid] - run, tileAt = scanTiles[tid] - tiles, chunkAt = scanChunks[tid] - chunks;
+    for (int s = lo; s < hi; ++s) {
+        if (!selected(s)) continue;
+        selectedSystems[k] = s;
+        tilePrefix[k] = tileAt;
+        chunkPrefix[k] = chunkAt;
+        tileAt += systemTiles[s + 1] - systemTiles[s];
+        chunkAt += systemChunks[s + 1] - systemChunks[s];
+        k += 1;
+    }
+    const int count = scanRun[kTileBlock - 1];
+    const int tileTotal = scanTiles[kTileBlock - 1];
+    const int chunkTotal = kSelect == kRunning ? scanChunks[kTileBlock - 1] : 0;
+    if (tid == 0) {
+        tilePrefix[count] = tileTotal;
+        chunkPrefix[count] = chunkTotal;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) {
+        work.count[0] = count;
+        work.count[1] = tileTotal;
+        work.count[2] = chunkTotal;
+    }
+    const int i = int(blockIdx.x) * kTileBlock + tid;
+    if (i < tileTotal) {
+        const int j = locate(i, tilePrefix, count);
+        const int s = selectedSystems[j];
+        work.tileList[i] = make_int2(systemTiles[s] + (i - tilePrefix[j]), s);
+    }
+    if (i < chunkTotal) {
+        const int j = locate(i, chunkPrefix, count);
+        const int s = selectedSystems[j];
+        work.chunkList[i] = make_int2(systemChunks[s] + (i - chunkPrefix[j]), s);
     }
 }
 
@@ -932,11 +1017,17 @@ void launchTileRange(int code, int precision, int tiles, hipStream_t stream, con
 template <int kSelect>
 void launchSelect(cie_hip_batch batch, const ActiveWork& work) {
     const hipStream_t stream = batch->context->stream;
+    const int items = std::max(batch->tileCount, kSelect == kRunning ? batch->chunkCount : 0);
+    const dim3 grid(std::max(1, (items + kTileBlock - 1) / kTileBlock));
+    if (batch->systems <= kSelectSystems) {
+        hipLaunchKernelGGL((batchSelectKernel<kSelect>), grid, dim3(kTileBlock), 0, stream, batch->systems,
+                           batch->state, batch->systemTiles, batch->systemChunks, work);
+        return;
+    }
     hipLaunchKernelGGL((batchCompactKernel<kSelect>), dim3(1), dim3(kCompactThreads), 0, stream, batch->systems,
                        batch->state, batch->systemTiles, batch->systemChunks, work);
-    const int items = std::max(batch->tileCount, kSelect == kRunning ? batch->chunkCount : 0);
-    hipLaunchKernelGGL(batchExpandKernel, dim3(std::max(1, (items + kTileBlock - 1) / kTileBlock)), dim3(kTileBlock), 0,
-                       stream, batch->systemTiles, batch->systemChunks, work);
+    hipLaunchKernelGGL(batchExpandKernel, grid, dim3(kTileBlock), 0, stream, batch->systemTiles, batch->systemChunks,
+                       work);
 }
 
 // Enqueues the kernels of one iteration on the context stream.

@@ -69,6 +69,15 @@ struct SolverState {
     uint64_t historyCapacity;
 };
 
+// Mixed forms: after a refinement, a replaced (true) residual whose r^T z
+// exceeds the recursive one's by more than this factor restarts the search
+// direction. The recursion then ran below the FP32 floor of the true
+// residual (a system that one FP32 step solves, such as a single row), and
+// combining the stale direction with beta = rho_true / rho_recursive can
+// grow the residual without bound. At an ordinary refinement the two agree
+// closely, so the factor never triggers there.
+constexpr double kRestartGap = 4.0;
+
 enum SolveEnd : int32_t {
     kRunning = 0,
     kConverged = 1,

@@ -343,11 +343,15 @@ __global__ __launch_bounds__(kVectorThreads) void mixedUpdateKernel(int slot, in
         fold = refineDue(s);
     }
     const float pq = sumPartialsAll<kVectorThreads, float>(productPartials, productCount, 0, lds);
-    if (!(0.0f < pq)) {
+    // p.q = 0 on a trip that folds, or right after one: the FP32 recurrence solved its system exactly (r, then
+    // p, became zero). The trip takes no step (folding, the true residual then decides; after a fold, the
+    // direction restarts from the true residual); any other p.q <= 0 ends the solve.
+    const bool vanished = pq == 0.0f && (fold || replaced);
+    if (!(0.0f < pq) && !vanished) {
         if (firstThread()) s.done = kNotSpd;
         return;
     }
-    const float alpha = float(s.rhos[slot]) / pq;
+    const float alpha = vanished ? 0.0f : float(s.rhos[slot]) / pq;
     if (firstThread()) {
         s.alpha = double(alpha);
         s.refined[slot] = fold ? 1 : 0;
@@ -476,7 +480,13 @@ __global__ __launch_bounds__(kVectorThreads) void directionKernel(int slot, int6
         // residual replaces r on the next trip (levelRefineKernel).
         const float rr = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 0, ldsT);
         const float rz = sumPartialsAll<kVectorThreads, float>(updatePartials, updateCount, 1, ldsT);
-        beta = s.restart && s.refined[slot] ? 0.0 : double(rz / float(rhoPrevious));
+        // A vanished FP32 residual (rho 0) restarts the direction from the next residual, and so does a
+        // replaced residual (this trip's r started from the true one) that exceeds the recursive one by more
+        // than kRestartGap.
+        const bool gap = s.refined[slot ^ 1] && double(rz) > kRestartGap * rhoPrevious;
+        beta = (s.restart && s.refined[slot]) || float(rhoPrevious) == 0.0f || gap
+                   ? 0.0
+                   : double(rz / float(rhoPrevious));
         if (firstThread()) {
             const double absolute = sqrt(double(rr));
             s.rhos[slot ^ 1] = double(rz);

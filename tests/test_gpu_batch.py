@@ -235,3 +235,31 @@ def test_batch_keeps_its_column_blocks_through_the_matrix_autotuner(hip, torch_g
     assert np.all(status == 1)
     # (2A) x1 = b: x1 = x0 / 2 to the solve's tolerance
     assert np.linalg.norm(2.0 * x1.cpu().numpy() - x0.cpu().numpy()) <= 1e-8 * np.linalg.norm(x0.cpu().numpy())
+
+
+@pytest.mark.parametrize("systems_count", [1024, 1100])
+def test_batched_selection_of_many_systems(hip, torch_gpu, systems_count):
+    """The running and refining systems are listed by one kernel up to 1024 systems and by a compaction and an
+    expansion kernel above; both lists give each system the oracle's solution (FP64) and the FP64 tolerance
+    (mixed)."""
+    w = systems.q2_quad_batch(count=systems_count, seed=5, smallest=1, largest=4)
+    matrix = hip.CSRMatrix(w.matrix)
+    batch = hip.Batch(matrix, w.extra["offsets"])
+    b = torch_gpu.from_numpy(w.rhs).cuda()
+    x = torch_gpu.zeros_like(b)
+    stats, status, _, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=False)
+    assert np.all(status == 1)
+    got = x.cpu().numpy()
+    offsets = w.extra["offsets"]
+    for i in range(0, systems_count, 37):  # a sample of the systems against the oracle
+        a, c = int(offsets[i]), int(offsets[i + 1])
+        sub = systems.CSR(c - a, c - a, w.matrix.row_extents[a:c + 1] - w.matrix.row_extents[a],
+                          w.matrix.column_indices[w.matrix.row_extents[a]:w.matrix.row_extents[c]] - a,
+                          w.matrix.entries[w.matrix.row_extents[a]:w.matrix.row_extents[c]])
+        x_ref, s_ref, _, _ = oracle.pcg(sub, w.rhs[a:c], relative_tolerance=1e-10)
+        assert abs(int(stats[i].iterationCount) - int(s_ref.iterationCount)) <= 1
+        assert np.linalg.norm(got[a:c] - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+    x.zero_()
+    stats, status, refinements, _ = batch.solve(b, x, Statistics(20000, 0.0, 1e-10), mixed=True)
+    assert np.all(status == 1)
+    assert np.all(per_system_residuals(w, x.cpu().numpy()) < 1e-10)

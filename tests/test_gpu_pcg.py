@@ -189,6 +189,45 @@ def test_mixed_pcg_reaches_fp64_tolerance(hip, torch_gpu, restart, tiled):
 
 
 @pytest.mark.parametrize("restart", [False, True])
+@pytest.mark.parametrize("rows", [1, 2, 3, 300])
+def test_mixed_pcg_when_the_fp32_recurrence_solves_exactly(hip, torch_gpu, rows, restart):
+    """Diagonal systems with power-of-two entries: the first Jacobi-preconditioned FP32 step is exact (alpha = 1,
+    x = D^-1 b), so r and then p become zero and the next p.q is 0. The mixed solver folds without a step, and
+    the FP64 true residual (zero) ends the solve as converged, not as a breakdown; x equals the oracle's."""
+    diagonal = 2.0 ** (np.arange(rows) % 5 + 1)
+    csr = systems.CSR(rows, rows, np.arange(rows + 1, dtype=np.int32), np.arange(rows, dtype=np.int32), diagonal)
+    b = (np.arange(rows) % 7 + 1) * 0.5
+    x_ref, s_ref, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-10)
+    matrix = hip.CSRMatrix(csr)
+    solver = hip.MixedPCG(matrix, Statistics(100, 0.0, 1e-10), restart=restart)
+    x = torch_gpu.zeros(rows, dtype=torch_gpu.float64, device="cuda:0")
+    solver.product(0, to_dev(torch_gpu, b), 1, x)
+    assert solver.info.converged == 1
+    assert np.array_equal(x.cpu().numpy(), x_ref) and np.array_equal(x_ref, b / diagonal)
+    assert int(solver.getStats().iterationCount) <= int(s_ref.iterationCount) + 1
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mixed_pcg_restarts_below_the_fp32_floor(hip, torch_gpu, seed):
+    """Single rows and tiny systems with lognormal entries: one FP32 step takes the recursive residual far
+    below the FP32 floor of the true one (the correction is accumulated in FP32), so the first refinement's
+    replaced residual exceeds the recursive one many times over and the direction restarts, instead of
+    combining with the stale one; every solve converges to the oracle's solution."""
+    rng = np.random.default_rng(seed)
+    for elements in (1, 2):
+        csr = systems.lagrange(2, 2, elements, conductivity=np.exp(rng.normal(0.0, 1.0, elements * elements)))
+        b = rng.uniform(-1.0, 1.0, csr.rows)
+        x_ref, _, _, _ = oracle.pcg(csr, b, relative_tolerance=1e-14, max_iterations=1000)
+        solver = hip.MixedPCG(hip.CSRMatrix(csr), Statistics(1000, 0.0, 1e-10))
+        x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
+        solver.product(0, to_dev(torch_gpu, b), 1, x)
+        assert solver.info.converged == 1, (seed, elements)
+        got = x.cpu().numpy()
+        assert true_relative_residual(csr, got, b) < 1e-10
+        assert rel(got, x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("restart", [False, True])
 def test_mixed_pcg_iteration_cap_and_cadence(hip, torch_gpu, restart):
     """The mixed form stops at the cap with x holding every correction (the last trip folds) and reports the
     FP64 true residual of that x; refinements recompute r = b - A x on the trip after the fold."""
