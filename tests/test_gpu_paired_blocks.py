@@ -7,8 +7,8 @@ width's blocks column-sorted within its tile.
 - Products are checked against the oracle's row loop within the rounding bound of a reordered sum, for
   every pair and unroll, several scalings, both precisions; they are bitwise reproducible run to run and
   follow every change of the entries.
-- A matrix with a row of more blocks of one width than a tile holds is refused at its first product and
-  skipped by the autotuner.
+- A matrix with a row of more blocks of one width than a tile holds, or with a tile whose blocks of one
+  width span more columns than the key holds, is refused at its first product and skipped by the autotuner.
 - The fused FP64 and the mixed solvers on paired launches agree with the oracle's Jacobi-PCG.
 """
 import numpy as np
@@ -193,3 +193,21 @@ def test_fused_and_mixed_solves_on_paired_tiles(hip, torch_gpu, width):
     mixed.product(0, b, 1, xm)
     assert mixed.info.converged == 1 and mixed.info.trueRelativeResidual < 1e-10
     assert np.linalg.norm(xm.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
+def test_paired_refused_where_a_tile_spans_too_many_columns(hip, torch_gpu):
+    """Two rows in one tile whose 1 x 3 blocks lie 1.2M columns apart: the key's 20-bit column offset cannot
+    hold them, so the paired launch fails with an error and the autotuner keeps another kernel."""
+    cols = 1_200_000
+    csr = systems.CSR(2, cols, np.array([0, 3, 6], np.int32),
+                      np.array([0, 1, 2, cols - 3, cols - 2, cols - 1], np.int32), np.arange(1.0, 7.0))
+    x = np.ones(cols)
+    matrix = hip.CSRMatrix(csr)
+    matrix.set_launch(64, "paired", 53, 2)
+    with pytest.raises(InvalidArgument):
+        _product(hip, torch_gpu, matrix, x, np.zeros(2), 0, 1, torch_gpu.float64)
+    matrix = hip.CSRMatrix(csr)
+    matrix.autotune(64, 1, kernels=("paired", "tiled"))
+    assert matrix.launch(64)[0] == "tiled"
+    assert np.array_equal(_product(hip, torch_gpu, matrix, x, np.zeros(2), 0, 1, torch_gpu.float64),
+                          np.array([6.0, 15.0]))
