@@ -219,6 +219,21 @@ __global__ __launch_bounds__(Walker::kBlock) void applyUpdateKernel(RowSource so
     });
 }
 
+// The same on a compacted grid: grid row i is row keep[i].
+template <class Walker>
+__global__ __launch_bounds__(Walker::kBlock) void applyUpdateScatterKernel(RowSource source,
+                                                                          const double* __restrict__ entries,
+                                                                          const double* __restrict__ update,
+                                                                          const int32_t* __restrict__ keep,
+                                                                          double* __restrict__ residual,
+                                                                          double* __restrict__ solution) {
+    Walker::run(source, entries, update, [&](int row, double sum) {
+        const int32_t i = keep[row];
+        residual[i] = residual[i] - sum;
+        solution[i] = solution[i] + update[row];
+    });
+}
+
 template <template <class> class Kernel, class... Args>
 int launchSweep(cie_hip_csr m, Args... args) {
     int status = CIE_HIP_OK;
@@ -244,6 +259,10 @@ struct JacobiSweep {
 template <class Walker>
 struct ApplyUpdate {
     static constexpr auto kernel = applyUpdateKernel<Walker>;
+};
+template <class Walker>
+struct ApplyUpdateScatter {
+    static constexpr auto kernel = applyUpdateScatterKernel<Walker>;
 };
 
 // MaskedJacobiOperator (masked) and ApproximateSubstitution sweeps from
@@ -332,7 +351,11 @@ struct cie_hip_pmg_s {
     cie_hip_csr matrix = nullptr;
     cie_hip_mask_s* mask = nullptr;   // held by the plan
     int order = 0;
-    std::vector<cie_hip_csr> grids;   // grid g: the matrix masked to mask < g + 2 (owned unless it is `matrix`)
+    std::vector<cie_hip_csr> grids;   // grid g: the matrix masked to mask < g + 2 (owned unless it is `matrix`);
+                                      // for g >= 1 its kept rows and columns renumbered (keep[g] maps them back)
+    std::vector<int32_t*> keep;       // g >= 1, compacted grids: grid row -> row (null for grid 0 and the matrix)
+    std::vector<int64_t> keepRows;    // rows of each compacted grid
+    std::vector<double*> keepInverse; // the inverse diagonal on each compacted grid's rows (gathered per solve)
     cie_hip_csr coarse = nullptr;      // mask < 2, rows and columns renumbered
     int64_t coarseRows = 0;
     int32_t* coarseIndex = nullptr;    // coarse row -> row
@@ -355,6 +378,10 @@ void releasePmg(cie_hip_pmg plan) {
         if (g && g != plan->matrix) cie_hip_csr_destroy(g);
     if (plan->coarse) cie_hip_csr_destroy(plan->coarse);
     if (plan->coarseIndex) (void)hipFree(plan->coarseIndex);
+    for (int32_t* k : plan->keep)
+        if (k) (void)hipFree(k);
+    for (double* v : plan->keepInverse)
+        if (v) (void)hipFree(v);
     if (plan->work) (void)hipFree(plan->work);
     cie_hip_csr matrix = plan->matrix;
     cie_hip_mask mask = plan->mask;
@@ -498,15 +525,41 @@ int cie_hip_pmg_create(cie_hip_csr matrix, cie_hip_mask mask, int order, cie_hip
         return status;
     };
     const int64_t n = matrix->rows;
-    // Grid g keeps mask < g + 2 (solver.cpp:148-205); a grid that keeps every row is the matrix itself.
+    // Grid g keeps mask < g + 2 (solver.cpp:148-205); a grid that keeps every row is the matrix itself. The
+    // masked rows of a grid stay zero through its sweeps and update, so grids g >= 1 are compacted to their
+    // kept rows and columns (their sweeps then stream only those); grid 0 is solved on the coarse matrix.
     for (int g = 0; g < order; ++g) {
         const uint16_t threshold = uint16_t(g + 2);
         cie_hip_csr grid = matrix;
-        if (mask->largest >= threshold) {
+        int32_t* keep = nullptr;
+        int64_t keepRows = 0;
+        double* keepInverse = nullptr;
+        if (mask->largest >= threshold && g == 0) {
             const int s = cie_hip_csr_masked(matrix, mask, threshold, &grid);
             if (s != CIE_HIP_OK) return bail(s);
+        } else if (mask->largest >= threshold) {
+            std::vector<int32_t> rows, renumber(size_t(n), -1);
+            for (int64_t i = 0; i < n; ++i)
+                if (mask->host[size_t(i)] < threshold) {
+                    renumber[size_t(i)] = int32_t(rows.size());
+                    rows.push_back(int32_t(i));
+                }
+            keepRows = int64_t(rows.size());
+            if (const int s = compactMatrix(matrix, rows, renumber, keepRows, &grid)) return bail(s);
+            if (hipMalloc(&keep, sizeof(int32_t) * size_t(std::max<int64_t>(1, keepRows))) != hipSuccess ||
+                hipMalloc(&keepInverse, sizeof(double) * size_t(std::max<int64_t>(1, keepRows))) != hipSuccess ||
+                (keepRows && hipMemcpy(keep, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice) !=
+                                 hipSuccess)) {
+                if (keep) (void)hipFree(keep);
+                if (keepInverse) (void)hipFree(keepInverse);
+                p->grids.push_back(grid);
+                return bail(setError(CIE_HIP_ERROR_OUT_OF_MEMORY, "cie_hip_pmg_create: grid index upload failed"));
+            }
         }
         p->grids.push_back(grid);
+        p->keep.push_back(keep);
+        p->keepRows.push_back(keepRows);
+        p->keepInverse.push_back(keepInverse);
     }
     std::vector<int32_t> coarseIndex, renumber(size_t(n), -1);
     for (int64_t i = 0; i < n; ++i)
@@ -559,6 +612,11 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
     double* coarseX = coarseB + std::max<int64_t>(1, nc);
     const double* inverse = m->inverseDiagonal64;
     const uint16_t* maskValues = plan->mask->values;
+    for (size_t g = 0; g < plan->keep.size(); ++g)  // the matrix's entries may have changed since the last solve
+        if (plan->keep[g] && plan->keepRows[g])
+            hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(plan->keepRows[g])), dim3(kReduceBlock), 0, ctx->stream,
+                               plan->keepRows[g], static_cast<const int32_t*>(plan->keep[g]), inverse,
+                               plan->keepInverse[g]);
     constexpr double kRelaxation = 2.0 / 3.0;  // solver.cpp:196
     constexpr int kSweeps = 6;                 // solver.cpp:195
     ScopedEvents<2> events;  // start, stop
@@ -594,6 +652,29 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
                                    plan->coarseIndex, static_cast<const double*>(coarseX), update);
                 CIE_HIP_CHECK(hipGetLastError());
             }
+        } else if (const int32_t* keep = plan->keep[size_t(g)]) {
+            // NestedProductOperator(Jacobi, mask) of the restricted residual (solver.cpp:177-205) on the
+            // compacted grid: its right-hand side is the residual on the kept rows (in `update`).
+            const int64_t ng = plan->keepRows[size_t(g)];
+            const double* gridInverse = plan->keepInverse[size_t(g)];
+            double* gridB = update;
+            hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(ng)), dim3(kReduceBlock), 0, ctx->stream, ng, keep,
+                               static_cast<const double*>(residual), gridB);
+            hipLaunchKernelGGL(jacobiStartKernel, dim3(vectorGrid(ng)), dim3(kReduceBlock), 0, ctx->stream, ng,
+                               static_cast<const uint16_t*>(nullptr), threshold, static_cast<const double*>(gridB),
+                               gridInverse, kRelaxation, xa);
+            CIE_HIP_CHECK(hipGetLastError());
+            double *current = xa, *next = xb;
+            for (int k = 1; k < kSweeps; ++k) {
+                if (const int s = launchSweep<JacobiSweep>(grid, static_cast<const double*>(current),
+                                                           static_cast<const uint16_t*>(nullptr), threshold,
+                                                           static_cast<const double*>(gridB), gridInverse,
+                                                           kRelaxation, next))
+                    return s;
+                std::swap(current, next);
+            }
+            // solution += update; residual -= A_grid update on the kept rows (solver.cpp:217-218).
+            return launchSweep<ApplyUpdateScatter>(grid, static_cast<const double*>(current), keep, residual, x);
         } else {
             // NestedProductOperator(Jacobi, mask) of the restricted residual (solver.cpp:177-205).
             hipLaunchKernelGGL(jacobiStartKernel, dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, n, gridMask,
