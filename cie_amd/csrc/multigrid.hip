@@ -185,12 +185,6 @@ __global__ __launch_bounds__(kReduceBlock) void gatherKernel(int64_t n, const in
         out[i] = in[index[i]];
 }
 
-__global__ __launch_bounds__(kReduceBlock) void scatterKernel(int64_t n, const int32_t* __restrict__ index,
-                                                              const double* __restrict__ in, double* __restrict__ out) {
-    for (int64_t i = int64_t(blockIdx.x) * kReduceBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kReduceBlock)
-        out[index[i]] = in[i];
-}
-
 // ---- SpMV-based sweeps ------------------------------------------------------
 
 // x' = x + (w (b - A x)) / d for every row; b masked as in jacobiStartKernel.
@@ -375,7 +369,7 @@ void releaseMask(cie_hip_mask mask) {
 void releasePmg(cie_hip_pmg plan) {
     if (!plan) return;
     for (cie_hip_csr g : plan->grids)
-        if (g && g != plan->matrix) cie_hip_csr_destroy(g);
+        if (g && g != plan->matrix && g != plan->coarse) cie_hip_csr_destroy(g);
     if (plan->coarse) cie_hip_csr_destroy(plan->coarse);
     if (plan->coarseIndex) (void)hipFree(plan->coarseIndex);
     for (int32_t* k : plan->keep)
@@ -535,8 +529,7 @@ int cie_hip_pmg_create(cie_hip_csr matrix, cie_hip_mask mask, int order, cie_hip
         int64_t keepRows = 0;
         double* keepInverse = nullptr;
         if (mask->largest >= threshold && g == 0) {
-            const int s = cie_hip_csr_masked(matrix, mask, threshold, &grid);
-            if (s != CIE_HIP_OK) return bail(s);
+            grid = nullptr;  // grid 0 is the coarse matrix, set below
         } else if (mask->largest >= threshold) {
             std::vector<int32_t> rows, renumber(size_t(n), -1);
             for (int64_t i = 0; i < n; ++i)
@@ -570,6 +563,7 @@ int cie_hip_pmg_create(cie_hip_csr matrix, cie_hip_mask mask, int order, cie_hip
     p->coarseRows = int64_t(coarseIndex.size());
     if (p->coarseRows) {
         if (const int s = compactMatrix(matrix, coarseIndex, renumber, p->coarseRows, &p->coarse)) return bail(s);
+        if (!p->grids[0]) p->grids[0] = p->coarse;  // grid 0 keeps exactly the coarse rows
         if (hipMalloc(&p->coarseIndex, sizeof(int32_t) * coarseIndex.size()) != hipSuccess ||
             hipMemcpy(p->coarseIndex, coarseIndex.data(), sizeof(int32_t) * coarseIndex.size(), hipMemcpyHostToDevice) !=
                 hipSuccess)
@@ -638,20 +632,18 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
         const uint16_t* gridMask = grid == m ? nullptr : maskValues;
         double* result = update;
         if (g == 0) {
-            CIE_HIP_CHECK(hipMemsetAsync(update, 0, sizeof(double) * size_t(n), ctx->stream));
-            if (nc) {
-                hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(nc)), dim3(kReduceBlock), 0, ctx->stream, nc,
-                                   plan->coarseIndex, static_cast<const double*>(residual), coarseB);
-                CIE_HIP_CHECK(hipMemsetAsync(coarseX, 0, sizeof(double) * size_t(nc), ctx->stream));
-                cie_hip_solve_info coarseInfo{};
-                const int s = cie_hip_pcg_f64(ctx, plan->coarse, 0.0, coarseB, 1.0, coarseX, nc, &coarseSettings, 0,
-                                              &coarseInfo);
-                if (s != CIE_HIP_OK) return s;
-                coarseIterations += coarseInfo.stats.iterationCount;
-                hipLaunchKernelGGL(scatterKernel, dim3(vectorGrid(nc)), dim3(kReduceBlock), 0, ctx->stream, nc,
-                                   plan->coarseIndex, static_cast<const double*>(coarseX), update);
-                CIE_HIP_CHECK(hipGetLastError());
-            }
+            if (!nc) return CIE_HIP_OK;  // no row has mask < 2: grid 0 is empty and its update zero
+            hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(nc)), dim3(kReduceBlock), 0, ctx->stream, nc,
+                               plan->coarseIndex, static_cast<const double*>(residual), coarseB);
+            CIE_HIP_CHECK(hipMemsetAsync(coarseX, 0, sizeof(double) * size_t(nc), ctx->stream));
+            cie_hip_solve_info coarseInfo{};
+            const int s = cie_hip_pcg_f64(ctx, plan->coarse, 0.0, coarseB, 1.0, coarseX, nc, &coarseSettings, 0,
+                                          &coarseInfo);
+            if (s != CIE_HIP_OK) return s;
+            coarseIterations += coarseInfo.stats.iterationCount;
+            // solution += update; residual -= A_grid update on the coarse rows (solver.cpp:217-218).
+            return launchSweep<ApplyUpdateScatter>(plan->coarse, static_cast<const double*>(coarseX),
+                                                   static_cast<const int32_t*>(plan->coarseIndex), residual, x);
         } else if (const int32_t* keep = plan->keep[size_t(g)]) {
             // NestedProductOperator(Jacobi, mask) of the restricted residual (solver.cpp:177-205) on the
             // compacted grid: its right-hand side is the residual on the kept rows (in `update`).

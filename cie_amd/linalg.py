@@ -762,8 +762,8 @@ class PMultigrid:
 
     def grid(self, index):
         """The matrix of grid `index` (-1: the renumbered coarse matrix; None if there is none), a view whose
-        launch may be set or autotuned (cie_hip_pmg_grid). Grid 0 is masked in place, the grids between it and
-        the finest are compacted to their kept rows and columns, and the finest is the plan's own matrix."""
+        launch may be set or autotuned (cie_hip_pmg_grid). The masked grids are compacted to their kept rows and
+        columns (grid 0 is the coarse matrix), and the finest is the plan's own matrix."""
         handle = c_void_p()
         check(library().cie_hip_pmg_grid(self.handle, int(index), byref(handle)), "cie_hip_pmg_grid")
         return None if not handle.value else CSRMatrix._borrow(handle, self.matrix.context, self)

@@ -565,10 +565,10 @@ int cie_hip_pmg_destroy(cie_hip_pmg plan);
 int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size, double relativeTolerance,
                       uint64_t maxCycles, cie_hip_pmg_info* info);
 /* The matrix a plan uses on grid `grid`: 0 <= grid < order gives the matrix
- * masked to mask < grid + 2 (grid 0 in place; grids >= 1 compacted to their
- * kept rows and columns, in row order; the plan's own matrix where the mask
- * keeps every row), grid = -1 the renumbered coarse matrix (null when no row
- * has mask < 2).
+ * masked to mask < grid + 2, compacted to its kept rows and columns in row
+ * order (grid 0 is the coarse matrix; the plan's own matrix where the mask
+ * keeps every row), grid = -1 the coarse matrix as well (null, for both, when
+ * no row has mask < 2).
  * Borrowed: valid while the plan lives; not to be destroyed. Its SpMV launch
  * may be set or autotuned like any matrix's (cie_hip_csr_set_launch,
  * cie_hip_csr_autotune), which changes the grid's sweeps and products only in

@@ -151,7 +151,7 @@ def test_pmg_matches_oracle(hip, torch_gpu, order, n, blocks):
 @pytest.mark.parametrize("launch", [("blocks", 2, 4), ("sorted", 3, 8), ("tiled", 256, 16)])
 def test_pmg_grid_launches(hip, torch_gpu, launch):
     """Every grid's matrix is reachable (the finest grid, where the mask keeps every row, is the plan's own
-    matrix; grid 0 is masked in place; the grids between are compacted to their kept rows and columns; -1 the
+    matrix; the grids below it are compacted to their kept rows and columns, grid 0 and -1 being the
     renumbered coarse matrix), and a launch set on the masked grids and the coarse matrix changes only
     summation orders: the oracle's cycle and coarse CG counts, x to 1e-9."""
     order, n = 3, 6
@@ -162,14 +162,14 @@ def test_pmg_grid_launches(hip, torch_gpu, launch):
     pmg = hip.PMultigrid(matrix, hip.Mask(mask_values), order)
     grids = [pmg.grid(g) for g in range(order)]
     assert grids[order - 1].handle.value == matrix.handle.value  # mask < order + 1 keeps every row
-    assert grids[0].rows == csr.rows and grids[0].columns == csr.columns
-    for g in range(1, order - 1):
+    for g in range(0, order - 1):
         kept = int(np.count_nonzero(mask_values < g + 2))
         assert grids[g].rows == kept and grids[g].columns == kept
     assert [g.nnz for g in grids] == sorted(g.nnz for g in grids) and grids[0].nnz < csr.nnz
     coarse_matrix = pmg.grid(-1)
     assert coarse_matrix.rows == int(np.count_nonzero(mask_values < 2))
-    for g in grids[:order - 1] + [coarse_matrix]:
+    assert coarse_matrix.handle.value == grids[0].handle.value
+    for g in grids[:order - 1]:
         g.set_launch(64, *launch)
         assert tuple(g.launch(64)) == launch
     x = torch_gpu.zeros(csr.rows, dtype=torch_gpu.float64, device="cuda:0")
