@@ -605,8 +605,8 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
     double* coarseB = xb + n;
     double* coarseX = coarseB + std::max<int64_t>(1, nc);
     const double* inverse = m->inverseDiagonal64;
-    const uint16_t* maskValues = plan->mask->values;
-    for (size_t g = 0; g < plan->keep.size(); ++g)  // the matrix's entries may have changed since the last solve
+    // The matrix's current inverse diagonal onto each compacted grid's rows (what the sweeps read per row).
+    for (size_t g = 0; g < plan->keep.size(); ++g)
         if (plan->keep[g] && plan->keepRows[g])
             hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(plan->keepRows[g])), dim3(kReduceBlock), 0, ctx->stream,
                                plan->keepRows[g], static_cast<const int32_t*>(plan->keep[g]), inverse,
@@ -629,8 +629,6 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
     auto visit = [&](int g) -> int {
         cie_hip_csr grid = plan->grids[size_t(g)];
         const uint16_t threshold = uint16_t(g + 2);
-        const uint16_t* gridMask = grid == m ? nullptr : maskValues;
-        double* result = update;
         if (g == 0) {
             if (!nc) return CIE_HIP_OK;  // no row has mask < 2: grid 0 is empty and its update zero
             hipLaunchKernelGGL(gatherKernel, dim3(vectorGrid(nc)), dim3(kReduceBlock), 0, ctx->stream, nc,
@@ -667,23 +665,23 @@ int cie_hip_pmg_solve(cie_hip_pmg plan, const double* b, double* x, int64_t size
             }
             // solution += update; residual -= A_grid update on the kept rows (solver.cpp:217-218).
             return launchSweep<ApplyUpdateScatter>(grid, static_cast<const double*>(current), keep, residual, x);
-        } else {
-            // NestedProductOperator(Jacobi, mask) of the restricted residual (solver.cpp:177-205).
-            hipLaunchKernelGGL(jacobiStartKernel, dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, n, gridMask,
-                               threshold, static_cast<const double*>(residual), inverse, kRelaxation, xa);
-            CIE_HIP_CHECK(hipGetLastError());
-            double *current = xa, *next = xb;
-            for (int k = 1; k < kSweeps; ++k) {
-                if (const int s = launchSweep<JacobiSweep>(grid, static_cast<const double*>(current), gridMask,
-                                                           threshold, static_cast<const double*>(residual), inverse,
-                                                           kRelaxation, next))
-                    return s;
-                std::swap(current, next);
-            }
-            result = current;
+        }
+        // A grid that keeps every row (the matrix itself): the same Jacobi sweeps with nothing masked.
+        hipLaunchKernelGGL(jacobiStartKernel, dim3(vectorGrid(n)), dim3(kReduceBlock), 0, ctx->stream, n,
+                           static_cast<const uint16_t*>(nullptr), threshold, static_cast<const double*>(residual),
+                           inverse, kRelaxation, xa);
+        CIE_HIP_CHECK(hipGetLastError());
+        double *current = xa, *next = xb;
+        for (int k = 1; k < kSweeps; ++k) {
+            if (const int s = launchSweep<JacobiSweep>(grid, static_cast<const double*>(current),
+                                                       static_cast<const uint16_t*>(nullptr), threshold,
+                                                       static_cast<const double*>(residual), inverse, kRelaxation,
+                                                       next))
+                return s;
+            std::swap(current, next);
         }
         // solution += update; residual -= A_grid update (solver.cpp:217-218).
-        return launchSweep<ApplyUpdate>(grid, static_cast<const double*>(result), residual, x);
+        return launchSweep<ApplyUpdate>(grid, static_cast<const double*>(current), residual, x);
     };
 
     while (relativeTolerance * initial < norm && cycles < maxCycles) {  // solver.cpp:212
