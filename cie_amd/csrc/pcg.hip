@@ -669,8 +669,9 @@ int writeOutput(cie_hip_context ctx, double inScale, double outScale, double* ou
         hipLaunchKernelGGL((vectorKernel<double>), grid, block, 0, ctx->stream, int(kAdd), n, out, solution, outScale);
     } else if (inScale == 0.0) {
         hipLaunchKernelGGL((vectorKernel<double>), grid, block, 0, ctx->stream, int(kAssign), n, out, solution, 0.0);
-        hipLaunchKernelGGL((vectorKernel<double>), grid, block, 0, ctx->stream, int(kScaleUniform), n, out,
-                           static_cast<const double*>(nullptr), outScale);
+        if (outScale != 1.0)  // x * 1 is x, bit for bit
+            hipLaunchKernelGGL((vectorKernel<double>), grid, block, 0, ctx->stream, int(kScaleUniform), n, out,
+                               static_cast<const double*>(nullptr), outScale);
     } else {
         hipLaunchKernelGGL((vectorKernel<double>), grid, block, 0, ctx->stream, int(kScaleUniform), n, out,
                            static_cast<const double*>(nullptr), inScale);
