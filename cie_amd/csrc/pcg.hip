@@ -713,6 +713,8 @@ struct Timer {
 };
 
 constexpr int kChunk = 16;
+constexpr int kShortChunk = 4;            // trips per chunk once the residual is near the tolerance
+constexpr double kNearTolerance = 100.0;  // "near": within this factor of the relative or absolute tolerance
 
 // Enqueues chunks of iterations until the device state says done. The state
 // of chunk k is read back while chunk k + 1 is already queued, so the device
@@ -722,8 +724,9 @@ int runLoop(cie_hip_context ctx, Timer& timer, Enqueue&& enqueueIteration, doubl
     SolverState* pinned = ctx->scratch.pinnedState;
     int slot = 0;
     bool pending = false;
+    int chunk = kChunk;
     for (;;) {
-        for (int i = 0; i < kChunk; ++i) {
+        for (int i = 0; i < chunk; ++i) {
             const int status = enqueueIteration();
             if (status) return status;
         }
@@ -734,13 +737,19 @@ int runLoop(cie_hip_context ctx, Timer& timer, Enqueue&& enqueueIteration, doubl
         if (pending) {
             const int previous = slot ^ 1;
             CIE_HIP_CHECK(hipEventSynchronize(timer.marks[previous]));
-            if (pinned[previous].done) {
+            const SolverState& state = pinned[previous];
+            if (state.done) {
                 float ms = 0.0f;
                 CIE_HIP_CHECK(hipEventElapsedTime(&ms, timer.start, timer.marks[previous]));
                 *seconds = ms * 1e-3;
                 CIE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
                 return CIE_HIP_OK;
             }
+            // Near the tolerance, shorter chunks: fewer trips are queued past convergence (they only exit,
+            // but each still launches its grids). The state is device-decided, so results do not change.
+            if (state.relative < kNearTolerance * state.relativeTolerance ||
+                state.absolute < kNearTolerance * state.absoluteTolerance)
+                chunk = kShortChunk;
         }
         pending = true;
         slot ^= 1;
