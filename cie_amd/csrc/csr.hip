@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kReduceBlock) void columnBlockValuesKernel(
 // position in the tile). A tile of one row longer than the capacity is
 // skipped (it is walked unsorted). A tile whose columns span more than the
 // keys hold sets *failed.
-constexpr int kSortThreads = 1024;
+constexpr int kSortThreads = 256;
 constexpr int kSortMax = 1 << kSortedPositionBits;
 
 __global__ __launch_bounds__(kSortThreads) void sortedKeysKernel(const int2* __restrict__ tiles, int capacity,

@@ -40,7 +40,7 @@ using namespace ciehip;
 namespace {
 
 constexpr int kCoverLanes = 16;
-constexpr int kSortThreads = 1024;
+constexpr int kSortThreads = 256;
 constexpr int kPairedSortMax = 2048;  // blocks of one width per tile at most (kSpmvBlock * 8)
 constexpr int kMaxPairedWidth = 8;
 constexpr int kValuesThreads = 256;
