@@ -68,9 +68,11 @@ def kernel_stats(path, out, command):
 
 
 def _counter_means(directory, counter):
-    """Mean counter value per kernel name, and the value of the last dispatch of the copy
-    kernel: the calibration copy ends the program, and the runtime launches other copies
-    under that name before it."""
+    """Mean counter value per kernel name over its working dispatches (those at least
+    LONG_FRACTION of the kernel's largest: the solver loops enqueue a few trips past
+    convergence whose kernels return at once and move next to nothing, as in the kernel
+    time summaries), and the value of the last dispatch of the copy kernel: the calibration
+    copy ends the program, and the runtime launches other copies under that name before it."""
     files = glob.glob(str(Path(directory) / "*counter_collection.csv"))
     if not files:
         raise FileNotFoundError(f"no counter_collection.csv under {directory}")
@@ -87,7 +89,11 @@ def _counter_means(directory, counter):
             values[row["Kernel_Name"]].append(float(row["Counter_Value"]))
     if calibration is None:
         raise ValueError(f"{directory}: no {CALIBRATION_KERNEL} dispatch to calibrate {counter} with")
-    return {name: sum(v) / len(v) for name, v in values.items()}, calibration[1]
+    means = {}
+    for name, v in values.items():
+        working = [x for x in v if x >= LONG_FRACTION * max(v)] or v
+        means[name] = sum(working) / len(working)
+    return means, calibration[1]
 
 
 def traffic(fetch_dir, write_dir, out_json, out_md, command):
@@ -109,7 +115,9 @@ def traffic(fetch_dir, write_dir, out_json, out_md, command):
     result = {"command": command, "calibration": {"bytes": CALIBRATION_BYTES, "fetch_scale": fetch_scale,
                                                   "write_scale": write_scale}, "kernels": kernels}
     Path(out_json).write_text(json.dumps(result, indent=1) + "\n")
-    lines = [f"# HBM traffic per dispatch: `{command}`", "",
+    lines = [f"# HBM traffic per working dispatch: `{command}`", "",
+             f"Working dispatches: those that moved at least {LONG_FRACTION:g} of the kernel's largest (the solver "
+             "loops enqueue a few trips past convergence whose kernels return at once).", "",
              f"Calibration: a {CALIBRATION_BYTES} B device copy read FETCH_SIZE x{fetch_scale:.3f}, "
              f"WRITE_SIZE x{write_scale:.3f} of its true bytes; the figures below are scaled by those factors.", "",
              "| kernel | fetch (MB) | write (MB) | traffic (MB) |", "|---|---:|---:|---:|"]
@@ -131,7 +139,9 @@ def merge(out_json, out_md, command, *inputs):
             kernels.setdefault(name, entry)
     result = {"command": command, "calibration": calibration, "kernels": kernels}
     Path(out_json).write_text(json.dumps(result, indent=1) + "\n")
-    lines = [f"# HBM traffic per dispatch: `{command}`", "",
+    lines = [f"# HBM traffic per working dispatch: `{command}`", "",
+             f"Working dispatches: those that moved at least {LONG_FRACTION:g} of the kernel's largest (the solver "
+             "loops enqueue a few trips past convergence whose kernels return at once).", "",
              f"Calibration: a {CALIBRATION_BYTES} B device copy read FETCH_SIZE x{calibration['fetch_scale']:.3f}, "
              f"WRITE_SIZE x{calibration['write_scale']:.3f} of its true bytes in the first run (each run is scaled "
              "by its own copy); the figures below are scaled by those factors.", "",
